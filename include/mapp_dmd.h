@@ -1,0 +1,160 @@
+/*
+ * mapp_dmd.h -- C ABI of the MI355X-native DMD force field.
+ *
+ * One context holds one MPI rank's whole domain on one GPU: tables, atoms
+ * (owned + periodic ghosts), the half neighbor list and every per-atom
+ * vector stay resident in HBM between calls.  Every function returns an int
+ * status (0 = success, negative = error, message from mapp_dmd_last_error)
+ * where the reference calls error->abort (src/error.cpp:29-43).
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * force field is the C++ class ForceField (src/ff.h:12-29) with the DMD
+ * implementations ForceField_DMD / ForceField_DMD_I / ForceField_DMD_II
+ * (src/ff_dmd.cpp, src/ff_dmd_I.cpp, src/ff_dmd_II.cpp).
+ *
+ * Layouts follow the reference's per-atom vectors (src/atoms.h:21-131):
+ *   x   [natms][3+nt]  position (Angstrom) then alpha_t (1/Angstrom^2)
+ *   c   [natms][nt]    concentrations
+ *   f   [natms][3+nt]  -dU/dx, -dU/dalpha
+ *   E, dE, n, c_d ...  [natms][nt]
+ */
+#ifndef MAPP_DMD_H
+#define MAPP_DMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mapp_dmd mapp_dmd_t;
+
+enum {
+    MAPP_DMD_OK = 0,
+    MAPP_DMD_E_ARG = -1,   /* invalid argument */
+    MAPP_DMD_E_STATE = -2, /* call out of order (e.g. force before init) */
+    MAPP_DMD_E_HIP = -3,   /* HIP runtime error */
+    MAPP_DMD_E_RANGE = -4, /* value outside what the tables / kernels cover */
+    MAPP_DMD_E_IO = -5     /* file could not be read or parsed */
+};
+
+/* variants differ in the mobility of c_d_calc / calc_g only */
+enum {
+    MAPP_DMD_VARIANT_DMD = 0, /* ForceField_DMD    (src/ff_dmd.cpp)    Mat = 5.2 beta, rate exp(n + Mat) */
+    MAPP_DMD_VARIANT_I = 1,   /* ForceField_DMD_I  (src/ff_dmd_I.cpp)  Mat = 0.4 beta */
+    MAPP_DMD_VARIANT_II = 2   /* ForceField_DMD_II (src/ff_dmd_II.cpp) mat = max(f_i,f_j) + 0.5 */
+};
+
+/* arithmetic of the pair quadrature (per-site sums, embedding and the
+   energy total stay FP64 in every mode) */
+enum {
+    MAPP_DMD_PREC_FP64 = 0,     /* quadrature in FP64 */
+    MAPP_DMD_PREC_MIXED = 1,    /* quadrature sums in FP32 with FP32 accumulation */
+    MAPP_DMD_PREC_SURROGATE = 2 /* BF16-MFMA MLP surrogate for the averaged primitives */
+};
+
+/* per-atom vectors for mapp_dmd_get_vec / mapp_dmd_set_vec / mapp_dmd_device_ptr */
+enum {
+    MAPP_DMD_VEC_X = 0,   /* [natms][3+nt] */
+    MAPP_DMD_VEC_C = 1,   /* [natms][nt] */
+    MAPP_DMD_VEC_F = 2,   /* [natms][3+nt] */
+    MAPP_DMD_VEC_E = 3,   /* [natms][nt]  (after force_calc: F(rho_bar)) */
+    MAPP_DMD_VEC_DE = 4,  /* [natms][nt]  F'(rho_bar) */
+    MAPP_DMD_VEC_N = 5,   /* [natms][nt]  chemical potential of c_d_calc / calc_g */
+    MAPP_DMD_VEC_CD = 6,  /* [natms][nt]  dc/dt */
+    MAPP_DMD_VEC_DDE = 7, /* [natms][nt]  F''(rho_bar) (calc_g) */
+    MAPP_DMD_VEC_S = 8,   /* [natms][nt]  calc_g residual s */
+    MAPP_DMD_VEC_T = 9,   /* [natms][nt]  calc_g t */
+    MAPP_DMD_VEC_V = 10,  /* [natms][nt]  calc_g v */
+    MAPP_DMD_VEC_A = 11,  /* [natms][nt]  device buffer for calc_g's a (caller-filled) */
+    MAPP_DMD_VEC_G = 12   /* [natms][nt]  device buffer for calc_g's g */
+};
+
+/* Constructor: ForceField_DMD_I::ForceField_DMD_I (src/ff_dmd_I.cpp:10-40).
+   no_types = nt; device = HIP device ordinal. */
+int mapp_dmd_create(mapp_dmd_t** ctx, int no_types, int variant, int device);
+/* Destructor: ForceField_DMD_I::~ForceField_DMD_I (src/ff_dmd_I.cpp:44-85). */
+int mapp_dmd_destroy(mapp_dmd_t* ctx);
+const char* mapp_dmd_last_error(const mapp_dmd_t* ctx);
+
+/* Finnis-Sinclair tables, element order = type order:
+   ForceField_DMD_II::set_fs (src/ff_dmd_II.cpp:1253-1480) then set_arrays
+   (src/ff_dmd_I.cpp:1729-1757).  F: [nt][nrho]; rho: [a][b][nr] (density at
+   an a site from a b neighbour, the file's order); rphi: [i][j<=i][nr] holding
+   r*phi.  nr must equal nrho (the reference clamps F rows with nr-2,
+   src/ff_dmd_I.cpp:326). */
+int mapp_dmd_set_fs(mapp_dmd_t* ctx, int nrho, double drho, int nr, double dr,
+                    const double* F, const double* rho, const double* rphi);
+/* FS file reader, same format as ForceField_DMD_II::set_fs
+   (src/ff_dmd_II.cpp:1253-1480): elements[nt] names the type order. */
+int mapp_dmd_read_fs(mapp_dmd_t* ctx, const char* path, const char* const* elements);
+/* ff_coef r_crd no_i alpha_min alpha_max T: ForceField_DMD_I::coef
+   (src/ff_dmd_I.cpp:735-796); mass[nt] in amu (AtomTypes::mass). */
+int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min,
+                  double alpha_max, double T, const double* mass);
+int mapp_dmd_set_precision(mapp_dmd_t* ctx, int mode);
+
+/* System: lower-triangular H (row-major 3x3, rows are the box vectors,
+   src/read_cfg.cpp:425-453), natms sites whose x rows hold the reduced
+   coordinates s in [0,1) followed by alpha (what read_cfg leaves in x,
+   src/read_cfg.cpp:562-625), and c.  Host pointers. */
+int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms,
+                        const double* x_reduced, const double* c);
+/* ForceField_DMD_I::init (src/ff_dmd_I.cpp:672-699) + Atoms::ph_setup
+   (src/atoms.cpp:317-372) + Neighbor::create_list(0,1)
+   (src/neighbor.cpp:137-292) + Atoms::store_0, on the GPU. */
+int mapp_dmd_init(mapp_dmd_t* ctx, double skin);
+/* ForceField_DMD_I::fin (src/ff_dmd_I.cpp:703-725): drops lists and caches. */
+int mapp_dmd_fin(mapp_dmd_t* ctx);
+
+/* New owned x (cartesian + alpha; host pointer, or NULL when the device
+   vector was written in place), then Atoms::update_0 (src/atoms.cpp:1780-1865):
+   rebuild ghosts and lists if a site moved more than skin, else forward-update
+   ghost x.  *rebuilt (may be NULL) reports which. */
+int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt);
+/* New owned c (host pointer or NULL for in-place device writes), forwarded to
+   ghosts (Atoms::update of c). */
+int mapp_dmd_set_c(mapp_dmd_t* ctx, const double* c_host);
+
+/* virtual void force_calc(int st_clc, TYPE0* en_st)  (src/ff.h:20,
+   src/ff_dmd_I.cpp:88-450).  en_st (host, >= 7 doubles): [0] = energy;
+   when st_clc, [1..6] = virial xx yy zz yz zx xy. Writes f, E, dE. */
+int mapp_dmd_force_calc(mapp_dmd_t* ctx, int st_clc, double* en_st);
+/* virtual TYPE0 energy_calc()  (src/ff.h:21, src/ff_dmd_I.cpp:454-668). */
+int mapp_dmd_energy_calc(mapp_dmd_t* ctx, double* energy);
+/* virtual void create_2nd_neigh_lst()  (src/ff.h:27, src/ff_dmd_I.cpp:2138-2217). */
+int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx);
+/* virtual void c_d_calc()  (src/ff.h:29; src/ff_dmd_I.cpp:2241-2398,
+   src/ff_dmd_II.cpp:2186-2337, src/ff_dmd.cpp:2241-2398).  Writes c_d, E, dE, n. */
+int mapp_dmd_c_d_calc(mapp_dmd_t* ctx);
+/* virtual TYPE0 calc_g(int chk, TYPE0 alpha, TYPE0* a, TYPE0* g)  (src/ff.h:28;
+   src/ff_dmd_I.cpp:2402-2674, src/ff_dmd_II.cpp:2341-2630).  a and g are
+   DEVICE pointers of natms*nt doubles; g is written only when chk == 0. */
+int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
+                    double* g_dev, double* cost);
+
+/* host <-> device copies of per-atom vectors (owned rows only) */
+int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count);
+int mapp_dmd_device_ptr(mapp_dmd_t* ctx, int which, void** dev_ptr);
+/* host -> device copy into the owned rows of X (staging for set_x(NULL)),
+   A or G; count = number of doubles supplied */
+int mapp_dmd_set_vec(mapp_dmd_t* ctx, int which, const double* host, long count);
+int mapp_dmd_sync(mapp_dmd_t* ctx);
+
+/* sizes: out[0] natms, out[1] nghost, out[2] half-list pairs, out[3] 2nd-list
+   pairs, out[4] number of ghost swaps, out[5] pairs inside the quadrature cut */
+int mapp_dmd_sizes(mapp_dmd_t* ctx, long* out);
+/* Neighbor::neighbor_list as CSR (offsets natms+1, entries npairs) */
+int mapp_dmd_get_list(mapp_dmd_t* ctx, int* offsets, int* entries);
+int mapp_dmd_get_list2(mapp_dmd_t* ctx, int* offsets, int* entries);
+/* ghost rows: x (cartesian + alpha) [nghost][3+nt] and source index [nghost] */
+int mapp_dmd_get_ghosts(mapp_dmd_t* ctx, double* x, int* src);
+/* Gauss-Hermite rule in use: xi, wi_0, wi_1, wi_2 (no_i each) */
+int mapp_dmd_get_gh(mapp_dmd_t* ctx, double* xi, double* w0, double* w1, double* w2);
+/* device time in ms of the kernels of the last force_calc:
+   [0] pass 1, [1] embedding, [2] pass 2, [3] gathers + reduction, [4] total */
+int mapp_dmd_last_timing(mapp_dmd_t* ctx, double* ms);
+/* enable (1) / disable (0) per-kernel event timing of force_calc */
+int mapp_dmd_enable_timing(mapp_dmd_t* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1149 @@
+// C ABI of the MI355X DMD force field (declarations and reference citations in
+// include/mapp_dmd.h).  Host orchestration: tables, parameters, the GPU build
+// of ghosts and lists, and the kernel sequences of force_calc / energy_calc /
+// c_d_calc / calc_g.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "context.hpp"
+#include "geometry.hpp"
+#include "force.hpp"
+#include "mobility.hpp"
+#include "gh_table.h"
+#include "tables.hpp"
+
+using namespace mappdmd;
+
+mapp_dmd::~mapp_dmd() {
+    for (auto* b : scan_levels) delete b;
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+template <typename F>
+int guarded(mapp_dmd* ctx, F&& fn) {
+    if (!ctx) return MAPP_DMD_E_ARG;
+    try {
+        int rc = fn();
+        return rc;
+    } catch (const HipError& e) {
+        ctx->err = std::string("HIP error ") + hipGetErrorString(e.code) + " at " + e.what + " (line " +
+                   std::to_string(e.line) + ")";
+        return MAPP_DMD_E_HIP;
+    } catch (const ApiError& e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return MAPP_DMD_E_HIP;
+    }
+}
+
+void launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw HipError(e, what, 0);
+}
+
+// exclusive scan of n ints; returns the total (synchronizes the stream)
+long exclusive_scan(mapp_dmd* ctx, const int* in, int* out, long n, int depth = 0) {
+    if (n <= 0) return 0;
+    const int nb = (int)((n + kScanTile - 1) / kScanTile);
+    while ((int)ctx->scan_levels.size() < 2 * (depth + 1)) ctx->scan_levels.push_back(new DevBuf<int>());
+    DevBuf<int>& sums = *ctx->scan_levels[2 * depth];
+    DevBuf<int>& sums_x = *ctx->scan_levels[2 * depth + 1];
+    sums.reserve(nb);
+    sums_x.reserve(nb);
+    hipLaunchKernelGGL(k_scan_tile, dim3(nb), dim3(kScanThreads), 0, ctx->stream, in, out, sums.p, n);
+    launch_check("k_scan_tile");
+    if (nb > 1) {
+        exclusive_scan(ctx, sums.p, sums_x.p, nb, depth + 1);
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanThreads), 0, ctx->stream, out, sums_x.p, n);
+        launch_check("k_scan_add");
+    }
+    int last[2];
+    HIP_OK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(&last[1], in + n - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return (long)last[0] + last[1];
+}
+
+// CSR offsets (n+1) from counts (n)
+long counts_to_offsets(mapp_dmd* ctx, const int* cnt, DevBuf<int>& off, long n) {
+    off.reserve(n + 1);
+    long tot = exclusive_scan(ctx, cnt, off.p, n);
+    int t = (int)tot;
+    HIP_OK(hipMemcpyAsync(off.p + n, &t, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return tot;
+}
+
+void require(bool cond, int code, const char* msg) {
+    if (!cond) throw ApiError(code, msg);
+}
+
+// Atoms::set_ph(TYPE0) for one rank (s_lo = 0, s_hi = 1), src/atoms.cpp:249-280
+void set_ph(mapp_dmd* ctx, double cut) {
+    BoxParams& bp = ctx->bp;
+    const double tot_cut_ph = cut + ctx->skin;
+    for (int i = 0; i < 3; ++i) {
+        double tmp = 0.0;
+        for (int j = i; j < 3; ++j) tmp += bp.B[j][i] * bp.B[j][i];
+        bp.cut_ph_s[i] = tot_cut_ph * std::sqrt(tmp);
+    }
+    for (int i = 0; i < 3; ++i) {
+        bp.s_ph_hi[i] = 1.0 - bp.cut_ph_s[i];
+        bp.s_ph_lo[i] = 0.0 + bp.cut_ph_s[i];
+    }
+    // Neighbor::create_bin_list, src/neighbor.cpp:311-326
+    bp.tot_bin = 1;
+    for (int i = 0; i < 3; ++i) {
+        bp.bin_size[i] = bp.cut_ph_s[i];
+        bp.grid[i] = (int)((2.0 * bp.cut_ph_s[i] + 1.0 - 0.0) / bp.bin_size[i]) + 1;
+        bp.tot_bin *= bp.grid[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        int no = 1;
+        for (int j = 0; j < i; ++j) no *= bp.grid[j];
+        bp.denom[i] = no;
+    }
+}
+
+template <int NT>
+void build_ghosts(mapp_dmd* ctx) {
+    BoxParams& bp = ctx->bp;
+    const int natms = ctx->natms;
+    ctx->nghost = 0;
+    ctx->swap_off.assign(1, 0);
+    ctx->swap_dim.clear();
+    ctx->swap_dir.clear();
+    int lo_dim = 0, hi_dim = natms;
+    for (int idim = 0; idim < 3; ++idim) {
+        const int comm_need = (int)(bp.cut_ph_s[idim] / (1.0 - 0.0)) + 1;
+        for (int dir = 0; dir < 2; ++dir) {
+            int lo = lo_dim, hi = hi_dim;
+            for (int k = 0; k < comm_need; ++k) {
+                const int tmp_r = natms + ctx->nghost;
+                const int n = hi - lo;
+                int cnt = 0;
+                if (n > 0) {
+                    ctx->d_flag.reserve(n);
+                    ctx->d_offs.reserve(n);
+                    double thr = dir == 0 ? bp.s_ph_lo[idim] : bp.s_ph_hi[idim];
+                    hipLaunchKernelGGL(k_ghost_flag, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, lo,
+                                       hi, idim, dir, thr, ctx->d_flag.p);
+                    launch_check("k_ghost_flag");
+                    cnt = (int)exclusive_scan(ctx, ctx->d_flag.p, ctx->d_offs.p, n);
+                }
+                const int need = natms + ctx->nghost + cnt;
+                ctx->d_pos.reserve(need, true, ctx->stream);
+                ctx->d_alpha.reserve((size_t)need * NT, true, ctx->stream);
+                ctx->d_conc.reserve((size_t)need * NT, true, ctx->stream);
+                ctx->d_ghost_src.reserve(ctx->nghost + cnt + 1, true, ctx->stream);
+                ctx->d_ghost_owner.reserve(ctx->nghost + cnt + 1, true, ctx->stream);
+                if (cnt > 0)
+                    hipLaunchKernelGGL((k_ghost_emit<NT>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream,
+                                       ctx->d_pos.p, ctx->d_alpha.p, ctx->d_conc.p, lo, hi, idim, dir,
+                                       ctx->d_flag.p, ctx->d_offs.p, natms + ctx->nghost, natms,
+                                       ctx->d_ghost_src.p, ctx->d_ghost_owner.p);
+                launch_check("k_ghost_emit");
+                ctx->nghost += cnt;
+                ctx->swap_dim.push_back(idim);
+                ctx->swap_dir.push_back(dir);
+                ctx->swap_off.push_back(ctx->nghost);
+                hi = natms + ctx->nghost;
+                lo = tmp_r;
+            }
+        }
+        lo_dim = 0;
+        hi_dim = natms + ctx->nghost;
+    }
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+}
+
+// ghosts + bins + s->x + half list + reverse CSR; positions hold s on entry
+template <int NT>
+void rebuild(mapp_dmd* ctx) {
+    BoxParams& bp = ctx->bp;
+    const int natms = ctx->natms;
+    build_ghosts<NT>(ctx);
+    const int ntot = natms + ctx->nghost;
+    hipStream_t st = ctx->stream;
+    // bins
+    ctx->d_bin_of.reserve(ntot);
+    ctx->d_bin_cnt.reserve(bp.tot_bin + 1);
+    ctx->d_bin_atoms.reserve(ntot);
+    HIP_OK(hipMemsetAsync(ctx->d_bin_cnt.p, 0, sizeof(int) * (bp.tot_bin + 1), st));
+    hipLaunchKernelGGL(k_s2bin, dim3(nblk(ntot)), dim3(kBlock), 0, st, ctx->d_pos.p, ntot, bp, ctx->d_bin_of.p,
+                       ctx->d_bin_cnt.p);
+    launch_check("k_s2bin");
+    counts_to_offsets(ctx, ctx->d_bin_cnt.p, ctx->d_bin_start, bp.tot_bin);
+    HIP_OK(hipMemsetAsync(ctx->d_bin_cnt.p, 0, sizeof(int) * (bp.tot_bin + 1), st));
+    hipLaunchKernelGGL(k_bin_fill, dim3(nblk(ntot)), dim3(kBlock), 0, st, ctx->d_bin_of.p, ntot,
+                       ctx->d_bin_start.p, ctx->d_bin_cnt.p, ctx->d_bin_atoms.p);
+    launch_check("k_bin_fill");
+    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(bp.tot_bin)), dim3(kBlock), 0, st, ctx->d_bin_start.p,
+                       bp.tot_bin, ctx->d_bin_atoms.p);
+    launch_check("k_segment_sort");
+    // Neighbor::bin_atoms_s ends with s2x of every atom
+    hipLaunchKernelGGL(k_s2x, dim3(nblk(ntot)), dim3(kBlock), 0, st, ctx->d_pos.p, ntot, bp);
+    launch_check("k_s2x");
+    // half list
+    ctx->d_cnt.reserve(natms + 1);
+    hipLaunchKernelGGL((k_half_list<false>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
+                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_cnt.p, (int*)nullptr);
+    launch_check("k_half_list count");
+    ctx->npairs = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_off, natms);
+    ctx->d_nl.reserve(ctx->npairs + 1);
+    hipLaunchKernelGGL((k_half_list<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
+                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_off.p, ctx->d_nl.p);
+    launch_check("k_half_list fill");
+    // reverse CSR (partner j owned)
+    HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
+    hipLaunchKernelGGL(k_rev_count, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
+                       ctx->d_cnt.p);
+    launch_check("k_rev_count");
+    long nrev = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_roff, natms);
+    ctx->d_rev.reserve(nrev + 1);
+    HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
+    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
+                       ctx->d_roff.p, ctx->d_cnt.p, ctx->d_rev.p);
+    launch_check("k_rev_fill");
+    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff.p, natms,
+                       ctx->d_rev.p);
+    launch_check("k_segment_sort rev");
+    ctx->d_pair_i.reserve(ctx->npairs + 1);
+    hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_pair_i.p);
+    launch_check("k_pair_owner");
+    // per-atom vectors and pair buffers
+    const size_t nv = (size_t)ntot * NT;
+    for (auto* b : {&ctx->d_E, &ctx->d_dE, &ctx->d_ddE, &ctx->d_n, &ctx->d_s, &ctx->d_t, &ctx->d_v})
+        b->reserve(nv);
+    ctx->d_cd.reserve((size_t)natms * NT);
+    ctx->d_a.reserve((size_t)natms * NT);
+    ctx->d_g.reserve((size_t)natms * NT);
+    ctx->d_fa.reserve((size_t)natms * NT);
+    ctx->d_Eown.reserve((size_t)natms * NT);
+    ctx->d_f.reserve((size_t)natms * (3 + NT));
+    ctx->d_fown.reserve((size_t)natms * (3 + NT));
+    ctx->d_pairE.reserve((size_t)(ctx->npairs + 1) * NT);
+    ctx->d_pairF.reserve((size_t)(ctx->npairs + 1) * (3 + NT));
+    ctx->d_crd.reserve(ntot);
+    ctx->d_x0.reserve(natms);
+    HIP_OK(hipMemcpyAsync(ctx->d_x0.p, ctx->d_pos.p, sizeof(double4) * natms, hipMemcpyDeviceToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    ctx->have_list2 = false;
+    ctx->cache_valid = false;
+}
+
+void rebuild_any(mapp_dmd* ctx) {
+    if (ctx->nt == 1) rebuild<1>(ctx);
+    else rebuild<2>(ctx);
+}
+
+template <int NT>
+void ghost_update_x(mapp_dmd* ctx) {
+    for (size_t sw = 0; sw + 1 < ctx->swap_off.size(); ++sw) {
+        int g0 = ctx->swap_off[sw], g1 = ctx->swap_off[sw + 1];
+        if (g1 <= g0) continue;
+        int idim = ctx->swap_dim[sw], dir = ctx->swap_dir[sw];
+        hipLaunchKernelGGL((k_ghost_update_x<NT>), dim3(nblk(g1 - g0)), dim3(kBlock), 0, ctx->stream,
+                           ctx->d_pos.p, ctx->d_alpha.p, ctx->natms, g0, g1, ctx->d_ghost_src.p, idim, dir,
+                           ctx->bp.H[idim][0], ctx->bp.H[idim][1], ctx->bp.H[idim][2]);
+        launch_check("k_ghost_update_x");
+    }
+}
+
+void ghost_copy(mapp_dmd* ctx, double* v, int d) {
+    if (ctx->nghost <= 0) return;
+    hipLaunchKernelGGL((k_ghost_copy<double>), dim3(nblk(ctx->nghost)), dim3(kBlock), 0, ctx->stream, v, d,
+                       ctx->natms, ctx->nghost, ctx->d_ghost_owner.p);
+    launch_check("k_ghost_copy");
+}
+
+// AoS (3+nt) owned rows <-> positions / alpha
+template <int NT>
+__global__ void k_scatter_x(const double* __restrict__ x, int natms, double4* __restrict__ pos,
+                            double* __restrict__ alpha) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    const double* r = x + (size_t)i * (3 + NT);
+    pos[i] = make_double4(r[0], r[1], r[2], 0.0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) alpha[(size_t)i * NT + t] = r[3 + t];
+}
+template <int NT>
+__global__ void k_gather_x(double* __restrict__ x, int n, const double4* __restrict__ pos,
+                           const double* __restrict__ alpha) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* r = x + (size_t)i * (3 + NT);
+    double4 p = pos[i];
+    r[0] = p.x;
+    r[1] = p.y;
+    r[2] = p.z;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) r[3 + t] = alpha[(size_t)i * NT + t];
+}
+
+template <int NT>
+void scatter_x(mapp_dmd* ctx, const double* dev_aos) {
+    hipLaunchKernelGGL((k_scatter_x<NT>), dim3(nblk(ctx->natms)), dim3(kBlock), 0, ctx->stream, dev_aos,
+                       ctx->natms, ctx->d_pos.p, ctx->d_alpha.p);
+    launch_check("k_scatter_x");
+}
+
+// ------------------------------------------------------------ force kernels
+template <int NT, typename Q, int NQ>
+void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st) {
+    hipStream_t st = ctx->stream;
+    const int natms = ctx->natms;
+    const int nb = nblk(natms);
+    PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
+    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_out.reserve(16);
+    double* part1 = ctx->d_part.p;
+    double* part2 = part1 + (size_t)nb * 2;
+    double* part3 = part2 + (size_t)nb;
+    if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
+    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                       ctx->d_pairE.p, part1);
+    launch_check("k_pass1");
+    if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
+    hipLaunchKernelGGL((k_embed<NT, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p,
+                       ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p,
+                       ctx->d_fa.p, part2);
+    launch_check("k_embed");
+    ghost_copy(ctx, ctx->d_dE.p, NT);
+    if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
+    hipLaunchKernelGGL((k_pass2<NT, Q, NQ>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_dE.p,
+                       ctx->d_fown.p, ctx->d_pairF.p, part3);
+    launch_check("k_pass2");
+    if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
+    hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
+                       ctx->d_fa.p, ctx->d_f.p);
+    launch_check("k_fgather");
+    double* out = ctx->d_out.p;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part1, nb, 2, 2, 0, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part2, nb, 1, 1, 0, out + 2);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part3, nb, 6, 6, 0, out + 3);
+    launch_check("k_reduce_partials");
+    if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[4], st));
+    double h[9];
+    HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    ctx->n_inrange = (long)h[1];
+    en_st[0] = h[0] + h[2];
+    if (st_clc)
+        for (int k = 0; k < 6; ++k) en_st[1 + k] = h[3 + k];
+    if (ctx->timing) {
+        float ms;
+        for (int k = 0; k < 4; ++k) {
+            HIP_OK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+            ctx->last_ms[k] = ms;
+        }
+        HIP_OK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));
+        ctx->last_ms[4] = ms;
+    }
+}
+
+template <int NT, typename Q, int NQ>
+double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
+    hipStream_t st = ctx->stream;
+    const int nb = nblk(ctx->natms);
+    PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
+    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_out.reserve(16);
+    double* part1 = ctx->d_part.p;
+    double* part2 = part1 + (size_t)nb * 2;
+    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                       ctx->d_pairE.p, part1);
+    launch_check("k_pass1 energy");
+    hipLaunchKernelGGL((k_embed<NT, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
+                       L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p, ctx->d_fa.p,
+                       part2);
+    launch_check("k_embed energy");
+    double* out = ctx->d_out.p;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part1, nb, 1, 2, 0, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part2, nb, 1, 1, 0, out + 1);
+    launch_check("k_reduce_partials energy");
+    double h[2];
+    HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return h[0] + h[1];
+}
+
+template <int NT>
+void dispatch_force(mapp_dmd* ctx, int st_clc, double* en_st) {
+    const int q = ctx->P.no_i;
+    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+        if (q == 3) run_force<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
+        else if (q == 5) run_force<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
+        else run_force<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
+    } else {
+        if (q == 3) run_force<NT, double, 3>(ctx, ctx->d_quad64.p, ctx->P, st_clc, en_st);
+        else if (q == 5) run_force<NT, double, 5>(ctx, ctx->d_quad64.p, ctx->P, st_clc, en_st);
+        else run_force<NT, double, 0>(ctx, ctx->d_quad64.p, ctx->P, st_clc, en_st);
+    }
+}
+
+template <int NT>
+double dispatch_energy(mapp_dmd* ctx) {
+    const int q = ctx->P.no_i;
+    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+        if (q == 3) return run_energy<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf);
+        if (q == 5) return run_energy<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf);
+        return run_energy<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf);
+    }
+    if (q == 3) return run_energy<NT, double, 3>(ctx, ctx->d_quad64.p, ctx->P);
+    if (q == 5) return run_energy<NT, double, 5>(ctx, ctx->d_quad64.p, ctx->P);
+    return run_energy<NT, double, 0>(ctx, ctx->d_quad64.p, ctx->P);
+}
+
+void upload_tables(mapp_dmd* ctx) {
+    const int nt = ctx->nt, nr = ctx->nr;
+    std::vector<double> q64((size_t)nt * nt * nr * kQuadRow, 0.0);
+    std::vector<float> q32(q64.size(), 0.0f);
+    for (int it = 0; it < nt; ++it)
+        for (int jt = 0; jt < nt; ++jt) {
+            const double* rs = ctx->rho7.data() + (size_t)type2rho(nt, it, jt) * nr * 7;
+            const double* ps = ctx->phi7.data() + (size_t)type2phi(it, jt) * nr * 7;
+            double* d = q64.data() + (size_t)(it * nt + jt) * nr * kQuadRow;
+            for (int m = 0; m < nr; ++m)
+                for (int k = 0; k < 4; ++k) {
+                    d[(size_t)m * kQuadRow + k] = rs[(size_t)m * 7 + k];
+                    d[(size_t)m * kQuadRow + 4 + k] = ps[(size_t)m * 7 + k];
+                }
+        }
+    for (size_t k = 0; k < q64.size(); ++k) q32[k] = (float)q64[k];
+    ctx->d_quad64.reserve(q64.size());
+    ctx->d_quad32.reserve(q32.size());
+    ctx->d_rho7.reserve(ctx->rho7.size());
+    ctx->d_phi7.reserve(ctx->phi7.size());
+    ctx->d_F9.reserve(ctx->F9.size());
+    HIP_OK(hipMemcpy(ctx->d_quad64.p, q64.data(), q64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_quad32.p, q32.data(), q32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_rho7.p, ctx->rho7.data(), ctx->rho7.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_phi7.p, ctx->phi7.data(), ctx->phi7.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ctx->d_F9.p, ctx->F9.data(), ctx->F9.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+void fill_float_params(mapp_dmd* ctx) {
+    const FFParams<double>& P = ctx->P;
+    FFParams<float>& F = ctx->Pf;
+    F.natms = P.natms;
+    F.ntot = P.ntot;
+    F.nr = P.nr;
+    F.nrho = P.nrho;
+    F.no_i = P.no_i;
+    F.dr_inv = (float)P.dr_inv;
+    F.drho_inv = (float)P.drho_inv;
+    F.rho_max = (float)P.rho_max;
+    F.rc = (float)P.rc;
+    F.cut_sq = (float)P.cut_sq;
+    F.cut_sq_mod = (float)P.cut_sq_mod;
+    F.alpha_min = (float)P.alpha_min;
+    F.alpha_max = (float)P.alpha_max;
+    F.kbT = (float)P.kbT;
+    F.beta = (float)P.beta;
+    for (int k = 0; k < kMaxPoints; ++k) {
+        F.xi[k] = (float)P.xi[k];
+        F.w0[k] = (float)P.w0[k];
+        F.w1[k] = (float)P.w1[k];
+        F.w2[k] = (float)P.w2[k];
+    }
+    for (int t = 0; t < kMaxTypes; ++t) F.c0[t] = (float)P.c0[t];
+}
+
+void sync_sizes(mapp_dmd* ctx) {
+    ctx->P.natms = ctx->natms;
+    ctx->P.ntot = ctx->natms + ctx->nghost;
+    ctx->Pf.natms = ctx->P.natms;
+    ctx->Pf.ntot = ctx->P.ntot;
+}
+
+
+void ensure_pair_cache(mapp_dmd* ctx);
+
+template <int NT, typename Q, int NQ>
+void run_pair_values(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
+    const size_t np = (size_t)ctx->npairs + 1;
+    ctx->d_rhoc.reserve(np * NT * NT);
+    ctx->d_phic.reserve(np * NT * (NT + 1) / 2);
+    hipLaunchKernelGGL((k_pair_values<NT, Q, NQ>), dim3(nblk(ctx->natms)), dim3(kBlock), 0, ctx->stream, ctx->P, PQ,
+                       ctx->d_pos.p, ctx->d_alpha.p, ctx->d_off.p, ctx->d_nl.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
+                       ctx->d_rhoc.p, ctx->d_phic.p);
+    launch_check("k_pair_values");
+}
+
+template <int NT>
+void pair_values_dispatch(mapp_dmd* ctx) {
+    const int q = ctx->P.no_i;
+    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+        if (q == 3) run_pair_values<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf);
+        else if (q == 5) run_pair_values<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf);
+        else run_pair_values<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf);
+    } else {
+        if (q == 3) run_pair_values<NT, double, 3>(ctx, ctx->d_quad64.p, ctx->P);
+        else if (q == 5) run_pair_values<NT, double, 5>(ctx, ctx->d_quad64.p, ctx->P);
+        else run_pair_values<NT, double, 0>(ctx, ctx->d_quad64.p, ctx->P);
+    }
+}
+
+void ensure_pair_cache(mapp_dmd* ctx) {
+    if (ctx->cache_valid) return;
+    if (ctx->nt == 1) pair_values_dispatch<1>(ctx);
+    else pair_values_dispatch<2>(ctx);
+    ctx->cache_valid = true;
+}
+
+template <int NT>
+void build_list2(mapp_dmd* ctx) {
+    hipStream_t st = ctx->stream;
+    const int natms = ctx->natms;
+    ctx->d_cnt.reserve(natms + 1);
+    hipLaunchKernelGGL((k_list2<false>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
+                       ctx->d_nl.p, ctx->rsq_crd, ctx->d_cnt.p, (int*)nullptr);
+    launch_check("k_list2 count");
+    ctx->npairs2 = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_off2, natms);
+    ctx->d_nl2.reserve(ctx->npairs2 + 1);
+    hipLaunchKernelGGL((k_list2<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
+                       ctx->d_nl.p, ctx->rsq_crd, ctx->d_off2.p, ctx->d_nl2.p);
+    launch_check("k_list2 fill");
+    HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
+    hipLaunchKernelGGL(k_rev_count, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
+                       ctx->d_cnt.p);
+    launch_check("k_rev_count 2");
+    long nrev = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_roff2, natms);
+    ctx->d_rev2.reserve(nrev + 1);
+    HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
+    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
+                       ctx->d_roff2.p, ctx->d_cnt.p, ctx->d_rev2.p);
+    launch_check("k_rev_fill 2");
+    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff2.p, natms, ctx->d_rev2.p);
+    launch_check("k_segment_sort 2");
+    ctx->d_pair_i2.reserve(ctx->npairs2 + 1);
+    hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, natms, ctx->d_pair_i2.p);
+    launch_check("k_pair_owner 2");
+    const int ntot = natms + ctx->nghost;
+    ctx->d_crd.reserve(ntot);
+    hipLaunchKernelGGL(k_crd, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_roff2.p, natms, ctx->d_crd.p);
+    launch_check("k_crd");
+    if (ctx->nghost > 0) {
+        hipLaunchKernelGGL((k_ghost_copy<int>), dim3(nblk(ctx->nghost)), dim3(kBlock), 0, st, ctx->d_crd.p, 1, natms,
+                           ctx->nghost, ctx->d_ghost_owner.p);
+        launch_check("k_ghost_copy crd");
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    ctx->have_list2 = true;
+}
+
+RevBufs rev_half(mapp_dmd* ctx) { return RevBufs{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p, ctx->d_pair_i.p}; }
+RevBufs rev_second(mapp_dmd* ctx) {
+    return RevBufs{ctx->d_off2.p, ctx->d_nl2.p, ctx->d_roff2.p, ctx->d_rev2.p, ctx->d_pair_i2.p};
+}
+
+// c_d_calc: density and n from the pair cache, then rates over the 2nd list
+template <int NT, int VAR>
+void run_c_d(mapp_dmd* ctx) {
+    hipStream_t st = ctx->stream;
+    const int nb = nblk(ctx->natms);
+    ensure_pair_cache(ctx);
+    RevBufs L = rev_half(ctx), L2 = rev_second(ctx);
+    hipLaunchKernelGGL((k_density_cache<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L, ctx->d_rhoc.p,
+                       ctx->d_F9.p, ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, (double*)nullptr);
+    launch_check("k_density_cache");
+    ghost_copy(ctx, ctx->d_dE.p, NT);
+    constexpr int CLOSE = VAR == 2 ? 1 : 0;
+    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
+                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_n.p);
+    launch_check("k_chem_pot");
+    ghost_copy(ctx, ctx->d_n.p, NT);
+    hipLaunchKernelGGL((k_c_d<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L2, ctx->d_n.p,
+                       ctx->d_cd.p);
+    launch_check("k_c_d");
+    HIP_OK(hipStreamSynchronize(st));
+}
+
+template <int NT, int VAR>
+double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g) {
+    hipStream_t st = ctx->stream;
+    const int nb = nblk(ctx->natms);
+    ensure_pair_cache(ctx);
+    RevBufs L = rev_half(ctx), L2 = rev_second(ctx);
+    hipLaunchKernelGGL((k_density_cache<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L, ctx->d_rhoc.p,
+                       ctx->d_F9.p, ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, ctx->d_ddE.p);
+    launch_check("k_density_cache g");
+    ghost_copy(ctx, ctx->d_dE.p, NT);
+    constexpr int CLOSE = VAR == 2 ? 3 : 2;
+    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
+                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_n.p);
+    launch_check("k_chem_pot g");
+    ghost_copy(ctx, ctx->d_n.p, NT);
+    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_out.reserve(16);
+    hipLaunchKernelGGL((k_calc_g_s<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, a, L2,
+                       ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, ctx->d_part.p);
+    launch_check("k_calc_g_s");
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, ctx->d_part.p, nb, 1, 1, 0, ctx->d_out.p);
+    launch_check("k_reduce_partials g");
+    double inner = 0.0;
+    HIP_OK(hipMemcpyAsync(&inner, ctx->d_out.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (!chk) {
+        ghost_copy(ctx, ctx->d_s.p, NT);
+        hipLaunchKernelGGL((k_calc_g_t<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, L2,
+                           ctx->d_n.p, ctx->d_s.p, ctx->d_t.p, g);
+        launch_check("k_calc_g_t");
+        ghost_copy(ctx, ctx->d_t.p, NT);
+        hipLaunchKernelGGL((k_calc_g_v<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, L, ctx->d_rhoc.p,
+                           ctx->d_phic.p, ctx->d_dE.p, ctx->d_t.p, ctx->d_v.p, g);
+        launch_check("k_calc_g_v");
+        ghost_copy(ctx, ctx->d_v.p, NT);
+        ghost_copy(ctx, ctx->d_ddE.p, NT);
+        hipLaunchKernelGGL((k_calc_g_w<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, L,
+                           ctx->d_rhoc.p, ctx->d_ddE.p, ctx->d_v.p, g);
+        launch_check("k_calc_g_w");
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return inner * 0.25;
+}
+
+template <int NT>
+void c_d_variant(mapp_dmd* ctx) {
+    if (ctx->variant == 0) run_c_d<NT, 0>(ctx);
+    else if (ctx->variant == 1) run_c_d<NT, 1>(ctx);
+    else run_c_d<NT, 2>(ctx);
+}
+template <int NT>
+double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g) {
+    if (ctx->variant == 0) return run_calc_g<NT, 0>(ctx, chk, alpha_t, a, g);
+    if (ctx->variant == 1) return run_calc_g<NT, 1>(ctx, chk, alpha_t, a, g);
+    return run_calc_g<NT, 2>(ctx, chk, alpha_t, a, g);
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
+    if (!out) return MAPP_DMD_E_ARG;
+    *out = nullptr;
+    if (no_types < 1 || no_types > 2) return MAPP_DMD_E_RANGE;
+    if (variant < 0 || variant > 2) return MAPP_DMD_E_ARG;
+    std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
+    ctx->nt = no_types;
+    ctx->variant = variant;
+    ctx->device = device;
+    int rc = guarded(ctx.get(), [&]() {
+        int ndev = 0;
+        HIP_OK(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, MAPP_DMD_E_ARG, "no such HIP device");
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto& e : ctx->ev) HIP_OK(hipEventCreate(&e));
+        ctx->d_iflag.reserve(4);
+        return 0;
+    });
+    if (rc != 0) return rc;
+    *out = ctx.release();
+    return 0;
+}
+
+int mapp_dmd_destroy(mapp_dmd_t* ctx) {
+    if (!ctx) return MAPP_DMD_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return 0;
+}
+
+const char* mapp_dmd_last_error(const mapp_dmd_t* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mapp_dmd_set_fs(mapp_dmd_t* ctx, int nrho, double drho, int nr, double dr, const double* F, const double* rho,
+                    const double* rphi) {
+    return guarded(ctx, [&]() {
+        require(F && rho && rphi, MAPP_DMD_E_ARG, "null table");
+        require(nr >= 5 && nrho >= 5, MAPP_DMD_E_RANGE, "tables need at least 5 rows");
+        require(nr == nrho, MAPP_DMD_E_RANGE, "nr must equal nrho (F rows are clamped with nr-2)");
+        require(dr > 0 && drho > 0, MAPP_DMD_E_ARG, "non-positive grid spacing");
+        HIP_OK(hipSetDevice(ctx->device));
+        const int nt = ctx->nt, nphi = nt * (nt + 1) / 2;
+        ctx->nr = nr;
+        ctx->nrho = nrho;
+        ctx->dr = dr;
+        ctx->drho = drho;
+        ctx->F9.assign((size_t)nt * (nrho + 1) * 9, 0.0);
+        ctx->rho7.assign((size_t)nt * nt * nr * 7, 0.0);
+        ctx->phi7.assign((size_t)nphi * nr * 7, 0.0);
+        for (int a = 0; a < nt; ++a) {
+            for (int i = 0; i < nrho; ++i) ctx->F9[((size_t)a * (nrho + 1) + i) * 9] = F[(size_t)a * nrho + i];
+            for (int b = 0; b < nt; ++b) {
+                int slot = type2rho(nt, a, b);
+                for (int i = 0; i < nr; ++i)
+                    ctx->rho7[((size_t)slot * nr + i) * 7] = rho[((size_t)a * nt + b) * nr + i];
+            }
+        }
+        size_t k = 0;
+        for (int i = 0; i < nt; ++i)
+            for (int j = 0; j <= i; ++j, ++k) {
+                int slot = type2phi(i, j);
+                for (int r = 0; r < nr; ++r) ctx->phi7[((size_t)slot * nr + r) * 7] = rphi[k * nr + r];
+            }
+        for (int a = 0; a < nt; ++a) spline_rows(nrho, drho, ctx->F9.data() + (size_t)a * (nrho + 1) * 9, 9, true);
+        for (int s = 0; s < nt * nt; ++s) spline_rows(nr, dr, ctx->rho7.data() + (size_t)s * nr * 7, 7, false);
+        for (int s = 0; s < nphi; ++s) spline_rows(nr, dr, ctx->phi7.data() + (size_t)s * nr * 7, 7, false);
+        ctx->P.nr = nr;
+        ctx->P.nrho = nrho;
+        ctx->P.dr_inv = 1.0 / dr;
+        ctx->P.drho_inv = 1.0 / drho;
+        upload_tables(ctx);
+        ctx->have_tables = true;
+        ctx->have_params = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_read_fs(mapp_dmd_t* ctx, const char* path, const char* const* elements) {
+    return guarded(ctx, [&]() {
+        require(path && elements, MAPP_DMD_E_ARG, "null path or element names");
+        std::vector<std::string> names;
+        for (int t = 0; t < ctx->nt; ++t) {
+            require(elements[t] != nullptr, MAPP_DMD_E_ARG, "null element name");
+            names.emplace_back(elements[t]);
+        }
+        FsData fs;
+        std::string e = read_fs_file(path, names, fs);
+        if (!e.empty()) throw ApiError(MAPP_DMD_E_IO, e);
+        return mapp_dmd_set_fs(ctx, fs.nrho, fs.drho, fs.nr, fs.dr, fs.F.data(), fs.rho.data(), fs.rphi.data());
+    });
+}
+
+int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, double alpha_max, double T,
+                  const double* mass) {
+    return guarded(ctx, [&]() {
+        require(ctx->have_tables, MAPP_DMD_E_STATE, "tables must be set before coef");
+        require(no_i >= 1 && no_i <= 14, MAPP_DMD_E_RANGE, "n should be between 1 and 14");
+        require(alpha_min != 0.0, MAPP_DMD_E_ARG, "minimum alpha cannot be zero");
+        require(mass != nullptr, MAPP_DMD_E_ARG, "null masses");
+        FFParams<double>& P = ctx->P;
+        ctx->rsq_crd = r_crd;
+        ctx->rsq_crd *= ctx->rsq_crd;
+        P.no_i = no_i;
+        for (int k = 0; k < kMaxPoints; ++k) P.xi[k] = P.w0[k] = P.w1[k] = P.w2[k] = 0.0;
+        for (int k = 0; k < no_i; ++k) {
+            P.xi[k] = GH_NODES[no_i - 1][k];
+            P.w0[k] = GH_WEIGHTS[no_i - 1][k];
+        }
+        for (int k = 0; k < no_i; ++k) {
+            P.w1[k] = P.w0[k] * P.xi[k];
+            P.w2[k] = P.w0[k] * P.xi[k] * P.xi[k];
+        }
+        P.alpha_min = alpha_min;
+        P.alpha_max = alpha_max;
+        const double kb = 8.617332478e-5, hbar = 6.5821192815e-16;
+        for (int t = 0; t < kMaxTypes; ++t) P.c0[t] = 0.0;
+        for (int t = 0; t < ctx->nt; ++t) {
+            double m = mass[t];
+            m *= 1.0364269184093291236e-28;
+            double deb_l = hbar * hbar * 2.0 / (m * kb * T);
+            P.c0[t] = 1.5 * kb * T * (std::log(deb_l) - 1.0);
+        }
+        P.kbT = kb * T;
+        P.beta = 1.0 / P.kbT;
+        P.rc = ((double)ctx->nr - 1.0) * ctx->dr;
+        P.rho_max = ((double)ctx->nrho - 1.0) * ctx->drho;
+        P.cut_sq = P.rc * P.rc;
+        ctx->mod_rc = P.rc + P.xi[no_i - 1] / std::sqrt(alpha_min);
+        P.cut_sq_mod = ctx->mod_rc * ctx->mod_rc;
+        fill_float_params(ctx);
+        ctx->have_params = true;
+        ctx->cache_valid = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_set_precision(mapp_dmd_t* ctx, int mode) {
+    return guarded(ctx, [&]() {
+        require(mode == MAPP_DMD_PREC_FP64 || mode == MAPP_DMD_PREC_MIXED || mode == MAPP_DMD_PREC_SURROGATE,
+                MAPP_DMD_E_ARG, "unknown precision mode");
+        require(mode != MAPP_DMD_PREC_SURROGATE, MAPP_DMD_E_STATE,
+                "surrogate mode needs a trained network (mapp_dmd_surrogate_*)");
+        ctx->prec = mode;
+        ctx->cache_valid = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms, const double* x, const double* c) {
+    return guarded(ctx, [&]() {
+        require(H && x && c && natms > 0, MAPP_DMD_E_ARG, "null input or empty system");
+        for (int i = 0; i < 3; ++i)
+            for (int j = i + 1; j < 3; ++j)
+                require(H[i * 3 + j] == 0.0, MAPP_DMD_E_ARG, "H must be lower triangular");
+        const int nt = ctx->nt;
+        BoxParams& bp = ctx->bp;
+        std::memcpy(bp.H, H, sizeof(bp.H));
+        // M3INV, src/init.h:72-85
+        double(*A)[3] = bp.H;
+        double(*B)[3] = bp.B;
+        B[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+        B[1][1] = A[2][2] * A[0][0] - A[2][0] * A[0][2];
+        B[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        B[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+        B[2][1] = A[2][0] * A[0][1] - A[2][1] * A[0][0];
+        B[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+        B[2][0] = A[1][0] * A[2][1] - A[2][0] * A[1][1];
+        B[0][1] = A[2][1] * A[0][2] - A[0][1] * A[2][2];
+        B[1][2] = A[0][2] * A[1][0] - A[1][2] * A[0][0];
+        double det = A[0][0] * B[0][0] + A[0][1] * B[1][0] + A[0][2] * B[2][0];
+        require(det != 0.0, MAPP_DMD_E_ARG, "singular box");
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) B[i][j] /= det;
+        ctx->natms = natms;
+        ctx->h_s.resize((size_t)natms * 4);
+        ctx->h_alpha.resize((size_t)natms * nt);
+        ctx->h_c.assign(c, c + (size_t)natms * nt);
+        for (int i = 0; i < natms; ++i) {
+            const double* r = x + (size_t)i * (3 + nt);
+            for (int d = 0; d < 3; ++d) {
+                require(r[d] >= 0.0 && r[d] < 1.0, MAPP_DMD_E_RANGE, "reduced coordinates must lie in [0,1)");
+                ctx->h_s[(size_t)i * 4 + d] = r[d];
+            }
+            ctx->h_s[(size_t)i * 4 + 3] = 0.0;
+            for (int t = 0; t < nt; ++t) ctx->h_alpha[(size_t)i * nt + t] = r[3 + t];
+        }
+        ctx->have_system = true;
+        ctx->inited = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
+    return guarded(ctx, [&]() {
+        require(ctx->have_params && ctx->have_system, MAPP_DMD_E_STATE, "coef and set_system must precede init");
+        require(skin >= 0.0, MAPP_DMD_E_ARG, "negative skin");
+        HIP_OK(hipSetDevice(ctx->device));
+        const int nt = ctx->nt, natms = ctx->natms;
+        ctx->skin = skin;
+        const FFParams<double>& P = ctx->P;
+        // ForceField_DMD_I::init, src/ff_dmd_I.cpp:675-685
+        ctx->bp.cut_sk_sq = P.cut_sq_mod + skin * skin + 2.0 * std::sqrt(P.cut_sq_mod) * skin;
+        set_ph(ctx, std::sqrt(P.cut_sq_mod));
+        for (int d = 0; d < 3; ++d)
+            require(ctx->bp.cut_ph_s[d] < 1.0, MAPP_DMD_E_RANGE,
+                    "box thinner than cutoff+skin: needs more than one ghost swap per face");
+        ctx->d_pos.reserve(natms);
+        ctx->d_alpha.reserve((size_t)natms * nt);
+        ctx->d_conc.reserve((size_t)natms * nt);
+        HIP_OK(hipMemcpy(ctx->d_pos.p, ctx->h_s.data(), sizeof(double4) * natms, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ctx->d_alpha.p, ctx->h_alpha.data(), sizeof(double) * natms * nt, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(ctx->d_conc.p, ctx->h_c.data(), sizeof(double) * natms * nt, hipMemcpyHostToDevice));
+        rebuild_any(ctx);
+        sync_sizes(ctx);
+        ctx->inited = true;
+        return 0;
+    });
+}
+
+int mapp_dmd_fin(mapp_dmd_t* ctx) {
+    return guarded(ctx, [&]() {
+        ctx->inited = false;
+        ctx->have_list2 = false;
+        ctx->cache_valid = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_x");
+        HIP_OK(hipSetDevice(ctx->device));
+        const int nt = ctx->nt, natms = ctx->natms;
+        const size_t n = (size_t)natms * (3 + nt);
+        ctx->d_xaos.reserve(n);
+        if (x_host)
+            HIP_OK(hipMemcpyAsync(ctx->d_xaos.p, x_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
+        else scatter_x<2>(ctx, ctx->d_xaos.p);
+        HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_skin_check, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p,
+                           ctx->d_x0.p, natms, ctx->skin * ctx->skin, ctx->d_iflag.p);
+        launch_check("k_skin_check");
+        int flag = 0;
+        HIP_OK(hipMemcpyAsync(&flag, ctx->d_iflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (flag) {
+            hipLaunchKernelGGL(k_x2s, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, natms, ctx->bp);
+            launch_check("k_x2s");
+            rebuild_any(ctx);
+            sync_sizes(ctx);
+        } else {
+            if (nt == 1) ghost_update_x<1>(ctx);
+            else ghost_update_x<2>(ctx);
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+        }
+        ctx->cache_valid = false;
+        if (rebuilt) *rebuilt = flag;
+        return 0;
+    });
+}
+
+int mapp_dmd_set_c(mapp_dmd_t* ctx, const double* c_host) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_c");
+        HIP_OK(hipSetDevice(ctx->device));
+        const size_t n = (size_t)ctx->natms * ctx->nt;
+        if (c_host)
+            HIP_OK(hipMemcpyAsync(ctx->d_conc.p, c_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        ghost_copy(ctx, ctx->d_conc.p, ctx->nt);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int mapp_dmd_force_calc(mapp_dmd_t* ctx, int st_clc, double* en_st) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede force_calc");
+        require(en_st != nullptr, MAPP_DMD_E_ARG, "null en_st");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en_st);
+        else dispatch_force<2>(ctx, st_clc, en_st);
+        return 0;
+    });
+}
+
+int mapp_dmd_energy_calc(mapp_dmd_t* ctx, double* energy) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede energy_calc");
+        require(energy != nullptr, MAPP_DMD_E_ARG, "null energy");
+        HIP_OK(hipSetDevice(ctx->device));
+        *energy = ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
+        return 0;
+    });
+}
+
+
+int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede create_2nd_neigh_lst");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (ctx->nt == 1) build_list2<1>(ctx);
+        else build_list2<2>(ctx);
+        return 0;
+    });
+}
+
+int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede c_d_calc");
+        HIP_OK(hipSetDevice(ctx->device));
+        if (ctx->nt == 1) c_d_variant<1>(ctx);
+        else c_d_variant<2>(ctx);
+        return 0;
+    });
+}
+
+int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev, double* g_dev, double* cost) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede calc_g");
+        require(a_dev && g_dev && cost, MAPP_DMD_E_ARG, "null a, g or cost");
+        HIP_OK(hipSetDevice(ctx->device));
+        *cost = ctx->nt == 1 ? calc_g_variant<1>(ctx, chk, alpha, a_dev, g_dev)
+                             : calc_g_variant<2>(ctx, chk, alpha, a_dev, g_dev);
+        return 0;
+    });
+}
+
+int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede get_vec");
+        require(host != nullptr, MAPP_DMD_E_ARG, "null destination");
+        HIP_OK(hipSetDevice(ctx->device));
+        const int nt = ctx->nt, natms = ctx->natms;
+        const double* src = nullptr;
+        long n = (long)natms * nt;
+        switch (which) {
+            case MAPP_DMD_VEC_X: {
+                n = (long)natms * (3 + nt);
+                ctx->d_xaos.reserve(n);
+                if (nt == 1)
+                    hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream,
+                                       ctx->d_xaos.p, natms, ctx->d_pos.p, ctx->d_alpha.p);
+                else
+                    hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream,
+                                       ctx->d_xaos.p, natms, ctx->d_pos.p, ctx->d_alpha.p);
+                launch_check("k_gather_x");
+                src = ctx->d_xaos.p;
+                break;
+            }
+            case MAPP_DMD_VEC_C: src = ctx->d_conc.p; break;
+            case MAPP_DMD_VEC_F: src = ctx->d_f.p; n = (long)natms * (3 + nt); break;
+            case MAPP_DMD_VEC_E: src = ctx->d_E.p; break;
+            case MAPP_DMD_VEC_DE: src = ctx->d_dE.p; break;
+            case MAPP_DMD_VEC_N: src = ctx->d_n.p; break;
+            case MAPP_DMD_VEC_CD: src = ctx->d_cd.p; break;
+            case MAPP_DMD_VEC_DDE: src = ctx->d_ddE.p; break;
+            case MAPP_DMD_VEC_S: src = ctx->d_s.p; break;
+            case MAPP_DMD_VEC_T: src = ctx->d_t.p; break;
+            case MAPP_DMD_VEC_V: src = ctx->d_v.p; break;
+            case MAPP_DMD_VEC_A: src = ctx->d_a.p; break;
+            case MAPP_DMD_VEC_G: src = ctx->d_g.p; break;
+            default: throw ApiError(MAPP_DMD_E_ARG, "unknown vector");
+        }
+        require(count >= n, MAPP_DMD_E_ARG, "destination too small");
+        HIP_OK(hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int mapp_dmd_device_ptr(mapp_dmd_t* ctx, int which, void** dev_ptr) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited && dev_ptr, MAPP_DMD_E_STATE, "init must precede device_ptr");
+        switch (which) {
+            case MAPP_DMD_VEC_X:
+                ctx->d_xaos.reserve((size_t)ctx->natms * (3 + ctx->nt));
+                *dev_ptr = ctx->d_xaos.p;
+                break;
+            case MAPP_DMD_VEC_C: *dev_ptr = ctx->d_conc.p; break;
+            case MAPP_DMD_VEC_F: *dev_ptr = ctx->d_f.p; break;
+            case MAPP_DMD_VEC_E: *dev_ptr = ctx->d_E.p; break;
+            case MAPP_DMD_VEC_DE: *dev_ptr = ctx->d_dE.p; break;
+            case MAPP_DMD_VEC_N: *dev_ptr = ctx->d_n.p; break;
+            case MAPP_DMD_VEC_CD: *dev_ptr = ctx->d_cd.p; break;
+            case MAPP_DMD_VEC_A: *dev_ptr = ctx->d_a.p; break;
+            case MAPP_DMD_VEC_G: *dev_ptr = ctx->d_g.p; break;
+            default: throw ApiError(MAPP_DMD_E_ARG, "no device pointer for this vector");
+        }
+        return 0;
+    });
+}
+
+int mapp_dmd_set_vec(mapp_dmd_t* ctx, int which, const double* host, long count) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_vec");
+        require(host != nullptr, MAPP_DMD_E_ARG, "null source");
+        HIP_OK(hipSetDevice(ctx->device));
+        const int nt = ctx->nt, natms = ctx->natms;
+        double* dst = nullptr;
+        long n = (long)natms * nt;
+        switch (which) {
+            case MAPP_DMD_VEC_X:
+                n = (long)natms * (3 + nt);
+                ctx->d_xaos.reserve(n);
+                dst = ctx->d_xaos.p;
+                break;
+            case MAPP_DMD_VEC_A: dst = ctx->d_a.p; break;
+            case MAPP_DMD_VEC_G: dst = ctx->d_g.p; break;
+            default: throw ApiError(MAPP_DMD_E_ARG, "set_vec accepts X, A or G");
+        }
+        require(count == n, MAPP_DMD_E_ARG, "count does not match the vector size");
+        HIP_OK(hipMemcpyAsync(dst, host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int mapp_dmd_sync(mapp_dmd_t* ctx) {
+    return guarded(ctx, [&]() {
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int mapp_dmd_sizes(mapp_dmd_t* ctx, long* out) {
+    return guarded(ctx, [&]() {
+        require(out != nullptr, MAPP_DMD_E_ARG, "null output");
+        out[0] = ctx->natms;
+        out[1] = ctx->nghost;
+        out[2] = ctx->npairs;
+        out[3] = ctx->have_list2 ? ctx->npairs2 : 0;
+        out[4] = ctx->swap_off.empty() ? 0 : (long)ctx->swap_off.size() - 1;
+        out[5] = ctx->n_inrange;
+        return 0;
+    });
+}
+
+int mapp_dmd_get_list(mapp_dmd_t* ctx, int* offsets, int* entries) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede get_list");
+        if (offsets)
+            HIP_OK(hipMemcpy(offsets, ctx->d_off.p, sizeof(int) * (ctx->natms + 1), hipMemcpyDeviceToHost));
+        if (entries && ctx->npairs > 0)
+            HIP_OK(hipMemcpy(entries, ctx->d_nl.p, sizeof(int) * ctx->npairs, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+int mapp_dmd_get_list2(mapp_dmd_t* ctx, int* offsets, int* entries) {
+    return guarded(ctx, [&]() {
+        require(ctx->have_list2, MAPP_DMD_E_STATE, "create_2nd_neigh_lst must precede get_list2");
+        if (offsets)
+            HIP_OK(hipMemcpy(offsets, ctx->d_off2.p, sizeof(int) * (ctx->natms + 1), hipMemcpyDeviceToHost));
+        if (entries && ctx->npairs2 > 0)
+            HIP_OK(hipMemcpy(entries, ctx->d_nl2.p, sizeof(int) * ctx->npairs2, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+int mapp_dmd_get_ghosts(mapp_dmd_t* ctx, double* x, int* src) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede get_ghosts");
+        const int nt = ctx->nt, ng = ctx->nghost;
+        if (ng == 0) return 0;
+        if (x) {
+            DevBuf<double> tmp;
+            tmp.reserve((size_t)ng * (3 + nt));
+            if (nt == 1)
+                hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(ng)), dim3(kBlock), 0, ctx->stream, tmp.p, ng,
+                                   ctx->d_pos.p + ctx->natms, ctx->d_alpha.p + (size_t)ctx->natms * nt);
+            else
+                hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(ng)), dim3(kBlock), 0, ctx->stream, tmp.p, ng,
+                                   ctx->d_pos.p + ctx->natms, ctx->d_alpha.p + (size_t)ctx->natms * nt);
+            launch_check("k_gather_x ghosts");
+            HIP_OK(hipMemcpyAsync(x, tmp.p, sizeof(double) * ng * (3 + nt), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+        }
+        if (src) HIP_OK(hipMemcpy(src, ctx->d_ghost_src.p, sizeof(int) * ng, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+int mapp_dmd_get_gh(mapp_dmd_t* ctx, double* xi, double* w0, double* w1, double* w2) {
+    return guarded(ctx, [&]() {
+        require(ctx->have_params, MAPP_DMD_E_STATE, "coef must precede get_gh");
+        for (int k = 0; k < ctx->P.no_i; ++k) {
+            if (xi) xi[k] = ctx->P.xi[k];
+            if (w0) w0[k] = ctx->P.w0[k];
+            if (w1) w1[k] = ctx->P.w1[k];
+            if (w2) w2[k] = ctx->P.w2[k];
+        }
+        return 0;
+    });
+}
+
+int mapp_dmd_last_timing(mapp_dmd_t* ctx, double* ms) {
+    return guarded(ctx, [&]() {
+        require(ms != nullptr, MAPP_DMD_E_ARG, "null output");
+        for (int k = 0; k < 5; ++k) ms[k] = ctx->last_ms[k];
+        return 0;
+    });
+}
+
+int mapp_dmd_enable_timing(mapp_dmd_t* ctx, int on) {
+    return guarded(ctx, [&]() {
+        ctx->timing = on != 0;
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// The context behind the C ABI: everything one MPI rank's domain needs, resident on one GPU.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "dmd_common.hpp"
+
+struct mapp_dmd {
+    int nt = 0, variant = 1, device = 0, prec = MAPP_DMD_PREC_FP64;
+    std::string err;
+    hipStream_t stream = nullptr;
+
+    // tables (host copies, reference layouts)
+    bool have_tables = false, have_params = false, have_system = false, inited = false;
+    int nr = 0, nrho = 0;
+    double dr = 0, drho = 0;
+    std::vector<double> F9, rho7, phi7;  // [nt][nrho+1][9], [nt*nt][nr][7], [nphi][nr][7]
+
+    // parameters
+    mappdmd::FFParams<double> P{};
+    mappdmd::FFParams<float> Pf{};
+    double rsq_crd = 0, mod_rc = 0;
+    double wi1_ref[mappdmd::kMaxPoints]{}, wi2_ref[mappdmd::kMaxPoints]{};
+
+    // box and atoms
+    mappdmd::BoxParams bp{};
+    double skin = 0.5;
+    int natms = 0, nghost = 0;
+    std::vector<double> h_s, h_alpha, h_c;  // owned input rows until init
+    std::vector<int> swap_off, swap_dim, swap_dir;
+
+    // device tables
+    mappdmd::DevBuf<double> d_quad64, d_rho7, d_phi7, d_F9;
+    mappdmd::DevBuf<float> d_quad32;
+    // device atoms (owned rows first, ghosts after)
+    mappdmd::DevBuf<double4> d_pos, d_x0;
+    mappdmd::DevBuf<double> d_alpha, d_conc;
+    mappdmd::DevBuf<int> d_ghost_src, d_ghost_owner;
+    // binning and lists
+    mappdmd::DevBuf<int> d_flag, d_offs, d_bin_of, d_bin_cnt, d_bin_start, d_bin_atoms;
+    mappdmd::DevBuf<int> d_cnt, d_off, d_nl, d_roff, d_rev;
+    mappdmd::DevBuf<int> d_off2, d_nl2, d_roff2, d_rev2, d_crd;
+    mappdmd::DevBuf<int> d_pair_i, d_pair_i2;
+    std::vector<mappdmd::DevBuf<int>*> scan_levels;
+    long npairs = 0, npairs2 = 0;
+    bool have_list2 = false;
+    // per-atom vectors ([ntot][nt] unless noted) and pair buffers
+    mappdmd::DevBuf<double> d_E, d_dE, d_ddE, d_n, d_s, d_t, d_v, d_cd, d_fa, d_Eown, d_a, d_g;
+    mappdmd::DevBuf<double> d_f, d_fown;        // [natms][3+nt]
+    mappdmd::DevBuf<double> d_pairE, d_pairF;   // [npairs][nt], [npairs][3+nt]
+    mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair value caches for c_d_calc / calc_g
+    bool cache_valid = false;
+    mappdmd::DevBuf<double> d_part, d_out, d_xaos;
+    mappdmd::DevBuf<int> d_iflag;
+    long n_inrange = 0;
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev[8]{};
+    double last_ms[5]{};
+
+    ~mapp_dmd();
+};

@@ -1,0 +1,253 @@
+"""Python view of the C ABI (include/mapp_dmd.h) through ctypes.
+
+ForceFieldDMD mirrors the reference's ForceField interface (src/ff.h:12-29):
+coef / init / fin / force_calc / energy_calc / create_2nd_neigh_lst /
+c_d_calc / calc_g, with the same argument meaning.  Errors raise DMDError
+carrying the library's status code and message (the reference aborts).
+The library must be built in-tree (mapp_amd/lib/libmapp_dmd.so); there is no
+fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmapp_dmd.so")
+
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2}
+PREC = {"fp64": 0, "mixed": 1, "surrogate": 2}
+VEC = {"x": 0, "c": 1, "f": 2, "E": 3, "dE": 4, "n": 5, "c_d": 6, "ddE": 7, "s": 8, "t": 9, "v": 10, "a": 11, "g": 12}
+
+_lib = None
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+LP = C.POINTER(C.c_long)
+VPP = C.POINTER(C.c_void_p)
+CTX = C.c_void_p
+
+_SIGS = {
+    "mapp_dmd_create": [C.POINTER(CTX), C.c_int, C.c_int, C.c_int],
+    "mapp_dmd_destroy": [CTX],
+    "mapp_dmd_set_fs": [CTX, C.c_int, D, C.c_int, D, DP, DP, DP],
+    "mapp_dmd_read_fs": [CTX, C.c_char_p, C.POINTER(C.c_char_p)],
+    "mapp_dmd_coef": [CTX, D, C.c_int, D, D, D, DP],
+    "mapp_dmd_set_precision": [CTX, C.c_int],
+    "mapp_dmd_set_system": [CTX, DP, C.c_int, DP, DP],
+    "mapp_dmd_init": [CTX, D],
+    "mapp_dmd_fin": [CTX],
+    "mapp_dmd_set_x": [CTX, DP, IP],
+    "mapp_dmd_set_c": [CTX, DP],
+    "mapp_dmd_force_calc": [CTX, C.c_int, DP],
+    "mapp_dmd_energy_calc": [CTX, DP],
+    "mapp_dmd_create_2nd_neigh_lst": [CTX],
+    "mapp_dmd_c_d_calc": [CTX],
+    "mapp_dmd_calc_g": [CTX, C.c_int, D, C.c_void_p, C.c_void_p, DP],
+    "mapp_dmd_get_vec": [CTX, C.c_int, DP, C.c_long],
+    "mapp_dmd_device_ptr": [CTX, C.c_int, VPP],
+    "mapp_dmd_set_vec": [CTX, C.c_int, DP, C.c_long],
+    "mapp_dmd_sync": [CTX],
+    "mapp_dmd_sizes": [CTX, LP],
+    "mapp_dmd_get_list": [CTX, IP, IP],
+    "mapp_dmd_get_list2": [CTX, IP, IP],
+    "mapp_dmd_get_ghosts": [CTX, DP, IP],
+    "mapp_dmd_get_gh": [CTX, DP, DP, DP, DP],
+    "mapp_dmd_last_timing": [CTX, DP],
+    "mapp_dmd_enable_timing": [CTX, C.c_int],
+}
+
+
+class DMDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mapp_dmd error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Load libmapp_dmd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python -m mapp_amd.build`")
+    lib = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.mapp_dmd_last_error.argtypes = [CTX]
+    lib.mapp_dmd_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(IP)
+
+
+class ForceFieldDMD:
+    """ForceField_DMD / _I / _II on one GPU through the C ABI."""
+
+    def __init__(self, no_types=2, variant="DMD_I", device=0):
+        self.lib = load_library()
+        self.nt = no_types
+        self.ctx = CTX()
+        self._call("mapp_dmd_create", C.byref(self.ctx), no_types, VARIANT[variant], device, noctx=True)
+
+    def _call(self, name, *args, noctx=False):
+        fn = getattr(self.lib, name)
+        rc = fn(*args) if noctx else fn(self.ctx, *args)
+        if rc != 0:
+            msg = self.lib.mapp_dmd_last_error(self.ctx).decode() if not noctx and self.ctx else "create failed"
+            raise DMDError(rc, msg)
+
+    def close(self):
+        if self.ctx:
+            self.lib.mapp_dmd_destroy(self.ctx)
+            self.ctx = CTX()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up (ff_coef, read, init) --
+    def set_fs(self, t):
+        F = np.ascontiguousarray(t.F, dtype=np.float64)
+        rho = np.ascontiguousarray(t.rho, dtype=np.float64)
+        rphi = np.ascontiguousarray(t.rphi, dtype=np.float64)
+        self._call("mapp_dmd_set_fs", t.nrho, t.drho, t.nr, t.dr, _dp(F), _dp(rho), _dp(rphi))
+
+    def read_fs(self, path, elements):
+        arr = (C.c_char_p * len(elements))(*[e.encode() for e in elements])
+        self._call("mapp_dmd_read_fs", path.encode(), arr)
+
+    def coef(self, r_crd, no_i, alpha_min, alpha_max, T, masses):
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        self._call("mapp_dmd_coef", r_crd, no_i, alpha_min, alpha_max, T, _dp(m))
+
+    def set_precision(self, mode):
+        self._call("mapp_dmd_set_precision", PREC[mode])
+
+    def set_system(self, H, x_reduced, c):
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        x = np.ascontiguousarray(x_reduced, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self._call("mapp_dmd_set_system", _dp(H), x.shape[0], _dp(x), _dp(c))
+
+    def init(self, skin=0.5):
+        self._call("mapp_dmd_init", skin)
+
+    def fin(self):
+        self._call("mapp_dmd_fin")
+
+    def set_x(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        flag = C.c_int(0)
+        self._call("mapp_dmd_set_x", _dp(x), C.byref(flag))
+        return bool(flag.value)
+
+    def set_c(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self._call("mapp_dmd_set_c", _dp(c))
+
+    # -- the ForceField interface --
+    def force_calc(self, st_clc=1):
+        en = np.zeros(7)
+        self._call("mapp_dmd_force_calc", int(st_clc), _dp(en))
+        return en
+
+    def energy_calc(self):
+        e = C.c_double(0.0)
+        self._call("mapp_dmd_energy_calc", C.byref(e))
+        return e.value
+
+    def create_2nd_neigh_lst(self):
+        self._call("mapp_dmd_create_2nd_neigh_lst")
+
+    def c_d_calc(self):
+        self._call("mapp_dmd_c_d_calc")
+
+    def calc_g(self, chk, alpha, a_dev_ptr=None, g_dev_ptr=None):
+        """calc_g on device vectors; by default the context's own A and G buffers"""
+        a_dev_ptr = self.device_ptr("a") if a_dev_ptr is None else a_dev_ptr
+        g_dev_ptr = self.device_ptr("g") if g_dev_ptr is None else g_dev_ptr
+        cost = C.c_double(0.0)
+        self._call("mapp_dmd_calc_g", int(chk), float(alpha), C.c_void_p(a_dev_ptr), C.c_void_p(g_dev_ptr),
+                   C.byref(cost))
+        return cost.value
+
+    def set_vec(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._call("mapp_dmd_set_vec", VEC[name], _dp(a), a.size)
+
+    # -- data access --
+    def sizes(self):
+        out = np.zeros(6, dtype=np.int64)
+        self._call("mapp_dmd_sizes", out.ctypes.data_as(LP))
+        return dict(natms=int(out[0]), nghost=int(out[1]), npairs=int(out[2]), npairs2=int(out[3]),
+                    nswap=int(out[4]), n_inrange=int(out[5]))
+
+    def get(self, name):
+        s = self.sizes()
+        width = 3 + self.nt if name in ("x", "f") else self.nt
+        out = np.zeros((s["natms"], width))
+        self._call("mapp_dmd_get_vec", VEC[name], _dp(out), out.size)
+        return out
+
+    def device_ptr(self, name):
+        p = C.c_void_p()
+        self._call("mapp_dmd_device_ptr", VEC[name], C.byref(p))
+        return p.value
+
+    def neighbor_list(self):
+        s = self.sizes()
+        off = np.zeros(s["natms"] + 1, dtype=np.int32)
+        nl = np.zeros(max(s["npairs"], 1), dtype=np.int32)
+        self._call("mapp_dmd_get_list", _ip(off), _ip(nl))
+        return off, nl[: s["npairs"]]
+
+    def neighbor_list2(self):
+        s = self.sizes()
+        off = np.zeros(s["natms"] + 1, dtype=np.int32)
+        nl = np.zeros(max(s["npairs2"], 1), dtype=np.int32)
+        self._call("mapp_dmd_get_list2", _ip(off), _ip(nl))
+        return off, nl[: s["npairs2"]]
+
+    def ghosts(self):
+        s = self.sizes()
+        x = np.zeros((max(s["nghost"], 1), 3 + self.nt))
+        src = np.zeros(max(s["nghost"], 1), dtype=np.int32)
+        self._call("mapp_dmd_get_ghosts", _dp(x), _ip(src))
+        return x[: s["nghost"]], src[: s["nghost"]]
+
+    def gh(self, no_i):
+        arr = [np.zeros(no_i) for _ in range(4)]
+        self._call("mapp_dmd_get_gh", *[_dp(a) for a in arr])
+        return arr
+
+    def enable_timing(self, on=True):
+        self._call("mapp_dmd_enable_timing", 1 if on else 0)
+
+    def last_timing(self):
+        ms = np.zeros(5)
+        self._call("mapp_dmd_last_timing", _dp(ms))
+        return ms
+
+
+def setup(lat, tables, no_i=3, variant="DMD_I", precision="fp64", skin=0.5, r_crd=3.0,
+          alpha_min=25.0, alpha_max=1000.0, T=600.0, device=0):
+    """ff + ff_coef + read + init in the order of the reference's scripts."""
+    ff = ForceFieldDMD(no_types=lat.c.shape[1], variant=variant, device=device)
+    ff.set_fs(tables)
+    ff.coef(r_crd, no_i, alpha_min, alpha_max, T, tables.masses[: lat.c.shape[1]])
+    ff.set_precision(precision)
+    ff.set_system(lat.H, lat.x, lat.c)
+    ff.init(skin)
+    return ff

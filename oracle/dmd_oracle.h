@@ -1,0 +1,88 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle.
+ *
+ * FP64 CPU restatement of MAPP's diffusive-molecular-dynamics force field
+ * (ForceField_DMD, ForceField_DMD_I, ForceField_DMD_II), its plain EAM loop
+ * (ForceField_eam), the single-rank periodic ghost construction and the cell
+ * list / half neighbor list, in the reference's operation order.  It is
+ * compiled with -O2 -ffp-contract=off so every product and sum is a separately
+ * rounded IEEE binary64 operation, as in the reference build.
+ *
+ * Only tests, the smoke check and the CPU-baseline leg of bench.py may load it;
+ * the product never links or calls it.
+ *
+ * Function-by-function citations are in dmd_oracle.c.
+ */
+#ifndef MAPP_DMD_ORACLE_H
+#define MAPP_DMD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc_t;
+
+/* variants of the DMD force field (they differ only in c_d_calc / calc_g) */
+enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3 };
+
+orc_t* orc_create(int nt, int variant);
+void orc_destroy(orc_t* o);
+
+/* Finnis-Sinclair tables in file order: F[a][nrho], rho[a][b][nr], rphi[i][j<=i][nr] */
+int orc_set_fs(orc_t* o, int nrho, double drho, int nr, double dr,
+               const double* F, const double* rho, const double* rphi);
+/* ff_coef r_crd no_i alpha_min alpha_max T; masses in amu */
+int orc_set_params(orc_t* o, double r_crd, int no_i, double alpha_min,
+                   double alpha_max, double T, const double* mass);
+/* H lower triangular (row-major 3x3); x: natms*(3+nt) with reduced coordinates
+   in the first three slots and alpha after them; c: natms*nt.  For ORC_EAM:
+   x is natms*3 reduced coordinates and c holds the type of each atom. */
+int orc_set_system(orc_t* o, const double* H, int natms, const double* x,
+                   const double* c);
+/* force-field init + ghosts + cell list + half list (Min_LBFGS::init order) */
+int orc_init(orc_t* o, double skin);
+/* replace owned x (cartesian + alpha), then Atoms::update_0 (rebuild or ghost update) */
+int orc_set_x_update(orc_t* o, const double* x_owned, int* rebuilt);
+/* 1: pair caches are zeroed before every force_calc ("fresh" semantics);
+   0: reference semantics (zeroed only when the caches grow) */
+void orc_set_fresh_caches(orc_t* o, int fresh);
+
+void orc_force_calc(orc_t* o, int st_clc, double* en_st);
+double orc_energy_calc(orc_t* o);
+void orc_create_2nd_neigh_lst(orc_t* o);
+void orc_c_d_calc(orc_t* o);
+double orc_calc_g(orc_t* o, int chk, double alpha, const double* a, double* g);
+
+/* introspection */
+int orc_natms(const orc_t* o);
+int orc_nghost(const orc_t* o);
+long orc_npairs(const orc_t* o);
+long orc_npairs2(const orc_t* o);
+const int* orc_list_offsets(const orc_t* o);   /* natms+1 */
+const int* orc_list(const orc_t* o);           /* npairs */
+const int* orc_list2_offsets(const orc_t* o);
+const int* orc_list2(const orc_t* o);
+const double* orc_x(const orc_t* o);           /* (natms+nghost)*xdim */
+const double* orc_c(const orc_t* o);           /* (natms+nghost)*nt */
+const int* orc_ghost_src(const orc_t* o);      /* nghost: source index of each ghost */
+int orc_nswap(const orc_t* o);
+const int* orc_swap_offsets(const orc_t* o);   /* nswap+1, ghost-relative */
+double* orc_vec(orc_t* o, const char* name);   /* f, E, dE, ddE, n, s, t, v, c_d */
+const int* orc_crd(const orc_t* o);
+void orc_scalars(const orc_t* o, double* out); /* rc, cut_sq, mod_rc, cut_sq_mod, cut_sk_sq, kbT, beta, rho_max, dr_inv, drho_inv, c_0[0..nt-1] */
+void orc_gh(const orc_t* o, double* xi, double* w0, double* w1, double* w2);
+/* spline rows: kind 0=F (9 per row), 1=rho (7), 2=r*phi (7) */
+const double* orc_table(const orc_t* o, int kind, int slot, int* nrows, int* ncoef);
+/* box helpers (restated M3INV, x2s, s2x) */
+void orc_m3inv(const double* H, double* B);
+void orc_x2s(const double* B, int n, int xdim, double* x);
+void orc_s2x(const double* H, int n, int xdim, double* x);
+
+/* Gaussian-averaged radial primitives for one pair (r, alpha) and table slots:
+   out = {rho, drho_dr, drho_dalpha, phi, dphi_dr, dphi_dalpha}; returns 1 if computed */
+int orc_pair_quadrature(const orc_t* o, double r, double alpha, int rho_slot,
+                        int phi_slot, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes view of the FP64 CPU oracle
+(oracle/dmd_oracle.c), a restatement of the reference force field used as the
+checker by tests, the smoke check and the CPU-baseline leg of bench.py.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_dmd.so")
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3}
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int, C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_fs.argtypes = [C.c_void_p, C.c_int, D, C.c_int, D, DP, DP, DP]
+        L.orc_set_params.argtypes = [C.c_void_p, D, C.c_int, D, D, D, DP]
+        L.orc_set_system.argtypes = [C.c_void_p, DP, C.c_int, DP, DP]
+        L.orc_init.argtypes = [C.c_void_p, D]
+        L.orc_set_x_update.argtypes = [C.c_void_p, DP, IP]
+        L.orc_set_fresh_caches.argtypes = [C.c_void_p, C.c_int]
+        L.orc_force_calc.argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_energy_calc.argtypes = [C.c_void_p]
+        L.orc_energy_calc.restype = D
+        L.orc_create_2nd_neigh_lst.argtypes = [C.c_void_p]
+        L.orc_c_d_calc.argtypes = [C.c_void_p]
+        L.orc_calc_g.argtypes = [C.c_void_p, C.c_int, D, DP, DP]
+        L.orc_calc_g.restype = D
+        for nm in ("orc_natms", "orc_nghost", "orc_nswap"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = C.c_int
+        for nm in ("orc_npairs", "orc_npairs2"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = C.c_long
+        for nm in ("orc_list_offsets", "orc_list", "orc_list2_offsets", "orc_list2", "orc_ghost_src",
+                   "orc_swap_offsets", "orc_crd"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = IP
+        for nm in ("orc_x", "orc_c"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = DP
+        L.orc_vec.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_vec.restype = DP
+        L.orc_scalars.argtypes = [C.c_void_p, DP]
+        L.orc_gh.argtypes = [C.c_void_p, DP, DP, DP, DP]
+        L.orc_table.argtypes = [C.c_void_p, C.c_int, C.c_int, IP, IP]
+        L.orc_table.restype = DP
+        L.orc_m3inv.argtypes = [DP, DP]
+        L.orc_x2s.argtypes = [DP, C.c_int, C.c_int, DP]
+        L.orc_s2x.argtypes = [DP, C.c_int, C.c_int, DP]
+        L.orc_pair_quadrature.argtypes = [C.c_void_p, D, D, C.c_int, C.c_int, DP]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _view(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+
+class Oracle:
+    """CPU restatement of ForceField_DMD* / ForceField_eam on one rank."""
+
+    def __init__(self, nt=2, variant="DMD_I"):
+        self.L = lib()
+        self.nt = nt
+        self.variant = variant
+        self.o = self.L.orc_create(nt, VARIANT[variant])
+        self.xdim = 3 if variant == "EAM" else 3 + nt
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.o)
+        except Exception:
+            pass
+
+    def set_fs(self, t):
+        F = np.ascontiguousarray(t.F, dtype=np.float64)
+        rho = np.ascontiguousarray(t.rho, dtype=np.float64)
+        rphi = np.ascontiguousarray(t.rphi, dtype=np.float64)
+        assert self.L.orc_set_fs(self.o, t.nrho, t.drho, t.nr, t.dr, _dp(F), _dp(rho), _dp(rphi)) == 0
+
+    def set_params(self, r_crd=3.0, no_i=3, alpha_min=25.0, alpha_max=1000.0, T=600.0, masses=(58.6934, 63.546)):
+        m = np.ascontiguousarray(masses[: self.nt], dtype=np.float64)
+        assert self.L.orc_set_params(self.o, r_crd, no_i, alpha_min, alpha_max, T, _dp(m)) == 0
+
+    def set_system(self, H, x, c):
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert self.L.orc_set_system(self.o, _dp(H), x.shape[0], _dp(x), _dp(c)) == 0
+
+    def init(self, skin=0.5):
+        assert self.L.orc_init(self.o, skin) == 0
+
+    def set_fresh_caches(self, fresh):
+        self.L.orc_set_fresh_caches(self.o, 1 if fresh else 0)
+
+    def set_x(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        flag = C.c_int(0)
+        self.L.orc_set_x_update(self.o, _dp(x), C.byref(flag))
+        return bool(flag.value)
+
+    def force_calc(self, st_clc=1):
+        en = np.zeros(7)
+        self.L.orc_force_calc(self.o, int(st_clc), _dp(en))
+        return en
+
+    def energy_calc(self):
+        return self.L.orc_energy_calc(self.o)
+
+    def create_2nd_neigh_lst(self):
+        self.L.orc_create_2nd_neigh_lst(self.o)
+
+    def c_d_calc(self):
+        self.L.orc_c_d_calc(self.o)
+
+    def calc_g(self, chk, alpha, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        g = np.zeros_like(a)
+        cost = self.L.orc_calc_g(self.o, int(chk), float(alpha), _dp(a), _dp(g))
+        return cost, g.reshape(-1, self.nt)
+
+    @property
+    def natms(self):
+        return self.L.orc_natms(self.o)
+
+    @property
+    def nghost(self):
+        return self.L.orc_nghost(self.o)
+
+    def neighbor_list(self):
+        n = self.natms
+        off = _view(self.L.orc_list_offsets(self.o), n + 1, np.int32)
+        nl = _view(self.L.orc_list(self.o), int(self.L.orc_npairs(self.o)), np.int32)
+        return off, nl
+
+    def neighbor_list2(self):
+        n = self.natms
+        off = _view(self.L.orc_list2_offsets(self.o), n + 1, np.int32)
+        nl = _view(self.L.orc_list2(self.o), int(self.L.orc_npairs2(self.o)), np.int32)
+        return off, nl
+
+    def ghosts(self):
+        ng = self.nghost
+        x = _view(self.L.orc_x(self.o), (self.natms + ng) * self.xdim, np.float64).reshape(-1, self.xdim)
+        src = _view(self.L.orc_ghost_src(self.o), ng, np.int32)
+        return x[self.natms:], src
+
+    def x_all(self):
+        ng = self.nghost
+        return _view(self.L.orc_x(self.o), (self.natms + ng) * self.xdim, np.float64).reshape(-1, self.xdim)
+
+    def get(self, name):
+        n = self.natms
+        if name == "f":
+            w = self.xdim
+        else:
+            w = 1 if self.variant == "EAM" else self.nt
+        ptr = self.L.orc_vec(self.o, name.encode())
+        return _view(ptr, n * w, np.float64).reshape(n, w)
+
+    def crd(self):
+        return _view(self.L.orc_crd(self.o), self.natms, np.int32)
+
+    def scalars(self):
+        out = np.zeros(10 + self.nt)
+        self.L.orc_scalars(self.o, _dp(out))
+        keys = ["rc", "cut_sq", "mod_rc", "cut_sq_mod", "cut_sk_sq", "kbT", "beta", "rho_max", "dr_inv", "drho_inv"]
+        d = dict(zip(keys, out[:10]))
+        d["c_0"] = out[10:]
+        return d
+
+    def gh(self, no_i):
+        arr = [np.zeros(no_i) for _ in range(4)]
+        self.L.orc_gh(self.o, *[_dp(a) for a in arr])
+        return arr
+
+    def table(self, kind, slot):
+        nr = C.c_int(0)
+        nc = C.c_int(0)
+        ptr = self.L.orc_table(self.o, kind, slot, C.byref(nr), C.byref(nc))
+        return _view(ptr, nr.value * nc.value, np.float64).reshape(nr.value, nc.value)
+
+    def pair_quadrature(self, r, alpha, rho_slot, phi_slot=-1):
+        out = np.zeros(6)
+        ok = self.L.orc_pair_quadrature(self.o, r, alpha, rho_slot, phi_slot, _dp(out))
+        return out if ok else None
+
+
+def make(lat, tables, no_i=3, variant="DMD_I", skin=0.5, fresh=True, **kw):
+    o = Oracle(nt=lat.c.shape[1], variant=variant)
+    o.set_fs(tables)
+    o.set_params(no_i=no_i, masses=tables.masses, **kw)
+    o.set_system(lat.H, lat.x, lat.c)
+    o.init(skin)
+    o.set_fresh_caches(fresh)
+    return o

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on the
+same seeded inputs (FCC Ni-Cu, 10^3 cells = 4000 sites unless noted).
+
+Bars: neighbour lists and ghosts bit-exact; FP64 energy, virial, forces,
+E, dE: relative error <= 1e-12 (infinity norm over the array, written per
+test); mixed precision (FP32 quadrature sums): <= 1e-5 relative on forces."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(lat, tables, no_i=3, variant="DMD_I", precision="fp64", fresh=True):
+    from mapp_amd.dmd import setup
+    from oracle import oracle as O
+    ff = setup(lat, tables, no_i=no_i, variant=variant, precision=precision)
+    orc = O.make(lat, tables, no_i=no_i, variant=variant, fresh=fresh)
+    return ff, orc
+
+
+@pytest.fixture(scope="module")
+def cfg0(tables):
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(10, seed=1)
+    ff, orc = _pair(lat, tables)
+    return lat, ff, orc
+
+
+def test_ghosts_bit_exact(cfg0):
+    lat, ff, orc = cfg0
+    gx, gsrc = ff.ghosts()
+    ox, osrc = orc.ghosts()
+    print("nghost", len(gsrc), len(osrc), flush=True)
+    assert np.array_equal(gsrc, osrc)
+    assert np.array_equal(gx.view(np.uint64), ox.view(np.uint64))
+
+
+def test_neighbor_list_bit_exact(cfg0):
+    lat, ff, orc = cfg0
+    goff, gnl = ff.neighbor_list()
+    ooff, onl = orc.neighbor_list()
+    print("pairs", len(gnl), len(onl), flush=True)
+    assert np.array_equal(goff, ooff)
+    assert np.array_equal(gnl, onl)
+
+
+def test_force_calc_fp64(cfg0):
+    lat, ff, orc = cfg0
+    en_g = ff.force_calc(1)
+    en_o = orc.force_calc(1)
+    print("energy", en_g[0], en_o[0], flush=True)
+    assert abs(en_g[0] - en_o[0]) <= 1e-12 * abs(en_o[0])
+    assert rel_inf(en_g[1:], en_o[1:]) <= 1e-12
+    for name in ("f", "E", "dE"):
+        err = rel_inf(ff.get(name), orc.get(name))
+        print(name, err, flush=True)
+        assert err <= 1e-12, name
+
+
+def test_energy_calc_fp64(cfg0):
+    lat, ff, orc = cfg0
+    eg = ff.energy_calc()
+    eo = orc.energy_calc()
+    assert abs(eg - eo) <= 1e-12 * abs(eo)
+
+
+def test_deterministic_repeat(cfg0):
+    lat, ff, orc = cfg0
+    e1 = ff.force_calc(1)
+    f1 = ff.get("f")
+    e2 = ff.force_calc(1)
+    f2 = ff.get("f")
+    assert np.array_equal(e1, e2)
+    assert np.array_equal(f1.view(np.uint64), f2.view(np.uint64))
+
+
+@pytest.mark.parametrize("variant", ["DMD", "DMD_I", "DMD_II"])
+def test_c_d_calc_fp64(tables, variant):
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(8, seed=3)
+    ff, orc = _pair(lat, tables, variant=variant)
+    ff.force_calc(1)
+    orc.force_calc(1)
+    ff.create_2nd_neigh_lst()
+    orc.create_2nd_neigh_lst()
+    goff, gnl = ff.neighbor_list2()
+    ooff, onl = orc.neighbor_list2()
+    assert np.array_equal(goff, ooff) and np.array_equal(gnl, onl)
+    ff.c_d_calc()
+    orc.c_d_calc()
+    for name in ("c_d", "n", "E", "dE"):
+        err = rel_inf(ff.get(name), orc.get(name))
+        print(variant, name, err, flush=True)
+        assert err <= 1e-12, name
+
+
+@pytest.mark.parametrize("variant", ["DMD", "DMD_I", "DMD_II"])
+def test_calc_g_fp64(tables, variant):
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(8, seed=4)
+    ff, orc = _pair(lat, tables, variant=variant)
+    ff.force_calc(1)
+    orc.force_calc(1)
+    ff.create_2nd_neigh_lst()
+    orc.create_2nd_neigh_lst()
+    rng = np.random.default_rng(0)
+    a = rng.uniform(-0.2, 0.2, size=lat.c.shape)
+    beta_t = 0.37
+    cost_o, g_o = orc.calc_g(0, beta_t, a)
+    ff.set_vec("a", a)
+    cost_g = ff.calc_g(0, beta_t)
+    g_g = ff.get("g")
+    print(variant, "cost", cost_g, cost_o, "g err", rel_inf(g_g, g_o), flush=True)
+    assert abs(cost_g - cost_o) <= 1e-12 * abs(cost_o)
+    assert rel_inf(g_g, g_o) <= 1e-12
+    cost_chk = ff.calc_g(1, beta_t)
+    assert abs(cost_chk - cost_o) <= 1e-12 * abs(cost_o)
+
+
+def test_mixed_precision_forces(tables):
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(10, seed=1)
+    ff, orc = _pair(lat, tables, no_i=5, precision="mixed")
+    en_g = ff.force_calc(1)
+    en_o = orc.force_calc(1)
+    ef = rel_inf(ff.get("f"), orc.get("f"))
+    ee = abs(en_g[0] - en_o[0]) / abs(en_o[0])
+    print("mixed: energy rel", ee, "force rel", ef, flush=True)
+    assert ee <= 1e-6
+    assert ef <= 1e-5
+
+
+def test_set_x_update_and_rebuild(tables):
+    """Atoms::update_0: small moves forward-update ghosts, large moves rebuild"""
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(6, seed=5)
+    ff, orc = _pair(lat, tables)
+    x = orc.x_all()[: lat.natms].copy()
+    rng = np.random.default_rng(1)
+    x1 = x.copy()
+    x1[:, :3] += rng.normal(0, 0.05, size=(lat.natms, 3))
+    assert ff.set_x(x1) == orc.set_x(x1) == False
+    gx, _ = ff.ghosts()
+    ox, _ = orc.ghosts()
+    assert np.array_equal(gx.view(np.uint64), ox.view(np.uint64))
+    assert rel_inf(ff.force_calc(1), orc.force_calc(1)) <= 1e-12
+    x2 = x1.copy()
+    x2[7, 0] += 0.8
+    assert ff.set_x(x2) == orc.set_x(x2) == True
+    goff, gnl = ff.neighbor_list()
+    ooff, onl = orc.neighbor_list()
+    assert np.array_equal(goff, ooff) and np.array_equal(gnl, onl)
+    assert rel_inf(ff.force_calc(1), orc.force_calc(1)) <= 1e-12
+    assert rel_inf(ff.get("f"), orc.get("f")) <= 1e-12
