@@ -84,6 +84,18 @@ long counts_to_offsets(mapp_dmd* ctx, const int* cnt, DevBuf<int>& off, long n) 
     return tot;
 }
 
+// out[0..nv) = sum over nparts partials (stride doubles each, starting at offset)
+void reduce_partials(mapp_dmd* ctx, const double* part, long nparts, int nv, int stride, int offset, double* out) {
+    const long chunk = 4096;
+    const long nb1 = (nparts + chunk - 1) / chunk;
+    ctx->d_stage.reserve((size_t)(nb1 + 1) * 8);
+    hipLaunchKernelGGL(k_reduce_stage, dim3((int)nb1), dim3(kBlock), 0, ctx->stream, part, nparts, nv, stride, offset,
+                       chunk, ctx->d_stage.p, 8);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stage.p, nb1, nv, 8, 0,
+                       nb1, out, 1);
+    launch_check("k_reduce_stage");
+}
+
 void require(bool cond, int code, const char* msg) {
     if (!cond) throw ApiError(code, msg);
 }
@@ -221,6 +233,10 @@ void rebuild(mapp_dmd* ctx) {
     ctx->d_pair_i.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_pair_i.p);
     launch_check("k_pair_owner");
+    ctx->nrev = nrev;
+    ctx->d_rpos.reserve(ctx->npairs + 1);
+    hipLaunchKernelGGL(k_rpos, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff.p, ctx->d_rev.p, natms, ctx->d_rpos.p);
+    launch_check("k_rpos");
     // per-atom vectors and pair buffers
     const size_t nv = (size_t)ntot * NT;
     for (auto* b : {&ctx->d_E, &ctx->d_dE, &ctx->d_ddE, &ctx->d_n, &ctx->d_s, &ctx->d_t, &ctx->d_v})
@@ -232,8 +248,9 @@ void rebuild(mapp_dmd* ctx) {
     ctx->d_Eown.reserve((size_t)natms * NT);
     ctx->d_f.reserve((size_t)natms * (3 + NT));
     ctx->d_fown.reserve((size_t)natms * (3 + NT));
-    ctx->d_pairE.reserve((size_t)(ctx->npairs + 1) * NT);
-    ctx->d_pairF.reserve((size_t)(ctx->npairs + 1) * (3 + NT));
+    ctx->d_pairE.reserve((size_t)(nrev + 1) * NT);
+    ctx->d_deriv.reserve((size_t)(ctx->npairs + 1) * NT * NT * 4);
+    ctx->d_pairF.reserve((size_t)(nrev + 1) * (3 + NT));
     ctx->d_crd.reserve(ntot);
     ctx->d_x0.reserve(natms);
     HIP_OK(hipMemcpyAsync(ctx->d_x0.p, ctx->d_pos.p, sizeof(double4) * natms, hipMemcpyDeviceToDevice, st));
@@ -300,42 +317,57 @@ void scatter_x(mapp_dmd* ctx, const double* dev_aos) {
 }
 
 // ------------------------------------------------------------ force kernels
+template <int NT>
+void pack_atoms(mapp_dmd* ctx) {
+    const long n = (long)(ctx->natms + ctx->nghost) * NT;
+    ctx->d_arec.reserve(n);
+    hipLaunchKernelGGL(k_pack_atoms, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, ctx->d_alpha.p, ctx->d_conc.p, n,
+                       ctx->d_arec.p);
+    launch_check("k_pack_atoms");
+}
+
 template <int NT, typename Q, int NQ>
 void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
     const int nb = nblk(natms);
+    const int nbw = nblk((long)natms * 64);  // one wavefront per site
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
-    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
-    double* part2 = part1 + (size_t)nb * 2;
+    double* part2 = part1 + (size_t)nbw * 2;
     double* part3 = part2 + (size_t)nb;
+    Q* deriv = reinterpret_cast<Q*>(ctx->d_deriv.p);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                       ctx->d_pairE.p, part1);
+    pack_atoms<NT>(ctx);
+    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                       ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                       ctx->d_pairE.p, deriv, part1);
     launch_check("k_pass1");
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
     hipLaunchKernelGGL((k_embed<NT, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p,
                        ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p,
-                       ctx->d_fa.p, part2);
+                       ctx->d_fa.p, ctx->d_arec.p, part2);
     launch_check("k_embed");
     ghost_copy(ctx, ctx->d_dE.p, NT);
+    if (ctx->nghost > 0) {
+        hipLaunchKernelGGL((k_ghost_copy<double4>), dim3(nblk(ctx->nghost)), dim3(kBlock), 0, st, ctx->d_arec.p, NT,
+                           natms, ctx->nghost, ctx->d_ghost_owner.p);
+        launch_check("k_ghost_copy arec");
+    }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
-    hipLaunchKernelGGL((k_pass2<NT, Q, NQ>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_dE.p,
-                       ctx->d_fown.p, ctx->d_pairF.p, part3);
+    hipLaunchKernelGGL((k_pass2<NT, Q>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, ctx->d_pos.p, ctx->d_arec.p, L,
+                       ctx->d_rpos.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
     launch_check("k_pass2");
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
                        ctx->d_fa.p, ctx->d_f.p);
     launch_check("k_fgather");
     double* out = ctx->d_out.p;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part1, nb, 2, 2, 0, out);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part2, nb, 1, 1, 0, out + 2);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part3, nb, 6, 6, 0, out + 3);
-    launch_check("k_reduce_partials");
+    reduce_partials(ctx, part1, nbw, 2, 2, 0, out);
+    reduce_partials(ctx, part2, nb, 1, 1, 0, out + 2);
+    reduce_partials(ctx, part3, nbw, 6, 6, 0, out + 3);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[4], st));
     double h[9];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -359,23 +391,24 @@ template <int NT, typename Q, int NQ>
 double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     hipStream_t st = ctx->stream;
     const int nb = nblk(ctx->natms);
+    const int nbw = nblk((long)ctx->natms * 64);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
-    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
-    double* part2 = part1 + (size_t)nb * 2;
-    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                       ctx->d_alpha.p, ctx->d_conc.p, L, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                       ctx->d_pairE.p, part1);
+    double* part2 = part1 + (size_t)nbw * 2;
+    pack_atoms<NT>(ctx);
+    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                       ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                       ctx->d_pairE.p, (Q*)nullptr, part1);
     launch_check("k_pass1 energy");
     hipLaunchKernelGGL((k_embed<NT, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
                        L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p, ctx->d_fa.p,
-                       part2);
+                       ctx->d_arec.p, part2);
     launch_check("k_embed energy");
     double* out = ctx->d_out.p;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part1, nb, 1, 2, 0, out);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, part2, nb, 1, 1, 0, out + 1);
-    launch_check("k_reduce_partials energy");
+    reduce_partials(ctx, part1, nbw, 1, 2, 0, out);
+    reduce_partials(ctx, part2, nb, 1, 1, 0, out + 1);
     double h[2];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -453,6 +486,8 @@ void fill_float_params(mapp_dmd* ctx) {
     F.cut_sq_mod = (float)P.cut_sq_mod;
     F.alpha_min = (float)P.alpha_min;
     F.alpha_max = (float)P.alpha_max;
+    F.inv_amin = (float)P.inv_amin;
+    F.inv_amax = (float)P.inv_amax;
     F.kbT = (float)P.kbT;
     F.beta = (float)P.beta;
     for (int k = 0; k < kMaxPoints; ++k) {
@@ -594,8 +629,7 @@ double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, doubl
     hipLaunchKernelGGL((k_calc_g_s<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, a, L2,
                        ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, ctx->d_part.p);
     launch_check("k_calc_g_s");
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, st, ctx->d_part.p, nb, 1, 1, 0, ctx->d_out.p);
-    launch_check("k_reduce_partials g");
+    reduce_partials(ctx, ctx->d_part.p, nb, 1, 1, 0, ctx->d_out.p);
     double inner = 0.0;
     HIP_OK(hipMemcpyAsync(&inner, ctx->d_out.p, sizeof(double), hipMemcpyDeviceToHost, st));
     if (!chk) {
@@ -749,6 +783,8 @@ int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, dou
         }
         P.alpha_min = alpha_min;
         P.alpha_max = alpha_max;
+        P.inv_amin = 1.0 / alpha_min;
+        P.inv_amax = 1.0 / alpha_max;
         const double kb = 8.617332478e-5, hbar = 6.5821192815e-16;
         for (int t = 0; t < kMaxTypes; ++t) P.c0[t] = 0.0;
         for (int t = 0; t < ctx->nt; ++t) {

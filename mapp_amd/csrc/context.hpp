@@ -40,7 +40,10 @@ struct mapp_dmd {
     mappdmd::DevBuf<int> d_flag, d_offs, d_bin_of, d_bin_cnt, d_bin_start, d_bin_atoms;
     mappdmd::DevBuf<int> d_cnt, d_off, d_nl, d_roff, d_rev;
     mappdmd::DevBuf<int> d_off2, d_nl2, d_roff2, d_rev2, d_crd;
-    mappdmd::DevBuf<int> d_pair_i, d_pair_i2;
+    mappdmd::DevBuf<int> d_pair_i, d_pair_i2, d_rpos;
+    mappdmd::DevBuf<double> d_inva, d_deriv;
+    mappdmd::DevBuf<double4> d_arec;  // [ntot][nt] {alpha, 1/alpha, c, dE}  // deriv: [npairs][nt*nt][4] of the quadrature type
+    long nrev = 0;
     std::vector<mappdmd::DevBuf<int>*> scan_levels;
     long npairs = 0, npairs2 = 0;
     bool have_list2 = false;
@@ -50,7 +53,7 @@ struct mapp_dmd {
     mappdmd::DevBuf<double> d_pairE, d_pairF;   // [npairs][nt], [npairs][3+nt]
     mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair value caches for c_d_calc / calc_g
     bool cache_valid = false;
-    mappdmd::DevBuf<double> d_part, d_out, d_xaos;
+    mappdmd::DevBuf<double> d_part, d_out, d_xaos, d_stage;
     mappdmd::DevBuf<int> d_iflag;
     long n_inrange = 0;
 

@@ -19,7 +19,7 @@ constexpr double kPiInSq = 0.564189583547756286948079451561;  // 1/sqrt(pi)
 template <typename T>
 struct FFParams {
     int natms, ntot, nr, nrho, no_i;
-    T dr_inv, drho_inv, rho_max, rc, cut_sq, cut_sq_mod, alpha_min, alpha_max;
+    T dr_inv, drho_inv, rho_max, rc, cut_sq, cut_sq_mod, alpha_min, alpha_max, inv_amin, inv_amax;
     T kbT, beta;
     T xi[kMaxPoints], w0[kMaxPoints], w1[kMaxPoints], w2[kMaxPoints];
     T c0[kMaxTypes];

@@ -23,6 +23,12 @@
 #pragma once
 #include "dmd_common.hpp"
 
+// The force kernels do not need the reference's bitwise operation order (the
+// bar is 1e-12 relative), so a*b+c may fuse into FMA here; the geometry
+// kernels, which must reproduce the reference's list bit for bit, are
+// compiled before this region with contraction off.
+#pragma clang fp contract(fast)
+
 namespace mappdmd {
 
 constexpr int kBlock = 256;
@@ -193,92 +199,364 @@ struct PairBufs {
     const int* rev;
 };
 
+// ---------------------------------------------------------------- lane mapping
+// NT*NT lanes per site, lane q = it*NT + jt owns the ordered type pair
+// (it, jt) of every pair of the site's list.  For NT = 2 the four lanes of a
+// site form a DPP quad: the transposed pair (jt, it) is quad_perm [0,2,1,3],
+// sums over jt are xor 1, sums over it are xor 2.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+constexpr int kDppTranspose = 0xD8;  // quad_perm [0,2,1,3]
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
+
+template <int NT>
+__device__ __forceinline__ double transpose_tp(double v) {
+    if constexpr (NT == 2) return dpp_f64<kDppTranspose>(v);
+    else return v;
+}
+template <int NT>
+__device__ __forceinline__ double sum_over_jt(double v) {
+    if constexpr (NT == 2) return v + dpp_f64<kDppXor1>(v);
+    else return v;
+}
+template <int NT>
+__device__ __forceinline__ double sum_over_it(double v) {
+    if constexpr (NT == 2) return v + dpp_f64<kDppXor2>(v);
+    else return v;
+}
+
+// which branch of the reference applies to the pair width alpha = 1/ia:
+// 1 quadrature (alpha_min < alpha < alpha_max), 2 classical (alpha >= alpha_max),
+// 0 nothing.  ia = 1/alpha_i + 1/alpha_j equals the reference's
+// alpha_i alpha_j / (alpha_i + alpha_j) to within two ulp; within 1e-12 of a
+// bound the reference's own expression decides.
+__device__ __forceinline__ int alpha_branch(const FFParams<double>& P, double ia, double ai, double aj) {
+    const double lo = P.inv_amax, hi = P.inv_amin;
+    if (fabs(ia - hi) <= 1e-12 * hi || fabs(ia - lo) <= 1e-12 * lo) {
+        const double a = (ai * aj) / (ai + aj);
+        return (P.alpha_min < a && a < P.alpha_max) ? 1 : (P.alpha_max <= a ? 2 : 0);
+    }
+    return (ia > lo && ia < hi) ? 1 : (ia <= lo ? 2 : 0);
+}
+
+// Gauss-Hermite average with sqrt(alpha) = asq and 1/alpha = ia given
+// (src/ff_dmd_I.cpp:180-242); no divisions and no branches: a point outside
+// the window (xi_k <= lower or xi_k >= upper) or a rule beyond the cutoff
+// (lower >= xi_max) contributes zero, its row index being clamped in range.
+// g, f = {value, d/dr, d/dalpha}; returns whether the rule is inside.
+template <typename Q, int NQ, bool DERIV>
+__device__ __forceinline__ bool gh_avg(const FFParams<Q>& P, Q r, Q r_inv, Q asq, Q ia, const Q* __restrict__ tab,
+                                       Q* g, Q* f) {
+    const int nq = NQ > 0 ? NQ : P.no_i;
+    const Q inv = ia * asq;
+    const Q upper = (r + P.rc) * asq;
+    const Q lower = (r - P.rc) * asq;
+    const bool rule = lower < P.xi[nq - 1];
+    Q g0 = 0, g1 = 0, g2 = 0, f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll
+    for (int k = 0; k < (NQ > 0 ? NQ : kMaxPoints); ++k) {
+        if (NQ == 0 && k >= nq) break;
+        const Q xk = P.xi[k];
+        const bool inside = rule && xk > lower && xk < upper;
+        const Q t = r - xk * inv;
+        Q p = fabs(t) * P.dr_inv;
+        int m = (int)p;
+        m = m < P.nr - 2 ? m : P.nr - 2;
+        m = m > 0 ? m : 0;
+        p -= (Q)m;
+        p = p < Q(1) ? p : Q(1);
+        const Q* row = tab + (size_t)m * kQuadRow;
+        Q rv = horner3(row, p) * t;
+        Q pv = horner3(row + 4, p);
+        pv = t < Q(0) ? -pv : pv;
+        rv = inside ? rv : Q(0);
+        pv = inside ? pv : Q(0);
+        g0 += P.w0[k] * rv;
+        f0 += P.w0[k] * pv;
+        if (DERIV) {
+            g1 += P.w1[k] * rv;
+            g2 += P.w2[k] * rv;
+            f1 += P.w1[k] * pv;
+            f2 += P.w2[k] * pv;
+        }
+    }
+    const Q sc = Q(kPiInSq) * r_inv;
+    g0 *= sc;
+    f0 *= sc;
+    g[0] = g0;
+    f[0] = f0;
+    if (DERIV) {
+        g1 *= sc;
+        g2 *= sc;
+        f1 *= sc;
+        f2 *= sc;
+        g[1] = -g0 * r_inv - Q(2) * asq * g1;
+        g[2] = (Q(0.5) * g0 - g2) * ia;
+        f[1] = -f0 * r_inv - Q(2) * asq * f1;
+        f[2] = (Q(0.5) * f0 - f2) * ia;
+    }
+    return rule;
+}
+
+// 1/alpha per (site, type), ghosts included
+__global__ void k_inv_alpha(const double* __restrict__ alpha, long n, double* __restrict__ inva) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) inva[k] = 1.0 / alpha[k];
+}
+
+// pair values (and r-, alpha-derivatives when DERIV) of the lane's type pair
+template <int NT, typename Q, int NQ, bool DERIV>
+__device__ __forceinline__ void lane_pair(const FFParams<double>& P, const FFParams<Q>& PQ, int it, int jt,
+                                          double rsq, double r, double r_inv, double ai, double aj, double ia,
+                                          double asq, const Q* __restrict__ quad, const double* __restrict__ rho7,
+                                          const double* __restrict__ phi7, double* g, double* f) {
+    const int br = alpha_branch(P, ia, ai, aj);
+    if (br == 1) {
+        Q gq[3], fq[3];
+        const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
+        if (gh_avg<Q, NQ, DERIV>(PQ, (Q)r, (Q)r_inv, (Q)asq, (Q)ia, tab, gq, fq)) {
+            g[0] = (double)gq[0];
+            f[0] = (double)fq[0];
+            if (DERIV) {
+                g[1] = (double)gq[1];
+                g[2] = (double)gq[2];
+                f[1] = (double)fq[1];
+                f[2] = (double)fq[2];
+            }
+        }
+    } else if (br == 2 && rsq < P.cut_sq) {
+        classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
+                       phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
+    }
+}
+
+// ---------------------------------------------------------------- wave per site
+// One wavefront per site i: lane l = k*LPP + q works on pair k of the current
+// chunk of i's half list (PPW = 64/LPP pairs per chunk) and on ordered type
+// pair q = it*NT + jt.  Streams indexed by the pair (list, rpos, derivative
+// records) are then contiguous across the wave, the site's own data is
+// wave-uniform, and each neighbour's attributes are one 32-byte record per type.
+template <int NT>
+struct WaveMap {
+    static constexpr int LPP = NT * NT;
+    static constexpr int PPW = 64 / LPP;
+};
+
+// per (site, type) record {alpha, 1/alpha, c, dE}, ghosts included
+__global__ void k_pack_atoms(const double* __restrict__ alpha, const double* __restrict__ conc, long n,
+                             double4* __restrict__ rec) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const double a = alpha[k];
+        rec[k] = make_double4(a, 1.0 / a, conc[k], 0.0);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// deterministic per-block sum of NV wave partials (lane 0 of each wave holds them)
+template <int NV>
+__device__ __forceinline__ void block_store_waves(const double* v, double* __restrict__ out) {
+    __shared__ double sh[kBlock / 64][NV];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[w][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int q = 0; q < kBlock / 64; ++q) s += sh[q][k];
+            out[(size_t)blockIdx.x * NV + k] = s;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- pass 1
+// Wave per site, 4 lanes per ordered type pair: lane l = k*LPP + q*4 + part.
+// A spline row of one type pair holds rho c0..c3 and r*phi c0..c3; the four
+// part lanes load one quarter each (16 bytes in FP64), so each table row costs
+// one cache-line access instead of four.  Each part lane evaluates half of a
+// cubic (c_lo + c_hi p); one DPP exchange joins the halves (val = h0 + p^2 h1)
+// in lane part 0 (rho) and lane part 2 (phi), which then carry the weighted
+// Gauss-Hermite sums of their function.
+//
 // ENERGY_ONLY restates energy_calc's triangle (jtype starts at itype, so phi
-// only for itype == jtype; src/ff_dmd_I.cpp:508)
+// only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
+// goes to pairE in reverse-CSR order (rpos), so the embedding reads it
+// contiguously.  Unless ENERGY_ONLY, the rho lane stores {drho/dr, drho/dalpha}
+// and the phi lane {dphi/dr, dphi/dalpha} at deriv[p][q] (the reference's
+// per-pair caches, src/ff_dmd_I.cpp:227-242, restricted to what the back-pass reads).
+template <int NT>
+struct Pass1Map {
+    static constexpr int LPP = NT * NT * 4;
+    static constexpr int PPW = 64 / LPP;
+};
+
+template <typename Q>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using T = double2;
+};
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+
+template <typename Q>
+__device__ __forceinline__ Q dpp_xor1_q(Q v);
+template <>
+__device__ __forceinline__ double dpp_xor1_q<double>(double v) {
+    return dpp_f64<kDppXor1>(v);
+}
+template <>
+__device__ __forceinline__ float dpp_xor1_q<float>(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppXor1, 0xF, 0xF, false));
+}
+
+constexpr int kDppRowRor8 = 0x128;  // row_ror:8 (rotate by 8 lanes within a row of 16)
+
 template <int NT, typename Q, int NQ, bool ENERGY_ONLY>
 __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
-                                                  const double* __restrict__ alpha,
-                                                  const double* __restrict__ conc, PairBufs L,
-                                                  const Q* __restrict__ quad, const double* __restrict__ rho7,
-                                                  const double* __restrict__ phi7, double* __restrict__ Eown,
-                                                  double* __restrict__ pairE, double* __restrict__ part) {
-    const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+                                                  const double4* __restrict__ arec, PairBufs L,
+                                                  const int* __restrict__ rpos, const Q* __restrict__ quad,
+                                                  const double* __restrict__ rho7, const double* __restrict__ phi7,
+                                                  double* __restrict__ Eown, double* __restrict__ pairE,
+                                                  Q* __restrict__ deriv, double* __restrict__ part) {
+    using Q2 = typename Vec2<Q>::T;
+    constexpr int LPP = Pass1Map<NT>::LPP, PPW = Pass1Map<NT>::PPW;
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6));
+    const int k = lane / LPP, q = (lane % LPP) >> 2, prt = lane & 3;
+    const int it = q / NT, jt = q % NT;
+    const bool active = !ENERGY_ONLY || jt >= it;
+    const bool want_phi = it >= jt;
+    const bool rho_lane = prt == 0, phi_lane = prt == 2;
     double acc[2] = {0.0, 0.0};  // energy, pairs inside the quadrature cut
     if (i < P.natms) {
         const double4 xi = pos[i];
-        double ai[NT], ci[NT], Ei[NT];
+        double ci[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            ai[t] = alpha[(size_t)i * NT + t];
-            ci[t] = conc[(size_t)i * NT + t];
-            Ei[t] = 0.0;
-        }
-        const int p1 = L.off[i + 1];
-        for (int p = L.off[i]; p < p1; ++p) {
-            const int j = L.nl[p];
-            const double4 xj = pos[j];
-            const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
-            const double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-            double Ej[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) Ej[t] = 0.0;
-            if (rsq < P.cut_sq_mod) {
-                acc[1] += 1.0;
-                double aj[NT], cj[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    aj[t] = alpha[(size_t)j * NT + t];
-                    cj[t] = conc[(size_t)j * NT + t];
+        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
+        const double4 ra = arec[(size_t)i * NT + it];
+        const double ai = ra.x, iai = ra.y;
+        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * 2;
+        double Ei = 0.0, en = 0.0, npair = 0.0;
+        const int p0 = L.off[i], p1 = L.off[i + 1];
+        for (int base = p0; base < p1; base += PPW) {
+            const int p = base + k;
+            const bool valid = p < p1;
+            int j = 0;
+            bool inr = false;
+            double rsq = 0.0;
+            if (valid) {
+                j = L.nl[p];
+                const double4 xj = pos[j];
+                const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
+                {
+#pragma clang fp contract(off)
+                    rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 }
-                const double r = sqrt(rsq);
-                const double r_inv = 1.0 / r;
-                const double half = j < P.natms ? 0.5 : 0.25;
+                inr = rsq < P.cut_sq_mod;
+            }
+            double ej = 0.0;
+            if (inr) {
+                if ((lane % LPP) == 0) npair += 1.0;
+                const double4 rj = arec[(size_t)j * NT + jt];
+                double cj[NT];
 #pragma unroll
-                for (int it = 0; it < NT; ++it) {
+                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : arec[(size_t)j * NT + t].z;
+                const double aj = rj.x, iaj = rj.y;
+                const double r_inv = rsqrt(rsq), r = rsq * r_inv;
+                const double ia = iai + iaj, asq = rsqrt(ia);
+                const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
+                double v[3] = {0.0, 0.0, 0.0};  // rho (part 0) or phi (part 2): value, d/dr, d/dalpha
+                if (br == 1) {
+                    const Q rq = (Q)r, iq = (Q)ia, aq = (Q)asq;
+                    const Q inv = iq * aq;
+                    const Q upper = (rq + PQ.rc) * aq, lower = (rq - PQ.rc) * aq;
+                    const int nq = NQ > 0 ? NQ : PQ.no_i;
+                    const bool rule = lower < PQ.xi[nq - 1];
+                    Q s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-                    for (int jt = (ENERGY_ONLY ? it : 0); jt < NT; ++jt) {
-                        const bool want_phi = it >= jt;
-                        const double a = (ai[it] * aj[jt]) / (ai[it] + aj[jt]);
-                        double g0 = 0.0, f0 = 0.0;
-                        bool hit = false;
-                        if (P.alpha_min < a && a < P.alpha_max) {
-                            Q g[3], f[3];
-                            const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
-                            if (want_phi)
-                                hit = gh_average<Q, NQ, false, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f);
-                            else
-                                hit = gh_average<Q, NQ, false, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f);
-                            if (hit) {
-                                g0 = (double)g[0];
-                                if (want_phi) f0 = (double)f[0];
-                            }
-                        } else if (P.alpha_max <= a && rsq < P.cut_sq) {
-                            double g[3], f[3];
-                            classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
-                                           phi7 + (size_t)type2phi(it, jt) * P.nr * 7, want_phi, g, f);
-                            g0 = g[0];
-                            if (want_phi) f0 = f[0];
-                            hit = true;
-                        }
-                        if (hit) {
-                            Ei[it] += g0 * cj[jt];
-                            Ej[jt] += g0 * ci[it];
-                            if (want_phi) acc[0] += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * f0;
+                    for (int kp = 0; kp < (NQ > 0 ? NQ : kMaxPoints); ++kp) {
+                        if (NQ == 0 && kp >= nq) break;
+                        const Q xk = PQ.xi[kp];
+                        const bool inside = rule && xk > lower && xk < upper;
+                        const Q t = rq - xk * inv;
+                        Q pp = fabs(t) * PQ.dr_inv;
+                        int m = (int)pp;
+                        m = m < P.nr - 2 ? m : P.nr - 2;
+                        m = m > 0 ? m : 0;
+                        pp -= (Q)m;
+                        pp = pp < Q(1) ? pp : Q(1);
+                        const Q2 c = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
+                        const Q h = c.x + c.y * pp;
+                        const Q hb = dpp_xor1_q<Q>(h);
+                        const Q val = h + pp * pp * hb;
+                        Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
+                        x = inside ? x : Q(0);
+                        s0 += PQ.w0[kp] * x;
+                        if (!ENERGY_ONLY) {
+                            s1 += PQ.w1[kp] * x;
+                            s2 += PQ.w2[kp] * x;
                         }
                     }
+                    if (rule) {
+                        const Q sc = Q(kPiInSq) * (Q)r_inv;
+                        const Q v0 = s0 * sc;
+                        v[0] = (double)v0;
+                        if (!ENERGY_ONLY) {
+                            v[1] = (double)(-v0 * (Q)r_inv - Q(2) * aq * (s1 * sc));
+                            v[2] = (double)((Q(0.5) * v0 - s2 * sc) * iq);
+                        }
+                    }
+                } else if (br == 2 && rsq < P.cut_sq) {
+                    double g[3], f[3];
+                    classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
+                                   phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
+                    for (int c = 0; c < 3; ++c) v[c] = prt == 0 ? g[c] : f[c];
+                }
+                if (rho_lane) {
+                    Ei += v[0] * cj[jt];
+                    ej = v[0] * ci[it];
+                }
+                if (phi_lane && want_phi && active) {
+                    const double half = j < P.natms ? 0.5 : 0.25;
+                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v[0];
+                }
+                if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
+                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + prt;
+                    d[0] = (Q)v[1];
+                    d[1] = (Q)v[2];
                 }
             }
-            if (j < P.natms) {
-#pragma unroll
-                for (int t = 0; t < NT; ++t) pairE[(size_t)p * NT + t] = Ej[t];
-            }
+            if constexpr (NT == 2) ej += dpp_f64<kDppRowRor8>(ej);  // sum over it (lanes 8 apart)
+            if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
         }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) Eown[(size_t)i * NT + t] = Ei[t];
+        for (int t = 0; t < NT; ++t) {
+            const double s = wave_sum(rho_lane && it == t ? Ei : 0.0);
+            if (lane == 0) Eown[(size_t)i * NT + t] = s;
+        }
+        acc[0] = wave_sum(en);
+        acc[1] = wave_sum(npair);
     }
-    block_sum_store<2>(acc, part);
+    block_store_waves<2>(acc, part);
 }
 
 // ---------------------------------------------------------------- embed
@@ -290,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
                                                   const double* __restrict__ pairE,
                                                   const double* __restrict__ F9, double* __restrict__ E,
                                                   double* __restrict__ dE, double* __restrict__ fa,
-                                                  double* __restrict__ part) {
+                                                  double4* __restrict__ arec, double* __restrict__ part) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double acc[1] = {0.0};
     if (i < P.natms) {
@@ -299,9 +577,8 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
         for (int t = 0; t < NT; ++t) e[t] = Eown[(size_t)i * NT + t];
         const int q1 = L.roff[i + 1];
         for (int q = L.roff[i]; q < q1; ++q) {
-            const size_t p = (size_t)L.rev[q];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) e[t] += pairE[p * NT + t];
+            for (int t = 0; t < NT; ++t) e[t] += pairE[(size_t)q * NT + t];
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -312,6 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
             E[(size_t)i * NT + t] = Fv;
             if (!ENERGY_ONLY) {
                 dE[(size_t)i * NT + t] = dF;
+                arec[(size_t)i * NT + t].w = dF;
                 fa[(size_t)i * NT + t] = -(1.5 * P.kbT * c / a);
             }
             acc[0] += c * Fv;
@@ -323,131 +601,118 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
 }
 
 // ---------------------------------------------------------------- pass 2
-template <int NT, typename Q, int NQ>
-__global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, FFParams<Q> PQ,
-                                                  const double4* __restrict__ pos,
-                                                  const double* __restrict__ alpha,
-                                                  const double* __restrict__ conc, PairBufs L,
-                                                  const Q* __restrict__ quad, const double* __restrict__ rho7,
-                                                  const double* __restrict__ phi7, const double* __restrict__ dE,
+// src/ff_dmd_I.cpp:357-436: fpair/apair of each ordered type pair from the
+// derivative records of pass 1 (slot type2rho[jt][it], i.e. the transposed
+// lane's record, pairs with dE_i; phi for it < jt lives in the transposed
+// lane's record as in the reference's shared COMP slot).  No table lookups.
+template <int NT, typename Q>
+__global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
+                                                  const double4* __restrict__ arec, PairBufs L,
+                                                  const int* __restrict__ rpos, const Q* __restrict__ deriv,
                                                   double* __restrict__ fown, double* __restrict__ pairF,
                                                   double* __restrict__ part) {
-    constexpr int NPHI = NT * (NT + 1) / 2;
+    constexpr int LPP = WaveMap<NT>::LPP, PPW = WaveMap<NT>::PPW;
     constexpr int XD = 3 + NT;
-    const int i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6));
+    const int k = lane / LPP, q = lane % LPP, it = q / NT, jt = q % NT;
+    const int qT = jt * NT + it;
+    const bool want_phi = it >= jt;
     double vir[6] = {0, 0, 0, 0, 0, 0};
     if (i < P.natms) {
         const double4 xi = pos[i];
-        double ai[NT], ci[NT], dEi[NT], fi[XD];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            ai[t] = alpha[(size_t)i * NT + t];
-            ci[t] = conc[(size_t)i * NT + t];
-            dEi[t] = dE[(size_t)i * NT + t];
-        }
-#pragma unroll
-        for (int k = 0; k < XD; ++k) fi[k] = 0.0;
-        const int p1 = L.off[i + 1];
-        for (int p = L.off[i]; p < p1; ++p) {
-            const int j = L.nl[p];
-            const double4 xj = pos[j];
-            const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
-            const double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+        const double4 ra = arec[(size_t)i * NT + it];
+        const double ai = ra.x, iai = ra.y, ci = ra.z, dEi = ra.w;
+        double fx = 0.0, fy = 0.0, fz = 0.0, fa = 0.0;
+        const int p0 = L.off[i], p1 = L.off[i + 1];
+        for (int base = p0; base < p1; base += PPW) {
+            const int p = base + k;
+            const bool valid = p < p1;
+            int j = 0;
+            bool inr = false;
+            double dx0 = 0, dx1 = 0, dx2 = 0, rsq = 0;
+            if (valid) {
+                j = L.nl[p];
+                const double4 xj = pos[j];
+                dx0 = xi.x - xj.x;
+                dx1 = xi.y - xj.y;
+                dx2 = xi.z - xj.z;
+                {
+#pragma clang fp contract(off)
+                    rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                }
+                inr = rsq < P.cut_sq_mod;
+            }
             const bool owned = j < P.natms;
-            double pf[XD];
-#pragma unroll
-            for (int k = 0; k < XD; ++k) pf[k] = 0.0;
-            if (rsq < P.cut_sq_mod) {
-                double aj[NT], cj[NT], dEj[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    aj[t] = alpha[(size_t)j * NT + t];
-                    cj[t] = conc[(size_t)j * NT + t];
-                    dEj[t] = dE[(size_t)j * NT + t];
-                }
-                const double r = sqrt(rsq);
-                const double r_inv = 1.0 / r;
-                // all ordered type pairs: rho {val, dr, dalpha}; phi for it >= jt
-                double gr[NT][NT][2], fr[NPHI][2];
-#pragma unroll
-                for (int it = 0; it < NT; ++it) {
-#pragma unroll
-                    for (int jt = 0; jt < NT; ++jt) {
-                        const bool want_phi = it >= jt;
-                        const double a = (ai[it] * aj[jt]) / (ai[it] + aj[jt]);
-                        double g[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-                        if (P.alpha_min < a && a < P.alpha_max) {
-                            Q gq[3], fq[3];
-                            const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
-                            bool hit;
-                            if (want_phi)
-                                hit = gh_average<Q, NQ, true, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, gq, fq);
-                            else
-                                hit = gh_average<Q, NQ, true, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, gq, fq);
-                            if (hit) {
-                                g[1] = (double)gq[1];
-                                g[2] = (double)gq[2];
-                                if (want_phi) {
-                                    f[1] = (double)fq[1];
-                                    f[2] = (double)fq[2];
-                                }
-                            }
-                        } else if (P.alpha_max <= a && rsq < P.cut_sq) {
-                            classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
-                                           phi7 + (size_t)type2phi(it, jt) * P.nr * 7, want_phi, g, f);
-                        }
-                        gr[it][jt][0] = g[1];
-                        gr[it][jt][1] = g[2];
-                        if (want_phi) {
-                            fr[type2phi(it, jt)][0] = f[1];
-                            fr[type2phi(it, jt)][1] = f[2];
-                        }
-                    }
-                }
-                // src/ff_dmd_I.cpp:374-410 (cache slot type2rho[jt][it] pairs with dE_i)
-#pragma unroll
-                for (int it = 0; it < NT; ++it) {
-#pragma unroll
-                    for (int jt = 0; jt < NT; ++jt) {
-                        const int ph = type2phi(it, jt);
-                        const double cc = ci[it] * cj[jt];
-                        const double fpair =
-                            -(gr[jt][it][0] * dEi[it] + gr[it][jt][0] * dEj[jt] + fr[ph][0]) * cc * r_inv;
-                        const double s = ai[it] + aj[jt];
-                        const double apair =
-                            -(gr[jt][it][1] * dEi[it] + gr[it][jt][1] * dEj[jt] + fr[ph][1]) * cc / (s * s);
-                        if (apair != 0.0 || fpair != 0.0) {
-                            fi[0] += dx0 * fpair;
-                            fi[1] += dx1 * fpair;
-                            fi[2] += dx2 * fpair;
-                            fi[3 + it] += apair * aj[jt] * aj[jt];
-                            const double w = owned ? 1.0 : 0.5;
-                            if (owned) {
-                                pf[0] -= dx0 * fpair;
-                                pf[1] -= dx1 * fpair;
-                                pf[2] -= dx2 * fpair;
-                                pf[3 + jt] += apair * ai[it] * ai[it];
-                            }
-                            const double wf = w * fpair;
-                            vir[0] -= wf * dx0 * dx0;
-                            vir[1] -= wf * dx1 * dx1;
-                            vir[2] -= wf * dx2 * dx2;
-                            vir[3] -= wf * dx1 * dx2;
-                            vir[4] -= wf * dx2 * dx0;
-                            vir[5] -= wf * dx0 * dx1;
-                        }
-                    }
-                }
+            double S = 0.0, aJ = 0.0;
+            if (inr) {
+                const double4 rj = arec[(size_t)j * NT + jt];
+                const double aj = rj.x, iaj = rj.y, cj = rj.z, dEj = rj.w;
+                const double r_inv = rsqrt(rsq);
+                const Q* rec = deriv + (size_t)p * LPP * 4;
+                const double g_dr = (double)rec[q * 4 + 0], g_da = (double)rec[q * 4 + 1];
+                const double gT_dr = (double)rec[qT * 4 + 0], gT_da = (double)rec[qT * 4 + 1];
+                const int qf = want_phi ? q : qT;
+                const double fr_dr = (double)rec[qf * 4 + 2], fr_da = (double)rec[qf * 4 + 3];
+                const double cc = ci * cj;
+                // 1/(alpha_i + alpha_j) = (1/alpha_i)(1/alpha_j) / (1/alpha_i + 1/alpha_j)
+                const double ia = iai + iaj;
+                double ria = __builtin_amdgcn_rcp(ia);
+                ria = ria * (2.0 - ia * ria);
+                ria = ria * (2.0 - ia * ria);
+                const double is = iai * iaj * ria;
+                const double fpair = -(gT_dr * dEi + g_dr * dEj + fr_dr) * cc * r_inv;
+                const double apair = -(gT_da * dEi + g_da * dEj + fr_da) * cc * (is * is);
+                fa += apair * aj * aj;
+                aJ = apair * ai * ai;
+                S = fpair;
             }
-            if (owned) {
-#pragma unroll
-                for (int k = 0; k < XD; ++k) pairF[(size_t)p * XD + k] = pf[k];
+            S = sum_over_it<NT>(sum_over_jt<NT>(S));
+            aJ = sum_over_it<NT>(aJ);
+            if (inr && q == 0) {
+                fx += dx0 * S;
+                fy += dx1 * S;
+                fz += dx2 * S;
+                const double wf = owned ? S : 0.5 * S;
+                vir[0] -= wf * dx0 * dx0;
+                vir[1] -= wf * dx1 * dx1;
+                vir[2] -= wf * dx2 * dx2;
+                vir[3] -= wf * dx1 * dx2;
+                vir[4] -= wf * dx2 * dx0;
+                vir[5] -= wf * dx0 * dx1;
+            }
+            if (valid && owned) {
+                double* dst = pairF + (size_t)rpos[p] * XD;
+                if (q == 0) {
+                    dst[0] = -dx0 * S;
+                    dst[1] = -dx1 * S;
+                    dst[2] = -dx2 * S;
+                }
+                if (it == 0) dst[3 + jt] = aJ;
             }
         }
+        const double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
+        double sa[NT];
 #pragma unroll
-        for (int k = 0; k < XD; ++k) fown[(size_t)i * XD + k] = fi[k];
+        for (int t = 0; t < NT; ++t) sa[t] = wave_sum(it == t ? fa : 0.0);
+        if (lane == 0) {
+            fown[(size_t)i * XD + 0] = sx;
+            fown[(size_t)i * XD + 1] = sy;
+            fown[(size_t)i * XD + 2] = sz;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) fown[(size_t)i * XD + 3 + t] = sa[t];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
     }
-    block_sum_store<6>(vir, part);
+    block_store_waves<6>(vir, part);
+}
+
+// reverse-CSR position of each pair whose partner is owned
+__global__ void k_rpos(const int* __restrict__ roff, const int* __restrict__ rev, int natms, int* __restrict__ rpos) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= natms) return;
+    for (int q = roff[j]; q < roff[j + 1]; ++q) rpos[rev[q]] = q;
 }
 
 // ---------------------------------------------------------------- force gather
@@ -465,30 +730,37 @@ __global__ __launch_bounds__(kBlock) void k_fgather(int natms, PairBufs L, const
     for (int t = 0; t < NT; ++t) acc[3 + t] += fa[(size_t)i * NT + t];
     const int q1 = L.roff[i + 1];
     for (int q = L.roff[i]; q < q1; ++q) {
-        const size_t p = (size_t)L.rev[q];
 #pragma unroll
-        for (int k = 0; k < XD; ++k) acc[k] += pairF[p * XD + k];
+        for (int k = 0; k < XD; ++k) acc[k] += pairF[(size_t)q * XD + k];
     }
 #pragma unroll
     for (int k = 0; k < XD; ++k) f[(size_t)i * XD + k] = acc[k];
 }
 
-// sums block partials in block order; out[k] for k < nv.  One block.
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double* __restrict__ part, int nblocks, int nv,
-                                                            int stride, int offset, double* __restrict__ out) {
+// Two-stage deterministic sum of per-block partials: stage 1 block b sums the
+// contiguous range [b*chunk, (b+1)*chunk) of partials in a fixed tree; stage 2
+// (one block) sums the stage-1 results the same way.
+__global__ __launch_bounds__(kBlock) void k_reduce_stage(const double* __restrict__ part, long nparts, int nv,
+                                                         int stride, int offset, long chunk,
+                                                         double* __restrict__ out, int out_stride) {
     __shared__ double sh[kBlock];
+    const long lo = (long)blockIdx.x * chunk;
+    long hi = lo + chunk;
+    hi = hi < nparts ? hi : nparts;
     for (int k = 0; k < nv; ++k) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += kBlock) s += part[(size_t)b * stride + offset + k];
+        for (long b = lo + threadIdx.x; b < hi; b += kBlock) s += part[(size_t)b * stride + offset + k];
         sh[threadIdx.x] = s;
         __syncthreads();
         for (int w = kBlock / 2; w > 0; w >>= 1) {
             if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[k] = sh[0];
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * out_stride + k] = sh[0];
         __syncthreads();
     }
 }
 
 }  // namespace mappdmd
+
+#pragma clang fp contract(off)

@@ -16,6 +16,8 @@
 #pragma once
 #include "force.hpp"
 
+#pragma clang fp contract(fast)
+
 namespace mappdmd {
 
 struct RevBufs {
@@ -448,3 +450,5 @@ __global__ __launch_bounds__(kBlock) void k_calc_g_w(FFParams<double> P, double 
 }
 
 }  // namespace mappdmd
+
+#pragma clang fp contract(off)

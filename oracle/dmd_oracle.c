@@ -89,6 +89,7 @@ struct orc {
     long max_pairs;
     double *rho, *drho_dr, *drho_dalpha, *phi, *dphi_dr, *dphi_dalpha;
     int fresh;
+    int fix_dE_index; /* 1: dE[atom*nt+type] in the chemical-potential sweep */
     /* 2nd list */
     int* nl2_off;
     int* nl2;
@@ -134,6 +135,7 @@ void orc_destroy(orc_t* o) {
 }
 
 void orc_set_fresh_caches(orc_t* o, int fresh) { o->fresh = fresh; }
+void orc_set_fix_dE_index(orc_t* o, int fix) { o->fix_dE_index = fix; }
 
 /* ---------------------------------------------------------------- tables */
 
@@ -579,6 +581,13 @@ int orc_set_x_update(orc_t* o, const double* x_owned, int* rebuilt) {
     }
     if (rebuilt) *rebuilt = check;
     return 0;
+}
+
+/* new owned c, forwarded to the ghosts (Atoms::update of c) */
+void orc_set_c(orc_t* o, const double* c_owned) {
+    int cd = o->variant == ORC_EAM ? 1 : o->nt;
+    memcpy(o->c, c_owned, (size_t)o->natms * cd * sizeof(double));
+    update_vec(o, o->c, cd);
 }
 
 /* ------------------------------------------------------------------- DMD */
@@ -1034,9 +1043,11 @@ static void chem_pot_sweep(orc_t* o) {
             for (int it = 0; it < nt; it++)
                 for (int jt = 0; jt < nt; jt++) {
                     double ph = o->phi[phi_start + comp(it, jt)];
-                    n[icomp + it] += c[jcomp + jt] * (o->rho[rho_start + type2rho(nt, it, jt)] * dE[jatm + jt] + ph);
+                    long dj = o->fix_dE_index ? jcomp + jt : (long)jatm + jt;
+                    long di = o->fix_dE_index ? icomp + it : (long)iatm + it;
+                    n[icomp + it] += c[jcomp + jt] * (o->rho[rho_start + type2rho(nt, it, jt)] * dE[dj] + ph);
                     if (jatm < natms)
-                        n[jcomp + jt] += c[icomp + it] * (o->rho[rho_start + type2rho(nt, jt, it)] * dE[iatm + it] + ph);
+                        n[jcomp + jt] += c[icomp + it] * (o->rho[rho_start + type2rho(nt, jt, it)] * dE[di] + ph);
                 }
             rho_start += nt * nt;
             phi_start += nphi;

@@ -42,9 +42,14 @@ int orc_set_system(orc_t* o, const double* H, int natms, const double* x,
 int orc_init(orc_t* o, double skin);
 /* replace owned x (cartesian + alpha), then Atoms::update_0 (rebuild or ghost update) */
 int orc_set_x_update(orc_t* o, const double* x_owned, int* rebuilt);
+/* replace owned c and forward it to the ghosts */
+void orc_set_c(orc_t* o, const double* c_owned);
 /* 1: pair caches are zeroed before every force_calc ("fresh" semantics);
    0: reference semantics (zeroed only when the caches grow) */
 void orc_set_fresh_caches(orc_t* o, int fresh);
+/* 1: index dE by [atom*nt+type] in c_d_calc / calc_g instead of the reference's
+   dE[jatm+jtype] (src/ff_dmd_I.cpp:2336,2339) -- a switch for tests only */
+void orc_set_fix_dE_index(orc_t* o, int fix);
 
 void orc_force_calc(orc_t* o, int st_clc, double* en_st);
 double orc_energy_calc(orc_t* o);

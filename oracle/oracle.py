@@ -38,6 +38,8 @@ def lib():
         L.orc_init.argtypes = [C.c_void_p, D]
         L.orc_set_x_update.argtypes = [C.c_void_p, DP, IP]
         L.orc_set_fresh_caches.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_c.argtypes = [C.c_void_p, DP]
+        L.orc_set_fix_dE_index.argtypes = [C.c_void_p, C.c_int]
         L.orc_force_calc.argtypes = [C.c_void_p, C.c_int, DP]
         L.orc_energy_calc.argtypes = [C.c_void_p]
         L.orc_energy_calc.restype = D
@@ -125,6 +127,13 @@ class Oracle:
         flag = C.c_int(0)
         self.L.orc_set_x_update(self.o, _dp(x), C.byref(flag))
         return bool(flag.value)
+
+    def set_fix_dE_index(self, fix):
+        self.L.orc_set_fix_dE_index(self.o, 1 if fix else 0)
+
+    def set_c(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.L.orc_set_c(self.o, _dp(c))
 
     def force_calc(self, st_clc=1):
         en = np.zeros(7)
