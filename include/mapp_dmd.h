@@ -39,7 +39,11 @@ enum {
 enum {
     MAPP_DMD_VARIANT_DMD = 0, /* ForceField_DMD    (src/ff_dmd.cpp)    Mat = 5.2 beta, rate exp(n + Mat) */
     MAPP_DMD_VARIANT_I = 1,   /* ForceField_DMD_I  (src/ff_dmd_I.cpp)  Mat = 0.4 beta */
-    MAPP_DMD_VARIANT_II = 2   /* ForceField_DMD_II (src/ff_dmd_II.cpp) mat = max(f_i,f_j) + 0.5 */
+    MAPP_DMD_VARIANT_II = 2,  /* ForceField_DMD_II (src/ff_dmd_II.cpp) mat = max(f_i,f_j) + 0.5 */
+    /* plain EAM, ForceField_eam (src/ff_eam.cpp): x rows are [3] (positions),
+       c holds the type of each site, coef ignores its DMD arguments, and
+       force_calc / energy_calc follow src/ff_eam.cpp:44-356 */
+    MAPP_DMD_VARIANT_EAM = 3
 };
 
 /* arithmetic of the pair quadrature (per-site sums, embedding and the
@@ -129,6 +133,30 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx);
    DEVICE pointers of natms*nt doubles; g is written only when chk == 0. */
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
                     double* g_dev, double* cost);
+
+/* Learned surrogate (no counterpart in the reference; BASELINE cfg3 workload):
+   one MLP 3->64->64->64->6 (SiLU) per ordered type pair maps (r, alpha_i,
+   alpha_j) to {rho_bar, drho_bar/dr, drho_bar/dalpha, phi_bar, dphi_bar/dr,
+   dphi_bar/dalpha}.  Trains on the GPU on n_per_tp samples per type pair
+   labelled by the FP32 quadrature (current rule and tables), Adam with a cosine
+   schedule; then marks cells of (r, alpha_i, alpha_j) whose validation error
+   (max over outputs of |y - t| / std) exceeds tol for quadrature fallback.
+   box = {r_lo, r_hi, alpha_lo, alpha_hi} or NULL for {1.8, quadrature cut, 60, 120}.
+   report (8 doubles): final train loss, validation mean and max error, masked
+   cell fraction, steps, seconds, training samples, validation samples.
+   Afterwards MAPP_DMD_PREC_SURROGATE runs pass 1 of force_calc on BF16 MFMA. */
+int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int batch_per_tp, double lr,
+                             unsigned long long seed, double tol, const double* box, double* report);
+/* gradient of each network's mean squared (standardised) error over n_per_tp
+   samples (a multiple of 64) by the training kernel: params [4][8966], norm
+   [56] (box, input scaling, output means, output stds), X [4*n][3], T [4*n][6]
+   -> grad [4][8966], loss [4] */
+int mapp_dmd_surrogate_gradient(mapp_dmd_t* ctx, const float* params, const float* norm, long n_per_tp,
+                                const float* X, const float* T, float* grad, float* loss);
+/* trained parameters [4][8966] and normalisation [56] */
+int mapp_dmd_surrogate_params(mapp_dmd_t* ctx, float* params, float* norm);
+/* number of (pair, type pair) samples of the last surrogate force_calc that fell back to the quadrature */
+int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count);
 
 /* host <-> device copies of per-atom vectors (owned rows only) */
 int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count);

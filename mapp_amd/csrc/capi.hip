@@ -11,6 +11,11 @@
 #include "geometry.hpp"
 #include "force.hpp"
 #include "mobility.hpp"
+#include "eam.hpp"
+#include "surrogate.hpp"
+#include <chrono>
+#include <numeric>
+#include <random>
 #include "gh_table.h"
 #include "tables.hpp"
 
@@ -259,8 +264,11 @@ void rebuild(mapp_dmd* ctx) {
     ctx->cache_valid = false;
 }
 
+// columns of the per-site alpha / c arrays: nt for DMD, one (the type) for EAM
+inline int acols(const mapp_dmd* ctx) { return ctx->variant == MAPP_DMD_VARIANT_EAM ? 1 : ctx->nt; }
+
 void rebuild_any(mapp_dmd* ctx) {
-    if (ctx->nt == 1) rebuild<1>(ctx);
+    if (acols(ctx) == 1) rebuild<1>(ctx);
     else rebuild<2>(ctx);
 }
 
@@ -285,35 +293,104 @@ void ghost_copy(mapp_dmd* ctx, double* v, int d) {
 }
 
 // AoS (3+nt) owned rows <-> positions / alpha
-template <int NT>
+// NA = alpha columns carried in x (nt for DMD, 0 for EAM); NT = columns of the alpha array
+template <int NT, int NA = NT>
 __global__ void k_scatter_x(const double* __restrict__ x, int natms, double4* __restrict__ pos,
                             double* __restrict__ alpha) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natms) return;
-    const double* r = x + (size_t)i * (3 + NT);
+    const double* r = x + (size_t)i * (3 + NA);
     pos[i] = make_double4(r[0], r[1], r[2], 0.0);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) alpha[(size_t)i * NT + t] = r[3 + t];
+    for (int t = 0; t < NA; ++t) alpha[(size_t)i * NT + t] = r[3 + t];
 }
-template <int NT>
+template <int NT, int NA = NT>
 __global__ void k_gather_x(double* __restrict__ x, int n, const double4* __restrict__ pos,
                            const double* __restrict__ alpha) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double* r = x + (size_t)i * (3 + NT);
+    double* r = x + (size_t)i * (3 + NA);
     double4 p = pos[i];
     r[0] = p.x;
     r[1] = p.y;
     r[2] = p.z;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) r[3 + t] = alpha[(size_t)i * NT + t];
+    for (int t = 0; t < NA; ++t) r[3 + t] = alpha[(size_t)i * NT + t];
 }
 
-template <int NT>
+template <int NT, int NA = NT>
 void scatter_x(mapp_dmd* ctx, const double* dev_aos) {
-    hipLaunchKernelGGL((k_scatter_x<NT>), dim3(nblk(ctx->natms)), dim3(kBlock), 0, ctx->stream, dev_aos,
+    hipLaunchKernelGGL((k_scatter_x<NT, NA>), dim3(nblk(ctx->natms)), dim3(kBlock), 0, ctx->stream, dev_aos,
                        ctx->natms, ctx->d_pos.p, ctx->d_alpha.p);
     launch_check("k_scatter_x");
+}
+
+// x columns of the public layout: 3 + nt (DMD) or 3 (EAM)
+inline int xcols(const mapp_dmd* ctx) { return ctx->variant == MAPP_DMD_VARIANT_EAM ? 3 : 3 + ctx->nt; }
+
+void gather_x_any(mapp_dmd* ctx, double* dst, int n, const double4* pos, const double* alpha) {
+    if (ctx->variant == MAPP_DMD_VARIANT_EAM)
+        hipLaunchKernelGGL((k_gather_x<1, 0>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
+    else if (ctx->nt == 1)
+        hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
+    else
+        hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
+    launch_check("k_gather_x");
+}
+
+// ------------------------------------------------------------------ plain EAM
+double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
+    hipStream_t st = ctx->stream;
+    const int natms = ctx->natms, nb = nblk(natms);
+    PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
+    EamBufs T{ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_F7.p, ctx->nt};
+    ctx->d_part.reserve((size_t)nb * 14 + 16);
+    ctx->d_out.reserve(16);
+    ctx->d_drho.reserve(ctx->npairs + 1);
+    ctx->d_pairF2.reserve((size_t)(ctx->nrev + 1) * 3);
+    double* part1 = ctx->d_part.p;
+    double* part2 = part1 + (size_t)nb * 7;
+    double* part3 = part2 + (size_t)nb;
+    double2* drho = reinterpret_cast<double2*>(ctx->d_drho.p);
+    if (energy_only)
+        hipLaunchKernelGGL((k_eam_pass1<true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_pos.p, ctx->d_conc.p,
+                           L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p, ctx->d_pairE.p, ctx->d_pairF.p, drho,
+                           part1);
+    else
+        hipLaunchKernelGGL((k_eam_pass1<false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_pos.p,
+                           ctx->d_conc.p, L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p, ctx->d_pairE.p,
+                           ctx->d_pairF.p, drho, part1);
+    launch_check("k_eam_pass1");
+    if (energy_only)
+        hipLaunchKernelGGL((k_eam_embed<true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_conc.p, L,
+                           ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
+    else
+        hipLaunchKernelGGL((k_eam_embed<false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_conc.p, L,
+                           ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
+    launch_check("k_eam_embed");
+    double* out = ctx->d_out.p;
+    reduce_partials(ctx, part1, nb, 7, 7, 0, out);
+    reduce_partials(ctx, part2, nb, 1, 1, 0, out + 7);
+    if (!energy_only) {
+        ghost_copy(ctx, ctx->d_dE.p, 1);
+        hipLaunchKernelGGL(k_eam_pass2, dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_pos.p, L, ctx->d_rpos.p, drho,
+                           ctx->d_dE.p, ctx->d_fown.p, ctx->d_pairF2.p, part3);
+        launch_check("k_eam_pass2");
+        hipLaunchKernelGGL(k_eam_gather, dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
+                           ctx->d_pairF2.p, ctx->d_f.p);
+        launch_check("k_eam_gather");
+        reduce_partials(ctx, part3, nb, 6, 6, 0, out + 8);
+    }
+    double h[14];
+    HIP_OK(hipMemcpyAsync(h, out, sizeof(double) * (energy_only ? 8 : 14), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const double energy = h[0] + h[7];
+    if (en_st) {
+        en_st[0] = energy;
+        if (st_clc)
+            for (int k = 0; k < 6; ++k) en_st[1 + k] = h[1 + k] + h[8 + k];
+    }
+    return energy;
 }
 
 // ------------------------------------------------------------ force kernels
@@ -327,24 +404,51 @@ void pack_atoms(mapp_dmd* ctx) {
 }
 
 template <int NT, typename Q, int NQ>
-void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st) {
+void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st, bool sur = false) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
     const int nb = nblk(natms);
     const int nbw = nblk((long)natms * 64);  // one wavefront per site
+    const int nb1 = sur ? natms : nbw;         // the surrogate's pass 1 runs one wavefront per block
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
-    ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);
+    ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
-    double* part2 = part1 + (size_t)nbw * 2;
+    double* part2 = part1 + (size_t)nb1 * 2;
     double* part3 = part2 + (size_t)nb;
     Q* deriv = reinterpret_cast<Q*>(ctx->d_deriv.p);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
     pack_atoms<NT>(ctx);
-    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                       ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                       ctx->d_pairE.p, deriv, part1);
-    launch_check("k_pass1");
+    if constexpr (NT == 2 && std::is_same<Q, double>::value) {
+        if (sur) {
+            SurDev S{ctx->d_sp.p, reinterpret_cast<const __bf16*>(ctx->d_sfrag.p), ctx->d_smask.p, ctx->snorm};
+            ctx->d_nfall.reserve(natms);
+            HIP_OK(hipMemsetAsync(ctx->d_nfall.p, 0, sizeof(int) * natms, st));
+            const int q = ctx->Pf.no_i;
+            if (q == 5)
+                hipLaunchKernelGGL((k_pass1_sur<5>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
+                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
+                                   ctx->d_nfall.p);
+            else if (q == 3)
+                hipLaunchKernelGGL((k_pass1_sur<3>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
+                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
+                                   ctx->d_nfall.p);
+            else
+                hipLaunchKernelGGL((k_pass1_sur<0>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
+                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
+                                   ctx->d_nfall.p);
+            launch_check("k_pass1_sur");
+        }
+    }
+    if (!sur) {
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, deriv, part1);
+        launch_check("k_pass1");
+    }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
     hipLaunchKernelGGL((k_embed<NT, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p,
                        ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p,
@@ -365,7 +469,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
                        ctx->d_fa.p, ctx->d_f.p);
     launch_check("k_fgather");
     double* out = ctx->d_out.p;
-    reduce_partials(ctx, part1, nbw, 2, 2, 0, out);
+    reduce_partials(ctx, part1, nb1, 2, 2, 0, out);
     reduce_partials(ctx, part2, nb, 1, 1, 0, out + 2);
     reduce_partials(ctx, part3, nbw, 6, 6, 0, out + 3);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[4], st));
@@ -418,6 +522,10 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
 template <int NT>
 void dispatch_force(mapp_dmd* ctx, int st_clc, double* en_st) {
     const int q = ctx->P.no_i;
+    if (ctx->prec == MAPP_DMD_PREC_SURROGATE) {
+        run_force<NT, double, 0>(ctx, ctx->d_quad64.p, ctx->P, st_clc, en_st, true);
+        return;
+    }
     if (ctx->prec == MAPP_DMD_PREC_MIXED) {
         if (q == 3) run_force<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
         else if (q == 5) run_force<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
@@ -468,6 +576,8 @@ void upload_tables(mapp_dmd* ctx) {
     HIP_OK(hipMemcpy(ctx->d_rho7.p, ctx->rho7.data(), ctx->rho7.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_phi7.p, ctx->phi7.data(), ctx->phi7.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(ctx->d_F9.p, ctx->F9.data(), ctx->F9.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->d_F7.reserve(ctx->F7.size());
+    HIP_OK(hipMemcpy(ctx->d_F7.p, ctx->F7.data(), ctx->F7.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void fill_float_params(mapp_dmd* ctx) {
@@ -664,6 +774,52 @@ double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, d
     return run_calc_g<NT, 2>(ctx, chk, alpha_t, a, g);
 }
 
+
+// ------------------------------------------------------------------ surrogate
+constexpr int kNormFloats = 8 + 2 * 4 * kOut;
+
+void norm_to_flat(const SurNorm& N, float* f) {
+    const float hdr[8] = {N.r_lo, N.r_hi, N.a_lo, N.a_hi, N.r_c, N.r_s, N.a_c, N.a_s};
+    std::memcpy(f, hdr, sizeof(hdr));
+    std::memcpy(f + 8, N.y_mean, sizeof(N.y_mean));
+    std::memcpy(f + 8 + 4 * kOut, N.y_std, sizeof(N.y_std));
+}
+void norm_from_flat(SurNorm& N, const float* f) {
+    N.r_lo = f[0]; N.r_hi = f[1]; N.a_lo = f[2]; N.a_hi = f[3];
+    N.r_c = f[4]; N.r_s = f[5]; N.a_c = f[6]; N.a_s = f[7];
+    std::memcpy(N.y_mean, f + 8, sizeof(N.y_mean));
+    std::memcpy(N.y_std, f + 8 + 4 * kOut, sizeof(N.y_std));
+}
+
+void sur_samples(mapp_dmd* ctx, const SurNorm& N, long n_per_tp, unsigned long long seed, DevBuf<float>& X,
+                 DevBuf<float>& T) {
+    const long n = n_per_tp * 4;
+    X.reserve(n * 3);
+    T.reserve(n * kOut);
+    const int q = ctx->Pf.no_i;
+    const dim3 g(nblk(n)), b(kBlock);
+    if (q == 3)
+        hipLaunchKernelGGL((k_sur_samples<2, 3>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
+                           X.p, T.p);
+    else if (q == 5)
+        hipLaunchKernelGGL((k_sur_samples<2, 5>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
+                           X.p, T.p);
+    else
+        hipLaunchKernelGGL((k_sur_samples<2, 0>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
+                           X.p, T.p);
+    launch_check("k_sur_samples");
+}
+
+// one gradient evaluation: grad of each network's mean squared error over its n_per_tp samples
+void sur_gradient(mapp_dmd* ctx, const SurNorm& N, const float* params, const float* X, const float* T,
+                  const int* idx, long n_per_tp, float inv_count, float* gpart, float* lpart) {
+    const int bpt = (int)(n_per_tp / kMB);
+    HIP_OK(hipFuncSetAttribute((const void*)k_sur_train, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kSurTrainLds));
+    hipLaunchKernelGGL(k_sur_train, dim3(4 * bpt), dim3(256), kSurTrainLds, ctx->stream, params, N, X, T, idx, bpt,
+                       inv_count, gpart, lpart);
+    launch_check("k_sur_train");
+}
 }  // namespace
 
 // =================================================================== C ABI
@@ -673,7 +829,7 @@ int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
     if (!out) return MAPP_DMD_E_ARG;
     *out = nullptr;
     if (no_types < 1 || no_types > 2) return MAPP_DMD_E_RANGE;
-    if (variant < 0 || variant > 2) return MAPP_DMD_E_ARG;
+    if (variant < 0 || variant > 3) return MAPP_DMD_E_ARG;
     std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
     ctx->nt = no_types;
     ctx->variant = variant;
@@ -733,13 +889,19 @@ int mapp_dmd_set_fs(mapp_dmd_t* ctx, int nrho, double drho, int nr, double dr, c
                 for (int r = 0; r < nr; ++r) ctx->phi7[((size_t)slot * nr + r) * 7] = rphi[k * nr + r];
             }
         for (int a = 0; a < nt; ++a) spline_rows(nrho, drho, ctx->F9.data() + (size_t)a * (nrho + 1) * 9, 9, true);
+        // ForceField_eam::set_arrays uses the cubic rows for F (src/ff_eam.cpp:1297-1321)
+        ctx->F7.assign((size_t)nt * (nrho + 1) * 7, 0.0);
+        for (int a = 0; a < nt; ++a) {
+            for (int i = 0; i < nrho; ++i) ctx->F7[((size_t)a * (nrho + 1) + i) * 7] = F[(size_t)a * nrho + i];
+            spline_rows(nrho, drho, ctx->F7.data() + (size_t)a * (nrho + 1) * 7, 7, false);
+        }
         for (int s = 0; s < nt * nt; ++s) spline_rows(nr, dr, ctx->rho7.data() + (size_t)s * nr * 7, 7, false);
         for (int s = 0; s < nphi; ++s) spline_rows(nr, dr, ctx->phi7.data() + (size_t)s * nr * 7, 7, false);
         ctx->P.nr = nr;
         ctx->P.nrho = nrho;
         ctx->P.dr_inv = 1.0 / dr;
         ctx->P.drho_inv = 1.0 / drho;
-        upload_tables(ctx);
+        upload_tables(ctx);  // after F7 is built
         ctx->have_tables = true;
         ctx->have_params = false;
         return 0;
@@ -765,6 +927,13 @@ int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, dou
                   const double* mass) {
     return guarded(ctx, [&]() {
         require(ctx->have_tables, MAPP_DMD_E_STATE, "tables must be set before coef");
+        if (ctx->variant == MAPP_DMD_VARIANT_EAM) {
+            // ForceField_eam::coef, src/ff_eam.cpp:424-425
+            ctx->P.cut_sq = (ctx->nr - 1.0) * (ctx->nr - 1.0) * ctx->dr * ctx->dr;
+            ctx->P.rho_max = (ctx->nrho - 1.0) * ctx->drho;
+            ctx->have_params = true;
+            return 0;
+        }
         require(no_i >= 1 && no_i <= 14, MAPP_DMD_E_RANGE, "n should be between 1 and 14");
         require(alpha_min != 0.0, MAPP_DMD_E_ARG, "minimum alpha cannot be zero");
         require(mass != nullptr, MAPP_DMD_E_ARG, "null masses");
@@ -811,8 +980,8 @@ int mapp_dmd_set_precision(mapp_dmd_t* ctx, int mode) {
     return guarded(ctx, [&]() {
         require(mode == MAPP_DMD_PREC_FP64 || mode == MAPP_DMD_PREC_MIXED || mode == MAPP_DMD_PREC_SURROGATE,
                 MAPP_DMD_E_ARG, "unknown precision mode");
-        require(mode != MAPP_DMD_PREC_SURROGATE, MAPP_DMD_E_STATE,
-                "surrogate mode needs a trained network (mapp_dmd_surrogate_*)");
+        require(mode != MAPP_DMD_PREC_SURROGATE || (ctx->sur_trained && ctx->nt == 2), MAPP_DMD_E_STATE,
+                "surrogate mode needs a trained network (mapp_dmd_surrogate_train) and two types");
         ctx->prec = mode;
         ctx->cache_valid = false;
         return 0;
@@ -825,7 +994,7 @@ int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms, const doubl
         for (int i = 0; i < 3; ++i)
             for (int j = i + 1; j < 3; ++j)
                 require(H[i * 3 + j] == 0.0, MAPP_DMD_E_ARG, "H must be lower triangular");
-        const int nt = ctx->nt;
+        const int nt = acols(ctx), xc = xcols(ctx);
         BoxParams& bp = ctx->bp;
         std::memcpy(bp.H, H, sizeof(bp.H));
         // M3INV, src/init.h:72-85
@@ -849,14 +1018,18 @@ int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms, const doubl
         ctx->h_alpha.resize((size_t)natms * nt);
         ctx->h_c.assign(c, c + (size_t)natms * nt);
         for (int i = 0; i < natms; ++i) {
-            const double* r = x + (size_t)i * (3 + nt);
+            const double* r = x + (size_t)i * xc;
             for (int d = 0; d < 3; ++d) {
                 require(r[d] >= 0.0 && r[d] < 1.0, MAPP_DMD_E_RANGE, "reduced coordinates must lie in [0,1)");
                 ctx->h_s[(size_t)i * 4 + d] = r[d];
             }
             ctx->h_s[(size_t)i * 4 + 3] = 0.0;
-            for (int t = 0; t < nt; ++t) ctx->h_alpha[(size_t)i * nt + t] = r[3 + t];
+            for (int t = 0; t < nt; ++t) ctx->h_alpha[(size_t)i * nt + t] = xc > 3 ? r[3 + t] : 1.0;
         }
+        if (ctx->variant == MAPP_DMD_VARIANT_EAM)
+            for (int i = 0; i < natms; ++i)
+                require(c[i] >= 0.0 && c[i] < ctx->nt && c[i] == (double)(int)c[i], MAPP_DMD_E_ARG,
+                        "EAM types must be integers in [0, nt)");
         ctx->have_system = true;
         ctx->inited = false;
         return 0;
@@ -868,12 +1041,13 @@ int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
         require(ctx->have_params && ctx->have_system, MAPP_DMD_E_STATE, "coef and set_system must precede init");
         require(skin >= 0.0, MAPP_DMD_E_ARG, "negative skin");
         HIP_OK(hipSetDevice(ctx->device));
-        const int nt = ctx->nt, natms = ctx->natms;
+        const int nt = acols(ctx), natms = ctx->natms;
         ctx->skin = skin;
         const FFParams<double>& P = ctx->P;
-        // ForceField_DMD_I::init, src/ff_dmd_I.cpp:675-685
-        ctx->bp.cut_sk_sq = P.cut_sq_mod + skin * skin + 2.0 * std::sqrt(P.cut_sq_mod) * skin;
-        set_ph(ctx, std::sqrt(P.cut_sq_mod));
+        // ForceField_DMD_I::init, src/ff_dmd_I.cpp:675-685; ForceField_eam::init, src/ff_eam.cpp:362-370
+        const double base = ctx->variant == MAPP_DMD_VARIANT_EAM ? P.cut_sq : P.cut_sq_mod;
+        ctx->bp.cut_sk_sq = base + skin * skin + 2.0 * std::sqrt(base) * skin;
+        set_ph(ctx, std::sqrt(base));
         for (int d = 0; d < 3; ++d)
             require(ctx->bp.cut_ph_s[d] < 1.0, MAPP_DMD_E_RANGE,
                     "box thinner than cutoff+skin: needs more than one ghost swap per face");
@@ -903,12 +1077,13 @@ int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_x");
         HIP_OK(hipSetDevice(ctx->device));
-        const int nt = ctx->nt, natms = ctx->natms;
-        const size_t n = (size_t)natms * (3 + nt);
+        const int nt = acols(ctx), natms = ctx->natms;
+        const size_t n = (size_t)natms * xcols(ctx);
         ctx->d_xaos.reserve(n);
         if (x_host)
             HIP_OK(hipMemcpyAsync(ctx->d_xaos.p, x_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
+        if (ctx->variant == MAPP_DMD_VARIANT_EAM) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
+        else if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
         else scatter_x<2>(ctx, ctx->d_xaos.p);
         HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
         hipLaunchKernelGGL(k_skin_check, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p,
@@ -924,7 +1099,7 @@ int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt) {
             sync_sizes(ctx);
         } else {
             if (nt == 1) ghost_update_x<1>(ctx);
-            else ghost_update_x<2>(ctx);
+            else ghost_update_x<2>(ctx);  // EAM: alpha column unused
             HIP_OK(hipStreamSynchronize(ctx->stream));
         }
         ctx->cache_valid = false;
@@ -937,10 +1112,11 @@ int mapp_dmd_set_c(mapp_dmd_t* ctx, const double* c_host) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_c");
         HIP_OK(hipSetDevice(ctx->device));
-        const size_t n = (size_t)ctx->natms * ctx->nt;
+        const size_t n = (size_t)ctx->natms * acols(ctx);
         if (c_host)
             HIP_OK(hipMemcpyAsync(ctx->d_conc.p, c_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        ghost_copy(ctx, ctx->d_conc.p, ctx->nt);
+        ghost_copy(ctx, ctx->d_conc.p, acols(ctx));
+        ctx->cache_valid = false;
         HIP_OK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
@@ -951,7 +1127,8 @@ int mapp_dmd_force_calc(mapp_dmd_t* ctx, int st_clc, double* en_st) {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede force_calc");
         require(en_st != nullptr, MAPP_DMD_E_ARG, "null en_st");
         HIP_OK(hipSetDevice(ctx->device));
-        if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en_st);
+        if (ctx->variant == MAPP_DMD_VARIANT_EAM) run_eam(ctx, false, st_clc, en_st);
+        else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en_st);
         else dispatch_force<2>(ctx, st_clc, en_st);
         return 0;
     });
@@ -962,7 +1139,8 @@ int mapp_dmd_energy_calc(mapp_dmd_t* ctx, double* energy) {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede energy_calc");
         require(energy != nullptr, MAPP_DMD_E_ARG, "null energy");
         HIP_OK(hipSetDevice(ctx->device));
-        *energy = ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
+        if (ctx->variant == MAPP_DMD_VARIANT_EAM) *energy = run_eam(ctx, true, 0, nullptr);
+        else *energy = ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
         return 0;
     });
 }
@@ -980,6 +1158,7 @@ int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx) {
 
 int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
     return guarded(ctx, [&]() {
+        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE, "c_d_calc is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede c_d_calc");
         HIP_OK(hipSetDevice(ctx->device));
         if (ctx->nt == 1) c_d_variant<1>(ctx);
@@ -990,6 +1169,7 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
 
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev, double* g_dev, double* cost) {
     return guarded(ctx, [&]() {
+        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE, "calc_g is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede calc_g");
         require(a_dev && g_dev && cost, MAPP_DMD_E_ARG, "null a, g or cost");
         HIP_OK(hipSetDevice(ctx->device));
@@ -999,30 +1179,249 @@ int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
     });
 }
 
+
+int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int batch_per_tp, double lr,
+                             unsigned long long seed, double tol, const double* box, double* report) {
+    return guarded(ctx, [&]() {
+        require(ctx->have_params && ctx->nt == 2 && ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE,
+                "the surrogate covers the two-type DMD force field after coef");
+        require(n_per_tp >= kMB && batch_per_tp >= kMB && batch_per_tp % kMB == 0 && n_per_tp >= batch_per_tp &&
+                    epochs >= 1 && lr > 0,
+                MAPP_DMD_E_ARG, "bad training sizes (batch a multiple of 64, samples >= batch)");
+        HIP_OK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        SurNorm N{};
+        N.r_lo = box ? (float)box[0] : 1.8f;
+        N.r_hi = box ? (float)box[1] : (float)ctx->mod_rc;
+        N.a_lo = box ? (float)box[2] : 60.0f;
+        N.a_hi = box ? (float)box[3] : 120.0f;
+        N.r_c = 0.5f * (N.r_lo + N.r_hi);
+        N.r_s = 0.5f * (N.r_hi - N.r_lo);
+        N.a_c = 0.5f * (N.a_lo + N.a_hi);
+        N.a_s = 0.5f * (N.a_hi - N.a_lo);
+        const long n_val = std::max<long>(n_per_tp / 8, 4096);
+        sur_samples(ctx, N, n_per_tp, seed, ctx->d_sX, ctx->d_sT);
+        sur_samples(ctx, N, n_val, seed ^ 0x5bd1e995ull, ctx->d_sXv, ctx->d_sTv);
+        // output standardisation per type pair
+        std::vector<float> hT((size_t)n_per_tp * 4 * kOut);
+        HIP_OK(hipMemcpyAsync(hT.data(), ctx->d_sT.p, hT.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (int tp = 0; tp < 4; ++tp)
+            for (int o = 0; o < kOut; ++o) {
+                double m = 0, m2 = 0;
+                for (long s2 = 0; s2 < n_per_tp; ++s2) {
+                    const double v = hT[((size_t)tp * n_per_tp + s2) * kOut + o];
+                    m += v;
+                    m2 += v * v;
+                }
+                m /= n_per_tp;
+                const double var = m2 / n_per_tp - m * m;
+                N.y_mean[tp][o] = (float)m;
+                N.y_std[tp][o] = var > 1e-30 ? (float)std::sqrt(var) : 1.0f;
+            }
+        // Glorot-uniform weights, zero biases
+        std::mt19937_64 rng(seed);
+        std::vector<float> hp((size_t)4 * kNParam, 0.0f);
+        auto fill = [&](float* w, int fin, int fout) {
+            std::uniform_real_distribution<float> U(-std::sqrt(6.0f / (fin + fout)), std::sqrt(6.0f / (fin + fout)));
+            for (int e = 0; e < fin * fout; ++e) w[e] = U(rng);
+        };
+        for (int tp = 0; tp < 4; ++tp) {
+            float* P = hp.data() + (size_t)tp * kNParam;
+            fill(P + kW1, kIn, kHid);
+            fill(P + kW2, kHid, kHid);
+            fill(P + kW3, kHid, kHid);
+            fill(P + kW4, kHid, kOut);
+        }
+        ctx->d_sp.reserve(hp.size());
+        ctx->d_sm.reserve(hp.size());
+        ctx->d_sv.reserve(hp.size());
+        HIP_OK(hipMemcpy(ctx->d_sp.p, hp.data(), hp.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(ctx->d_sm.p, 0, hp.size() * sizeof(float)));
+        HIP_OK(hipMemset(ctx->d_sv.p, 0, hp.size() * sizeof(float)));
+        const int bpt = batch_per_tp / kMB;
+        const long steps_per_epoch = n_per_tp / batch_per_tp;
+        ctx->d_sgpart.reserve((size_t)4 * bpt * kNParam);
+        ctx->d_slpart.reserve((size_t)4 * bpt);
+        ctx->d_sidx.reserve((size_t)n_per_tp * 4);
+        std::vector<int> perm(n_per_tp), idx((size_t)n_per_tp * 4);
+        const long total = steps_per_epoch * epochs;
+        long step = 0;
+        double last_loss = 0.0;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int ep = 0; ep < epochs; ++ep) {
+            // per type pair shuffle, laid out step-major: [step][tp][batch]
+            for (int tp = 0; tp < 4; ++tp) {
+                std::iota(perm.begin(), perm.end(), 0);
+                std::shuffle(perm.begin(), perm.end(), rng);
+                for (long s2 = 0; s2 < steps_per_epoch; ++s2)
+                    for (int b = 0; b < batch_per_tp; ++b)
+                        idx[((size_t)s2 * 4 + tp) * batch_per_tp + b] =
+                            (int)((long)tp * n_per_tp + perm[(size_t)s2 * batch_per_tp + b]);
+            }
+            HIP_OK(hipMemcpyAsync(ctx->d_sidx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            for (long s2 = 0; s2 < steps_per_epoch; ++s2, ++step) {
+                sur_gradient(ctx, N, ctx->d_sp.p, ctx->d_sX.p, ctx->d_sT.p,
+                             ctx->d_sidx.p + (size_t)s2 * 4 * batch_per_tp, batch_per_tp,
+                             1.0f / (float)(batch_per_tp * kOut), ctx->d_sgpart.p, ctx->d_slpart.p);
+                const double frac = (double)step / (double)std::max<long>(1, total - 1);
+                const float lr_t = (float)(lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.14159265358979 * frac))));
+                const float bc1 = 1.0f - std::pow(0.9f, (float)(step + 1));
+                const float bc2 = 1.0f - std::pow(0.999f, (float)(step + 1));
+                hipLaunchKernelGGL(k_sur_adam, dim3(nblk(4L * kNParam)), dim3(kBlock), 0, st, ctx->d_sp.p,
+                                   ctx->d_sm.p, ctx->d_sv.p, ctx->d_sgpart.p, 4, bpt, lr_t, 0.9f, 0.999f, 1e-8f, bc1,
+                                   bc2);
+                launch_check("k_sur_adam");
+            }
+            std::vector<float> lp((size_t)4 * bpt);
+            HIP_OK(hipMemcpyAsync(lp.data(), ctx->d_slpart.p, lp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            last_loss = 0.0;
+            for (float v : lp) last_loss += v;
+            last_loss /= (double)(4 * batch_per_tp * kOut);
+        }
+        HIP_OK(hipStreamSynchronize(st));
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // validation error per cell -> fallback mask
+        ctx->d_serr.reserve((size_t)n_val * 4);
+        hipLaunchKernelGGL(k_sur_eval, dim3(nblk(n_val * 4)), dim3(kBlock), 0, st, ctx->d_sp.p, N, ctx->d_sXv.p,
+                           ctx->d_sTv.p, n_val, 4, ctx->d_serr.p);
+        launch_check("k_sur_eval");
+        std::vector<float> err((size_t)n_val * 4), hX((size_t)n_val * 4 * 3);
+        HIP_OK(hipMemcpyAsync(err.data(), ctx->d_serr.p, err.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hX.data(), ctx->d_sXv.p, hX.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        const int cells = kCellR * kCellA * kCellA;
+        std::vector<float> cmax((size_t)4 * cells, 0.0f);
+        double emean = 0.0, emax = 0.0;
+        for (long s2 = 0; s2 < n_val * 4; ++s2) {
+            const int tp = (int)(s2 / n_val);
+            const float r = hX[s2 * 3], ai = hX[s2 * 3 + 1], aj = hX[s2 * 3 + 2];
+            int cr = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kCellR);
+            int ci = (int)((ai - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
+            int cj = (int)((aj - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
+            cr = std::min(std::max(cr, 0), kCellR - 1);
+            ci = std::min(std::max(ci, 0), kCellA - 1);
+            cj = std::min(std::max(cj, 0), kCellA - 1);
+            const int cell = (cr * kCellA + ci) * kCellA + cj;
+            cmax[(size_t)tp * cells + cell] = std::max(cmax[(size_t)tp * cells + cell], err[s2]);
+            emean += err[s2];
+            emax = std::max(emax, (double)err[s2]);
+        }
+        emean /= (double)(n_val * 4);
+        std::vector<unsigned> mask((size_t)4 * cells / 32, 0u);
+        long masked = 0;
+        for (int e = 0; e < 4 * cells; ++e)
+            if (cmax[e] > tol) {
+                mask[e / 32] |= 1u << (e % 32);
+                ++masked;
+            }
+        ctx->d_smask.reserve(mask.size());
+        HIP_OK(hipMemcpy(ctx->d_smask.p, mask.data(), mask.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+        ctx->d_sfrag.reserve((size_t)4 * kFragPerTp * 64 * 8);
+        hipLaunchKernelGGL(k_sur_pack_frags, dim3(nblk(4L * kFragPerTp * 64)), dim3(kBlock), 0, st, ctx->d_sp.p, 4,
+                           reinterpret_cast<__bf16*>(ctx->d_sfrag.p));
+        launch_check("k_sur_pack_frags");
+        HIP_OK(hipStreamSynchronize(st));
+        ctx->snorm = N;
+        ctx->sur_trained = true;
+        if (report) {
+            report[0] = last_loss;
+            report[1] = emean;
+            report[2] = emax;
+            report[3] = (double)masked / (4.0 * cells);
+            report[4] = (double)total;
+            report[5] = secs;
+            report[6] = (double)(n_per_tp * 4);
+            report[7] = (double)(n_val * 4);
+        }
+        return 0;
+    });
+}
+
+int mapp_dmd_surrogate_gradient(mapp_dmd_t* ctx, const float* params, const float* norm, long n_per_tp,
+                                const float* X, const float* T, float* grad, float* loss) {
+    return guarded(ctx, [&]() {
+        require(params && norm && X && T && grad && n_per_tp >= kMB && n_per_tp % kMB == 0, MAPP_DMD_E_ARG,
+                "bad gradient request (samples per type pair a multiple of 64)");
+        HIP_OK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        SurNorm N{};
+        norm_from_flat(N, norm);
+        DevBuf<float> p, x, t, gp, lp, g;
+        DevBuf<int> idx;
+        const long n = n_per_tp * 4;
+        const int bpt = (int)(n_per_tp / kMB);
+        p.reserve((size_t)4 * kNParam);
+        x.reserve(n * 3);
+        t.reserve(n * kOut);
+        gp.reserve((size_t)4 * bpt * kNParam);
+        lp.reserve((size_t)4 * bpt);
+        g.reserve((size_t)4 * kNParam);
+        idx.reserve(n);
+        std::vector<int> id(n);
+        std::iota(id.begin(), id.end(), 0);
+        HIP_OK(hipMemcpy(p.p, params, sizeof(float) * 4 * kNParam, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(x.p, X, sizeof(float) * n * 3, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(t.p, T, sizeof(float) * n * kOut, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(idx.p, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        sur_gradient(ctx, N, p.p, x.p, t.p, idx.p, n_per_tp, 1.0f / (float)(n_per_tp * kOut), gp.p, lp.p);
+        hipLaunchKernelGGL(k_sur_sum_grad, dim3(nblk(4L * kNParam)), dim3(kBlock), 0, st, gp.p, 4, bpt, g.p);
+        launch_check("k_sur_sum_grad");
+        std::vector<float> hl((size_t)4 * bpt);
+        HIP_OK(hipMemcpyAsync(grad, g.p, sizeof(float) * 4 * kNParam, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hl.data(), lp.p, sizeof(float) * hl.size(), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (loss)
+            for (int tp = 0; tp < 4; ++tp) {
+                double acc = 0.0;
+                for (int b = 0; b < bpt; ++b) acc += hl[(size_t)tp * bpt + b];
+                loss[tp] = (float)(acc / (double)(n_per_tp * kOut));
+            }
+        return 0;
+    });
+}
+
+int mapp_dmd_surrogate_params(mapp_dmd_t* ctx, float* params, float* norm) {
+    return guarded(ctx, [&]() {
+        require(ctx->sur_trained, MAPP_DMD_E_STATE, "no trained surrogate");
+        if (params) HIP_OK(hipMemcpy(params, ctx->d_sp.p, sizeof(float) * 4 * kNParam, hipMemcpyDeviceToHost));
+        if (norm) norm_to_flat(ctx->snorm, norm);
+        return 0;
+    });
+}
+
+int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count) {
+    return guarded(ctx, [&]() {
+        require(ctx->sur_trained && count, MAPP_DMD_E_STATE, "no trained surrogate");
+        std::vector<int> h(ctx->natms);
+        ctx->d_nfall.reserve(ctx->natms);
+        HIP_OK(hipMemcpy(h.data(), ctx->d_nfall.p, sizeof(int) * ctx->natms, hipMemcpyDeviceToHost));
+        long c = 0;
+        for (int v : h) c += v;
+        *count = c;
+        return 0;
+    });
+}
+
 int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede get_vec");
         require(host != nullptr, MAPP_DMD_E_ARG, "null destination");
         HIP_OK(hipSetDevice(ctx->device));
-        const int nt = ctx->nt, natms = ctx->natms;
+        const int nt = acols(ctx), natms = ctx->natms;
         const double* src = nullptr;
         long n = (long)natms * nt;
         switch (which) {
             case MAPP_DMD_VEC_X: {
-                n = (long)natms * (3 + nt);
+                n = (long)natms * xcols(ctx);
                 ctx->d_xaos.reserve(n);
-                if (nt == 1)
-                    hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream,
-                                       ctx->d_xaos.p, natms, ctx->d_pos.p, ctx->d_alpha.p);
-                else
-                    hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream,
-                                       ctx->d_xaos.p, natms, ctx->d_pos.p, ctx->d_alpha.p);
-                launch_check("k_gather_x");
+                gather_x_any(ctx, ctx->d_xaos.p, natms, ctx->d_pos.p, ctx->d_alpha.p);
                 src = ctx->d_xaos.p;
                 break;
             }
             case MAPP_DMD_VEC_C: src = ctx->d_conc.p; break;
-            case MAPP_DMD_VEC_F: src = ctx->d_f.p; n = (long)natms * (3 + nt); break;
+            case MAPP_DMD_VEC_F: src = ctx->d_f.p; n = (long)natms * xcols(ctx); break;
             case MAPP_DMD_VEC_E: src = ctx->d_E.p; break;
             case MAPP_DMD_VEC_DE: src = ctx->d_dE.p; break;
             case MAPP_DMD_VEC_N: src = ctx->d_n.p; break;
@@ -1074,7 +1473,7 @@ int mapp_dmd_set_vec(mapp_dmd_t* ctx, int which, const double* host, long count)
         long n = (long)natms * nt;
         switch (which) {
             case MAPP_DMD_VEC_X:
-                n = (long)natms * (3 + nt);
+                n = (long)natms * xcols(ctx);
                 ctx->d_xaos.reserve(n);
                 dst = ctx->d_xaos.p;
                 break;
@@ -1134,19 +1533,13 @@ int mapp_dmd_get_list2(mapp_dmd_t* ctx, int* offsets, int* entries) {
 int mapp_dmd_get_ghosts(mapp_dmd_t* ctx, double* x, int* src) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede get_ghosts");
-        const int nt = ctx->nt, ng = ctx->nghost;
+        const int nt = acols(ctx), ng = ctx->nghost;
         if (ng == 0) return 0;
         if (x) {
             DevBuf<double> tmp;
-            tmp.reserve((size_t)ng * (3 + nt));
-            if (nt == 1)
-                hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(ng)), dim3(kBlock), 0, ctx->stream, tmp.p, ng,
-                                   ctx->d_pos.p + ctx->natms, ctx->d_alpha.p + (size_t)ctx->natms * nt);
-            else
-                hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(ng)), dim3(kBlock), 0, ctx->stream, tmp.p, ng,
-                                   ctx->d_pos.p + ctx->natms, ctx->d_alpha.p + (size_t)ctx->natms * nt);
-            launch_check("k_gather_x ghosts");
-            HIP_OK(hipMemcpyAsync(x, tmp.p, sizeof(double) * ng * (3 + nt), hipMemcpyDeviceToHost, ctx->stream));
+            tmp.reserve((size_t)ng * xcols(ctx));
+            gather_x_any(ctx, tmp.p, ng, ctx->d_pos.p + ctx->natms, ctx->d_alpha.p + (size_t)ctx->natms * nt);
+            HIP_OK(hipMemcpyAsync(x, tmp.p, sizeof(double) * ng * xcols(ctx), hipMemcpyDeviceToHost, ctx->stream));
             HIP_OK(hipStreamSynchronize(ctx->stream));
         }
         if (src) HIP_OK(hipMemcpy(src, ctx->d_ghost_src.p, sizeof(int) * ng, hipMemcpyDeviceToHost));

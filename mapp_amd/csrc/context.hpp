@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "dmd_common.hpp"
+#include "surrogate_types.hpp"
 
 struct mapp_dmd {
     int nt = 0, variant = 1, device = 0, prec = MAPP_DMD_PREC_FP64;
@@ -14,7 +15,7 @@ struct mapp_dmd {
     bool have_tables = false, have_params = false, have_system = false, inited = false;
     int nr = 0, nrho = 0;
     double dr = 0, drho = 0;
-    std::vector<double> F9, rho7, phi7;  // [nt][nrho+1][9], [nt*nt][nr][7], [nphi][nr][7]
+    std::vector<double> F9, rho7, phi7, F7;  // [nt][nrho+1][9], [nt*nt][nr][7], [nphi][nr][7], EAM F [nt][nrho+1][7]
 
     // parameters
     mappdmd::FFParams<double> P{};
@@ -30,7 +31,7 @@ struct mapp_dmd {
     std::vector<int> swap_off, swap_dim, swap_dir;
 
     // device tables
-    mappdmd::DevBuf<double> d_quad64, d_rho7, d_phi7, d_F9;
+    mappdmd::DevBuf<double> d_quad64, d_rho7, d_phi7, d_F9, d_F7, d_drho, d_pairF2;
     mappdmd::DevBuf<float> d_quad32;
     // device atoms (owned rows first, ghosts after)
     mappdmd::DevBuf<double4> d_pos, d_x0;
@@ -56,6 +57,14 @@ struct mapp_dmd {
     mappdmd::DevBuf<double> d_part, d_out, d_xaos, d_stage;
     mappdmd::DevBuf<int> d_iflag;
     long n_inrange = 0;
+
+    // learned surrogate
+    mappdmd::DevBuf<float> d_sp, d_sm, d_sv, d_sX, d_sT, d_sXv, d_sTv, d_sgpart, d_slpart, d_serr;
+    mappdmd::DevBuf<int> d_sidx, d_nfall;
+    mappdmd::DevBuf<unsigned short> d_sfrag;  // bf16 MFMA weight fragments
+    mappdmd::DevBuf<unsigned> d_smask;
+    mappdmd::SurNorm snorm{};
+    bool sur_trained = false;
 
     // timing
     bool timing = false;

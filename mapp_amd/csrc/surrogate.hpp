@@ -1,0 +1,504 @@
+// Learned surrogate of the Gauss-Hermite averaged primitives.
+//
+// One MLP per ordered type pair (it, jt) maps (r, alpha_i^it, alpha_j^jt) to
+// {rho_bar, d rho_bar/dr, d rho_bar/dalpha, phi_bar, d phi_bar/dr,
+// d phi_bar/dalpha} -- exactly what pass 1 of the force computes per
+// (pair, type pair) by quadrature (src/ff_dmd_I.cpp:180-242).  Shape
+// 3 -> 64 -> 64 -> 64 -> 6 with SiLU, inputs and outputs standardised.
+//
+//   k_sur_samples   training / validation samples labelled by the FP32
+//                   quadrature itself (the mixed path's tables and rule)
+//   k_sur_train     one micro-batch of 64 samples of one type pair per block:
+//                   forward, loss, backward in FP32 with everything in LDS;
+//                   writes the block's gradient (fixed-order sums, no atomics)
+//   k_sur_adam      sums block gradients in block order, Adam update
+//   k_sur_eval      FP32 forward for validation (error per cell -> fallback mask)
+//   k_pass1_sur     pass 1 of the force with the networks on BF16 MFMA
+//                   (v_mfma_f32_16x16x32_bf16, FP32 accumulate): one wavefront
+//                   per site, lane = type pair * 16 + pair slot, so each MFMA
+//                   M tile holds the 16 samples of one type pair; samples
+//                   outside the trained box or in a cell whose validation error
+//                   exceeded the tolerance fall back to the quadrature.
+#pragma once
+#include "force.hpp"
+#include "surrogate_types.hpp"
+#include <type_traits>
+
+#pragma clang fp contract(fast)
+
+namespace mappdmd {
+
+constexpr int kHid = 64, kIn = 3, kOut = 6;
+constexpr int kW1 = 0;
+constexpr int kB1 = kW1 + kIn * kHid;
+constexpr int kW2 = kB1 + kHid;
+constexpr int kB2 = kW2 + kHid * kHid;
+constexpr int kW3 = kB2 + kHid;
+constexpr int kB3 = kW3 + kHid * kHid;
+constexpr int kW4 = kB3 + kHid;
+constexpr int kB4 = kW4 + kHid * kOut;
+constexpr int kNParam = kB4 + kOut;  // 8966 per network
+constexpr int kMB = 64;              // training micro-batch per block
+constexpr int kCellR = 64, kCellA = 4;  // fallback cells: r x alpha_i x alpha_j
+
+
+__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_df(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u01(unsigned long long h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+// samples n_per_tp per type pair: X[s] = {r, alpha_i, alpha_j}, T[s] = quadrature outputs
+template <int NT, int NQ>
+__global__ void k_sur_samples(FFParams<float> PQ, SurNorm N, const float* __restrict__ quad, long n_per_tp,
+                              unsigned long long seed, float* __restrict__ X, float* __restrict__ T) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_per_tp * NT * NT) return;
+    const int tp = (int)(s / n_per_tp);
+    const unsigned long long h = splitmix64(seed ^ (unsigned long long)s * 0x632BE59BD9B4E019ull);
+    const double r = N.r_lo + (N.r_hi - N.r_lo) * u01(h);
+    const double ai = N.a_lo + (N.a_hi - N.a_lo) * u01(splitmix64(h + 1));
+    const double aj = N.a_lo + (N.a_hi - N.a_lo) * u01(splitmix64(h + 2));
+    const double ia = 1.0 / ai + 1.0 / aj;
+    const float asq = (float)rsqrt(ia);
+    float g[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    const float* tab = quad + (size_t)tp * PQ.nr * kQuadRow;
+    gh_avg<float, NQ, true>(PQ, (float)r, (float)(1.0 / r), asq, (float)ia, tab, g, f);
+    X[s * 3 + 0] = (float)r;
+    X[s * 3 + 1] = (float)ai;
+    X[s * 3 + 2] = (float)aj;
+    for (int k = 0; k < 3; ++k) {
+        T[s * kOut + k] = g[k];
+        T[s * kOut + 3 + k] = f[k];
+    }
+}
+
+// one dense layer over the block's micro-batch: out[s][o] = b[o] + sum_k in[s][k] W[k][o]
+template <int K>
+__device__ __forceinline__ void dense_fwd(const float* W, const float* b, const float* in, int in_stride, float* z,
+                                          float* h) {
+    for (int e = threadIdx.x; e < kMB * kHid; e += blockDim.x) {
+        const int s = e / kHid, o = e % kHid;
+        float acc = b[o];
+#pragma unroll 8
+        for (int k = 0; k < K; ++k) acc += in[s * in_stride + k] * W[k * kHid + o];
+        z[s * kHid + o] = acc;
+        h[s * kHid + o] = silu_f(acc);
+    }
+}
+
+// forward + backward of one micro-batch (64 samples of type pair tp); the
+// block's gradient goes to gpart[block][kNParam], its loss sum to lpart[block]
+__global__ __launch_bounds__(256) void k_sur_train(const float* __restrict__ params, SurNorm N,
+                                                   const float* __restrict__ X, const float* __restrict__ T,
+                                                   const int* __restrict__ idx, int blocks_per_tp, float inv_count,
+                                                   float* __restrict__ gpart, float* __restrict__ lpart) {
+    extern __shared__ float sm[];
+    float* W = sm;                       // kNParam
+    float* H0 = W + kNParam + 2;         // [64][4]
+    float* Z1 = H0 + kMB * 4;            // [64][64] each
+    float* Z2 = Z1 + kMB * kHid;
+    float* Z3 = Z2 + kMB * kHid;
+    float* H1 = Z3 + kMB * kHid;
+    float* H2 = H1 + kMB * kHid;
+    float* H3 = H2 + kMB * kHid;
+    float* DY = H3 + kMB * kHid;         // [64][8]
+    float* red = DY + kMB * 8;           // [256]
+    const int tp = blockIdx.x / blocks_per_tp;
+    const float* P = params + (size_t)tp * kNParam;
+    for (int e = threadIdx.x; e < kNParam; e += blockDim.x) W[e] = P[e];
+    for (int s = threadIdx.x; s < kMB; s += blockDim.x) {
+        const int id = idx[(size_t)blockIdx.x * kMB + s];
+        H0[s * 4 + 0] = (X[(size_t)id * 3 + 0] - N.r_c) / N.r_s;
+        H0[s * 4 + 1] = (X[(size_t)id * 3 + 1] - N.a_c) / N.a_s;
+        H0[s * 4 + 2] = (X[(size_t)id * 3 + 2] - N.a_c) / N.a_s;
+        H0[s * 4 + 3] = 0.0f;
+    }
+    __syncthreads();
+    dense_fwd<kIn>(W + kW1, W + kB1, H0, 4, Z1, H1);
+    __syncthreads();
+    dense_fwd<kHid>(W + kW2, W + kB2, H1, kHid, Z2, H2);
+    __syncthreads();
+    dense_fwd<kHid>(W + kW3, W + kB3, H2, kHid, Z3, H3);
+    __syncthreads();
+    // output layer, loss and dL/dy
+    float lsum = 0.0f;
+    for (int e = threadIdx.x; e < kMB * kOut; e += blockDim.x) {
+        const int s = e / kOut, o = e % kOut;
+        float y = W[kB4 + o];
+#pragma unroll 8
+        for (int k = 0; k < kHid; ++k) y += H3[s * kHid + k] * W[kW4 + k * kOut + o];
+        const int id = idx[(size_t)blockIdx.x * kMB + s];
+        const float t = (T[(size_t)id * kOut + o] - N.y_mean[tp][o]) / N.y_std[tp][o];
+        const float d = y - t;
+        lsum += d * d;
+        DY[s * 8 + o] = 2.0f * d * inv_count;
+    }
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+    float* G = gpart + (size_t)blockIdx.x * kNParam;
+    // layer 4 gradients
+    for (int e = threadIdx.x; e < kHid * kOut; e += blockDim.x) {
+        const int k = e / kOut, o = e % kOut;
+        float acc = 0.0f;
+        for (int s = 0; s < kMB; ++s) acc += H3[s * kHid + k] * DY[s * 8 + o];
+        G[kW4 + e] = acc;
+    }
+    if (threadIdx.x < kOut) {
+        float acc = 0.0f;
+        for (int s = 0; s < kMB; ++s) acc += DY[s * 8 + threadIdx.x];
+        G[kB4 + threadIdx.x] = acc;
+    }
+    // dZ3 = (dY W4^T) * silu'(Z3), written over Z3
+    for (int e = threadIdx.x; e < kMB * kHid; e += blockDim.x) {
+        const int s = e / kHid, k = e % kHid;
+        float acc = 0.0f;
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) acc += DY[s * 8 + o] * W[kW4 + k * kOut + o];
+        Z3[e] = acc * silu_df(Z3[e]);
+    }
+    __syncthreads();
+    // hidden layers 3 and 2: dW = H^T dZ, db = sum dZ, dZ_prev = (dZ W^T) * silu'(Z_prev)
+    float* Zs[3] = {Z1, Z2, Z3};
+    float* Hs[3] = {H1, H2, H3};
+    const int wofs[3] = {kW1, kW2, kW3}, bofs[3] = {kB1, kB2, kB3};
+    for (int l = 2; l >= 1; --l) {
+        const float* dZ = Zs[l];
+        const float* Hin = Hs[l - 1];
+        for (int e = threadIdx.x; e < kHid * kHid; e += blockDim.x) {
+            const int k = e / kHid, o = e % kHid;
+            float acc = 0.0f;
+            for (int s = 0; s < kMB; ++s) acc += Hin[s * kHid + k] * dZ[s * kHid + o];
+            G[wofs[l] + e] = acc;
+        }
+        if (threadIdx.x < kHid) {
+            float acc = 0.0f;
+            for (int s = 0; s < kMB; ++s) acc += dZ[s * kHid + threadIdx.x];
+            G[bofs[l] + threadIdx.x] = acc;
+        }
+        float* Zp = Zs[l - 1];
+        const float* Wl = W + wofs[l];
+        // each (s, k) of the previous layer is read then overwritten by one thread
+        for (int e = threadIdx.x; e < kMB * kHid; e += blockDim.x) {
+            const int s = e / kHid, k = e % kHid;
+            float acc = 0.0f;
+#pragma unroll 8
+            for (int o = 0; o < kHid; ++o) acc += dZ[s * kHid + o] * Wl[k * kHid + o];
+            Zp[e] = acc * silu_df(Zp[e]);
+        }
+        __syncthreads();
+    }
+    // layer 1
+    for (int e = threadIdx.x; e < kIn * kHid; e += blockDim.x) {
+        const int k = e / kHid, o = e % kHid;
+        float acc = 0.0f;
+        for (int s = 0; s < kMB; ++s) acc += H0[s * 4 + k] * Z1[s * kHid + o];
+        G[kW1 + e] = acc;
+    }
+    if (threadIdx.x < kHid) {
+        float acc = 0.0f;
+        for (int s = 0; s < kMB; ++s) acc += Z1[s * kHid + threadIdx.x];
+        G[kB1 + threadIdx.x] = acc;
+    }
+}
+
+constexpr size_t kSurTrainLds = sizeof(float) * (kNParam + 2 + kMB * 4 + 6 * kMB * kHid + kMB * 8 + 256);
+
+// Adam: gradient of each network = sum of its blocks' gradients in block order
+__global__ void k_sur_adam(float* __restrict__ params, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ gpart, int ntp, int blocks_per_tp, float lr, float b1, float b2,
+                           float eps, float bc1, float bc2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntp * kNParam) return;
+    const int tp = e / kNParam, k = e % kNParam;
+    float g = 0.0f;
+    for (int b = 0; b < blocks_per_tp; ++b) g += gpart[(size_t)(tp * blocks_per_tp + b) * kNParam + k];
+    m[e] = b1 * m[e] + (1.0f - b1) * g;
+    v[e] = b2 * v[e] + (1.0f - b2) * g * g;
+    const float mh = m[e] / bc1, vh = v[e] / bc2;
+    params[e] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// sums of block gradients (for the gradient check): G[tp][k]
+__global__ void k_sur_sum_grad(const float* __restrict__ gpart, int ntp, int blocks_per_tp, float* __restrict__ G) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntp * kNParam) return;
+    const int tp = e / kNParam, k = e % kNParam;
+    float g = 0.0f;
+    for (int b = 0; b < blocks_per_tp; ++b) g += gpart[(size_t)(tp * blocks_per_tp + b) * kNParam + k];
+    G[e] = g;
+}
+
+// FP32 forward of one sample (thread per sample), de-standardised outputs
+__device__ __forceinline__ void sur_forward_f32(const float* __restrict__ P, const SurNorm& N, int tp, float r,
+                                                float ai, float aj, float* y) {
+    float u[3] = {(r - N.r_c) / N.r_s, (ai - N.a_c) / N.a_s, (aj - N.a_c) / N.a_s};
+    float h1[kHid], h2[kHid];
+    for (int o = 0; o < kHid; ++o) {
+        float z = P[kB1 + o];
+        for (int k = 0; k < kIn; ++k) z += u[k] * P[kW1 + k * kHid + o];
+        h1[o] = silu_f(z);
+    }
+    for (int o = 0; o < kHid; ++o) {
+        float z = P[kB2 + o];
+        for (int k = 0; k < kHid; ++k) z += h1[k] * P[kW2 + k * kHid + o];
+        h2[o] = silu_f(z);
+    }
+    for (int o = 0; o < kHid; ++o) {
+        float z = P[kB3 + o];
+        for (int k = 0; k < kHid; ++k) z += h2[k] * P[kW3 + k * kHid + o];
+        h1[o] = silu_f(z);
+    }
+    for (int o = 0; o < kOut; ++o) {
+        float z = P[kB4 + o];
+        for (int k = 0; k < kHid; ++k) z += h1[k] * P[kW4 + k * kOut + o];
+        y[o] = z * N.y_std[tp][o] + N.y_mean[tp][o];
+    }
+}
+
+// validation: per-sample error (max over the 6 outputs of |y - t| / std), for the cell statistics
+__global__ void k_sur_eval(const float* __restrict__ params, SurNorm N, const float* __restrict__ X,
+                           const float* __restrict__ T, long n_per_tp, int ntp, float* __restrict__ err) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_per_tp * ntp) return;
+    const int tp = (int)(s / n_per_tp);
+    float y[kOut];
+    sur_forward_f32(params + (size_t)tp * kNParam, N, tp, X[s * 3], X[s * 3 + 1], X[s * 3 + 2], y);
+    float e = 0.0f;
+    for (int o = 0; o < kOut; ++o) e = fmaxf(e, fabsf(y[o] - T[s * kOut + o]) / N.y_std[tp][o]);
+    err[s] = e;
+}
+
+__device__ __forceinline__ int sur_cell(const SurNorm& N, float r, float ai, float aj) {
+    int cr = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kCellR);
+    int ci = (int)((ai - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
+    int cj = (int)((aj - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
+    cr = cr < 0 ? 0 : (cr >= kCellR ? kCellR - 1 : cr);
+    ci = ci < 0 ? 0 : (ci >= kCellA ? kCellA - 1 : ci);
+    cj = cj < 0 ? 0 : (cj >= kCellA ? kCellA - 1 : cj);
+    return (cr * kCellA + ci) * kCellA + cj;
+}
+
+// ---------------------------------------------------------------- BF16 MFMA inference
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// weight fragments for v_mfma_f32_16x16x32_bf16, B operand (K x N = 32 x 16 per
+// fragment): lane l holds column n*16 + l%16 and rows ks*32 + (l/16)*8 + [0,8).
+// frag[((tp*3 + layer)*4 + n)*2 + ks][lane][8]; layer 2 (64->6) uses n = 0 only,
+// columns >= 6 zero.
+constexpr int kFragPerTp = 3 * 4 * 2;
+
+__global__ void k_sur_pack_frags(const float* __restrict__ params, int ntp, __bf16* __restrict__ frag) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (fragment, lane)
+    if (idx >= ntp * kFragPerTp * 64) return;
+    const int lane = idx % 64, fr = idx / 64;
+    const int ks = fr % 2, n = (fr / 2) % 4, layer = (fr / 8) % 3, tp = fr / 24;
+    const float* P = params + (size_t)tp * kNParam;
+    const int col = n * 16 + lane % 16;
+    for (int e = 0; e < 8; ++e) {
+        const int k = ks * 32 + (lane / 16) * 8 + e;
+        float w = 0.0f;
+        if (layer == 0) w = P[kW2 + k * kHid + col];
+        else if (layer == 1) w = P[kW3 + k * kHid + col];
+        else w = (n == 0 && col < kOut) ? P[kW4 + k * kOut + col] : 0.0f;
+        frag[(size_t)idx * 8 + e] = (__bf16)w;
+    }
+}
+
+struct SurDev {
+    const float* params;   // [ntp][kNParam] FP32 (layer 1, biases)
+    const __bf16* frag;    // MFMA weight fragments
+    const unsigned* mask;  // fallback bit per cell, [ntp][cells/32]
+    SurNorm N;
+};
+
+// pass 1 with the surrogate (DMD force, NT == 2): one wavefront per block and per
+// site, 16 pairs per chunk.  A block holds a single wavefront, so the LDS
+// hand-offs between the VALU layer and the MFMA layers synchronise with
+// __syncthreads() whatever the site's list length.
+constexpr int kSurBlock = 64;
+
+template <int NQ>
+__global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
+                                                         const double4* __restrict__ pos,
+                                                         const double4* __restrict__ arec, PairBufs L,
+                                                         const int* __restrict__ rpos, const float* __restrict__ quad,
+                                                         const double* __restrict__ rho7,
+                                                         const double* __restrict__ phi7, double* __restrict__ Eown,
+                                                         double* __restrict__ pairE, double* __restrict__ deriv,
+                                                         double* __restrict__ part, int* __restrict__ nfall) {
+    constexpr int NT = 2;
+    __shared__ __attribute__((aligned(16))) __bf16 act[64][kHid + 8];  // 64 samples x 64 features (+pad)
+    __shared__ float yout[64][8];
+    const int lane = threadIdx.x;
+    const int i = xcd_block(gridDim.x);
+    const int tp = lane / 16, k = lane % 16, it = tp / NT, jt = tp % NT;
+    const bool want_phi = it >= jt;
+    const float* Pp = S.params + (size_t)tp * kNParam;
+    double en = 0.0, npair = 0.0;
+    int fallbacks = 0;
+    if (i < P.natms) {
+        const double4 xi = pos[i];
+        double ci[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
+        const double4 ra = arec[(size_t)i * NT + it];
+        const double ai = ra.x, iai = ra.y;
+        double Ei = 0.0;
+        const int p0 = L.off[i], p1 = L.off[i + 1];
+        for (int base = p0; base < p1; base += 16) {
+            const int p = base + k;
+            const bool valid = p < p1;
+            int j = 0;
+            bool inr = false;
+            double rsq = 0.0;
+            if (valid) {
+                j = L.nl[p];
+                const double4 xj = pos[j];
+                const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
+                {
+#pragma clang fp contract(off)
+                    rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                }
+                inr = rsq < P.cut_sq_mod;
+            }
+            double aj = 1.0, cj[NT] = {0.0, 0.0}, r = 1.0, r_inv = 1.0, ia = 1.0;
+            int br = 0;
+            if (inr) {
+                const double4 rj = arec[(size_t)j * NT + jt];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : arec[(size_t)j * NT + t].z;
+                aj = rj.x;
+                r_inv = rsqrt(rsq);
+                r = rsq * r_inv;
+                ia = iai + rj.y;
+                br = alpha_branch(P, ia, ai, aj);
+            }
+            const float rf = (float)r, aif = (float)ai, ajf = (float)aj;
+            bool use_net = inr && br == 1 && rf >= S.N.r_lo && rf <= S.N.r_hi && aif >= S.N.a_lo &&
+                           aif <= S.N.a_hi && ajf >= S.N.a_lo && ajf <= S.N.a_hi;
+            if (use_net) {
+                const int cell = sur_cell(S.N, rf, aif, ajf);
+                const int per_tp = kCellR * kCellA * kCellA / 32;
+                if ((S.mask[tp * per_tp + cell / 32] >> (cell % 32)) & 1u) use_net = false;
+            }
+            // layer 1 on VALU: this lane's sample, 64 features -> LDS (bf16)
+            {
+                const float u0 = (rf - S.N.r_c) / S.N.r_s, u1 = (aif - S.N.a_c) / S.N.a_s,
+                            u2 = (ajf - S.N.a_c) / S.N.a_s;
+                for (int o = 0; o < kHid; ++o) {
+                    const float z = Pp[kB1 + o] + u0 * Pp[kW1 + o] + u1 * Pp[kW1 + kHid + o] +
+                                    u2 * Pp[kW1 + 2 * kHid + o];
+                    act[lane][o] = (__bf16)silu_f(z);
+                }
+            }
+            __syncthreads();
+            // layers 2, 3 (64 -> 64) and 4 (64 -> 6) on MFMA; M tile m = type pair m
+            for (int layer = 0; layer < 3; ++layer) {
+                const int nN = layer < 2 ? 4 : 1;
+                floatx4_t cacc[4][4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    bf16x8_t a[2];
+#pragma unroll
+                    for (int ks = 0; ks < 2; ++ks)
+                        a[ks] = *reinterpret_cast<const bf16x8_t*>(&act[m * 16 + lane % 16][ks * 32 + (lane / 16) * 8]);
+                    for (int n = 0; n < nN; ++n) {
+                        floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(
+                                S.frag + ((size_t)(((m * 3 + layer) * 4 + n) * 2 + ks) * 64 + lane) * 8);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b, c, 0, 0, 0);
+                        }
+                        cacc[m][n] = c;
+                    }
+                }
+                __syncthreads();
+                // D: lane holds rows (lane/16)*4 + e of tile m, column n*16 + lane%16
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float* Pm = S.params + (size_t)m * kNParam;
+                    for (int n = 0; n < nN; ++n)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int row = m * 16 + (lane / 16) * 4 + e, col = n * 16 + lane % 16;
+                            if (layer < 2) {
+                                const float z = cacc[m][n][e] + Pm[(layer == 0 ? kB2 : kB3) + col];
+                                act[row][col] = (__bf16)silu_f(z);
+                            } else if (col < kOut) {
+                                yout[row][col] = cacc[m][n][e] + Pm[kB4 + col];
+                            }
+                        }
+                }
+                __syncthreads();
+            }
+            double g[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+            if (use_net) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    g[c] = yout[lane][c] * S.N.y_std[tp][c] + S.N.y_mean[tp][c];
+                    f[c] = yout[lane][3 + c] * S.N.y_std[tp][3 + c] + S.N.y_mean[tp][3 + c];
+                }
+            } else if (inr && br == 1) {
+                ++fallbacks;
+                float gq[3] = {0, 0, 0}, fq[3] = {0, 0, 0};
+                gh_avg<float, NQ, true>(PQ, rf, (float)r_inv, (float)rsqrt(ia), (float)ia,
+                                        quad + (size_t)tp * P.nr * kQuadRow, gq, fq);
+                for (int c = 0; c < 3; ++c) {
+                    g[c] = gq[c];
+                    f[c] = fq[c];
+                }
+            } else if (inr && br == 2 && rsq < P.cut_sq) {
+                classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
+                               phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
+            }
+            double ej = 0.0;
+            if (inr) {
+                if (tp == 0) npair += 1.0;
+                Ei += g[0] * cj[jt];
+                ej = g[0] * ci[it];
+                if (want_phi) {
+                    const double half = j < P.natms ? 0.5 : 0.25;
+                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * f[0];
+                }
+                double* d = deriv + ((size_t)p * NT * NT + tp) * 4;
+                d[0] = g[1];
+                d[1] = g[2];
+                d[2] = f[1];
+                d[3] = f[2];
+            }
+            ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
+            if (valid && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double s = wave_sum(it == t ? Ei : 0.0);
+            if (lane == 0) Eown[(size_t)i * NT + t] = s;
+        }
+    }
+    const double se = wave_sum(en), sn = wave_sum(npair), sf = wave_sum((double)fallbacks);
+    if (lane == 0) {
+        part[(size_t)blockIdx.x * 2 + 0] = se;
+        part[(size_t)blockIdx.x * 2 + 1] = sn;
+        if (i < P.natms) nfall[i] = (int)sf;
+    }
+}
+
+}  // namespace mappdmd
+
+#pragma clang fp contract(off)

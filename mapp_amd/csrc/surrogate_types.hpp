@@ -1,0 +1,15 @@
+// Plain-data types of the learned surrogate shared by the context and the kernels.
+#pragma once
+
+namespace mappdmd {
+
+constexpr int kSurOut = 6;
+
+// trained box and standardisation of the surrogate networks
+struct SurNorm {
+    float r_lo, r_hi, a_lo, a_hi;  // trained box
+    float r_c, r_s, a_c, a_s;      // input standardisation
+    float y_mean[4][kSurOut], y_std[4][kSurOut];
+};
+
+}  // namespace mappdmd

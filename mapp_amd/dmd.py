@@ -14,7 +14,7 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmapp_dmd.so")
 
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2}
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3}
 PREC = {"fp64": 0, "mixed": 1, "surrogate": 2}
 VEC = {"x": 0, "c": 1, "f": 2, "E": 3, "dE": 4, "n": 5, "c_d": 6, "ddE": 7, "s": 8, "t": 9, "v": 10, "a": 11, "g": 12}
 
@@ -24,6 +24,7 @@ D = C.c_double
 DP = C.POINTER(C.c_double)
 IP = C.POINTER(C.c_int)
 LP = C.POINTER(C.c_long)
+FP = C.POINTER(C.c_float)
 VPP = C.POINTER(C.c_void_p)
 CTX = C.c_void_p
 
@@ -55,6 +56,10 @@ _SIGS = {
     "mapp_dmd_get_gh": [CTX, DP, DP, DP, DP],
     "mapp_dmd_last_timing": [CTX, DP],
     "mapp_dmd_enable_timing": [CTX, C.c_int],
+    "mapp_dmd_surrogate_train": [CTX, C.c_long, C.c_int, C.c_int, D, C.c_ulonglong, D, DP, DP],
+    "mapp_dmd_surrogate_gradient": [CTX, FP, FP, C.c_long, FP, FP, FP, FP],
+    "mapp_dmd_surrogate_params": [CTX, FP, FP],
+    "mapp_dmd_surrogate_fallbacks": [CTX, LP],
 }
 
 
@@ -90,12 +95,22 @@ def _ip(a):
     return a.ctypes.data_as(IP)
 
 
+def _fp(a):
+    return a.ctypes.data_as(FP)
+
+
+SUR_NPARAM = 8966
+SUR_NNORM = 56
+SUR_REPORT = ("loss", "val_mean_err", "val_max_err", "masked_frac", "steps", "seconds", "n_train", "n_val")
+
+
 class ForceFieldDMD:
     """ForceField_DMD / _I / _II on one GPU through the C ABI."""
 
     def __init__(self, no_types=2, variant="DMD_I", device=0):
         self.lib = load_library()
         self.nt = no_types
+        self.variant = variant
         self.ctx = CTX()
         self._call("mapp_dmd_create", C.byref(self.ctx), no_types, VARIANT[variant], device, noctx=True)
 
@@ -196,7 +211,8 @@ class ForceFieldDMD:
 
     def get(self, name):
         s = self.sizes()
-        width = 3 + self.nt if name in ("x", "f") else self.nt
+        eam = self.variant == "EAM"
+        width = (3 if eam else 3 + self.nt) if name in ("x", "f") else (1 if eam else self.nt)
         out = np.zeros((s["natms"], width))
         self._call("mapp_dmd_get_vec", VEC[name], _dp(out), out.size)
         return out
@@ -222,7 +238,7 @@ class ForceFieldDMD:
 
     def ghosts(self):
         s = self.sizes()
-        x = np.zeros((max(s["nghost"], 1), 3 + self.nt))
+        x = np.zeros((max(s["nghost"], 1), 3 if self.variant == "EAM" else 3 + self.nt))
         src = np.zeros(max(s["nghost"], 1), dtype=np.int32)
         self._call("mapp_dmd_get_ghosts", _dp(x), _ip(src))
         return x[: s["nghost"]], src[: s["nghost"]]
@@ -239,6 +255,47 @@ class ForceFieldDMD:
         ms = np.zeros(5)
         self._call("mapp_dmd_last_timing", _dp(ms))
         return ms
+
+    # learned surrogate of the pair quadrature (two types)
+    def surrogate_train(self, n_per_tp=1 << 20, epochs=4, batch_per_tp=256, lr=2e-3, seed=1, tol=0.05, box=None):
+        rep = np.zeros(8)
+        b = None if box is None else _dp(np.ascontiguousarray(box, dtype=np.float64))
+        self._call("mapp_dmd_surrogate_train", int(n_per_tp), int(epochs), int(batch_per_tp), float(lr), int(seed),
+                   float(tol), b, _dp(rep))
+        return dict(zip(SUR_REPORT, rep.tolist()))
+
+    def surrogate_gradient(self, params, norm, X, T):
+        """gradient of each network's mean squared standardised error by the training kernel"""
+        params = np.ascontiguousarray(params, dtype=np.float32).reshape(4, SUR_NPARAM)
+        norm = np.ascontiguousarray(norm, dtype=np.float32).reshape(SUR_NNORM)
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 3)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, 6)
+        n = X.shape[0] // 4
+        grad = np.zeros((4, SUR_NPARAM), dtype=np.float32)
+        loss = np.zeros(4, dtype=np.float32)
+        self._call("mapp_dmd_surrogate_gradient", _fp(params), _fp(norm), n, _fp(X), _fp(T), _fp(grad), _fp(loss))
+        return grad, loss
+
+    def surrogate_params(self):
+        params = np.zeros((4, SUR_NPARAM), dtype=np.float32)
+        norm = np.zeros(SUR_NNORM, dtype=np.float32)
+        self._call("mapp_dmd_surrogate_params", _fp(params), _fp(norm))
+        return params, norm
+
+    def surrogate_fallbacks(self):
+        n = C.c_long(0)
+        self._call("mapp_dmd_surrogate_fallbacks", C.byref(n))
+        return n.value
+
+
+def setup_eam(H, s3, types, tables, skin=0.5, device=0):
+    """plain EAM (ForceField_eam): reduced positions [natms][3] and integer types"""
+    ff = ForceFieldDMD(no_types=len(tables.elements), variant="EAM", device=device)
+    ff.set_fs(tables)
+    ff.coef(0.0, 1, 1.0, 1.0, 1.0, np.ones(len(tables.elements)))
+    ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    ff.init(skin)
+    return ff
 
 
 def setup(lat, tables, no_i=3, variant="DMD_I", precision="fp64", skin=0.5, r_crd=3.0,

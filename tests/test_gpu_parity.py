@@ -154,3 +154,57 @@ def test_set_x_update_and_rebuild(tables):
     assert np.array_equal(goff, ooff) and np.array_equal(gnl, onl)
     assert rel_inf(ff.force_calc(1), orc.force_calc(1)) <= 1e-12
     assert rel_inf(ff.get("f"), orc.get("f")) <= 1e-12
+
+
+def test_plain_eam_parity(tables):
+    """ForceField_eam (src/ff_eam.cpp:44-356) on the GPU against the oracle's EAM:
+    list bit-exact, energy / virial / forces <= 1e-12 relative"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_eam
+    from oracle import oracle as O
+    lat = W.fcc_alloy(8, seed=21)
+    types = (np.arange(lat.natms) % 3 == 0).astype(np.float64)
+    ff = setup_eam(lat.H, lat.x[:, :3], types, tables)
+    orc = O.Oracle(nt=2, variant="EAM")
+    orc.set_fs(tables)
+    orc.set_params(masses=tables.masses)
+    orc.set_system(lat.H, lat.x[:, :3].copy(), types.reshape(-1, 1))
+    orc.init(0.5)
+    goff, gnl = ff.neighbor_list()
+    ooff, onl = orc.neighbor_list()
+    assert np.array_equal(goff, ooff) and np.array_equal(gnl, onl)
+    eg, eo = ff.force_calc(1), orc.force_calc(1)
+    print("eam energy", eg[0], eo[0], flush=True)
+    assert abs(eg[0] - eo[0]) <= 1e-12 * abs(eo[0])
+    assert rel_inf(eg[1:], eo[1:]) <= 1e-12
+    assert rel_inf(ff.get("f"), orc.get("f")) <= 1e-12
+    assert abs(ff.energy_calc() - orc.energy_calc()) <= 1e-12 * abs(eo[0])
+
+
+def test_read_fs_matches_set_fs(tmp_path, tables):
+    """the FS file reader (ForceField_DMD_II::set_fs layout) gives the same force field"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import ForceFieldDMD, setup
+    path = str(tmp_path / "NiCu.fs")
+    W.write_fs(path, tables)
+    lat = W.fcc_alloy(6, seed=22)
+    a = setup(lat, tables)
+    b = ForceFieldDMD(no_types=2, variant="DMD_I")
+    b.read_fs(path, ["Ni", "Cu"])
+    b.coef(3.0, 3, 25.0, 1000.0, 600.0, tables.masses)
+    b.set_system(lat.H, lat.x, lat.c)
+    b.init(0.5)
+    ea, eb = a.force_calc(1), b.force_calc(1)
+    assert np.array_equal(ea, eb)
+    assert np.array_equal(a.get("f"), b.get("f"))
+    # element order swapped in the call: types follow the names given
+    c = ForceFieldDMD(no_types=2, variant="DMD_I")
+    c.read_fs(path, ["Cu", "Ni"])
+    c.coef(3.0, 3, 25.0, 1000.0, 600.0, tables.masses[::-1])
+    lat2 = W.fcc_alloy(6, seed=22)
+    lat2.x[:, 3:] = lat.x[:, 4:2:-1]
+    lat2.c = np.ascontiguousarray(lat.c[:, ::-1])
+    c.set_system(lat2.H, lat2.x, lat2.c)
+    c.init(0.5)
+    ec = c.force_calc(1)
+    assert abs(ec[0] - ea[0]) <= 1e-12 * abs(ea[0])
