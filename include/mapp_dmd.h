@@ -134,6 +134,29 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx);
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
                     double* g_dev, double* cost);
 
+/* energy mode (no counterpart in the reference; a switch):
+   0 (default) the reference: energy_calc loops over type pairs jtype >= itype
+   (src/ff_dmd_I.cpp:508), so its energy differs from force_calc's for two or
+   more types, and both weight cross-type pair terms by 0.5 (c c + c c)
+   (:244-255) while the force loop weights each ordered pair by c_i c_j
+   (:375-380);
+   1 consistent energy: energy_calc over every ordered type pair and cross-type
+   pair terms weighted as in the forces, in both energies; then, for symmetric
+   density tables, f = -grad U in x and in a per-site alpha (sum of the types'
+   alpha forces). */
+int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full);
+
+/* Min_LBFGS::run on a fixed box with LineSearch_BackTrack (replaces the
+   minimiser's host loops, src/min_l-bfgs.cpp:302-787, src/line_search.cpp:
+   32-341): x / f / h and the m_it (s, y) pairs stay on the device, dot
+   products are FP64 deterministic device reductions; energy_calc, force_calc
+   and Atoms::update_0 (skin test, rebuild or ghost update) as in the reference.
+   energies[max_iter + 2]: force_calc energy after each step (energies[0]
+   initial); counts[4]: steps, force_calc calls, energy_calc calls, exit code
+   (1 not downhill, 2 zero gradient, 3 step below alpha_min, 4 energy change
+   below e_tol, 5 max_iter reached). */
+int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, double* energies, long* counts);
+
 /* Learned surrogate (no counterpart in the reference; BASELINE cfg3 workload):
    one MLP 3->64->64->64->6 (SiLU) per ordered type pair maps (r, alpha_i,
    alpha_j) to {rho_bar, drho_bar/dr, drho_bar/dalpha, phi_bar, dphi_bar/dr,

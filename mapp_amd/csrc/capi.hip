@@ -13,6 +13,7 @@
 #include "mobility.hpp"
 #include "eam.hpp"
 #include "surrogate.hpp"
+#include "lbfgs.hpp"
 #include <chrono>
 #include <numeric>
 #include <random>
@@ -502,9 +503,14 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     double* part1 = ctx->d_part.p;
     double* part2 = part1 + (size_t)nbw * 2;
     pack_atoms<NT>(ctx);
-    hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                       ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                       ctx->d_pairE.p, (Q*)nullptr, part1);
+    if (ctx->energy_full)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ,
+                           ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
+                           ctx->d_Eown.p, ctx->d_pairE.p, (Q*)nullptr, part1);
+    else
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, (Q*)nullptr, part1);
     launch_check("k_pass1 energy");
     hipLaunchKernelGGL((k_embed<NT, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
                        L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p, ctx->d_fa.p,
@@ -540,7 +546,8 @@ void dispatch_force(mapp_dmd* ctx, int st_clc, double* en_st) {
 template <int NT>
 double dispatch_energy(mapp_dmd* ctx) {
     const int q = ctx->P.no_i;
-    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+    // the surrogate's energy_calc runs its fallback, the FP32 quadrature
+    if (ctx->prec == MAPP_DMD_PREC_MIXED || ctx->prec == MAPP_DMD_PREC_SURROGATE) {
         if (q == 3) return run_energy<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf);
         if (q == 5) return run_energy<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf);
         return run_energy<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf);
@@ -600,6 +607,7 @@ void fill_float_params(mapp_dmd* ctx) {
     F.inv_amax = (float)P.inv_amax;
     F.kbT = (float)P.kbT;
     F.beta = (float)P.beta;
+    F.xw = (float)P.xw;
     for (int k = 0; k < kMaxPoints; ++k) {
         F.xi[k] = (float)P.xi[k];
         F.w0[k] = (float)P.w0[k];
@@ -820,6 +828,119 @@ void sur_gradient(mapp_dmd* ctx, const SurNorm& N, const float* params, const fl
                        inv_count, gpart, lpart);
     launch_check("k_sur_train");
 }
+
+// Atoms::update_0 (src/atoms.cpp:1780-1865) for owned x already in d_xaos
+// (cartesian + alpha rows): full-skin test against the positions of the last
+// build, then a rebuild (x2s, ghosts, bins, lists) or a forward ghost update.
+int update_x_device(mapp_dmd* ctx) {
+    const int nt = acols(ctx), natms = ctx->natms;
+    if (ctx->variant == MAPP_DMD_VARIANT_EAM) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
+    else if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
+    else scatter_x<2>(ctx, ctx->d_xaos.p);
+    HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_skin_check, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, ctx->d_x0.p, natms,
+                       ctx->skin * ctx->skin, ctx->d_iflag.p);
+    launch_check("k_skin_check");
+    int flag = 0;
+    HIP_OK(hipMemcpyAsync(&flag, ctx->d_iflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (flag) {
+        hipLaunchKernelGGL(k_x2s, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, natms, ctx->bp);
+        launch_check("k_x2s");
+        rebuild_any(ctx);
+        sync_sizes(ctx);
+    } else {
+        if (nt == 1) ghost_update_x<1>(ctx);
+        else ghost_update_x<2>(ctx);  // EAM: alpha column unused
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->cache_valid = false;
+    return flag;
+}
+
+// ------------------------------------------------------------------ minimiser
+void force_any(mapp_dmd* ctx, int st_clc, double* en) {
+    if (ctx->variant == MAPP_DMD_VARIANT_EAM) run_eam(ctx, false, st_clc, en);
+    else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en);
+    else dispatch_force<2>(ctx, st_clc, en);
+}
+double energy_any(mapp_dmd* ctx) {
+    if (ctx->variant == MAPP_DMD_VARIANT_EAM) return run_eam(ctx, true, 0, nullptr);
+    return ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
+}
+
+constexpr int kVecBlocks = 1024;  // grid of the grid-stride vector kernels (fixed: deterministic sums)
+
+template <int ND>
+void dots(mapp_dmd* ctx, long n, const DotArgs& D, double* res) {
+    ctx->d_part.reserve((size_t)kVecBlocks * 8 + 16);
+    ctx->d_out.reserve(16);
+    hipLaunchKernelGGL((k_dots<ND>), dim3(kVecBlocks), dim3(kBlock), 0, ctx->stream, D, n, ctx->d_part.p);
+    launch_check("k_dots");
+    reduce_partials(ctx, ctx->d_part.p, kVecBlocks, ND, ND, 0, ctx->d_out.p);
+    HIP_OK(hipMemcpyAsync(res, ctx->d_out.p, sizeof(double) * ND, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+}
+double dot1(mapp_dmd* ctx, long n, const double* a, const double* b) {
+    DotArgs D{{a, nullptr, nullptr, nullptr}, {b, nullptr, nullptr, nullptr}};
+    double r;
+    dots<1>(ctx, n, D, &r);
+    return r;
+}
+
+// LineSearch_BackTrack::line_min, fixed box (src/line_search.cpp:244-341)
+int line_min(mapp_dmd* ctx, long n, double* nrgy, double* alph, long* n_energy) {
+    const double c = 0.4, rho = 0.5, a_max = 1.0, d_max = 1.0;
+    hipStream_t st = ctx->stream;
+    const int xd = xcols(ctx);
+    double* h = ctx->d_lh.p;
+    const double* x0 = ctx->d_lx0.p;
+    // min alpha of the current x (x0 at this point) and max |h| (after normalising)
+    double a_min = 0.0;
+    double hh = dot1(ctx, n, h, h);
+    hh = d_max / std::sqrt(hh);
+    hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(kBlock), 0, st, n, 0.0, h, hh, h);
+    launch_check("k_axpby");
+    const double inner = dot1(ctx, n, ctx->d_f.p, h);
+    ctx->d_part.reserve((size_t)kVecBlocks * 8 + 16);
+    hipLaunchKernelGGL(k_extrema, dim3(kVecBlocks), dim3(kBlock), 0, st, n, xd, h, x0, ctx->d_part.p);
+    launch_check("k_extrema");
+    std::vector<double> ex((size_t)kVecBlocks * 2);
+    HIP_OK(hipMemcpyAsync(ex.data(), ctx->d_part.p, ex.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    double max_h = 0.0, min_a = INFINITY;
+    for (int b = 0; b < kVecBlocks; ++b) {
+        max_h = std::max(max_h, ex[b * 2]);
+        min_a = std::min(min_a, ex[b * 2 + 1]);
+    }
+    if (ctx->variant != MAPP_DMD_VARIANT_EAM) a_min = std::min(a_min, 0.99 * min_a);
+    *alph = 0.0;
+    if (inner <= 0) return 1;
+    if (max_h == 0) return 2;
+    double a = std::min(a_max, max_h);
+    if (a <= a_min) return 3;
+    auto energy_at = [&](double am) {
+        hipLaunchKernelGGL(k_step, dim3(nblk(n)), dim3(kBlock), 0, st, n, x0, am, h, ctx->d_xaos.p);
+        launch_check("k_step");
+        update_x_device(ctx);
+        ++*n_energy;
+        return energy_any(ctx);
+    };
+    while (true) {
+        const double ideal = *nrgy - a * c * inner;
+        const double cur = energy_at(a);
+        if (cur <= ideal) {
+            *nrgy = cur;
+            *alph = a;
+            return 0;
+        }
+        a *= rho;
+        if (a <= a_min) {
+            *nrgy = energy_at(0.0);
+            return 3;
+        }
+    }
+}
 }  // namespace
 
 // =================================================================== C ABI
@@ -963,6 +1084,7 @@ int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, dou
             P.c0[t] = 1.5 * kb * T * (std::log(deb_l) - 1.0);
         }
         P.kbT = kb * T;
+        P.xw = ctx->energy_full ? 2.0 : 1.0;
         P.beta = 1.0 / P.kbT;
         P.rc = ((double)ctx->nr - 1.0) * ctx->dr;
         P.rho_max = ((double)ctx->nrho - 1.0) * ctx->drho;
@@ -1077,32 +1199,11 @@ int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_x");
         HIP_OK(hipSetDevice(ctx->device));
-        const int nt = acols(ctx), natms = ctx->natms;
-        const size_t n = (size_t)natms * xcols(ctx);
+        const size_t n = (size_t)ctx->natms * xcols(ctx);
         ctx->d_xaos.reserve(n);
         if (x_host)
             HIP_OK(hipMemcpyAsync(ctx->d_xaos.p, x_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        if (ctx->variant == MAPP_DMD_VARIANT_EAM) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
-        else if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
-        else scatter_x<2>(ctx, ctx->d_xaos.p);
-        HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
-        hipLaunchKernelGGL(k_skin_check, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p,
-                           ctx->d_x0.p, natms, ctx->skin * ctx->skin, ctx->d_iflag.p);
-        launch_check("k_skin_check");
-        int flag = 0;
-        HIP_OK(hipMemcpyAsync(&flag, ctx->d_iflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_OK(hipStreamSynchronize(ctx->stream));
-        if (flag) {
-            hipLaunchKernelGGL(k_x2s, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, natms, ctx->bp);
-            launch_check("k_x2s");
-            rebuild_any(ctx);
-            sync_sizes(ctx);
-        } else {
-            if (nt == 1) ghost_update_x<1>(ctx);
-            else ghost_update_x<2>(ctx);  // EAM: alpha column unused
-            HIP_OK(hipStreamSynchronize(ctx->stream));
-        }
-        ctx->cache_valid = false;
+        const int flag = update_x_device(ctx);
         if (rebuilt) *rebuilt = flag;
         return 0;
     });
@@ -1400,6 +1501,107 @@ int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count) {
         long c = 0;
         for (int v : h) c += v;
         *count = c;
+        return 0;
+    });
+}
+
+
+int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full) {
+    return guarded(ctx, [&]() {
+        ctx->energy_full = full != 0;
+        ctx->P.xw = ctx->energy_full ? 2.0 : 1.0;
+        ctx->Pf.xw = (float)ctx->P.xw;
+        return 0;
+    });
+}
+
+int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, double* energies, long* counts) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited, MAPP_DMD_E_STATE, "init must precede min_lbfgs");
+        require(max_iter >= 0 && m_it >= 0 && energies, MAPP_DMD_E_ARG, "bad min_lbfgs arguments");
+        HIP_OK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const long n = (long)ctx->natms * xcols(ctx);
+        const int mm = std::max(m_it, 1);
+        ctx->d_lh.reserve(n);
+        ctx->d_lx0.reserve(n);
+        ctx->d_lf0.reserve(n);
+        ctx->d_lS.reserve((size_t)n * mm);
+        ctx->d_lY.reserve((size_t)n * mm);
+        ctx->d_xaos.reserve(n);
+        std::vector<int> sl(mm);
+        std::vector<double> rh(mm, 0.0), al(mm, 0.0);
+        for (int i = 0; i < mm; ++i) sl[i] = i;
+        auto Sv = [&](int i) { return ctx->d_lS.p + (size_t)sl[i] * n; };
+        auto Yv = [&](int i) { return ctx->d_lY.p + (size_t)sl[i] * n; };
+        long n_force = 0, n_energy = 0;
+        double en[7];
+        force_any(ctx, 1, en);
+        ++n_force;
+        double curr = en[0], prev = 0.0, ratio = 1.0, a_m = 0.0;
+        energies[0] = curr;
+        int k_it = 0, istp = 0, err = 0;
+        const dim3 g(nblk(n)), b(kBlock);
+        while (err == 0) {
+            // h = f, x0 = x, f0 = f
+            gather_x_any(ctx, ctx->d_xaos.p, ctx->natms, ctx->d_pos.p, ctx->d_alpha.p);
+            HIP_OK(hipMemcpyAsync(ctx->d_lh.p, ctx->d_f.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(ctx->d_lx0.p, ctx->d_xaos.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(ctx->d_lf0.p, ctx->d_f.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            for (int i = k_it - 1; i > -1; --i) {
+                al[i] = dot1(ctx, n, ctx->d_lh.p, Sv(i)) * rh[i];
+                hipLaunchKernelGGL(k_axpby, g, b, 0, st, n, -al[i], Yv(i), 1.0, ctx->d_lh.p);
+            }
+            hipLaunchKernelGGL(k_axpby, g, b, 0, st, n, 0.0, ctx->d_lh.p, ratio, ctx->d_lh.p);
+            for (int i = 0; i < k_it; ++i) {
+                const double be = dot1(ctx, n, ctx->d_lh.p, Yv(i)) * rh[i];
+                hipLaunchKernelGGL(k_axpby, g, b, 0, st, n, -(be + al[i]), Sv(i), 1.0, ctx->d_lh.p);
+            }
+            launch_check("k_axpby");
+            prev = curr;
+            err = line_min(ctx, n, &curr, &a_m, &n_energy);
+            if (err == 0 && prev - curr < e_tol) err = 4;
+            if (err == 0 && istp == max_iter) err = 5;
+            force_any(ctx, 1, en);
+            ++n_force;
+            curr = en[0];
+            energies[istp + 1] = curr;
+            if (err) break;
+            double r2[2];
+            if (m_it) {
+                if (k_it == m_it) {
+                    const int t = sl[0];
+                    for (int i = 0; i < m_it - 1; ++i) {
+                        sl[i] = sl[i + 1];
+                        rh[i] = rh[i + 1];
+                    }
+                    sl[m_it - 1] = t;
+                } else {
+                    ++k_it;
+                }
+                hipLaunchKernelGGL(k_sy, g, b, 0, st, n, a_m, ctx->d_lh.p, ctx->d_f.p, ctx->d_lf0.p, Sv(k_it - 1),
+                                   Yv(k_it - 1));
+                launch_check("k_sy");
+                DotArgs D{{Sv(k_it - 1), Yv(k_it - 1), nullptr, nullptr}, {Yv(k_it - 1), Yv(k_it - 1), nullptr, nullptr}};
+                dots<2>(ctx, n, D, r2);
+                rh[k_it - 1] = 1.0 / r2[0];
+            } else {
+                // s and y into the first slots (no history is kept)
+                hipLaunchKernelGGL(k_sy, g, b, 0, st, n, a_m, ctx->d_lh.p, ctx->d_f.p, ctx->d_lf0.p, ctx->d_lS.p,
+                                   ctx->d_lY.p);
+                launch_check("k_sy");
+                DotArgs D{{ctx->d_lS.p, ctx->d_lY.p, nullptr, nullptr}, {ctx->d_lY.p, ctx->d_lY.p, nullptr, nullptr}};
+                dots<2>(ctx, n, D, r2);
+            }
+            ratio = -r2[0] / r2[1];
+            ++istp;
+        }
+        if (counts) {
+            counts[0] = istp;
+            counts[1] = n_force;
+            counts[2] = n_energy;
+            counts[3] = err;
+        }
         return 0;
     });
 }

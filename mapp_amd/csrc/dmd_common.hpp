@@ -21,6 +21,7 @@ struct FFParams {
     int natms, ntot, nr, nrho, no_i;
     T dr_inv, drho_inv, rho_max, rc, cut_sq, cut_sq_mod, alpha_min, alpha_max, inv_amin, inv_amax;
     T kbT, beta;
+    T xw;  // energy weight of cross-type pair terms: 1 (reference) or 2 (consistent energy)
     T xi[kMaxPoints], w0[kMaxPoints], w1[kMaxPoints], w2[kMaxPoints];
     T c0[kMaxTypes];
 };

@@ -429,7 +429,10 @@ __device__ __forceinline__ float dpp_xor1_q<float>(float v) {
 
 constexpr int kDppRowRor8 = 0x128;  // row_ror:8 (rotate by 8 lanes within a row of 16)
 
-template <int NT, typename Q, int NQ, bool ENERGY_ONLY>
+// ENERGY_ONLY: the loop of ForceField_DMD_I::energy_calc (src/ff_dmd_I.cpp:454-668),
+// type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
+// energy equals force_calc's.
+template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false>
 __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
     const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6));
     const int k = lane / LPP, q = (lane % LPP) >> 2, prt = lane & 3;
     const int it = q / NT, jt = q % NT;
-    const bool active = !ENERGY_ONLY || jt >= it;
+    const bool active = !ENERGY_ONLY || FULL || jt >= it;
     const bool want_phi = it >= jt;
     const bool rho_lane = prt == 0, phi_lane = prt == 2;
     double acc[2] = {0.0, 0.0};  // energy, pairs inside the quadrature cut
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                     ej = v[0] * ci[it];
                 }
                 if (phi_lane && want_phi && active) {
-                    const double half = j < P.natms ? 0.5 : 0.25;
+                    const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
                     en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v[0];
                 }
                 if (!ENERGY_ONLY && (rho_lane || phi_lane)) {

@@ -473,7 +473,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 Ei += g[0] * cj[jt];
                 ej = g[0] * ci[it];
                 if (want_phi) {
-                    const double half = j < P.natms ? 0.5 : 0.25;
+                    const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
                     en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * f[0];
                 }
                 double* d = deriv + ((size_t)p * NT * NT + tp) * 4;

@@ -56,6 +56,8 @@ _SIGS = {
     "mapp_dmd_get_gh": [CTX, DP, DP, DP, DP],
     "mapp_dmd_last_timing": [CTX, DP],
     "mapp_dmd_enable_timing": [CTX, C.c_int],
+    "mapp_dmd_set_energy_mode": [CTX, C.c_int],
+    "mapp_dmd_min_lbfgs": [CTX, C.c_int, C.c_int, D, DP, LP],
     "mapp_dmd_surrogate_train": [CTX, C.c_long, C.c_int, C.c_int, D, C.c_ulonglong, D, DP, DP],
     "mapp_dmd_surrogate_gradient": [CTX, FP, FP, C.c_long, FP, FP, FP, FP],
     "mapp_dmd_surrogate_params": [CTX, FP, FP],
@@ -255,6 +257,18 @@ class ForceFieldDMD:
         ms = np.zeros(5)
         self._call("mapp_dmd_last_timing", _dp(ms))
         return ms
+
+    def set_energy_mode(self, full):
+        """False: the reference's energy_calc type loop; True: every ordered type pair"""
+        self._call("mapp_dmd_set_energy_mode", 1 if full else 0)
+
+    def min_lbfgs(self, max_iter, m_it=2, e_tol=1e-8):
+        """Min_LBFGS::run on the device: (exit code, energies per step, force and energy call counts)"""
+        en = np.zeros(max_iter + 2)
+        cnt = np.zeros(4, dtype=np.int64)
+        self._call("mapp_dmd_min_lbfgs", int(max_iter), int(m_it), float(e_tol), _dp(en),
+                   cnt.ctypes.data_as(LP))
+        return int(cnt[3]), en[: cnt[0] + 2], int(cnt[1]), int(cnt[2])
 
     # learned surrogate of the pair quadrature (two types)
     def surrogate_train(self, n_per_tp=1 << 20, epochs=4, batch_per_tp=256, lr=2e-3, seed=1, tol=0.05, box=None):

@@ -41,8 +41,12 @@ class FSTables:
         return (self.nr - 1.0) * self.dr
 
 
-def johnson_tables(elements=("Ni", "Cu"), nr=10000, nrho=10000, rc=5.0, rho_max=40.0):
-    """Tabulate the alloy on nr/nrho-point grids with a smooth cutoff at rc."""
+def johnson_tables(elements=("Ni", "Cu"), nr=10000, nrho=10000, rc=5.0, rho_max=40.0, symmetric_density=False):
+    """Tabulate the alloy on nr/nrho-point grids with a smooth cutoff at rc.
+
+    rho[a][b] (density at a site of type a due to a neighbour of type b) is
+    the density function of b; symmetric_density=True uses the mean of the
+    two functions instead, so rho[a][b] == rho[b][a]."""
     nt = len(elements)
     dr = rc / (nr - 1)
     drho = rho_max / (nrho - 1)
@@ -64,7 +68,7 @@ def johnson_tables(elements=("Ni", "Cu"), nr=10000, nrho=10000, rc=5.0, rho_max=
     rho = np.zeros((nt, nt, nr))
     for a in range(nt):
         for b in range(nt):
-            rho[a, b] = dens[b]
+            rho[a, b] = 0.5 * (dens[a] + dens[b]) if symmetric_density else dens[b]
     rphi = np.zeros((nt * (nt + 1) // 2, nr))
     k = 0
     for i in range(nt):
@@ -144,9 +148,11 @@ def x2s(cart, B):
     return s
 
 
-def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0, c_per_site=None):
+def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0, c_per_site=None,
+              alpha_per_site=False):
     """n^3 FCC cells of lattice constant a; positions jittered N(0, jitter);
-    alpha ~ U[alpha_lo, alpha_hi] per (site, type); c_A ~ U[0,1], c_B = 1 - c_A."""
+    alpha ~ U[alpha_lo, alpha_hi] per (site, type) (one value per site, shared
+    by the types, when alpha_per_site); c_A ~ U[0,1], c_B = 1 - c_A."""
     rng = np.random.default_rng(seed)
     basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
     g = np.arange(n)
@@ -158,6 +164,8 @@ def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0
     s = x2s(cart, B)
     natms = s.shape[0]
     alpha = rng.uniform(alpha_lo, alpha_hi, size=(natms, nt))
+    if alpha_per_site:
+        alpha[:, 1:] = alpha[:, :1]
     if nt == 1:
         c = rng.uniform(0.3, 1.0, size=(natms, 1)) if c_per_site is None else np.full((natms, 1), c_per_site)
     else:

@@ -90,6 +90,8 @@ struct orc {
     double *rho, *drho_dr, *drho_dalpha, *phi, *dphi_dr, *dphi_dalpha;
     int fresh;
     int fix_dE_index; /* 1: dE[atom*nt+type] in the chemical-potential sweep */
+    int energy_full;  /* 1: consistent energy: energy_calc over all type pairs, cross-type
+                         pair terms weighted as in the forces (force_calc's energy too) */
     /* 2nd list */
     int* nl2_off;
     int* nl2;
@@ -97,6 +99,11 @@ struct orc {
 };
 
 static int type2rho(int nt, int i, int j) { return j * nt + i; }
+/* weight of a cross-type pair term in the energy: 1 in the reference
+   (0.5 (c c + c c) phi for itype > jtype, src/ff_dmd_I.cpp:244-255), 2 in the
+   consistent energy mode, where the energy is the one whose gradient the
+   forces are (the force loop weights each ordered type pair by c_i c_j, :375-380) */
+#define XW(o, it, jt) (((o)->energy_full && (it) != (jt)) ? 2.0 : 1.0)
 static int comp(int i, int j) { return i < j ? (j + 1) * j / 2 + i : (i + 1) * i / 2 + j; }
 
 orc_t* orc_create(int nt, int variant) {
@@ -136,6 +143,7 @@ void orc_destroy(orc_t* o) {
 
 void orc_set_fresh_caches(orc_t* o, int fresh) { o->fresh = fresh; }
 void orc_set_fix_dE_index(orc_t* o, int fix) { o->fix_dE_index = fix; }
+void orc_set_energy_full(orc_t* o, int full) { o->energy_full = full; }
 
 /* ---------------------------------------------------------------- tables */
 
@@ -750,9 +758,9 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
                                 E[(long)iatm * nt + it] += g[0] * cj;
                                 if (jatm < natms) {
                                     E[(long)jatm * nt + jt] += g[0] * ci;
-                                    if (it >= jt) ns[0] += 0.5 * pairc * fq[0];
+                                    if (it >= jt) ns[0] += 0.5 * XW(o, it, jt) * pairc * fq[0];
                                 } else {
-                                    if (it >= jt) ns[0] += 0.25 * pairc * fq[0];
+                                    if (it >= jt) ns[0] += 0.25 * XW(o, it, jt) * pairc * fq[0];
                                 }
                             }
                         } else if (o->alpha_max <= alpha) {
@@ -780,9 +788,9 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
                                 E[(long)iatm * nt + it] += o->rho[rho_start + rs] * cj;
                                 if (jatm < natms) {
                                     E[(long)jatm * nt + jt] += o->rho[rho_start + rs] * ci;
-                                    if (it >= jt) ns[0] += 0.5 * pairc * o->phi[phi_start + ps];
+                                    if (it >= jt) ns[0] += 0.5 * XW(o, it, jt) * pairc * o->phi[phi_start + ps];
                                 } else {
-                                    if (it >= jt) ns[0] += 0.25 * pairc * o->phi[phi_start + ps];
+                                    if (it >= jt) ns[0] += 0.25 * XW(o, it, jt) * pairc * o->phi[phi_start + ps];
                                 }
                             }
                         }
@@ -885,7 +893,7 @@ double orc_energy_calc(orc_t* o) {
             double r = sqrt(rsq);
             double r_inv = 1.0 / r;
             for (int it = 0; it < nt; it++)
-                for (int jt = it; jt < nt; jt++) {
+                for (int jt = o->energy_full ? 0 : it; jt < nt; jt++) {
                     double ai = x[icomp + 3 + it], aj = x[jcomp + 3 + jt];
                     double alpha = (ai * aj) / (ai + aj);
                     int rs = type2rho(nt, it, jt), ps = comp(it, jt);
@@ -925,9 +933,9 @@ double orc_energy_calc(orc_t* o) {
                             E[(long)iatm * nt + it] += g_0 * cj;
                             if (jatm < natms) {
                                 E[(long)jatm * nt + jt] += g_0 * ci;
-                                if (it >= jt) en += 0.5 * pairc * f_0;
+                                if (it >= jt) en += 0.5 * XW(o, it, jt) * pairc * f_0;
                             } else {
-                                if (it >= jt) en += 0.25 * pairc * f_0;
+                                if (it >= jt) en += 0.25 * XW(o, it, jt) * pairc * f_0;
                             }
                         }
                     } else if (o->alpha_max <= alpha) {
@@ -950,9 +958,9 @@ double orc_energy_calc(orc_t* o) {
                             E[(long)iatm * nt + it] += g_0 * cj;
                             if (jatm < natms) {
                                 E[(long)jatm * nt + jt] += g_0 * ci;
-                                if (it >= jt) en += 0.5 * pairc * f_0;
+                                if (it >= jt) en += 0.5 * XW(o, it, jt) * pairc * f_0;
                             } else {
-                                if (it >= jt) en += 0.25 * pairc * f_0;
+                                if (it >= jt) en += 0.25 * XW(o, it, jt) * pairc * f_0;
                             }
                         }
                     }
@@ -1479,6 +1487,157 @@ static double energy_calc_eam(orc_t* o) {
 }
 
 /* ------------------------------------------------------------ accessors */
+
+/* ------------------------------------------------------------------ L-BFGS */
+
+/* LineSearch_BackTrack::line_min without box change (src/line_search.cpp:244-341;
+   c = 0.4, rho = 0.5, alpha_max = 1, d_max = 1 from the constructor :214-224),
+   LineSearch::energy (:117-196), inner_f_h (:32-45), normalize_h (:64-76).
+   x = x_prev + a h over all (3+nt) columns, Atoms::update_0, energy_calc. */
+enum { ORC_LS_S = 0, ORC_LS_F_DOWNHILL = 1, ORC_LS_F_GRAD0 = 2, ORC_LS_F_ALPHAMIN = 3,
+       ORC_MIN_F_TOLERANCE = 4, ORC_MIN_F_MAX_ITER = 5 };
+
+static double ls_energy(orc_t* o, const double* x_prev, const double* h, double a, double* xbuf, long n) {
+    for (long i = 0; i < n; i++) xbuf[i] = x_prev[i] + a * h[i];
+    orc_set_x_update(o, xbuf, NULL);
+    return orc_energy_calc(o);
+}
+
+static int line_min(orc_t* o, const double* x_prev, double* h, double* xbuf, long n, double* nrgy,
+                    double* alph, int* n_energy) {
+    const double c = 0.4, rho = 0.5, a_max = 1.0, d_max = 1.0;
+    double a_min = 0.0;
+    double min_h = INFINITY;
+    for (int i = 0; i < o->natms; i++)
+        for (int j = 0; j < o->nt; j++) {
+            double v = o->x[(long)i * o->xdim + 3 + j];
+            min_h = v < min_h ? v : min_h;
+        }
+    min_h *= 0.99;
+    a_min = a_min < min_h ? a_min : min_h;
+    *alph = 0.0;
+    double hh = 0.0;
+    for (long i = 0; i < n; i++) hh += h[i] * h[i];
+    hh = d_max / sqrt(hh);
+    for (long i = 0; i < n; i++) h[i] *= hh;
+    double inner = 0.0;
+    for (long i = 0; i < n; i++) inner += o->f[i] * h[i];
+    if (inner <= 0) return ORC_LS_F_DOWNHILL;
+    double max_h = 0.0;
+    for (long i = 0; i < n; i++) max_h = fabs(h[i]) > max_h ? fabs(h[i]) : max_h;
+    if (max_h == 0) return ORC_LS_F_GRAD0;
+    double a = a_max < max_h ? a_max : max_h;
+    if (a <= a_min) return ORC_LS_F_ALPHAMIN;
+    while (1) {
+        double ideal = *nrgy - a * c * inner;
+        double cur = ls_energy(o, x_prev, h, a, xbuf, n);
+        (*n_energy)++;
+        if (cur <= ideal) {
+            *nrgy = cur;
+            *alph = a;
+            return ORC_LS_S;
+        }
+        a *= rho;
+        if (a <= a_min) {
+            *nrgy = ls_energy(o, x_prev, h, 0.0, xbuf, n);
+            (*n_energy)++;
+            return ORC_LS_F_ALPHAMIN;
+        }
+    }
+}
+
+/* Min_LBFGS::init tail + Min_LBFGS::run, fixed box (src/min_l-bfgs.cpp:274-289,
+   :302-323, :606-787): two-loop recursion over m_it pairs, ratio
+   -<s,y>/<y,y>, y = f - f_prev, s = alpha_m h; stops on a failed line search,
+   an energy decrease below e_tol or max_iter steps.  energies[k] is the
+   force_calc energy after step k (energies[0]: initial); returns the exit code. */
+int orc_min_lbfgs(orc_t* o, int max_iter, int m_it, double e_tol, double* energies, int* nsteps, int* n_force,
+                  int* n_energy) {
+    long n = (long)o->natms * o->xdim;
+    double* h = (double*)calloc(n, sizeof(double));
+    double* x0 = (double*)calloc(n, sizeof(double));
+    double* f0 = (double*)calloc(n, sizeof(double));
+    double* xb = (double*)calloc(n, sizeof(double));
+    double* S = (double*)calloc((size_t)n * (m_it > 0 ? m_it : 1), sizeof(double));
+    double* Y = (double*)calloc((size_t)n * (m_it > 0 ? m_it : 1), sizeof(double));
+    int* sl = (int*)calloc(m_it > 0 ? m_it : 1, sizeof(int));
+    double* rh = (double*)calloc(m_it > 0 ? m_it : 1, sizeof(double));
+    double* al = (double*)calloc(m_it > 0 ? m_it : 1, sizeof(double));
+    for (int i = 0; i < m_it; i++) sl[i] = i;
+    double en[7];
+    *n_force = 0;
+    *n_energy = 0;
+    orc_force_calc(o, 1, en);
+    (*n_force)++;
+    double curr = en[0], prev, ratio = 1.0, a_m = 0.0;
+    energies[0] = curr;
+    int k_it = 0, istp = 0, err = ORC_LS_S;
+    while (err == ORC_LS_S) {
+        memcpy(h, o->f, n * sizeof(double));
+        memcpy(x0, o->x, n * sizeof(double));
+        memcpy(f0, o->f, n * sizeof(double));
+        for (int i = k_it - 1; i > -1; i--) {
+            const double* s = S + (size_t)sl[i] * n;
+            const double* y = Y + (size_t)sl[i] * n;
+            double in = 0.0;
+            for (long j = 0; j < n; j++) in += h[j] * s[j];
+            al[i] = in * rh[i];
+            for (long j = 0; j < n; j++) h[j] -= al[i] * y[j];
+        }
+        for (long j = 0; j < n; j++) h[j] *= ratio;
+        for (int i = 0; i < k_it; i++) {
+            const double* s = S + (size_t)sl[i] * n;
+            const double* y = Y + (size_t)sl[i] * n;
+            double in = 0.0;
+            for (long j = 0; j < n; j++) in += h[j] * y[j];
+            in *= rh[i];
+            for (long j = 0; j < n; j++) h[j] -= (in + al[i]) * s[j];
+        }
+        prev = curr;
+        err = line_min(o, x0, h, xb, n, &curr, &a_m, n_energy);
+        if (err == ORC_LS_S && prev - curr < e_tol) err = ORC_MIN_F_TOLERANCE;
+        if (err == ORC_LS_S && istp == max_iter) err = ORC_MIN_F_MAX_ITER;
+        orc_force_calc(o, 1, en);
+        (*n_force)++;
+        curr = en[0];
+        energies[istp + 1] = curr;
+        if (err) break;
+        double in = 0.0, in2 = 0.0;
+        if (m_it) {
+            if (k_it == m_it) {
+                int t = sl[0];
+                for (int i = 0; i < m_it - 1; i++) {
+                    sl[i] = sl[i + 1];
+                    rh[i] = rh[i + 1];
+                }
+                sl[m_it - 1] = t;
+            } else {
+                k_it++;
+            }
+            double* s = S + (size_t)sl[k_it - 1] * n;
+            double* y = Y + (size_t)sl[k_it - 1] * n;
+            for (long i = 0; i < n; i++) {
+                s[i] = a_m * h[i];
+                y[i] = o->f[i] - f0[i];
+                in += s[i] * y[i];
+                in2 += y[i] * y[i];
+            }
+            ratio = -in / in2;
+            rh[k_it - 1] = 1.0 / in;
+        } else {
+            for (long i = 0; i < n; i++) {
+                in += (a_m * h[i]) * (o->f[i] - f0[i]);
+                in2 += (o->f[i] - f0[i]) * (o->f[i] - f0[i]);
+            }
+            ratio = -in / in2;
+        }
+        istp++;
+    }
+    *nsteps = istp;
+    free(h); free(x0); free(f0); free(xb); free(S); free(Y); free(sl); free(rh); free(al);
+    return err;
+}
+
 int orc_natms(const orc_t* o) { return o->natms; }
 int orc_nghost(const orc_t* o) { return o->nghost; }
 long orc_npairs(const orc_t* o) { return o->no_pairs; }

@@ -50,12 +50,24 @@ void orc_set_fresh_caches(orc_t* o, int fresh);
 /* 1: index dE by [atom*nt+type] in c_d_calc / calc_g instead of the reference's
    dE[jatm+jtype] (src/ff_dmd_I.cpp:2336,2339) -- a switch for tests only */
 void orc_set_fix_dE_index(orc_t* o, int fix);
+/* 1: consistent energy -- energy_calc sums every ordered type pair and both
+   energies weight cross-type pair terms as the force loop does, so for
+   symmetric density tables and one alpha per site f = -grad U;
+   0 (default): the reference (jtype >= itype loop in energy_calc,
+   src/ff_dmd_I.cpp:508; cross-type pair energy 0.5 (c c + c c) phi, :244-255) */
+void orc_set_energy_full(orc_t* o, int full);
 
 void orc_force_calc(orc_t* o, int st_clc, double* en_st);
 double orc_energy_calc(orc_t* o);
 void orc_create_2nd_neigh_lst(orc_t* o);
 void orc_c_d_calc(orc_t* o);
 double orc_calc_g(orc_t* o, int chk, double alpha, const double* a, double* g);
+
+/* Min_LBFGS::run with LineSearch_BackTrack on a fixed box; energies has
+   max_iter + 2 slots; returns the exit code (1 downhill, 2 grad0, 3 alpha_min,
+   4 energy tolerance, 5 max iterations) */
+int orc_min_lbfgs(orc_t* o, int max_iter, int m_it, double e_tol, double* energies, int* nsteps, int* n_force,
+                  int* n_energy);
 
 /* introspection */
 int orc_natms(const orc_t* o);

@@ -40,6 +40,7 @@ def lib():
         L.orc_set_fresh_caches.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_c.argtypes = [C.c_void_p, DP]
         L.orc_set_fix_dE_index.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_energy_full.argtypes = [C.c_void_p, C.c_int]
         L.orc_force_calc.argtypes = [C.c_void_p, C.c_int, DP]
         L.orc_energy_calc.argtypes = [C.c_void_p]
         L.orc_energy_calc.restype = D
@@ -70,6 +71,7 @@ def lib():
         L.orc_x2s.argtypes = [DP, C.c_int, C.c_int, DP]
         L.orc_s2x.argtypes = [DP, C.c_int, C.c_int, DP]
         L.orc_pair_quadrature.argtypes = [C.c_void_p, D, D, C.c_int, C.c_int, DP]
+        L.orc_min_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, D, DP, IP, IP, IP]
         _lib = L
     return _lib
 
@@ -130,6 +132,9 @@ class Oracle:
 
     def set_fix_dE_index(self, fix):
         self.L.orc_set_fix_dE_index(self.o, 1 if fix else 0)
+
+    def set_energy_full(self, full):
+        self.L.orc_set_energy_full(self.o, 1 if full else 0)
 
     def set_c(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
@@ -215,6 +220,13 @@ class Oracle:
         nc = C.c_int(0)
         ptr = self.L.orc_table(self.o, kind, slot, C.byref(nr), C.byref(nc))
         return _view(ptr, nr.value * nc.value, np.float64).reshape(nr.value, nc.value)
+
+    def min_lbfgs(self, max_iter, m_it=2, e_tol=1e-8):
+        """Min_LBFGS::run on the fixed box: (exit code, energies per step, force and energy call counts)"""
+        en = np.zeros(max_iter + 2)
+        ns, nf, ne = C.c_int(0), C.c_int(0), C.c_int(0)
+        code = self.L.orc_min_lbfgs(self.o, max_iter, m_it, e_tol, _dp(en), C.byref(ns), C.byref(nf), C.byref(ne))
+        return code, en[: ns.value + 2], nf.value, ne.value
 
     def pair_quadrature(self, r, alpha, rho_slot, phi_slot=-1):
         out = np.zeros(6)
