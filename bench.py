@@ -6,11 +6,15 @@ whole system with inputs resident in HBM: binary FCC Ni-Cu, 64^3 cells
 Gauss-Hermite rule, FP64 quadrature.  value = sites / step time.
 
 Also reported on the same system: the mixed path (5-point rule, FP32
-quadrature sums) and, when trained weights are present, the surrogate path;
-the roofline of the dominant kernel (pass 2, event-timed on the library's
-stream); and the FP64 CPU oracle timed on a bounded sample (32^3 cells).
+quadrature sums), the learned surrogate (trained here on 4M quadrature
+samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
+with neighbour rebuilds (consistent energy mode), the roofline of the
+dominant kernel (pass 1, timed with HIP events on the library's stream) and
+the FP64 CPU oracle timed on a bounded sample (32^3 cells, one thread).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--no-surrogate] [--no-lbfgs] [--fp64-only]
+    python bench.py --copy-calibration   (plain copy kernel for HBM counter scaling)
 """
 import argparse
 import json
@@ -25,17 +29,23 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def flops_per_site(P, nt, q):
-    """Algorithmic FP64 operations of one site-evaluation of the reference's
-    force_calc: per in-range pair nt^2 density quadratures (20 + 19 q), nt(nt+1)/2
-    pair quadratures (20 + 12 q) and nt^2 back-pass terms (30); 40 per (site,type)."""
-    return P * (nt * nt * (20 + 19 * q) + nt * (nt + 1) // 2 * (20 + 12 * q) + 30 * nt * nt) + 40 * nt
+def pass1_flops_per_site(P, nt, q):
+    """Algorithmic FP64 operations of pass 1 per site (the reference's force_calc
+    quadrature): per in-range pair nt^2 density quadratures of (20 + 19 q) and
+    nt(nt+1)/2 pair quadratures of (20 + 12 q) operations."""
+    return P * (nt * nt * (20 + 19 * q) + nt * (nt + 1) // 2 * (20 + 12 * q))
+
+
+def site_flops(P, nt, q):
+    """whole force_calc per site: pass 1 + back-pass (30 per pair and type pair) + embedding (40 per type)"""
+    return pass1_flops_per_site(P, nt, q) + P * 30 * nt * nt + 40 * nt
 
 
 def bytes_per_site(P_list, nt):
@@ -43,9 +53,7 @@ def bytes_per_site(P_list, nt):
     return 8 * (3 + nt) + 8 * nt + 8 * (3 + nt) + 16 * nt + 8 * nt + 2 * 4 * P_list
 
 
-def run_path(lat, tables, no_i, precision, steps, warmup):
-    from mapp_amd.dmd import setup
-    ff = setup(lat, tables, no_i=no_i, precision=precision)
+def time_force(ff, steps, warmup):
     ff.enable_timing(True)
     for _ in range(warmup):
         ff.force_calc(1)
@@ -55,10 +63,47 @@ def run_path(lat, tables, no_i, precision, steps, warmup):
         en = ff.force_calc(1)
         ms_kernels += ff.last_timing()
     t1 = time.perf_counter()
-    sz = ff.sizes()
+    return dict(ms_per_step=(t1 - t0) * 1e3 / steps, kernel_ms=(ms_kernels / steps).tolist(), energy=float(en[0]))
+
+
+def run_path(lat, tables, no_i, precision, steps, warmup):
+    from mapp_amd.dmd import setup
+    ff = setup(lat, tables, no_i=no_i, precision=precision)
+    r = time_force(ff, steps, warmup)
+    r["sizes"] = ff.sizes()
+    r["f"] = ff.get("f")
     ff.close()
-    return dict(ms_per_step=(t1 - t0) * 1e3 / steps, kernel_ms=(ms_kernels / steps).tolist(), energy=float(en[0]),
-                sizes=sz)
+    return r
+
+
+def run_surrogate(lat, tables, steps, warmup, n_train, f_ref):
+    """train on n_train quadrature samples (4 type pairs), then time force_calc"""
+    from mapp_amd.dmd import setup
+    ff = setup(lat, tables, no_i=5, precision="mixed")
+    t0 = time.perf_counter()
+    rep = ff.surrogate_train(n_per_tp=n_train // 4, epochs=2, batch_per_tp=1024, lr=2e-3, seed=3, tol=0.05)
+    train_s = time.perf_counter() - t0
+    ff.set_precision("surrogate")
+    r = time_force(ff, steps, warmup)
+    f = ff.get("f")
+    r.update(train_seconds=train_s, train_report=rep, fallbacks=ff.surrogate_fallbacks(),
+             force_rel_err_vs_fp64=float(np.max(np.abs(f - f_ref)) / np.max(np.abs(f_ref))))
+    ff.close()
+    return r
+
+
+def run_lbfgs(lat, tables, iters):
+    """Min_LBFGS on the device from the same start, consistent energy mode, no energy tolerance"""
+    from mapp_amd.dmd import setup
+    ff = setup(lat, tables, no_i=3)
+    ff.set_energy_mode(True)
+    t0 = time.perf_counter()
+    code, en, nf, ne = ff.min_lbfgs(iters - 1, 2, -1.0)
+    dt = time.perf_counter() - t0
+    ff.close()
+    steps = len(en) - 1
+    return dict(steps=steps, exit_code=code, force_calls=nf, energy_calls=ne, seconds=dt,
+                ms_per_step=dt * 1e3 / max(steps, 1), energy_first=float(en[0]), energy_last=float(en[-1]))
 
 
 def cpu_baseline(tables, n=32, reps=3):
@@ -77,6 +122,28 @@ def cpu_baseline(tables, n=32, reps=3):
                        f"median of {reps}, 1 thread")
 
 
+def copy_calibration(mib=1024, reps=5):
+    """a plain device copy of known size, profiled with the same HBM counters as the kernels"""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "reps": reps}))
+
+
+def traffic_for(workload):
+    """HBM bytes per pass-1 launch from the committed counter summary, if it covers this workload"""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return t.get(workload)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,29 +151,38 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-lbfgs", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and this path keeps one rank's whole "
               f"domain on one GPU (requested --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from mapp_amd import workload as W
     tables = W.johnson_tables()
     lat = W.fcc_alloy(args.cells, seed=1)
     nt = lat.c.shape[1]
-    log(f"system: {lat.natms} sites")
+    natms = lat.natms
+    log(f"system: {natms} sites")
 
     r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup)
-    log("fp64", r64)
-    natms = lat.natms
+    log("fp64", {k: v for k, v in r64.items() if k != "f"})
     value = natms / (r64["ms_per_step"] / 1e3)
     P = r64["sizes"]["n_inrange"] / natms
     P_list = r64["sizes"]["npairs"] / natms
-    pass2_ms = r64["kernel_ms"][2]
-    # pass 2 carries the reference's quadrature-with-derivatives and back-pass work
-    f_site = flops_per_site(P, nt, 3) - 40 * nt
-    achieved = f_site * natms / (pass2_ms / 1e3) / 1e12
+    names = ["pass1", "embed", "pass2", "gather_reduce"]
+    kms = dict(zip(names, r64["kernel_ms"][:4]))
+    pass1_ms = kms["pass1"]
+    f1 = pass1_flops_per_site(P, nt, 3)
+    achieved = f1 * natms / (pass1_ms / 1e3) / 1e12
+    workload = f"DMD-I force_calc, FCC Ni-Cu {args.cells}^3 cells ({natms} sites), 3-point Gauss-Hermite, FP64"
+    traffic = traffic_for(f"k_pass1 fp64 q3 {args.cells}^3")
     out = {
         "metric": "DMD energy+grad site-evals/s at 1M sites: surrogate/FP32 vs FP64 quadrature",
         "value": value,
@@ -120,24 +196,44 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"DMD-I force_calc, FCC Ni-Cu {args.cells}^3 cells ({natms} sites), "
-                               f"3-point Gauss-Hermite, FP64 quadrature",
-                   "sites": natms, "pairs_half_list": r64["sizes"]["npairs"],
+        "config": {"workload": workload, "sites": natms, "pairs_half_list": r64["sizes"]["npairs"],
                    "pairs_in_quadrature_cut": r64["sizes"]["n_inrange"]},
-        "roofline": {"bound": "mfma", "pipe": "fp64 (vector and matrix peak both 78.6 TFLOP/s)",
-                     "kernel": "k_pass2", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_site": f_site, "kernel_ms": pass2_ms},
-        "kernel_ms": dict(zip(["pass1", "embed", "pass2", "gather_reduce", "total"], r64["kernel_ms"])),
+        "roofline": {"bound": "mfma", "pipe": "fp64 vector (78.6 TFLOP/s, same as the fp64 matrix peak)",
+                     "kernel": "k_pass1", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": None if traffic is None else traffic["bytes_per_launch"],
+                     "flops_per_site": f1, "kernel_ms": pass1_ms,
+                     "hbm_gbs": None if traffic is None else traffic["bytes_per_launch"] / (pass1_ms / 1e3) / 1e9},
+        "kernel_ms": kms,
+        "site_flops": site_flops(P, nt, 3),
+        "site_bytes": bytes_per_site(P_list, nt),
     }
     if not args.fp64_only:
         rmix = run_path(lat, tables, 5, "mixed", args.steps, args.warmup)
-        log("mixed", rmix)
-        out["paths"] = {
+        f_ref = r64["f"]
+        r64q5 = run_path(lat, tables, 5, "fp64", max(2, args.steps // 4), 1)
+        paths = {
             "fp64_q3": {"site_evals_per_s": value, "ms_per_step": r64["ms_per_step"]},
+            "fp64_q5": {"site_evals_per_s": natms / (r64q5["ms_per_step"] / 1e3), "ms_per_step": r64q5["ms_per_step"],
+                        "kernel_ms": r64q5["kernel_ms"][:4]},
             "mixed_fp32_q5": {"site_evals_per_s": natms / (rmix["ms_per_step"] / 1e3),
-                              "ms_per_step": rmix["ms_per_step"], "kernel_ms": rmix["kernel_ms"]},
+                              "ms_per_step": rmix["ms_per_step"], "kernel_ms": rmix["kernel_ms"][:4],
+                              "force_rel_err_vs_fp64_q5": float(np.max(np.abs(rmix["f"] - r64q5["f"])) /
+                                                                np.max(np.abs(r64q5["f"])))},
         }
+        if not args.no_surrogate:
+            rs = run_surrogate(lat, tables, args.steps, args.warmup, 4 << 20, r64q5["f"])
+            log("surrogate", rs)
+            paths["surrogate_q5_fallback"] = {
+                "site_evals_per_s": natms / (rs["ms_per_step"] / 1e3), "ms_per_step": rs["ms_per_step"],
+                "kernel_ms": rs["kernel_ms"][:4], "force_rel_err_vs_fp64_q5": rs["force_rel_err_vs_fp64"],
+                "fallback_pair_evals": rs["fallbacks"], "train_seconds": rs["train_seconds"],
+                "train_report": rs["train_report"],
+                "speedup_vs_fp64_q5": r64q5["ms_per_step"] / rs["ms_per_step"]}
+        out["paths"] = paths
+        if not args.no_lbfgs:
+            out["lbfgs"] = run_lbfgs(lat, tables, 50)
+            log("lbfgs", out["lbfgs"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tables)
     print(json.dumps(out), flush=True)
