@@ -157,6 +157,20 @@ int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full);
    below e_tol, 5 max_iter reached). */
 int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, double* energies, long* counts);
 
+/* Extended CFG files of DMD runs (host only, no context, no GPU).
+   mapp_dmd_cfg_read follows ReadCFG (src/read_cfg.cpp:15-178 header and atom
+   loop, :228-350 read_header, :356-418 set_box without eta, :562-657
+   read_atom_dmd/add_atom_read_x): H comes back lower triangular (row-major),
+   x as [natms][3+nt] reduced position wrapped into [0,1) and alpha, c as
+   [natms][nt], mass[nt], names '\n'-separated.  With x == NULL only natms and
+   nt are filled.  mapp_dmd_cfg_write writes the layout of
+   WriteCFG::write_file_dmd (src/write_cfg.cpp:262-433) with %.17g, so a round
+   trip is exact.  err receives the message of a failed call. */
+int mapp_dmd_cfg_read(const char* path, int cap_atoms, int* natms, int* nt, double* H, double* x, double* c,
+                      double* mass, char* names, int names_cap, char* err, int err_cap);
+int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, const double* x, const double* c,
+                       int ntypes, const double* mass, const char* const* names, char* err, int err_cap);
+
 /* Learned surrogate (no counterpart in the reference; BASELINE cfg3 workload):
    one MLP 3->64->64->64->6 (SiLU) per ordered type pair maps (r, alpha_i,
    alpha_j) to {rho_bar, drho_bar/dr, drho_bar/dalpha, phi_bar, dphi_bar/dr,

@@ -14,6 +14,7 @@
 #include "eam.hpp"
 #include "surrogate.hpp"
 #include "lbfgs.hpp"
+#include "cfg.hpp"
 #include <chrono>
 #include <numeric>
 #include <random>
@@ -1604,6 +1605,68 @@ int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, do
         }
         return 0;
     });
+}
+
+
+// ------------------------------------------------------------------ extended CFG (host only)
+namespace {
+void put_err(char* err, int cap, const std::string& m) {
+    if (err && cap > 0) {
+        std::strncpy(err, m.c_str(), (size_t)cap - 1);
+        err[cap - 1] = '\0';
+    }
+}
+}  // namespace
+
+int mapp_dmd_cfg_read(const char* path, int cap_atoms, int* natms, int* nt, double* H, double* x, double* c,
+                      double* mass, char* names, int names_cap, char* err, int err_cap) {
+    if (!path || !natms || !nt) return MAPP_DMD_E_ARG;
+    CfgData d;
+    const std::string e = read_cfg_file(path, d);
+    if (!e.empty()) {
+        put_err(err, err_cap, e);
+        return MAPP_DMD_E_IO;
+    }
+    *natms = d.natms;
+    *nt = d.nt;
+    if (!x) return MAPP_DMD_OK;  // sizes only
+    if (cap_atoms < d.natms) {
+        put_err(err, err_cap, "buffer holds fewer atoms than the file");
+        return MAPP_DMD_E_RANGE;
+    }
+    if (H) std::memcpy(H, d.H, sizeof(d.H));
+    std::memcpy(x, d.x.data(), d.x.size() * sizeof(double));
+    if (c) std::memcpy(c, d.c.data(), d.c.size() * sizeof(double));
+    if (mass)
+        for (int k = 0; k < d.nt; ++k) mass[k] = k < (int)d.mass.size() ? d.mass[k] : 0.0;
+    if (names && names_cap > 0) {
+        std::string all;
+        for (size_t k = 0; k < d.names.size(); ++k) all += (k ? "\n" : "") + d.names[k];
+        put_err(names, names_cap, all);
+    }
+    return MAPP_DMD_OK;
+}
+
+int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, const double* x, const double* c,
+                       int ntypes, const double* mass, const char* const* names, char* err, int err_cap) {
+    if (!path || !H || !x || !c || natms < 0 || nt < 1 || ntypes < 0 || (ntypes && (!mass || !names)))
+        return MAPP_DMD_E_ARG;
+    CfgData d;
+    d.natms = natms;
+    d.nt = nt;
+    std::memcpy(d.H, H, sizeof(d.H));
+    d.x.assign(x, x + (size_t)natms * (3 + nt));
+    d.c.assign(c, c + (size_t)natms * nt);
+    for (int k = 0; k < ntypes; ++k) {
+        d.mass.push_back(mass[k]);
+        d.names.emplace_back(names[k]);
+    }
+    const std::string e = write_cfg_file(path, d);
+    if (!e.empty()) {
+        put_err(err, err_cap, e);
+        return MAPP_DMD_E_IO;
+    }
+    return MAPP_DMD_OK;
 }
 
 int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count) {

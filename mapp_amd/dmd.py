@@ -65,6 +65,13 @@ _SIGS = {
 }
 
 
+_NOCTX_SIGS = {
+    "mapp_dmd_cfg_read": [C.c_char_p, C.c_int, IP, IP, DP, DP, DP, DP, C.c_char_p, C.c_int, C.c_char_p, C.c_int],
+    "mapp_dmd_cfg_write": [C.c_char_p, DP, C.c_int, C.c_int, DP, DP, C.c_int, DP, C.POINTER(C.c_char_p),
+                           C.c_char_p, C.c_int],
+}
+
+
 class DMDError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mapp_dmd error {code}: {msg}")
@@ -80,6 +87,10 @@ def load_library(path=LIB_PATH):
         raise FileNotFoundError(f"{path} is missing: build it with `python -m mapp_amd.build`")
     lib = C.CDLL(path)
     for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    for name, args in _NOCTX_SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -300,6 +311,43 @@ class ForceFieldDMD:
         n = C.c_long(0)
         self._call("mapp_dmd_surrogate_fallbacks", C.byref(n))
         return n.value
+
+
+def read_cfg(path):
+    """extended CFG of a DMD run -> (H lower triangular, x [natms][3+nt] reduced + alpha, c, masses, names)"""
+    lib = load_library()
+    natms, nt = C.c_int(0), C.c_int(0)
+    err = C.create_string_buffer(512)
+    rc = lib.mapp_dmd_cfg_read(path.encode(), 0, C.byref(natms), C.byref(nt), None, None, None, None, None, 0, err, 512)
+    if rc != 0:
+        raise DMDError(rc, err.value.decode())
+    n, t = natms.value, nt.value
+    H = np.zeros((3, 3))
+    x = np.zeros((n, 3 + t))
+    c = np.zeros((n, t))
+    mass = np.zeros(t)
+    names = C.create_string_buffer(4096)
+    rc = lib.mapp_dmd_cfg_read(path.encode(), n, C.byref(natms), C.byref(nt), _dp(H), _dp(x), _dp(c), _dp(mass),
+                               names, 4096, err, 512)
+    if rc != 0:
+        raise DMDError(rc, err.value.decode())
+    nm = names.value.decode().split("\n") if names.value else []
+    return H, x, c, mass[: len(nm)], nm
+
+
+def write_cfg(path, H, x, c, masses, names):
+    """write an extended CFG (x: reduced positions + alpha) with %.17g"""
+    lib = load_library()
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    err = C.create_string_buffer(512)
+    rc = lib.mapp_dmd_cfg_write(path.encode(), _dp(H), x.shape[0], c.shape[1], _dp(x), _dp(c), len(names), _dp(m),
+                                arr, err, 512)
+    if rc != 0:
+        raise DMDError(rc, err.value.decode())
 
 
 def setup_eam(H, s3, types, tables, skin=0.5, device=0):
