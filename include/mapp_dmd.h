@@ -130,7 +130,8 @@ int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx);
 int mapp_dmd_c_d_calc(mapp_dmd_t* ctx);
 /* virtual TYPE0 calc_g(int chk, TYPE0 alpha, TYPE0* a, TYPE0* g)  (src/ff.h:28;
    src/ff_dmd_I.cpp:2402-2674, src/ff_dmd_II.cpp:2341-2630).  a and g are
-   DEVICE pointers of natms*nt doubles; g is written only when chk == 0. */
+   DEVICE pointers of natms*nt doubles, or NULL for the library's own buffers
+   (MAPP_DMD_VEC_A / MAPP_DMD_VEC_G); g is written only when chk == 0. */
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
                     double* g_dev, double* cost);
 
@@ -156,6 +157,23 @@ int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full);
    (1 not downhill, 2 zero gradient, 3 step below alpha_min, 4 energy change
    below e_tol, 5 max_iter reached). */
 int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, double* energies, long* counts);
+
+/* Clock_BDF::init + Clock_BDF::run (src/clock_bdf.cpp:201-275, :479-672)
+   on the device: force_calc, create_2nd_neigh_lst and c_d_calc, then nsteps
+   steps of the variable-step, variable-order (up to max_order) BDF method on
+   c with the nonlinear conjugate-gradient solve of calc_g's cost
+   (Clock_BDF::solve, :677-883) and the order estimates of hi_lo_err_est
+   (:372-478).  History, residual and CG vectors stay in HBM; dot products
+   are FP64 deterministic device reductions.  initial_del_t <= 0 takes the
+   reference's a_tol / rms(c_d) rule.  Deviations: solve's uninitialised
+   curr_t is t[0] + del_t; a step retried more than max_retry times ends the
+   run (the reference loops).  flags bit 0 corrects the sign of the
+   (t_new - t[0]) c1 c2 term of dalpha_y[0] (:327; the reference's corrector
+   then drops y_n for orders >= 2); default off.  report[6*nsteps]: per step order, del_t, cost
+   ratio, error estimate, CG iterations, solve calls; counts[2]: steps taken
+   (or -1 when c left [0,1], -2 after too many retries), calc_g calls. */
+int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_del_t, int max_retry, int flags,
+                     double* report, long* counts);
 
 /* Extended CFG files of DMD runs (host only, no context, no GPU).
    mapp_dmd_cfg_read follows ReadCFG (src/read_cfg.cpp:15-178 header and atom

@@ -15,6 +15,7 @@
 #include "surrogate.hpp"
 #include "lbfgs.hpp"
 #include "cfg.hpp"
+#include "bdf.hpp"
 #include <chrono>
 #include <numeric>
 #include <random>
@@ -942,6 +943,227 @@ int line_min(mapp_dmd* ctx, long n, double* nrgy, double* alph, long* n_energy) 
         }
     }
 }
+
+// ------------------------------------------------------------------ BDF clock
+template <int NT>
+void c_d_variant(mapp_dmd* ctx);
+template <int NT>
+double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g);
+
+struct BdfState {
+    mapp_dmd* ctx;
+    long n;
+    int nt;
+    double t[kBdfMaxOrder + 1] = {0};
+    double* y[kBdfMaxOrder] = {nullptr};
+    double *dy, *a, *y0, *g, *g0, *h, *c0, *e_n;
+    double beta = 0.0;
+    long n_calc_g = 0;
+    bool fix = false;  // correct the sign of the c1 c2 term of dalpha_y[0]
+
+    double calc_g(int chk) {
+        ++n_calc_g;
+        return nt == 1 ? calc_g_variant<1>(ctx, chk, beta, a, g) : calc_g_variant<2>(ctx, chk, beta, a, g);
+    }
+    void set_c_from(const double* src) {
+        if (src != ctx->d_conc.p)
+            HIP_OK(hipMemcpyAsync(ctx->d_conc.p, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        ghost_copy(ctx, ctx->d_conc.p, nt);
+    }
+    double min_blocks(const double* part, int nb, double init) {
+        std::vector<double> h_(nb);
+        HIP_OK(hipMemcpyAsync(h_.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        double r = init;
+        for (double v : h_) r = v < r ? v : r;
+        return r;
+    }
+    double bound(const double* base, const double* dir, double scale, double init) {
+        ctx->d_part.reserve((size_t)kVecBlocks * 8 + 16);
+        hipLaunchKernelGGL(k_bdf_bound, dim3(kVecBlocks), dim3(kBlock), 0, ctx->stream, n, base, dir, scale,
+                           ctx->d_part.p);
+        launch_check("k_bdf_bound");
+        return min_blocks(ctx->d_part.p, kVecBlocks, init);
+    }
+    double dot(const double* u, const double* v) { return dot1(ctx, n, u, v); }
+
+    // Clock_BDF::interpolate (src/clock_bdf.cpp:295-369)
+    int interpolate(int q, double del_t) {
+        double alpha_y[kBdfMaxOrder], dalpha_y[kBdfMaxOrder];
+        const double t_new = t[0] + del_t;
+        double c0_ = 1.0, c1 = 0.0, c2 = 0.0;
+        for (int i = 1; i < q; ++i) {
+            double tmp0 = (t_new - t[0]) / (t[i] - t[0]);
+            double tmp1 = tmp0 * tmp0;
+            for (int j = 1; j < q; ++j)
+                if (i != j) tmp1 *= (t_new - t[j]) / (t[i] - t[j]);
+            alpha_y[i] = tmp1;
+            c0_ *= (t_new - t[i]) / (t[0] - t[i]);
+            c1 += 1.0 / (t[0] - t[i]);
+            c2 += 1.0 / (t_new - t[i]);
+        }
+        alpha_y[0] = (1.0 - (t_new - t[0]) * c1) * c0_;
+        const double alpha_dy_0 = c0_ * (t_new - t[0]);
+        const double c3 = 2.0 / (t_new - t[0]);
+        const double dalpha_dy_0 = alpha_dy_0 * (c2 + 0.5 * c3);
+        // the reference adds the (t_new - t[0]) c1 c2 term (src/clock_bdf.cpp:327); the
+        // derivative of alpha_y[0] subtracts it
+        dalpha_y[0] = fix ? c0_ * (c2 - c1 - (t_new - t[0]) * c1 * c2) : c0_ * (c2 - c1 + (t_new - t[0]) * c1 * c2);
+        for (int i = 1; i < q; ++i) dalpha_y[i] = alpha_y[i] * (c2 + c3 - 1.0 / (t_new - t[i]));
+        double tmp0 = 0.0;
+        for (int i = 0; i < q; ++i) tmp0 += 1.0 / (double)(i + 1);
+        beta = del_t / tmp0;
+        BdfInterp C{};
+        C.b_ady0 = beta * dalpha_dy_0;
+        C.ady0 = alpha_dy_0;
+        BdfHist Y{};
+        for (int j = 0; j < q; ++j) {
+            C.b_ay[j] = beta * dalpha_y[j];
+            C.ay[j] = alpha_y[j];
+            Y.y[j] = y[j];
+        }
+        HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_bdf_interp, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, n, q, C, Y, dy, a, y0,
+                           ctx->d_iflag.p);
+        launch_check("k_bdf_interp");
+        int flag = 0;
+        HIP_OK(hipMemcpyAsync(&flag, ctx->d_iflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        return flag ? -1 : 1;
+    }
+
+    // Clock_BDF::solve (src/clock_bdf.cpp:677-883), curr_t = t[0] + del_t; -1 if c left [0,1]
+    double solve(int q, double del_t, double* err, int* iters) {
+        const double min_gamma = 1.0e-10, gamma_red = 0.8, slope = 0.4, m_tol = 1.0e-10, a_tol = 1.0e-6;
+        const int max_iter = 50;
+        hipStream_t st = ctx->stream;
+        double* c = ctx->d_conc.p;
+        const double ratio0 = bound(y[0], dy, del_t, 1.0);
+        hipLaunchKernelGGL(k_bdf_pred, dim3(nblk(n)), dim3(kBlock), 0, st, n, y[0], dy, del_t, ratio0, c);
+        launch_check("k_bdf_pred");
+        set_c_from(c);
+        double curr_cost = calc_g(0);
+        HIP_OK(hipMemcpyAsync(h, g, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        double g0_g0 = dot(g, g), g_h = g0_g0, max_gamma = 1.0;
+        int iter = 0;
+        while (curr_cost > m_tol * (double)n && iter < max_iter && max_gamma > min_gamma) {
+            HIP_OK(hipMemcpyAsync(g0, g, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(c0, c, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            max_gamma = bound(c0, h, 1.0, 0.99) * 0.9;
+            int chk = 1;
+            const double cost = curr_cost;
+            while (chk && max_gamma > min_gamma) {
+                hipLaunchKernelGGL(k_step, dim3(nblk(n)), dim3(kBlock), 0, st, n, c0, max_gamma, h, c);
+                launch_check("k_step");
+                set_c_from(c);
+                curr_cost = calc_g(0);
+                const double ideal = cost - slope * max_gamma * g_h;
+                if (curr_cost < ideal) chk = 0;
+                max_gamma *= gamma_red;
+                if (max_gamma <= min_gamma) {
+                    set_c_from(c0);
+                    curr_cost = calc_g(1);
+                }
+            }
+            if (chk == 0) {
+                double r2[2];
+                DotArgs D{{g, g, nullptr, nullptr}, {g0, g, nullptr, nullptr}};
+                dots<2>(ctx, n, D, r2);
+                const double g_g0 = r2[0], g_g = r2[1];
+                const double r = (g_g - g_g0) / g0_g0;
+                g0_g0 = g_g;
+                hipLaunchKernelGGL(k_axpby, dim3(nblk(n)), dim3(kBlock), 0, st, n, 1.0, g, r, h);
+                launch_check("k_axpby");
+                g_h = dot(h, g);
+                if (g_h < 0.0) {
+                    HIP_OK(hipMemcpyAsync(h, g, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    g_h = g_g;
+                }
+            }
+            ++iter;
+        }
+        *iters = iter;
+        const double curr_t = t[0] + del_t;
+        double a_1 = 1.0, l1_q = 0.0;
+        for (int i = 0; i < q; ++i) {
+            l1_q += 1.0 / (curr_t - t[i]);
+            if (i != 0) a_1 *= (curr_t - t[i]) / (t[0] - t[i]);
+        }
+        const double err_pre_fac = -1.0 / (l1_q * (1.0 + a_1));
+        ctx->d_part.reserve((size_t)kVecBlocks * 8 + 16);
+        hipLaunchKernelGGL(k_bdf_err, dim3(kVecBlocks), dim3(kBlock), 0, st, n, y0, c, ctx->d_cd.p, ctx->d_part.p);
+        launch_check("k_bdf_err");
+        reduce_partials(ctx, ctx->d_part.p, kVecBlocks, 2, 2, 0, ctx->d_out.p);
+        double e2[2];
+        HIP_OK(hipMemcpyAsync(e2, ctx->d_out.p, sizeof(e2), hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_minmax, dim3(kVecBlocks), dim3(kBlock), 0, st, n, c, ctx->d_part.p);
+        launch_check("k_minmax");
+        std::vector<double> mm((size_t)kVecBlocks * 2);
+        HIP_OK(hipMemcpyAsync(mm.data(), ctx->d_part.p, mm.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        *err = std::sqrt(e2[0] / ((double)n * a_tol)) * std::fabs(err_pre_fac);
+        for (int b = 0; b < kVecBlocks; ++b)
+            if (mm[b * 2] < 0.0 || mm[b * 2 + 1] > 1.0) return -1.0;
+        return curr_cost / (m_tol * (double)n);
+    }
+
+    // Clock_BDF::hi_lo_err_est (src/clock_bdf.cpp:372-478)
+    void hi_lo(int q, double del_t, bool hi_avail, bool lo_avail, double* lo_err, double* hi_err) {
+        const double a_tol = 1.0e-6;
+        const double curr_t = t[0] + del_t;
+        double a_0 = 1.0, a_1 = 1.0, l1_q = 0.0;
+        for (int i = 0; i < q; ++i) {
+            l1_q += 1.0 / (curr_t - t[i]);
+            a_0 *= curr_t - t[i];
+            if (i != 0) a_1 *= (curr_t - t[i]) / (t[0] - t[i]);
+        }
+        BdfHiLo H{};
+        H.hi = hi_avail;
+        H.lo = lo_avail;
+        H.q = q;
+        H.Q_n = 1.0;
+        H.hi_pre = 1.0;
+        if (hi_avail) {
+            double hi_a_0 = 1.0, hi_a_1 = 1.0;
+            for (int i = 0; i < q; ++i) {
+                hi_a_0 *= t[0] - t[i + 1];
+                if (i != 0) hi_a_1 *= (t[0] - t[i + 1]) / (t[1] - t[i + 1]);
+            }
+            H.Q_n = (a_0 / hi_a_0) * (1.0 + a_1) / (1.0 + hi_a_1) * (del_t / (t[0] - t[1]));
+            const double hi_l1_q = l1_q + 1.0 / (curr_t - t[q]);
+            H.hi_pre = -(curr_t - t[q]) / (q + 2);
+            H.hi_pre *= 1.0 / (hi_l1_q * (1.0 + a_1) * del_t * del_t);
+        }
+        BdfHist Y{};
+        if (lo_avail) {
+            const double lo_pre = -del_t * a_0 / (curr_t - t[q - 1]);
+            double c0_ = 0.0, c1 = 1.0;
+            for (int i = 0; i < q - 1; ++i) {
+                const double tmp0 = 1.0 / (curr_t - t[i]);
+                c0_ += tmp0;
+                c1 *= tmp0;
+                H.la[i] = tmp0 * tmp0;
+                for (int j = 0; j < q - 1; ++j)
+                    if (i != j) H.la[i] *= 1.0 / (t[i] - t[j]);
+                H.la[i] *= lo_pre;
+                Y.y[i] = y[i];
+            }
+            H.la_dy = c1 * lo_pre;
+            H.la_y = -c0_ * c1 * lo_pre;
+            H.c1c0 = c1 / c0_;
+        }
+        ctx->d_part.reserve((size_t)kVecBlocks * 8 + 16);
+        hipLaunchKernelGGL(k_bdf_hilo, dim3(kVecBlocks), dim3(kBlock), 0, ctx->stream, n, H, Y, ctx->d_conc.p,
+                           ctx->d_cd.p, y0, e_n, ctx->d_part.p);
+        launch_check("k_bdf_hilo");
+        reduce_partials(ctx, ctx->d_part.p, kVecBlocks, 2, 2, 0, ctx->d_out.p);
+        double r[2];
+        HIP_OK(hipMemcpyAsync(r, ctx->d_out.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        *lo_err = std::sqrt(r[0] / ((double)n * a_tol));
+        *hi_err = std::sqrt(r[1] / ((double)n * a_tol));
+    }
+};
 }  // namespace
 
 // =================================================================== C ABI
@@ -1218,7 +1440,7 @@ int mapp_dmd_set_c(mapp_dmd_t* ctx, const double* c_host) {
         if (c_host)
             HIP_OK(hipMemcpyAsync(ctx->d_conc.p, c_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         ghost_copy(ctx, ctx->d_conc.p, acols(ctx));
-        ctx->cache_valid = false;
+        // the pair values of c_d_calc / calc_g depend on x and alpha only: they stay valid
         HIP_OK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
@@ -1273,8 +1495,10 @@ int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
     return guarded(ctx, [&]() {
         require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE, "calc_g is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede calc_g");
-        require(a_dev && g_dev && cost, MAPP_DMD_E_ARG, "null a, g or cost");
+        require(cost != nullptr, MAPP_DMD_E_ARG, "null cost");
         HIP_OK(hipSetDevice(ctx->device));
+        if (!a_dev) a_dev = ctx->d_a.p;  // the library's own buffers (mapp_dmd_set_vec / get_vec A, G)
+        if (!g_dev) g_dev = ctx->d_g.p;
         *cost = ctx->nt == 1 ? calc_g_variant<1>(ctx, chk, alpha, a_dev, g_dev)
                              : calc_g_variant<2>(ctx, chk, alpha, a_dev, g_dev);
         return 0;
@@ -1667,6 +1891,153 @@ int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, con
         return MAPP_DMD_E_IO;
     }
     return MAPP_DMD_OK;
+}
+
+
+int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_del_t, int max_retry, int flags,
+                     double* report, long* counts) {
+    return guarded(ctx, [&]() {
+        require(ctx->variant != MAPP_DMD_VARIANT_EAM && ctx->inited, MAPP_DMD_E_STATE,
+                "bdf_run needs an initialised DMD context");
+        require(nsteps >= 0 && max_order >= 1 && max_order <= kBdfMaxOrder && max_retry >= 0 && report,
+                MAPP_DMD_E_ARG, "bad bdf_run arguments");
+        HIP_OK(hipSetDevice(ctx->device));
+        const double a_tol = 1.0e-6, min_del_t = 1.0e-12, max_del_t = 1.0e4;
+        const int nt = ctx->nt;
+        const long n = (long)ctx->natms * nt;
+        hipStream_t st = ctx->stream;
+        // Clock_BDF::init (src/clock_bdf.cpp:201-275)
+        double en[7];
+        force_any(ctx, 1, en);
+        if (nt == 1) build_list2<1>(ctx);
+        else build_list2<2>(ctx);
+        if (nt == 1) c_d_variant<1>(ctx);
+        else c_d_variant<2>(ctx);
+        ctx->d_bdf.reserve((size_t)n * (max_order + 8));
+        BdfState S{ctx, n, nt};
+        S.fix = (flags & 1) != 0;
+        double* base = ctx->d_bdf.p;
+        for (int i = 0; i < max_order; ++i) S.y[i] = base + (size_t)i * n;
+        double* more = base + (size_t)max_order * n;
+        S.dy = more;
+        S.a = more + n;
+        S.y0 = more + 2 * n;
+        S.g = more + 3 * n;
+        S.g0 = more + 4 * n;
+        S.h = more + 5 * n;
+        S.c0 = more + 6 * n;
+        S.e_n = more + 7 * n;
+        HIP_OK(hipMemsetAsync(S.e_n, 0, n * sizeof(double), st));
+        if (initial_del_t <= 0.0) {
+            const double sum = std::sqrt(S.dot(ctx->d_cd.p, ctx->d_cd.p) / (double)n);
+            initial_del_t = std::max(a_tol / sum, min_del_t);
+            initial_del_t = std::min(initial_del_t, max_del_t);
+        }
+        HIP_OK(hipMemcpyAsync(S.y[0], ctx->d_conc.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(S.dy, ctx->d_cd.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        double del_t = initial_del_t, err = 0.0, cost = 0.0;
+        int ord = 1, const_stp = 0, istep = 0, rc = 0;
+        while (istep < nsteps) {
+            int chk = S.interpolate(ord, del_t);
+            while (chk == -1) {
+                del_t *= 0.8;
+                del_t = std::max(del_t, min_del_t);
+                chk = S.interpolate(ord, del_t);
+            }
+            int iters, it_sum = 0, solves = 1, retries = 0;
+            cost = S.solve(ord, del_t, &err, &iters);
+            it_sum += iters;
+            while (cost >= 0 && (err >= 1.0 || cost >= 1.0)) {
+                if (++retries > max_retry) {
+                    rc = -2;
+                    break;
+                }
+                const double err1 = std::max(err, cost);
+                double ratio = std::pow(0.5 / err1, 1.0 / (double)(ord + 1));
+                ratio = ratio < 0.5 ? 0.5 : (ratio > 0.9 ? 0.9 : ratio);
+                const double tmp = del_t * ratio;
+                del_t = tmp < min_del_t ? min_del_t : tmp;
+                chk = S.interpolate(ord, del_t);
+                while (chk == -1) {
+                    del_t *= 0.8;
+                    chk = S.interpolate(ord, del_t);
+                }
+                cost = S.solve(ord, del_t, &err, &iters);
+                it_sum += iters;
+                ++solves;
+            }
+            double* r = report + (size_t)istep * 6;
+            r[0] = ord;
+            r[1] = del_t;
+            r[2] = cost;
+            r[3] = err;
+            r[4] = it_sum;
+            r[5] = solves;
+            if (cost < 0) rc = -1;
+            if (rc) break;
+            // history shift (src/clock_bdf.cpp:566-579)
+            double* tmpy = S.y[max_order - 1];
+            for (int i = max_order - 1; i > 0; --i) {
+                S.t[i] = S.t[i - 1];
+                S.y[i] = S.y[i - 1];
+            }
+            S.y[0] = tmpy;
+            S.t[0] += del_t;
+            HIP_OK(hipMemcpyAsync(S.y[0], ctx->d_conc.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(S.dy, ctx->d_cd.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            // step and order control (src/clock_bdf.cpp:582-664)
+            const bool lo_avail = ord != 1 && const_stp > ord + 1;
+            const bool hi_avail = ord < max_order && istep != 0 && const_stp > ord + 1;
+            double lo_err, hi_err, ratio, lo_ratio, hi_ratio;
+            S.hi_lo(ord, del_t, hi_avail, lo_avail, &lo_err, &hi_err);
+            if (hi_avail && lo_avail) {
+                lo_ratio = std::pow(0.5 / lo_err, 1.0 / (double)ord);
+                hi_ratio = std::pow(0.5 / hi_err, 1.0 / (double)(ord + 2));
+                ratio = std::pow(0.5 / err, 1.0 / (double)(ord + 1));
+                if (hi_ratio > lo_ratio && hi_ratio > ratio) {
+                    ++ord;
+                    ratio = hi_ratio;
+                } else if (lo_ratio > hi_ratio && lo_ratio > ratio) {
+                    --ord;
+                    ratio = lo_ratio;
+                }
+            } else if (hi_avail && !lo_avail) {
+                hi_ratio = std::pow(0.5 / hi_err, 1.0 / (double)(ord + 2));
+                ratio = std::pow(0.5 / err, 1.0 / (double)(ord + 1));
+                if (hi_ratio > ratio) {
+                    ++ord;
+                    ratio = hi_ratio;
+                }
+            } else if (!hi_avail && lo_avail) {
+                lo_ratio = std::pow(0.5 / lo_err, 1.0 / (double)ord);
+                ratio = std::pow(0.5 / err, 1.0 / (double)(ord + 1));
+                if (lo_ratio > ratio) {
+                    --ord;
+                    ratio = lo_ratio;
+                }
+            } else {
+                ratio = std::pow(0.5 / err, 1.0 / (double)(ord + 1));
+            }
+            if (ratio >= 2.0) {
+                ratio = 2.0;
+                const_stp = 0;
+            } else if (ratio <= 0.5) {
+                ratio = 0.5;
+                const_stp = 0;
+            } else {
+                ratio = 1.0;
+                ++const_stp;
+            }
+            const double tmp = ratio * del_t;
+            del_t = tmp > max_del_t ? max_del_t : (tmp < min_del_t ? min_del_t : tmp);
+            ++istep;
+        }
+        if (counts) {
+            counts[0] = rc ? rc : istep;
+            counts[1] = S.n_calc_g;
+        }
+        return 0;
+    });
 }
 
 int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count) {

@@ -66,7 +66,8 @@ struct mapp_dmd {
     mappdmd::SurNorm snorm{};
     bool sur_trained = false;
     bool energy_full = false;
-    mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors  // energy_calc over every ordered type pair
+    mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors
+    mappdmd::DevBuf<double> d_bdf;                            // BDF history and solver vectors  // energy_calc over every ordered type pair
 
     // timing
     bool timing = false;

@@ -57,6 +57,7 @@ _SIGS = {
     "mapp_dmd_last_timing": [CTX, DP],
     "mapp_dmd_enable_timing": [CTX, C.c_int],
     "mapp_dmd_set_energy_mode": [CTX, C.c_int],
+    "mapp_dmd_bdf_run": [CTX, C.c_int, C.c_int, D, C.c_int, C.c_int, DP, LP],
     "mapp_dmd_min_lbfgs": [CTX, C.c_int, C.c_int, D, DP, LP],
     "mapp_dmd_surrogate_train": [CTX, C.c_long, C.c_int, C.c_int, D, C.c_ulonglong, D, DP, DP],
     "mapp_dmd_surrogate_gradient": [CTX, FP, FP, C.c_long, FP, FP, FP, FP],
@@ -280,6 +281,16 @@ class ForceFieldDMD:
         self._call("mapp_dmd_min_lbfgs", int(max_iter), int(m_it), float(e_tol), _dp(en),
                    cnt.ctypes.data_as(LP))
         return int(cnt[3]), en[: cnt[0] + 2], int(cnt[1]), int(cnt[2])
+
+    def bdf_run(self, nsteps, max_order=2, initial_del_t=-1.0, max_retry=100, fix=False):
+        """Clock_BDF::init + run on the device: (return code, report [steps][order, del_t, cost, err, iters,
+        solves], calc_g calls); the code is the number of steps or -1 / -2 as the C ABI documents"""
+        rep = np.zeros((max(nsteps, 1), 6))
+        cnt = np.zeros(2, dtype=np.int64)
+        self._call("mapp_dmd_bdf_run", int(nsteps), int(max_order), float(initial_del_t), int(max_retry),
+                   1 if fix else 0, _dp(rep), cnt.ctypes.data_as(LP))
+        code = int(cnt[0])
+        return code, rep[:code] if code >= 0 else rep, int(cnt[1])
 
     # learned surrogate of the pair quadrature (two types)
     def surrogate_train(self, n_per_tp=1 << 20, epochs=4, batch_per_tp=256, lr=2e-3, seed=1, tol=0.05, box=None):

@@ -1638,6 +1638,369 @@ int orc_min_lbfgs(orc_t* o, int max_iter, int m_it, double e_tol, double* energi
     return err;
 }
 
+
+/* -------------------------------------------------------------------- BDF */
+
+/* coefficients of Clock_BDF::interpolate (src/clock_bdf.cpp:295-341) for
+   order q and the step to t[0] + del_t: out = {beta, alpha_dy_0,
+   dalpha_dy_0, alpha_y[0..q-1], dalpha_y[0..q-1]}.  fix = 1 corrects the sign
+   of the (t_new - t[0]) c1 c2 term of dalpha_y[0] (:327), which the reference
+   adds instead of subtracting, so that the derivative weights sum to zero. */
+void orc_bdf_coef(int q, double del_t, const double* t, int fix, double* out) {
+    double alpha_y[8], dalpha_y[8];
+    double t_new = t[0] + del_t, c0 = 1.0, c1 = 0.0, c2 = 0.0;
+    for (int i = 1; i < q; i++) {
+        double tmp0 = (t_new - t[0]) / (t[i] - t[0]);
+        double tmp1 = tmp0 * tmp0;
+        for (int j = 1; j < q; j++)
+            if (i != j) tmp1 *= (t_new - t[j]) / (t[i] - t[j]);
+        alpha_y[i] = tmp1;
+        c0 *= (t_new - t[i]) / (t[0] - t[i]);
+        c1 += 1.0 / (t[0] - t[i]);
+        c2 += 1.0 / (t_new - t[i]);
+    }
+    alpha_y[0] = (1.0 - (t_new - t[0]) * c1) * c0;
+    double alpha_dy_0 = c0 * (t_new - t[0]);
+    double c3 = 2.0 / (t_new - t[0]);
+    double dalpha_dy_0 = alpha_dy_0 * (c2 + 0.5 * c3);
+    if (fix) dalpha_y[0] = c0 * (c2 - c1 - (t_new - t[0]) * c1 * c2);
+    else dalpha_y[0] = c0 * (c2 - c1 + (t_new - t[0]) * c1 * c2);
+    for (int i = 1; i < q; i++) dalpha_y[i] = alpha_y[i] * (c2 + c3 - 1.0 / (t_new - t[i]));
+    double tmp0 = 0.0;
+    for (int i = 0; i < q; i++) tmp0 += 1.0 / (double)(i + 1);
+    out[0] = del_t / tmp0;
+    out[1] = alpha_dy_0;
+    out[2] = dalpha_dy_0;
+    for (int i = 0; i < q; i++) {
+        out[3 + i] = alpha_y[i];
+        out[3 + q + i] = dalpha_y[i];
+    }
+}
+
+/* Clock_BDF::interpolate (src/clock_bdf.cpp:295-369): the predictor y_0 and
+   the constant part a of the residual; -1 when the predictor leaves [0,1]. */
+static int bdf_interpolate(int q, double del_t, const double* t, double* const* y, const double* dy, long n,
+                           double* a, double* y0, double* beta, int fix) {
+    double cf[3 + 2 * 8];
+    orc_bdf_coef(q, del_t, t, fix, cf);
+    *beta = cf[0];
+    const double alpha_dy_0 = cf[1], dalpha_dy_0 = cf[2];
+    const double* alpha_y = cf + 3;
+    const double* dalpha_y = cf + 3 + q;
+    for (long k = 0; k < n; k++) {
+        a[k] = *beta * dalpha_dy_0 * dy[k];
+        y0[k] = alpha_dy_0 * dy[k];
+        for (int j = 0; j < q; j++) {
+            a[k] += *beta * dalpha_y[j] * y[j][k];
+            y0[k] += alpha_y[j] * y[j][k];
+        }
+        a[k] -= y0[k];
+        if (y0[k] < 0.0 || y0[k] > 1.0) return -1;
+    }
+    return 1;
+}
+
+/* Clock_BDF::solve (src/clock_bdf.cpp:677-883) with curr_t = t[0] + del_t:
+   bounded predictor, then nonlinear conjugate gradients on calc_g's cost with a
+   backtracking step that keeps c inside [0,1].  Returns cost / (m_tol dof) or
+   -1 if c left [0,1]; *err is the local error estimate, *iters the CG count. */
+static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* const* y, const double* dy,
+                        const double* a, const double* y0, double beta, double* err, int* iters, int* n_calc_g) {
+    const double min_gamma = 1.0e-10, gamma_red = 0.8, slope = 0.4, m_tol = 1.0e-10, a_tol = 1.0e-6;
+    const int max_iter = 50;
+    int nt = o->nt, natms = o->natms;
+    long n = (long)natms * nt;
+    double* c = (double*)calloc(n, sizeof(double));
+    double* g = (double*)calloc(n, sizeof(double));
+    double* g0 = (double*)calloc(n, sizeof(double));
+    double* h = (double*)calloc(n, sizeof(double));
+    double* cc0 = (double*)calloc(n, sizeof(double));
+    double ratio = 1.0;
+    for (long i = 0; i < n; i++) {
+        double tmp = dy[i] * del_t + y[0][i];
+        if (tmp > 1.0) ratio = MINV((1.0 - y[0][i]) / (tmp - y[0][i]), ratio);
+        else if (tmp < 0.0) ratio = MINV((y[0][i] - 0.0) / (y[0][i] - tmp), ratio);
+    }
+    for (long i = 0; i < n; i++) c[i] = y[0][i] + dy[i] * del_t * ratio;
+    orc_set_c(o, c);
+    double curr_cost = orc_calc_g(o, 0, beta, a, g);
+    (*n_calc_g)++;
+    memcpy(h, g, n * sizeof(double));
+    double g0_g0 = 0.0;
+    for (long i = 0; i < n; i++) g0_g0 += g[i] * g[i];
+    double g_h = g0_g0, max_gamma = 1.0;
+    int iter = 0;
+    while (curr_cost > m_tol * (double)n && iter < max_iter && max_gamma > min_gamma) {
+        memcpy(g0, g, n * sizeof(double));
+        memcpy(cc0, o->c, n * sizeof(double));
+        double gamma = 0.99;
+        for (long i = 0; i < n; i++) {
+            double tmp = cc0[i] + h[i];
+            if (tmp > 1.0) gamma = MINV((1.0 - cc0[i]) / (tmp - cc0[i]), gamma);
+            else if (tmp < 0.0) gamma = MINV((cc0[i] - 0.0) / (cc0[i] - tmp), gamma);
+        }
+        max_gamma = gamma * 0.9;
+        int chk = 1;
+        double cost = curr_cost;
+        while (chk && max_gamma > min_gamma) {
+            for (long i = 0; i < n; i++) c[i] = cc0[i] + max_gamma * h[i];
+            orc_set_c(o, c);
+            curr_cost = orc_calc_g(o, 0, beta, a, g);
+            (*n_calc_g)++;
+            double ideal = cost - slope * max_gamma * g_h;
+            if (curr_cost < ideal) chk = 0;
+            max_gamma *= gamma_red;
+            if (max_gamma <= min_gamma) {
+                orc_set_c(o, cc0);
+                curr_cost = orc_calc_g(o, 1, beta, a, g);
+                (*n_calc_g)++;
+            }
+        }
+        if (chk == 0) {
+            double g_g0 = 0.0, g_g = 0.0;
+            for (long i = 0; i < n; i++) g_g0 += g[i] * g0[i];
+            for (long i = 0; i < n; i++) g_g += g[i] * g[i];
+            double r = (g_g - g_g0) / g0_g0;
+            g0_g0 = g_g;
+            double inner = 0.0;
+            for (long i = 0; i < n; i++) {
+                h[i] *= r;
+                h[i] += g[i];
+                inner += h[i] * g[i];
+            }
+            g_h = inner;
+            if (g_h < 0.0) {
+                memcpy(h, g, n * sizeof(double));
+                g_h = g_g;
+            }
+        }
+        iter++;
+    }
+    *iters = iter;
+    double curr_t = t[0] + del_t, a_1 = 1.0, l1_q = 0.0;
+    for (int i = 0; i < q; i++) {
+        l1_q += 1.0 / (curr_t - t[i]);
+        if (i != 0) a_1 *= (curr_t - t[i]) / (t[0] - t[i]);
+    }
+    double err_pre_fac = -1.0 / (l1_q * (1.0 + a_1));
+    double e = 0.0;
+    for (long i = 0; i < n; i++) {
+        double d = y0[i] - o->c[i];
+        e += d * d;
+    }
+    e = sqrt(e / ((double)n * a_tol));
+    *err = e * fabs(err_pre_fac);
+    double ret = curr_cost / (m_tol * (double)n);
+    for (long i = 0; i < n; i++)
+        if (o->c[i] < 0.0 || o->c[i] > 1.0) ret = -1.0;
+    free(c); free(g); free(g0); free(h); free(cc0);
+    return ret;
+}
+
+/* Clock_BDF::hi_lo_err_est (src/clock_bdf.cpp:372-478): error estimates of
+   the orders q+1 and q-1 for the step that starts at the current t[0]. */
+typedef struct {
+    double* e_n;
+    double lwr_alpha[8];
+    double lo_err, hi_err;
+    int lo_avail, hi_avail;
+} bdf_est_t;
+
+static void bdf_hi_lo(bdf_est_t* E, int q, double del_t, const double* t, double* const* y, const double* c,
+                      const double* c_d, const double* y0, long n) {
+    const double a_tol = 1.0e-6;
+    double curr_t = t[0] + del_t, a_0 = 1.0, a_1 = 1.0, l1_q = 0.0;
+    double hi_a_0, hi_a_1, Q_n = 1.0, hi_err_pre_fac = 1.0, lo_err_pre_fac = 0.0, c0 = 0.0, c1 = 1.0;
+    double lwr_alpha_dy = 0.0, lwr_alpha_y = 0.0;
+    for (int i = 0; i < q; i++) {
+        l1_q += 1.0 / (curr_t - t[i]);
+        a_0 *= curr_t - t[i];
+        if (i != 0) a_1 *= (curr_t - t[i]) / (t[0] - t[i]);
+    }
+    if (E->hi_avail) {
+        hi_a_0 = hi_a_1 = 1.0;
+        for (int i = 0; i < q; i++) {
+            hi_a_0 *= t[0] - t[i + 1];
+            if (i != 0) hi_a_1 *= (t[0] - t[i + 1]) / (t[1] - t[i + 1]);
+        }
+        Q_n = (a_0 / hi_a_0) * (1.0 + a_1) / (1.0 + hi_a_1) * (del_t / (t[0] - t[1]));
+        double hi_l1_q = l1_q + 1.0 / (curr_t - t[q]);
+        hi_err_pre_fac = -(curr_t - t[q]) / (q + 2);
+        hi_err_pre_fac *= 1.0 / (hi_l1_q * (1.0 + a_1) * del_t * del_t);
+    }
+    if (E->lo_avail) {
+        lo_err_pre_fac = -del_t * a_0 / (curr_t - t[q - 1]);
+        c0 = 0.0;
+        c1 = 1.0;
+        for (int i = 0; i < q - 1; i++) {
+            double tmp0 = 1.0 / (curr_t - t[i]);
+            c0 += tmp0;
+            c1 *= tmp0;
+            E->lwr_alpha[i] = tmp0 * tmp0;
+            for (int j = 0; j < q - 1; j++)
+                if (i != j) E->lwr_alpha[i] *= 1.0 / (t[i] - t[j]);
+            E->lwr_alpha[i] *= lo_err_pre_fac;
+        }
+        lwr_alpha_dy = c1 * lo_err_pre_fac;
+        lwr_alpha_y = -c0 * c1 * lo_err_pre_fac;
+    }
+    double lo = 0.0, hi = 0.0;
+    for (long i = 0; i < n; i++) {
+        if (E->hi_avail) {
+            double tmp0 = (c[i] - y0[i]) - Q_n * E->e_n[i];
+            tmp0 *= hi_err_pre_fac;
+            hi += tmp0 * tmp0;
+        }
+        if (E->lo_avail) {
+            double tmp0 = lwr_alpha_y * c[i];
+            tmp0 += lwr_alpha_dy * c_d[i];
+            for (int j = 0; j < q - 1; j++) tmp0 += E->lwr_alpha[j] * y[j][i];
+            tmp0 *= c1 / c0;
+            lo += tmp0 * tmp0;
+        }
+        E->e_n[i] = c[i] - y0[i];
+    }
+    E->lo_err = sqrt(lo / ((double)n * a_tol));
+    E->hi_err = sqrt(hi / ((double)n * a_tol));
+}
+
+/* Clock_BDF::init + Clock_BDF::run (src/clock_bdf.cpp:201-275, :479-672):
+   variable step and order up to max_order, retries with a smaller step while
+   the error estimate or the solver's cost ratio is >= 1, order changes from
+   hi_lo_err_est.  initial_del_t <= 0 takes the reference's a_tol / rms(c_d)
+   rule.  solve's uninitialised curr_t is taken as t[0] + del_t.  A step that
+   needs more than max_retry retries ends the run (the reference would loop).
+   report[step*6 + {0..5}] = order, del_t, cost ratio, error, CG iterations,
+   solve calls.  Returns the number of steps, -1 if c left [0,1], -2 on too
+   many retries. */
+int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
+                double* report, int* n_calc_g) {
+    const double a_tol = 1.0e-6, min_del_t = 1.0e-12, max_del_t = 1.0e4;
+    int nt = o->nt, natms = o->natms;
+    long n = (long)natms * nt;
+    if (max_order < 1 || max_order > 7) return -3;
+    double en[7];
+    orc_force_calc(o, 1, en);
+    orc_create_2nd_neigh_lst(o);
+    orc_c_d_calc(o);
+    double t[8] = {0};
+    double* y[8];
+    for (int i = 0; i < max_order; i++) y[i] = (double*)calloc(n, sizeof(double));
+    double* dy = (double*)calloc(n, sizeof(double));
+    double* a = (double*)calloc(n, sizeof(double));
+    double* y0 = (double*)calloc(n, sizeof(double));
+    bdf_est_t E;
+    memset(&E, 0, sizeof(E));
+    E.e_n = (double*)calloc(n, sizeof(double));
+    if (initial_del_t <= 0.0) {
+        double sum = 0.0;
+        for (long i = 0; i < n; i++) sum += o->c_d[i] * o->c_d[i];
+        sum = sqrt(sum / (double)n);
+        initial_del_t = a_tol / sum > min_del_t ? a_tol / sum : min_del_t;
+        initial_del_t = initial_del_t < max_del_t ? initial_del_t : max_del_t;
+    }
+    memcpy(y[0], o->c, n * sizeof(double));
+    memcpy(dy, o->c_d, n * sizeof(double));
+    *n_calc_g = 0;
+    double del_t = initial_del_t, beta = 0.0, err = 0.0, cost;
+    int ord = 1, const_stp = 0, istep = 0, rc = 0;
+    while (istep < nsteps) {
+        int chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
+        while (chk == -1) {
+            del_t *= 0.8;
+            del_t = del_t > min_del_t ? del_t : min_del_t;
+            chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
+        }
+        int iters, it_sum = 0, solves = 1, retries = 0;
+        cost = bdf_solve(o, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
+        it_sum += iters;
+        while (cost >= 0 && (err >= 1.0 || cost >= 1.0)) {
+            if (++retries > max_retry) {
+                rc = -2;
+                break;
+            }
+            double err1 = err > cost ? err : cost;
+            double ratio = pow(0.5 / err1, 1.0 / (double)(ord + 1));
+            ratio = ratio < 0.5 ? 0.5 : (ratio > 0.9 ? 0.9 : ratio);
+            double tmp = del_t * ratio;
+            del_t = tmp < min_del_t ? min_del_t : tmp;
+            chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
+            while (chk == -1) {
+                del_t *= 0.8;
+                chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
+            }
+            cost = bdf_solve(o, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
+            it_sum += iters;
+            solves++;
+        }
+        report[istep * 6 + 0] = ord;
+        report[istep * 6 + 1] = del_t;
+        report[istep * 6 + 2] = cost;
+        report[istep * 6 + 3] = err;
+        report[istep * 6 + 4] = it_sum;
+        report[istep * 6 + 5] = solves;
+        if (cost < 0) rc = -1;
+        if (rc) break;
+        double* tmpy = y[max_order - 1];
+        for (int i = max_order - 1; i > 0; i--) {
+            t[i] = t[i - 1];
+            y[i] = y[i - 1];
+        }
+        y[0] = tmpy;
+        t[0] += del_t;
+        memcpy(y[0], o->c, n * sizeof(double));
+        memcpy(dy, o->c_d, n * sizeof(double));
+        E.lo_avail = ord != 1 && const_stp > ord + 1;
+        E.hi_avail = ord < max_order && istep != 0 && const_stp > ord + 1;
+        bdf_hi_lo(&E, ord, del_t, t, y, o->c, o->c_d, y0, n);
+        double ratio, lo_ratio, hi_ratio;
+        if (E.hi_avail && E.lo_avail) {
+            lo_ratio = pow(0.5 / E.lo_err, 1.0 / (double)ord);
+            hi_ratio = pow(0.5 / E.hi_err, 1.0 / (double)(ord + 2));
+            ratio = pow(0.5 / err, 1.0 / (double)(ord + 1));
+            if (hi_ratio > lo_ratio && hi_ratio > ratio) {
+                ord++;
+                ratio = hi_ratio;
+            } else if (lo_ratio > hi_ratio && lo_ratio > ratio) {
+                ord--;
+                ratio = lo_ratio;
+            }
+        } else if (E.hi_avail && !E.lo_avail) {
+            hi_ratio = pow(0.5 / E.hi_err, 1.0 / (double)(ord + 2));
+            ratio = pow(0.5 / err, 1.0 / (double)(ord + 1));
+            if (hi_ratio > ratio) {
+                ord++;
+                ratio = hi_ratio;
+            }
+        } else if (!E.hi_avail && E.lo_avail) {
+            lo_ratio = pow(0.5 / E.lo_err, 1.0 / (double)ord);
+            ratio = pow(0.5 / err, 1.0 / (double)(ord + 1));
+            if (lo_ratio > ratio) {
+                ord--;
+                ratio = lo_ratio;
+            }
+        } else {
+            ratio = pow(0.5 / err, 1.0 / (double)(ord + 1));
+        }
+        if (ratio >= 2.0) {
+            ratio = 2.0;
+            const_stp = 0;
+        } else if (ratio <= 0.5) {
+            ratio = 0.5;
+            const_stp = 0;
+        } else {
+            ratio = 1.0;
+            const_stp++;
+        }
+        double tmp = ratio * del_t;
+        del_t = tmp > max_del_t ? max_del_t : (tmp < min_del_t ? min_del_t : tmp);
+        istep++;
+    }
+    for (int i = 0; i < max_order; i++) free(y[i]);
+    free(dy); free(a); free(y0); free(E.e_n);
+    return rc ? rc : istep;
+}
+
 int orc_natms(const orc_t* o) { return o->natms; }
 int orc_nghost(const orc_t* o) { return o->nghost; }
 long orc_npairs(const orc_t* o) { return o->no_pairs; }

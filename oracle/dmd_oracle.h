@@ -69,6 +69,18 @@ double orc_calc_g(orc_t* o, int chk, double alpha, const double* a, double* g);
 int orc_min_lbfgs(orc_t* o, int max_iter, int m_it, double e_tol, double* energies, int* nsteps, int* n_force,
                   int* n_energy);
 
+/* Clock_BDF::init + run: variable step and order (up to max_order), retries
+   while the error estimate or cost ratio is >= 1; initial_del_t <= 0 takes the
+   reference's a_tol / rms(c_d) rule; fix as in orc_bdf_coef.  report[6*nsteps]: per step order, del_t,
+   cost ratio, error, CG iterations, solve calls.  Returns the steps taken,
+   -1 if c left [0,1], -2 after more than max_retry retries of one step. */
+int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
+                double* report, int* n_calc_g);
+/* coefficients of Clock_BDF::interpolate: out[3 + 2q] = beta, alpha_dy_0,
+   dalpha_dy_0, alpha_y[q], dalpha_y[q]; fix = 1 corrects the sign of the
+   c1 c2 term of dalpha_y[0] (src/clock_bdf.cpp:327) */
+void orc_bdf_coef(int q, double del_t, const double* t, int fix, double* out);
+
 /* introspection */
 int orc_natms(const orc_t* o);
 int orc_nghost(const orc_t* o);

@@ -72,6 +72,8 @@ def lib():
         L.orc_s2x.argtypes = [DP, C.c_int, C.c_int, DP]
         L.orc_pair_quadrature.argtypes = [C.c_void_p, D, D, C.c_int, C.c_int, DP]
         L.orc_min_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, D, DP, IP, IP, IP]
+        L.orc_bdf_run.argtypes = [C.c_void_p, C.c_int, C.c_int, D, C.c_int, C.c_int, DP, IP]
+        L.orc_bdf_coef.argtypes = [C.c_int, D, DP, C.c_int, DP]
         _lib = L
     return _lib
 
@@ -228,10 +230,26 @@ class Oracle:
         code = self.L.orc_min_lbfgs(self.o, max_iter, m_it, e_tol, _dp(en), C.byref(ns), C.byref(nf), C.byref(ne))
         return code, en[: ns.value + 2], nf.value, ne.value
 
+    def bdf_run(self, nsteps, max_order=2, initial_del_t=-1.0, max_retry=100, fix=False):
+        """Clock_BDF::init + run: (return code, report [steps][order, del_t, cost, err, iters, solves], calc_g calls)"""
+        rep = np.zeros((nsteps, 6))
+        ng = C.c_int(0)
+        rc = self.L.orc_bdf_run(self.o, nsteps, max_order, initial_del_t, max_retry, 1 if fix else 0, _dp(rep),
+                                C.byref(ng))
+        return rc, rep[: max(rc, 0)] if rc >= 0 else rep, ng.value
+
     def pair_quadrature(self, r, alpha, rho_slot, phi_slot=-1):
         out = np.zeros(6)
         ok = self.L.orc_pair_quadrature(self.o, r, alpha, rho_slot, phi_slot, _dp(out))
         return out if ok else None
+
+
+def bdf_coef(q, del_t, t, fix=False):
+    """Clock_BDF::interpolate coefficients: dict beta, alpha_dy_0, dalpha_dy_0, alpha_y, dalpha_y"""
+    out = np.zeros(3 + 2 * q)
+    tt = np.ascontiguousarray(t, dtype=np.float64)
+    lib().orc_bdf_coef(q, del_t, _dp(tt), 1 if fix else 0, _dp(out))
+    return dict(beta=out[0], alpha_dy_0=out[1], dalpha_dy_0=out[2], alpha_y=out[3:3 + q], dalpha_y=out[3 + q:])
 
 
 def make(lat, tables, no_i=3, variant="DMD_I", skin=0.5, fresh=True, **kw):
