@@ -175,6 +175,32 @@ int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, do
 int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_del_t, int max_retry, int flags,
                      double* report, long* counts);
 
+/* A batch of independent periodic cells in one context (no counterpart in
+   the reference, which runs one domain per process): cell k has the lower-
+   triangular box H[9k..9k+8] and natms[k] sites; x (reduced positions +
+   alpha) and c are the cells' rows concatenated in cell order.  init builds
+   each cell's ghosts and lists as set_system + init would, then concatenates
+   them (owned sites of all cells, then all ghosts), so force_calc,
+   energy_calc, c_d_calc, calc_g, create_2nd_neigh_lst and bdf_run cover every
+   cell in one launch per kernel; per-site outputs come back in cell order and
+   energies / virials are batch totals.  Positions are fixed (set_x and
+   min_lbfgs are refused). */
+int mapp_dmd_set_cells(mapp_dmd_t* ctx, int ncell, const double* H, const int* natms, const double* x,
+                       const double* c);
+/* number of cells, and per cell owned sites and ghosts (up to cap) */
+int mapp_dmd_cells(mapp_dmd_t* ctx, int* ncell, int* natms, int* nghost, int cap);
+
+/* Launch schedules of the two force passes (no counterpart in the reference):
+   {waves per block (1, 2, 4), total waves (0 = one wavefront per site,
+   otherwise a persistent grid whose waves stride through XCD-contiguous
+   site ranges)} for pass 1 then pass 2.  mapp_dmd_autotune times every
+   candidate on the context's own system with HIP events (reps runs each
+   after one warm-up force_calc) and keeps the fastest per pass; report
+   [cap][4] receives {waves per block, total waves, pass-1 ms, pass-2 ms}. */
+int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* ncand);
+int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out);
+int mapp_dmd_set_schedule(mapp_dmd_t* ctx, const int* in);
+
 /* Extended CFG files of DMD runs (host only, no context, no GPU).
    mapp_dmd_cfg_read follows ReadCFG (src/read_cfg.cpp:15-178 header and atom
    loop, :228-350 read_header, :356-418 set_box without eta, :562-657

@@ -187,6 +187,9 @@ void build_ghosts(mapp_dmd* ctx) {
     HIP_OK(hipStreamSynchronize(ctx->stream));
 }
 
+template <int NT>
+void finish_lists(mapp_dmd* ctx);
+
 // ghosts + bins + s->x + half list + reverse CSR; positions hold s on entry
 template <int NT>
 void rebuild(mapp_dmd* ctx) {
@@ -224,6 +227,17 @@ void rebuild(mapp_dmd* ctx) {
     hipLaunchKernelGGL((k_half_list<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
                        ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_off.p, ctx->d_nl.p);
     launch_check("k_half_list fill");
+    finish_lists<NT>(ctx);
+}
+
+// reverse CSR, pair owners, reverse positions and per-site / per-pair buffers
+// for the half list in d_off / d_nl
+template <int NT>
+void finish_lists(mapp_dmd* ctx) {
+    const int natms = ctx->natms;
+    const int ntot = natms + ctx->nghost;
+    hipStream_t st = ctx->stream;
+    ctx->d_cnt.reserve(natms + 1);
     // reverse CSR (partner j owned)
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
     hipLaunchKernelGGL(k_rev_count, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
@@ -406,13 +420,19 @@ void pack_atoms(mapp_dmd* ctx) {
     launch_check("k_pack_atoms");
 }
 
+// blocks of a force-pass launch: one wave per site unless the shape caps the waves
+inline int shape_blocks(const mapp_dmd::Shape& sh, int natms) {
+    const long w = sh.waves > 0 ? std::min<long>(sh.waves, natms) : natms;
+    return (int)std::max<long>(1, (w + sh.wpb - 1) / sh.wpb);
+}
+
 template <int NT, typename Q, int NQ>
 void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st, bool sur = false) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
     const int nb = nblk(natms);
-    const int nbw = nblk((long)natms * 64);  // one wavefront per site
-    const int nb1 = sur ? natms : nbw;         // the surrogate's pass 1 runs one wavefront per block
+    const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
+    const int nb1 = sur ? natms : shape_blocks(ctx->sh1, natms);  // the surrogate's pass 1: one wave per block
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);
     ctx->d_out.reserve(16);
@@ -447,9 +467,9 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
         }
     }
     if (!sur) {
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                           ctx->d_pairE.p, deriv, part1);
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ,
+                           ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
+                           ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
         launch_check("k_pass1");
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
@@ -464,8 +484,8 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
         launch_check("k_ghost_copy arec");
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
-    hipLaunchKernelGGL((k_pass2<NT, Q>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, ctx->d_pos.p, ctx->d_arec.p, L,
-                       ctx->d_rpos.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
+    hipLaunchKernelGGL((k_pass2<NT, Q>), dim3(nbw), dim3(64 * ctx->sh2.wpb), 0, st, ctx->P, ctx->d_pos.p,
+                       ctx->d_arec.p, L, ctx->d_rpos.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
     launch_check("k_pass2");
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
@@ -498,7 +518,7 @@ template <int NT, typename Q, int NQ>
 double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     hipStream_t st = ctx->stream;
     const int nb = nblk(ctx->natms);
-    const int nbw = nblk((long)ctx->natms * 64);
+    const int nbw = shape_blocks(ctx->sh1, ctx->natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);
     ctx->d_out.reserve(16);
@@ -506,11 +526,11 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     double* part2 = part1 + (size_t)nbw * 2;
     pack_atoms<NT>(ctx);
     if (ctx->energy_full)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true>), dim3(nbw), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ,
                            ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
                            ctx->d_Eown.p, ctx->d_pairE.p, (Q*)nullptr, part1);
     else
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(kBlock), 0, st, ctx->P, PQ, ctx->d_pos.p,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, (Q*)nullptr, part1);
     launch_check("k_pass1 energy");
@@ -643,7 +663,8 @@ void run_pair_values(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
 template <int NT>
 void pair_values_dispatch(mapp_dmd* ctx) {
     const int q = ctx->P.no_i;
-    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+    // the surrogate's c_d_calc / calc_g pair values come from its fallback, the FP32 quadrature
+    if (ctx->prec == MAPP_DMD_PREC_MIXED || ctx->prec == MAPP_DMD_PREC_SURROGATE) {
         if (q == 3) run_pair_values<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf);
         else if (q == 5) run_pair_values<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf);
         else run_pair_values<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf);
@@ -1164,6 +1185,110 @@ struct BdfState {
         *hi_err = std::sqrt(r[1] / ((double)n * a_tol));
     }
 };
+
+// ------------------------------------------------------------------ batched cells
+// Cells are built one by one in scratch contexts with the single-cell pipeline,
+// then concatenated: the owned sites of all cells first (cell order), then all
+// ghosts (cell order).  Site j is owned iff j < natms, as for one cell, so every
+// force, mobility and BDF kernel runs unchanged in one launch over all cells;
+// lists keep each cell's order, so per-site sums match a cell run alone.
+template <int NT>
+__global__ void k_merge_sites(int n_own, int n_gh, const double4* __restrict__ pos, const double* __restrict__ alpha,
+                              const double* __restrict__ conc, const int* __restrict__ owner, int own_base,
+                              int gh_dst, int gh_index_base, double4* __restrict__ Pos, double* __restrict__ Alpha,
+                              double* __restrict__ Conc, int* __restrict__ Owner) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_own + n_gh) return;
+    const int dst = s < n_own ? own_base + s : gh_dst + (s - n_own);
+    Pos[dst] = pos[s];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        Alpha[(size_t)dst * NT + t] = alpha[(size_t)s * NT + t];
+        Conc[(size_t)dst * NT + t] = conc[(size_t)s * NT + t];
+    }
+    if (s >= n_own) Owner[gh_index_base + (s - n_own)] = own_base + owner[s - n_own];
+}
+
+__global__ void k_merge_list(int n_own, const int* __restrict__ off, const int* __restrict__ nl, int own_base,
+                             int pair_base, int gh_dst, int* __restrict__ Off, int* __restrict__ Nl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_own) return;
+    Off[own_base + i] = pair_base + off[i];
+    for (int p = off[i]; p < off[i + 1]; ++p) {
+        const int j = nl[p];
+        Nl[pair_base + p] = j < n_own ? own_base + j : gh_dst + (j - n_own);
+    }
+}
+
+template <int NT>
+void init_batch(mapp_dmd* ctx, double skin) {
+    const int ncell = (int)ctx->cell_natms.size();
+    std::vector<mapp_dmd*> sub(ncell, nullptr);
+    struct Guard {
+        std::vector<mapp_dmd*>& v;
+        ~Guard() {
+            for (auto* p : v)
+                if (p) mapp_dmd_destroy(p);
+        }
+    } guard{sub};
+    std::vector<int> own_base(ncell + 1, 0), gh_base(ncell + 1, 0);
+    std::vector<long> pair_base(ncell + 1, 0);
+    const int xc = xcols(ctx);
+    size_t xo = 0;
+    for (int k = 0; k < ncell; ++k) {
+        mapp_dmd_t* sp = nullptr;
+        if (mapp_dmd_create(&sp, ctx->nt, ctx->variant, ctx->device) != 0 || !sp)
+            throw ApiError(MAPP_DMD_E_HIP, "cannot create the context of a cell");
+        sub[k] = sp;
+        sp->P = ctx->P;
+        sp->Pf = ctx->Pf;
+        sp->mod_rc = ctx->mod_rc;
+        sp->rsq_crd = ctx->rsq_crd;
+        sp->have_params = true;
+        const int n = ctx->cell_natms[k];
+        if (mapp_dmd_set_system(sp, ctx->cell_H.data() + (size_t)k * 9, n, ctx->cell_x.data() + xo * xc,
+                                ctx->cell_c.data() + xo * NT) != 0 ||
+            mapp_dmd_init(sp, skin) != 0)
+            throw ApiError(MAPP_DMD_E_ARG, "cell " + std::to_string(k) + ": " + sp->err);
+        xo += n;
+        own_base[k + 1] = own_base[k] + sp->natms;
+        gh_base[k + 1] = gh_base[k] + sp->nghost;
+        pair_base[k + 1] = pair_base[k] + sp->npairs;
+    }
+    const int natms = own_base[ncell], nghost = gh_base[ncell];
+    require(pair_base[ncell] < (1L << 31), MAPP_DMD_E_RANGE, "batch exceeds 2^31 pairs");
+    hipStream_t st = ctx->stream;
+    ctx->natms = natms;
+    ctx->nghost = nghost;
+    ctx->npairs = pair_base[ncell];
+    const int ntot = natms + nghost;
+    ctx->d_pos.reserve(ntot);
+    ctx->d_alpha.reserve((size_t)ntot * NT);
+    ctx->d_conc.reserve((size_t)ntot * NT);
+    ctx->d_ghost_owner.reserve(nghost + 1);
+    ctx->d_off.reserve(natms + 1);
+    ctx->d_nl.reserve(ctx->npairs + 1);
+    for (int k = 0; k < ncell; ++k) {
+        mapp_dmd* c = sub[k];
+        const int n = c->natms, g = c->nghost;
+        hipLaunchKernelGGL((k_merge_sites<NT>), dim3(nblk(n + g)), dim3(kBlock), 0, st, n, g, c->d_pos.p,
+                           c->d_alpha.p, c->d_conc.p, c->d_ghost_owner.p, own_base[k], natms + gh_base[k], gh_base[k],
+                           ctx->d_pos.p, ctx->d_alpha.p, ctx->d_conc.p, ctx->d_ghost_owner.p);
+        launch_check("k_merge_sites");
+        hipLaunchKernelGGL(k_merge_list, dim3(nblk(n)), dim3(kBlock), 0, st, n, c->d_off.p, c->d_nl.p, own_base[k],
+                           (int)pair_base[k], natms + gh_base[k], ctx->d_off.p, ctx->d_nl.p);
+        launch_check("k_merge_list");
+    }
+    const int last = (int)ctx->npairs;
+    HIP_OK(hipMemcpyAsync(ctx->d_off.p + natms, &last, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    ctx->swap_off.assign(1, 0);  // no box-wide ghost swaps: positions stay fixed in a batch
+    ctx->swap_dim.clear();
+    ctx->swap_dir.clear();
+    finish_lists<NT>(ctx);
+    ctx->cell_nghost.assign(ncell, 0);
+    for (int k = 0; k < ncell; ++k) ctx->cell_nghost[k] = gh_base[k + 1] - gh_base[k];
+}
 }  // namespace
 
 // =================================================================== C ABI
@@ -1335,6 +1460,7 @@ int mapp_dmd_set_precision(mapp_dmd_t* ctx, int mode) {
 
 int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms, const double* x, const double* c) {
     return guarded(ctx, [&]() {
+        ctx->cell_natms.clear();
         require(H && x && c && natms > 0, MAPP_DMD_E_ARG, "null input or empty system");
         for (int i = 0; i < 3; ++i)
             for (int j = i + 1; j < 3; ++j)
@@ -1396,6 +1522,13 @@ int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
         for (int d = 0; d < 3; ++d)
             require(ctx->bp.cut_ph_s[d] < 1.0, MAPP_DMD_E_RANGE,
                     "box thinner than cutoff+skin: needs more than one ghost swap per face");
+        if (!ctx->cell_natms.empty()) {
+            if (nt == 1) init_batch<1>(ctx, skin);
+            else init_batch<2>(ctx, skin);
+            sync_sizes(ctx);
+            ctx->inited = true;
+            return 0;
+        }
         ctx->d_pos.reserve(natms);
         ctx->d_alpha.reserve((size_t)natms * nt);
         ctx->d_conc.reserve((size_t)natms * nt);
@@ -1421,6 +1554,7 @@ int mapp_dmd_fin(mapp_dmd_t* ctx) {
 int mapp_dmd_set_x(mapp_dmd_t* ctx, const double* x_host, int* rebuilt) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede set_x");
+        require(ctx->cell_natms.empty(), MAPP_DMD_E_STATE, "positions are fixed in a batch of cells");
         HIP_OK(hipSetDevice(ctx->device));
         const size_t n = (size_t)ctx->natms * xcols(ctx);
         ctx->d_xaos.reserve(n);
@@ -1743,6 +1877,7 @@ int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full) {
 int mapp_dmd_min_lbfgs(mapp_dmd_t* ctx, int max_iter, int m_it, double e_tol, double* energies, long* counts) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede min_lbfgs");
+        require(ctx->cell_natms.empty(), MAPP_DMD_E_STATE, "positions are fixed in a batch of cells");
         require(max_iter >= 0 && m_it >= 0 && energies, MAPP_DMD_E_ARG, "bad min_lbfgs arguments");
         HIP_OK(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;
@@ -2035,6 +2170,121 @@ int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_
         if (counts) {
             counts[0] = rc ? rc : istep;
             counts[1] = S.n_calc_g;
+        }
+        return 0;
+    });
+}
+
+
+int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* ncand) {
+    return guarded(ctx, [&]() {
+        require(ctx->inited && ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE,
+                "autotune needs an initialised DMD context");
+        require(reps >= 1, MAPP_DMD_E_ARG, "reps >= 1");
+        HIP_OK(hipSetDevice(ctx->device));
+        const bool timing0 = ctx->timing;
+        ctx->timing = true;
+        // candidate schedules: waves per block x total waves (0 = one wavefront per site;
+        // otherwise a persistent grid of 1..16 waves per SIMD over the chip's 1024 SIMDs)
+        std::vector<mapp_dmd::Shape> cand;
+        for (int wpb : {1, 2, 4})
+            for (long waves : {0L, 1024L, 2048L, 4096L, 8192L, 16384L})
+                if (waves == 0 || waves < ctx->natms) cand.push_back({wpb, waves});
+        double en[7];
+        double best1 = 1e300, best2 = 1e300;
+        mapp_dmd::Shape b1 = ctx->sh1, b2 = ctx->sh2;
+        int row = 0;
+        for (const auto& c : cand) {
+            ctx->sh1 = c;
+            ctx->sh2 = c;
+            double t1 = 1e300, t2 = 1e300;
+            for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
+                if (ctx->nt == 1) dispatch_force<1>(ctx, 1, en);
+                else dispatch_force<2>(ctx, 1, en);
+                if (r == 0) continue;
+                t1 = std::min(t1, ctx->last_ms[0]);
+                t2 = std::min(t2, ctx->last_ms[2]);
+            }
+            if (report && row < cap) {
+                report[row * 4 + 0] = c.wpb;
+                report[row * 4 + 1] = (double)c.waves;
+                report[row * 4 + 2] = t1;
+                report[row * 4 + 3] = t2;
+            }
+            ++row;
+            if (t1 < best1) {
+                best1 = t1;
+                b1 = c;
+            }
+            if (t2 < best2) {
+                best2 = t2;
+                b2 = c;
+            }
+        }
+        ctx->sh1 = b1;
+        ctx->sh2 = b2;
+        ctx->timing = timing0;
+        if (ncand) *ncand = row;
+        return 0;
+    });
+}
+
+int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out) {
+    return guarded(ctx, [&]() {
+        require(out != nullptr, MAPP_DMD_E_ARG, "null out");
+        out[0] = ctx->sh1.wpb;
+        out[1] = (int)ctx->sh1.waves;
+        out[2] = ctx->sh2.wpb;
+        out[3] = (int)ctx->sh2.waves;
+        return 0;
+    });
+}
+
+int mapp_dmd_set_schedule(mapp_dmd_t* ctx, const int* in) {
+    return guarded(ctx, [&]() {
+        require(in != nullptr, MAPP_DMD_E_ARG, "null schedule");
+        for (int k : {0, 2}) require(in[k] == 1 || in[k] == 2 || in[k] == 4, MAPP_DMD_E_ARG, "waves per block 1, 2 or 4");
+        for (int k : {1, 3}) require(in[k] >= 0, MAPP_DMD_E_ARG, "negative wave count");
+        ctx->sh1 = {in[0], (long)in[1]};
+        ctx->sh2 = {in[2], (long)in[3]};
+        return 0;
+    });
+}
+
+
+int mapp_dmd_set_cells(mapp_dmd_t* ctx, int ncell, const double* H, const int* natms, const double* x,
+                       const double* c) {
+    return guarded(ctx, [&]() {
+        require(ncell >= 1 && H && natms && x && c, MAPP_DMD_E_ARG, "bad cells");
+        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_ARG, "batches hold DMD cells");
+        const int nt = ctx->nt, xc = xcols(ctx);
+        long tot = 0;
+        for (int k = 0; k < ncell; ++k) {
+            require(natms[k] > 0, MAPP_DMD_E_ARG, "empty cell");
+            tot += natms[k];
+        }
+        require(tot < (1L << 30), MAPP_DMD_E_RANGE, "too many sites");
+        ctx->cell_natms.assign(natms, natms + ncell);
+        ctx->cell_H.assign(H, H + (size_t)ncell * 9);
+        ctx->cell_x.assign(x, x + (size_t)tot * xc);
+        ctx->cell_c.assign(c, c + (size_t)tot * nt);
+        ctx->natms = (int)tot;
+        ctx->inited = false;
+        ctx->have_list2 = false;
+        ctx->cache_valid = false;
+        ctx->have_system = true;
+        return 0;
+    });
+}
+
+int mapp_dmd_cells(mapp_dmd_t* ctx, int* ncell, int* natms, int* nghost, int cap) {
+    return guarded(ctx, [&]() {
+        require(ncell != nullptr, MAPP_DMD_E_ARG, "null ncell");
+        const int n = (int)ctx->cell_natms.size();
+        *ncell = n;
+        for (int k = 0; k < n && k < cap; ++k) {
+            if (natms) natms[k] = ctx->cell_natms[k];
+            if (nghost) nghost[k] = k < (int)ctx->cell_nghost.size() ? ctx->cell_nghost[k] : 0;
         }
         return 0;
     });

@@ -28,6 +28,9 @@ struct mapp_dmd {
     double skin = 0.5;
     int natms = 0, nghost = 0;
     std::vector<double> h_s, h_alpha, h_c;  // owned input rows until init
+    // batch of independent cells (empty: one cell from set_system)
+    std::vector<int> cell_natms, cell_nghost;
+    std::vector<double> cell_H, cell_x, cell_c;
     std::vector<int> swap_off, swap_dim, swap_dir;
 
     // device tables
@@ -66,6 +69,11 @@ struct mapp_dmd {
     mappdmd::SurNorm snorm{};
     bool sur_trained = false;
     bool energy_full = false;
+    // launch shapes of the two force passes (waves per block; total waves, 0 = one per site)
+    struct Shape {
+        int wpb = 4;
+        long waves = 0;
+    } sh1, sh2;
     mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors
     mappdmd::DevBuf<double> d_bdf;                            // BDF history and solver vectors  // energy_calc over every ordered type pair
 

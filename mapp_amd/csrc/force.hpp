@@ -368,7 +368,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int NV>
 __device__ __forceinline__ void block_store_waves(const double* v, double* __restrict__ out) {
     __shared__ double sh[kBlock / 64][NV];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) sh[w][k] = v[k];
@@ -378,11 +378,37 @@ __device__ __forceinline__ void block_store_waves(const double* v, double* __res
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
-            for (int q = 0; q < kBlock / 64; ++q) s += sh[q][k];
+            for (int q = 0; q < nw; ++q) s += sh[q][k];
             out[(size_t)blockIdx.x * NV + k] = s;
         }
     }
 }
+
+// Sites of one wavefront.  Dispatch places block b on XCD b % 8; the sites are
+// split into 8 contiguous ranges, one per XCD, so spatial neighbours share an
+// L2, and the waves of an XCD stride through its range.  With one wave per
+// site (the default grid) each wave takes one site; a smaller grid makes the
+// launch persistent.  Every wave of a block runs the same number of
+// iterations or fewer; kernels that synchronise a block inside the loop must
+// not use it.
+struct SiteLoop {
+    int first, end, stride;
+    __device__ __forceinline__ explicit SiteLoop(int n) {
+        const int wpb = blockDim.x >> 6, w = threadIdx.x >> 6;
+        const int nb = gridDim.x, b = blockIdx.x;
+        if (nb >= 8) {
+            const int x = b & 7, lb = b >> 3;
+            const int nbx = (nb - x + 7) >> 3;
+            first = (int)((long long)n * x / 8) + lb * wpb + w;
+            end = (int)((long long)n * (x + 1) / 8);
+            stride = nbx * wpb;
+        } else {
+            first = b * wpb + w;
+            end = n;
+            stride = nb * wpb;
+        }
+    }
+};
 
 // ---------------------------------------------------------------- pass 1
 // Wave per site, 4 lanes per ordered type pair: lane l = k*LPP + q*4 + part.
@@ -443,14 +469,15 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
     using Q2 = typename Vec2<Q>::T;
     constexpr int LPP = Pass1Map<NT>::LPP, PPW = Pass1Map<NT>::PPW;
     const int lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6));
     const int k = lane / LPP, q = (lane % LPP) >> 2, prt = lane & 3;
     const int it = q / NT, jt = q % NT;
     const bool active = !ENERGY_ONLY || FULL || jt >= it;
     const bool want_phi = it >= jt;
     const bool rho_lane = prt == 0, phi_lane = prt == 2;
-    double acc[2] = {0.0, 0.0};  // energy, pairs inside the quadrature cut
-    if (i < P.natms) {
+    double en = 0.0, npair = 0.0;  // energy, pairs inside the quadrature cut (this lane's share)
+    const SiteLoop sl(P.natms);
+    for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
+        const int i = __builtin_amdgcn_readfirstlane(i0);
         const double4 xi = pos[i];
         double ci[NT];
 #pragma unroll
@@ -458,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y;
         const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * 2;
-        double Ei = 0.0, en = 0.0, npair = 0.0;
+        double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int base = p0; base < p1; base += PPW) {
             const int p = base + k;
@@ -556,9 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
             const double s = wave_sum(rho_lane && it == t ? Ei : 0.0);
             if (lane == 0) Eown[(size_t)i * NT + t] = s;
         }
-        acc[0] = wave_sum(en);
-        acc[1] = wave_sum(npair);
     }
+    double acc[2] = {wave_sum(en), wave_sum(npair)};
     block_store_waves<2>(acc, part);
 }
 
@@ -617,12 +643,13 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
     constexpr int LPP = WaveMap<NT>::LPP, PPW = WaveMap<NT>::PPW;
     constexpr int XD = 3 + NT;
     const int lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane(xcd_block(gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6));
     const int k = lane / LPP, q = lane % LPP, it = q / NT, jt = q % NT;
     const int qT = jt * NT + it;
     const bool want_phi = it >= jt;
     double vir[6] = {0, 0, 0, 0, 0, 0};
-    if (i < P.natms) {
+    const SiteLoop sl(P.natms);
+    for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
+        const int i = __builtin_amdgcn_readfirstlane(i0);
         const double4 xi = pos[i];
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y, ci = ra.z, dEi = ra.w;
@@ -705,9 +732,9 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
 #pragma unroll
             for (int t = 0; t < NT; ++t) fown[(size_t)i * XD + 3 + t] = sa[t];
         }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
     block_store_waves<6>(vir, part);
 }
 

@@ -417,7 +417,9 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
 #pragma unroll
                     for (int ks = 0; ks < 2; ++ks)
                         a[ks] = *reinterpret_cast<const bf16x8_t*>(&act[m * 16 + lane % 16][ks * 32 + (lane / 16) * 8]);
-                    for (int n = 0; n < nN; ++n) {
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) {
+                        if (n >= nN) break;
                         floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                         for (int ks = 0; ks < 2; ++ks) {
@@ -433,7 +435,9 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const float* Pm = S.params + (size_t)m * kNParam;
-                    for (int n = 0; n < nN; ++n)
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) {
+                        if (n >= nN) break;
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
                             const int row = m * 16 + (lane / 16) * 4 + e, col = n * 16 + lane % 16;
@@ -444,6 +448,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                                 yout[row][col] = cacc[m][n][e] + Pm[kB4 + col];
                             }
                         }
+                    }
                 }
                 __syncthreads();
             }

@@ -57,6 +57,11 @@ _SIGS = {
     "mapp_dmd_last_timing": [CTX, DP],
     "mapp_dmd_enable_timing": [CTX, C.c_int],
     "mapp_dmd_set_energy_mode": [CTX, C.c_int],
+    "mapp_dmd_set_cells": [CTX, C.c_int, DP, IP, DP, DP],
+    "mapp_dmd_cells": [CTX, IP, IP, IP, C.c_int],
+    "mapp_dmd_autotune": [CTX, C.c_int, DP, C.c_int, IP],
+    "mapp_dmd_get_schedule": [CTX, IP],
+    "mapp_dmd_set_schedule": [CTX, IP],
     "mapp_dmd_bdf_run": [CTX, C.c_int, C.c_int, D, C.c_int, C.c_int, DP, LP],
     "mapp_dmd_min_lbfgs": [CTX, C.c_int, C.c_int, D, DP, LP],
     "mapp_dmd_surrogate_train": [CTX, C.c_long, C.c_int, C.c_int, D, C.c_ulonglong, D, DP, DP],
@@ -282,6 +287,39 @@ class ForceFieldDMD:
                    cnt.ctypes.data_as(LP))
         return int(cnt[3]), en[: cnt[0] + 2], int(cnt[1]), int(cnt[2])
 
+    def set_cells(self, Hs, xs, cs):
+        """a batch of independent cells: lists of boxes, reduced positions + alpha, concentrations"""
+        H = np.ascontiguousarray(np.stack([np.asarray(h, dtype=np.float64) for h in Hs]))
+        n = np.ascontiguousarray([len(x) for x in xs], dtype=np.int32)
+        x = np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)
+        c = np.ascontiguousarray(np.concatenate(cs), dtype=np.float64)
+        self._call("mapp_dmd_set_cells", len(xs), _dp(H), _ip(n), _dp(x), _dp(c))
+
+    def cells(self):
+        nc = C.c_int(0)
+        self._call("mapp_dmd_cells", C.byref(nc), None, None, 0)
+        na = np.zeros(nc.value, dtype=np.int32)
+        ng = np.zeros(nc.value, dtype=np.int32)
+        self._call("mapp_dmd_cells", C.byref(nc), _ip(na), _ip(ng), nc.value)
+        return na, ng
+
+    def autotune(self, reps=2):
+        """time the candidate launch schedules of the force passes; keeps the fastest.
+        Returns rows {waves per block, total waves (0: one per site), pass-1 ms, pass-2 ms}."""
+        rep = np.zeros((64, 4))
+        n = C.c_int(0)
+        self._call("mapp_dmd_autotune", int(reps), _dp(rep), 64, C.byref(n))
+        return rep[: n.value]
+
+    def schedule(self):
+        out = np.zeros(4, dtype=np.int32)
+        self._call("mapp_dmd_get_schedule", _ip(out))
+        return out.tolist()
+
+    def set_schedule(self, sched):
+        arr = np.ascontiguousarray(sched, dtype=np.int32)
+        self._call("mapp_dmd_set_schedule", _ip(arr))
+
     def bdf_run(self, nsteps, max_order=2, initial_del_t=-1.0, max_retry=100, fix=False):
         """Clock_BDF::init + run on the device: (return code, report [steps][order, del_t, cost, err, iters,
         solves], calc_g calls); the code is the number of steps or -1 / -2 as the C ABI documents"""
@@ -359,6 +397,19 @@ def write_cfg(path, H, x, c, masses, names):
                                 arr, err, 512)
     if rc != 0:
         raise DMDError(rc, err.value.decode())
+
+
+def setup_cells(lats, tables, no_i=3, variant="DMD_I", precision="fp64", skin=0.5, r_crd=3.0,
+                alpha_min=25.0, alpha_max=1000.0, T=600.0, device=0):
+    """a batch of independent cells (workload.Lattice each) in one context"""
+    nt = lats[0].c.shape[1]
+    ff = ForceFieldDMD(no_types=nt, variant=variant, device=device)
+    ff.set_fs(tables)
+    ff.coef(r_crd, no_i, alpha_min, alpha_max, T, tables.masses[:nt])
+    ff.set_precision(precision)
+    ff.set_cells([l.H for l in lats], [l.x for l in lats], [l.c for l in lats])
+    ff.init(skin)
+    return ff
 
 
 def setup_eam(H, s3, types, tables, skin=0.5, device=0):

@@ -1,0 +1,76 @@
+"""Batched cells (mapp_dmd_set_cells): one context, one launch per kernel
+over several independent cells of uneven size, against the FP64 oracle run
+on each cell alone.
+
+Bars: each cell's slice of the batch's f, E, dE, c_d within 1e-12 (infinity
+norm, relative) of that cell's oracle run; batch energy = sum of the cells'
+energies within 1e-12; ghosts counted per cell as the cell alone has them;
+results independent of the launch schedule picked by the autotuner (forces
+bit-identical: each site's sums run in its list order whatever the grid)."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_inf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch(tables):
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    from oracle import oracle as O
+    lats = [W.fcc_alloy(n, seed=10 + n) for n in (3, 5, 4, 6)]
+    ff = setup_cells(lats, tables, no_i=3, variant="DMD_I")
+    orcs = [O.make(l, tables, no_i=3, variant="DMD_I") for l in lats]
+    return lats, ff, orcs
+
+
+def test_batch_force_matches_cells(batch):
+    lats, ff, orcs = batch
+    en = ff.force_calc(1)
+    f = ff.get("f")
+    E = ff.get("E")
+    na, ng = ff.cells()
+    assert list(na) == [l.natms for l in lats]
+    assert list(ng) == [o.nghost() for o in orcs]
+    e_tot = 0.0
+    off = 0
+    for l, o in zip(lats, orcs):
+        eo = o.force_calc(1)
+        e_tot += eo[0]
+        sl = slice(off, off + l.natms)
+        assert rel_inf(f[sl], o.get("f")) <= 1e-12
+        assert rel_inf(E[sl], o.get("E")) <= 1e-12
+        off += l.natms
+    assert abs(en[0] - e_tot) <= 1e-12 * abs(e_tot)
+
+
+def test_batch_c_d_matches_cells(batch):
+    lats, ff, orcs = batch
+    ff.force_calc(1)
+    ff.create_2nd_neigh_lst()
+    ff.c_d_calc()
+    cd = ff.get("c_d")
+    off = 0
+    for l, o in zip(lats, orcs):
+        o.force_calc(1)
+        o.create_2nd_neigh_lst()
+        o.c_d_calc()
+        sl = slice(off, off + l.natms)
+        assert rel_inf(cd[sl], o.get("c_d")) <= 1e-12
+        off += l.natms
+
+
+def test_autotune_keeps_results(batch):
+    lats, ff, orcs = batch
+    ff.set_schedule([4, 0, 4, 0])
+    ff.force_calc(1)
+    f0 = ff.get("f").copy()
+    rows = ff.autotune(reps=1)
+    print("schedule", ff.schedule(), "candidates", len(rows), flush=True)
+    assert len(rows) >= 3
+    for sched in ([1, 64, 2, 128], [2, 0, 1, 0], ff.schedule()):
+        ff.set_schedule(sched)
+        ff.force_calc(1)
+        assert np.array_equal(ff.get("f").view(np.uint64), f0.view(np.uint64)), sched
