@@ -8,12 +8,13 @@ Gauss-Hermite rule, FP64 quadrature.  value = sites / step time.
 Also reported on the same system: the mixed path (5-point rule, FP32
 quadrature sums), the learned surrogate (trained here on 4M quadrature
 samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
-with neighbour rebuilds (consistent energy mode), the roofline of the
+with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
+(surrogate force in one launch per kernel, autotuned, 20 BDF steps), the roofline of the
 dominant kernel (pass 1, timed with HIP events on the library's stream) and
 the FP64 CPU oracle timed on a bounded sample (32^3 cells, one thread).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--no-surrogate] [--no-lbfgs] [--fp64-only]
+                    [--no-surrogate] [--no-lbfgs] [--no-batch] [--fp64-only]
     python bench.py --copy-calibration   (plain copy kernel for HBM counter scaling)
 """
 import argparse
@@ -106,6 +107,35 @@ def run_lbfgs(lat, tables, iters):
                 ms_per_step=dt * 1e3 / max(steps, 1), energy_first=float(en[0]), energy_last=float(en[-1]))
 
 
+def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
+    """96 independent cells, n^3 FCC cells with n log-uniform in [6, 40] (864 to 256,000 sites),
+    in one context: surrogate+FP32 force_calc over all cells in one launch per kernel (autotuned
+    schedule), then 20 BDF steps (orders up to 2) of dc/dt on the whole batch."""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    rng = np.random.default_rng(seed)
+    ns = np.clip(np.round(np.exp(rng.uniform(np.log(6), np.log(40), ncell))).astype(int), 6, 40)
+    lats = [W.fcc_alloy(int(n), seed=seed + k) for k, n in enumerate(ns)]
+    natms = int(sum(l.natms for l in lats))
+    t0 = time.perf_counter()
+    ff = setup_cells(lats, tables, no_i=5, precision="mixed")
+    build_s = time.perf_counter() - t0
+    rep = ff.surrogate_train(n_per_tp=1 << 20, epochs=2, batch_per_tp=1024, lr=2e-3, seed=3, tol=0.05)
+    ff.set_precision("surrogate")
+    rows = ff.autotune(reps=2)
+    r = time_force(ff, steps, warmup)
+    t0 = time.perf_counter()
+    code, brep, ncg = ff.bdf_run(bdf_steps, 2)
+    bdf_s = time.perf_counter() - t0
+    ff.close()
+    return dict(cells=ncell, sites=natms, sites_min=int(min(l.natms for l in lats)),
+                sites_max=int(max(l.natms for l in lats)), build_seconds=build_s,
+                force_site_evals_per_s=natms / (r["ms_per_step"] / 1e3), force_ms=r["ms_per_step"],
+                kernel_ms=r["kernel_ms"][:4], schedule_candidates=int(len(rows)), surrogate_val_err=rep["val_mean_err"],
+                bdf_steps=code, bdf_seconds=bdf_s, bdf_calc_g_calls=ncg,
+                bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [])
+
+
 def cpu_baseline(tables, n=32, reps=3):
     from mapp_amd import workload as W
     from oracle import oracle as O
@@ -153,6 +183,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -234,6 +265,9 @@ def main():
         if not args.no_lbfgs:
             out["lbfgs"] = run_lbfgs(lat, tables, 50)
             log("lbfgs", out["lbfgs"])
+        if not args.no_batch:
+            out["batch"] = run_batch(tables, max(3, args.steps // 4), 1)
+            log("batch", out["batch"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tables)
     print(json.dumps(out), flush=True)
