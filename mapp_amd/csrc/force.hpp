@@ -410,6 +410,21 @@ struct SiteLoop {
     }
 };
 
+// Pairs inside the cut, packed.  Each lane holds one list entry p of a 64-entry
+// chunk and whether it lies inside the cut; the inside entries move to the low
+// lanes in list order (a permutation through ds_permute: inside entries to
+// their rank, the others after them).  Returns their count; lane l < count of
+// *packed then holds the l-th inside entry.
+__device__ __forceinline__ int pack_in_range(int p, bool in, int* packed) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+    const int cnt = __popcll(m);
+    const int lane = threadIdx.x & 63;
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+    const int dest = in ? rank : cnt + (lane - rank);
+    *packed = __builtin_amdgcn_ds_permute(dest << 2, p);
+    return cnt;
+}
+
 // ---------------------------------------------------------------- pass 1
 // Wave per site, 4 lanes per ordered type pair: lane l = k*LPP + q*4 + part.
 // A spline row of one type pair holds rho c0..c3 and r*phi c0..c3; the four
@@ -487,9 +502,35 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
         const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * 2;
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
-        for (int base = p0; base < p1; base += PPW) {
-            const int p = base + k;
-            const bool valid = p < p1;
+        for (int chunk = p0; chunk < p1; chunk += 64) {
+            // one list entry per lane: entries outside the cut get a zero partner
+            // share; the entries inside it are packed to the low lanes
+            int packed;
+            int cnt;
+            {
+                const int pc = chunk + lane;
+                bool in = false;
+                if (pc < p1) {
+                    const int jc = L.nl[pc];
+                    const double4 xj = pos[jc];
+                    const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
+                    double r2;
+                    {
+#pragma clang fp contract(off)
+                        r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                    }
+                    in = r2 < P.cut_sq_mod;
+                    if (!in && jc < P.natms) {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) pairE[(size_t)rpos[pc] * NT + t] = 0.0;
+                    }
+                }
+                cnt = pack_in_range(pc, in, &packed);
+            }
+        for (int base = 0; base < cnt; base += PPW) {
+            const int slot = base + k;
+            const bool valid = slot < cnt;
+            const int p = __shfl(packed, valid ? slot : 0, 64);
             int j = 0;
             bool inr = false;
             double rsq = 0.0;
@@ -501,7 +542,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
 #pragma clang fp contract(off)
                     rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 }
-                inr = rsq < P.cut_sq_mod;
+                inr = true;  // inside the cut by construction
             }
             double ej = 0.0;
             if (inr) {
@@ -577,6 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
             }
             if constexpr (NT == 2) ej += dpp_f64<kDppRowRor8>(ej);  // sum over it (lanes 8 apart)
             if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
+        }
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -655,9 +697,33 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
         const double ai = ra.x, iai = ra.y, ci = ra.z, dEi = ra.w;
         double fx = 0.0, fy = 0.0, fz = 0.0, fa = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
-        for (int base = p0; base < p1; base += PPW) {
-            const int p = base + k;
-            const bool valid = p < p1;
+        for (int chunk = p0; chunk < p1; chunk += 64) {
+            int packed;
+            int cnt;
+            {
+                const int pc = chunk + lane;
+                bool in = false;
+                if (pc < p1) {
+                    const int jc = L.nl[pc];
+                    const double4 xj = pos[jc];
+                    const double e0 = xi.x - xj.x, e1 = xi.y - xj.y, e2 = xi.z - xj.z;
+                    double r2;
+                    {
+#pragma clang fp contract(off)
+                        r2 = e0 * e0 + e1 * e1 + e2 * e2;
+                    }
+                    in = r2 < P.cut_sq_mod;
+                    if (!in && jc < P.natms) {
+#pragma unroll
+                        for (int c = 0; c < XD; ++c) pairF[(size_t)rpos[pc] * XD + c] = 0.0;
+                    }
+                }
+                cnt = pack_in_range(pc, in, &packed);
+            }
+        for (int base = 0; base < cnt; base += PPW) {
+            const int slot = base + k;
+            const bool valid = slot < cnt;
+            const int p = __shfl(packed, valid ? slot : 0, 64);
             int j = 0;
             bool inr = false;
             double dx0 = 0, dx1 = 0, dx2 = 0, rsq = 0;
@@ -671,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
 #pragma clang fp contract(off)
                     rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 }
-                inr = rsq < P.cut_sq_mod;
+                inr = true;  // inside the cut by construction
             }
             const bool owned = j < P.natms;
             double S = 0.0, aJ = 0.0;
@@ -720,6 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
                 }
                 if (it == 0) dst[3 + jt] = aJ;
             }
+        }
         }
         const double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
         double sa[NT];
