@@ -360,9 +360,33 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
         const double ai = ra.x, iai = ra.y;
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
-        for (int base = p0; base < p1; base += 16) {
-            const int p = base + k;
-            const bool valid = p < p1;
+        for (int chunk = p0; chunk < p1; chunk += 64) {
+            int packed;
+            int cnt;
+            {
+                const int pc = chunk + lane;
+                bool in = false;
+                if (pc < p1) {
+                    const int jc = L.nl[pc];
+                    const double4 xj = pos[jc];
+                    const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
+                    double r2;
+                    {
+#pragma clang fp contract(off)
+                        r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                    }
+                    in = r2 < P.cut_sq_mod;
+                    if (!in && jc < P.natms) {
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) pairE[(size_t)rpos[pc] * NT + t] = 0.0;
+                    }
+                }
+                cnt = pack_in_range(pc, in, &packed);
+            }
+        for (int base = 0; base < cnt; base += 16) {
+            const int slot = base + k;
+            const bool valid = slot < cnt;
+            const int p = __shfl(packed, valid ? slot : 0, 64);
             int j = 0;
             bool inr = false;
             double rsq = 0.0;
@@ -374,7 +398,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
 #pragma clang fp contract(off)
                     rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 }
-                inr = rsq < P.cut_sq_mod;
+                inr = true;  // inside the cut by construction
             }
             double aj = 1.0, cj[NT] = {0.0, 0.0}, r = 1.0, r_inv = 1.0, ia = 1.0;
             int br = 0;
@@ -489,6 +513,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
             }
             ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
             if (valid && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
+        }
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
