@@ -191,12 +191,14 @@ int mapp_dmd_set_cells(mapp_dmd_t* ctx, int ncell, const double* H, const int* n
 int mapp_dmd_cells(mapp_dmd_t* ctx, int* ncell, int* natms, int* nghost, int cap);
 
 /* Launch schedules of the two force passes (no counterpart in the reference):
-   {waves per block (1, 2, 4), total waves (0 = one wavefront per site,
-   otherwise a persistent grid whose waves stride through XCD-contiguous
-   site ranges)} for pass 1 then pass 2.  mapp_dmd_autotune times every
+   {pass-1 lanes per (pair, type pair) (2 or 4), pass-1 waves per block
+   (1, 2, 4), pass-1 total waves (0 = one wavefront per site, otherwise a
+   persistent grid whose waves stride through XCD-contiguous site ranges),
+   pass-2 waves per block, pass-2 total waves}.  mapp_dmd_autotune times every
    candidate on the context's own system with HIP events (reps runs each
    after one warm-up force_calc) and keeps the fastest per pass; report
-   [cap][4] receives {waves per block, total waves, pass-1 ms, pass-2 ms}. */
+   [cap][5] receives {lanes per pair, waves per block, total waves, pass-1 ms,
+   pass-2 ms}. */
 int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* ncand);
 int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out);
 int mapp_dmd_set_schedule(mapp_dmd_t* ctx, const int* in);

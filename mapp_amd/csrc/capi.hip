@@ -467,9 +467,14 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
         }
     }
     if (!sur) {
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ,
-                           ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
-                           ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
+        if (ctx->sh1.parts == 2)
+            hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false, false, 2>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st,
+                               ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p,
+                               ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
+        else
+            hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false, false, 4>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st,
+                               ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p,
+                               ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
         launch_check("k_pass1");
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
@@ -525,12 +530,21 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     double* part1 = ctx->d_part.p;
     double* part2 = part1 + (size_t)nbw * 2;
     pack_atoms<NT>(ctx);
-    if (ctx->energy_full)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true>), dim3(nbw), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ,
-                           ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
-                           ctx->d_Eown.p, ctx->d_pairE.p, (Q*)nullptr, part1);
+    const dim3 g1(nbw), b1(64 * ctx->sh1.wpb);
+    if (ctx->energy_full && ctx->sh1.parts == 2)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true, 2>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p,
+                           L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p,
+                           (Q*)nullptr, part1);
+    else if (ctx->energy_full)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true, 4>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p,
+                           L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p,
+                           (Q*)nullptr, part1);
+    else if (ctx->sh1.parts == 2)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, false, 2>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, (Q*)nullptr, part1);
     else
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true>), dim3(nbw), dim3(64 * ctx->sh1.wpb), 0, st, ctx->P, PQ, ctx->d_pos.p,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, false, 4>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, (Q*)nullptr, part1);
     launch_check("k_pass1 energy");
@@ -2187,9 +2201,10 @@ int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* n
         // candidate schedules: waves per block x total waves (0 = one wavefront per site;
         // otherwise a persistent grid of 1..16 waves per SIMD over the chip's 1024 SIMDs)
         std::vector<mapp_dmd::Shape> cand;
-        for (int wpb : {1, 2, 4})
-            for (long waves : {0L, 1024L, 2048L, 4096L, 8192L, 16384L})
-                if (waves == 0 || waves < ctx->natms) cand.push_back({wpb, waves});
+        for (int parts : {4, 2})
+            for (int wpb : {1, 2, 4})
+                for (long waves : {0L, 1024L, 2048L, 4096L, 8192L, 16384L})
+                    if (waves == 0 || waves < ctx->natms) cand.push_back({wpb, waves, parts});
         double en[7];
         double best1 = 1e300, best2 = 1e300;
         mapp_dmd::Shape b1 = ctx->sh1, b2 = ctx->sh2;
@@ -2206,10 +2221,11 @@ int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* n
                 t2 = std::min(t2, ctx->last_ms[2]);
             }
             if (report && row < cap) {
-                report[row * 4 + 0] = c.wpb;
-                report[row * 4 + 1] = (double)c.waves;
-                report[row * 4 + 2] = t1;
-                report[row * 4 + 3] = t2;
+                report[row * 5 + 0] = c.parts;
+                report[row * 5 + 1] = c.wpb;
+                report[row * 5 + 2] = (double)c.waves;
+                report[row * 5 + 3] = t1;
+                report[row * 5 + 4] = t2;
             }
             ++row;
             if (t1 < best1) {
@@ -2232,10 +2248,11 @@ int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* n
 int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out) {
     return guarded(ctx, [&]() {
         require(out != nullptr, MAPP_DMD_E_ARG, "null out");
-        out[0] = ctx->sh1.wpb;
-        out[1] = (int)ctx->sh1.waves;
-        out[2] = ctx->sh2.wpb;
-        out[3] = (int)ctx->sh2.waves;
+        out[0] = ctx->sh1.parts;
+        out[1] = ctx->sh1.wpb;
+        out[2] = (int)ctx->sh1.waves;
+        out[3] = ctx->sh2.wpb;
+        out[4] = (int)ctx->sh2.waves;
         return 0;
     });
 }
@@ -2243,10 +2260,11 @@ int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out) {
 int mapp_dmd_set_schedule(mapp_dmd_t* ctx, const int* in) {
     return guarded(ctx, [&]() {
         require(in != nullptr, MAPP_DMD_E_ARG, "null schedule");
-        for (int k : {0, 2}) require(in[k] == 1 || in[k] == 2 || in[k] == 4, MAPP_DMD_E_ARG, "waves per block 1, 2 or 4");
-        for (int k : {1, 3}) require(in[k] >= 0, MAPP_DMD_E_ARG, "negative wave count");
-        ctx->sh1 = {in[0], (long)in[1]};
-        ctx->sh2 = {in[2], (long)in[3]};
+        require(in[0] == 2 || in[0] == 4, MAPP_DMD_E_ARG, "pass-1 lanes per (pair, type pair): 2 or 4");
+        for (int k : {1, 3}) require(in[k] == 1 || in[k] == 2 || in[k] == 4, MAPP_DMD_E_ARG, "waves per block 1, 2 or 4");
+        for (int k : {2, 4}) require(in[k] >= 0, MAPP_DMD_E_ARG, "negative wave count");
+        ctx->sh1 = {in[1], (long)in[2], in[0]};
+        ctx->sh2 = {in[3], (long)in[4], 4};
         return 0;
     });
 }

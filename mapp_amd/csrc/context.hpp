@@ -73,6 +73,7 @@ struct mapp_dmd {
     struct Shape {
         int wpb = 4;
         long waves = 0;
+        int parts = 4;  // pass 1: lanes per (pair, type pair)
     } sh1, sh2;
     mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors
     mappdmd::DevBuf<double> d_bdf;                            // BDF history and solver vectors  // energy_calc over every ordered type pair

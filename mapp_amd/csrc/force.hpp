@@ -426,13 +426,18 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* packed) {
 }
 
 // ---------------------------------------------------------------- pass 1
-// Wave per site, 4 lanes per ordered type pair: lane l = k*LPP + q*4 + part.
-// A spline row of one type pair holds rho c0..c3 and r*phi c0..c3; the four
-// part lanes load one quarter each (16 bytes in FP64), so each table row costs
-// one cache-line access instead of four.  Each part lane evaluates half of a
-// cubic (c_lo + c_hi p); one DPP exchange joins the halves (val = h0 + p^2 h1)
-// in lane part 0 (rho) and lane part 2 (phi), which then carry the weighted
-// Gauss-Hermite sums of their function.
+// Wave per site, PARTS lanes per ordered type pair: lane l = k*LPP + q*PARTS + part.
+// A spline row of one type pair holds rho c0..c3 and r*phi c0..c3.
+// PARTS = 4: the four part lanes load one quarter each (16 bytes in FP64), so
+// each table row costs one cache-line access; each evaluates half of a cubic
+// (c_lo + c_hi p) and one DPP exchange joins the halves (val = h0 + p^2 h1) in
+// lane part 0 (rho) and lane part 2 (phi).
+// PARTS = 2: part 0 loads the rho half of the row and evaluates its cubic,
+// part 1 the phi half: twice the pairs per wavefront iteration and half the
+// lanes repeating the point set-up, for one more cache-line access per row in
+// FP64 (in FP32 a half row is one 16-byte load).
+// The rho and phi lanes then carry the weighted Gauss-Hermite sums of their
+// function.
 //
 // ENERGY_ONLY restates energy_calc's triangle (jtype starts at itype, so phi
 // only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
@@ -440,9 +445,9 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* packed) {
 // contiguously.  Unless ENERGY_ONLY, the rho lane stores {drho/dr, drho/dalpha}
 // and the phi lane {dphi/dr, dphi/dalpha} at deriv[p][q] (the reference's
 // per-pair caches, src/ff_dmd_I.cpp:227-242, restricted to what the back-pass reads).
-template <int NT>
+template <int NT, int PARTS = 4>
 struct Pass1Map {
-    static constexpr int LPP = NT * NT * 4;
+    static constexpr int LPP = NT * NT * PARTS;
     static constexpr int PPW = 64 / LPP;
 };
 
@@ -473,7 +478,7 @@ constexpr int kDppRowRor8 = 0x128;  // row_ror:8 (rotate by 8 lanes within a row
 // ENERGY_ONLY: the loop of ForceField_DMD_I::energy_calc (src/ff_dmd_I.cpp:454-668),
 // type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
 // energy equals force_calc's.
-template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false>
+template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false, int PARTS = 4>
 __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
@@ -481,14 +486,17 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                                                   const double* __restrict__ rho7, const double* __restrict__ phi7,
                                                   double* __restrict__ Eown, double* __restrict__ pairE,
                                                   Q* __restrict__ deriv, double* __restrict__ part) {
+    static_assert(PARTS == 2 || PARTS == 4, "two or four lanes per (pair, type pair)");
     using Q2 = typename Vec2<Q>::T;
-    constexpr int LPP = Pass1Map<NT>::LPP, PPW = Pass1Map<NT>::PPW;
+    constexpr int LPP = Pass1Map<NT, PARTS>::LPP, PPW = Pass1Map<NT, PARTS>::PPW;
     const int lane = threadIdx.x & 63;
-    const int k = lane / LPP, q = (lane % LPP) >> 2, prt = lane & 3;
+    const int k = lane / LPP, q = (lane % LPP) / PARTS, prt = lane % PARTS;
     const int it = q / NT, jt = q % NT;
     const bool active = !ENERGY_ONLY || FULL || jt >= it;
     const bool want_phi = it >= jt;
-    const bool rho_lane = prt == 0, phi_lane = prt == 2;
+    // four parts: quarters of the row, rho in parts 0/1 and phi in 2/3 (results in 0 and 2);
+    // two parts: part 0 the rho cubic, part 1 the phi cubic
+    const bool rho_lane = prt == 0, phi_lane = prt == PARTS / 2;
     double en = 0.0, npair = 0.0;  // energy, pairs inside the quadrature cut (this lane's share)
     const SiteLoop sl(P.natms);
     for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
@@ -499,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
         for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y;
-        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * 2;
+        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * (8 / PARTS);
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
@@ -575,10 +583,17 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                         m = m > 0 ? m : 0;
                         pp -= (Q)m;
                         pp = pp < Q(1) ? pp : Q(1);
-                        const Q2 c = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
-                        const Q h = c.x + c.y * pp;
-                        const Q hb = dpp_xor1_q<Q>(h);
-                        const Q val = h + pp * pp * hb;
+                        Q val;
+                        if constexpr (PARTS == 4) {
+                            const Q2 c = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
+                            const Q h = c.x + c.y * pp;
+                            const Q hb = dpp_xor1_q<Q>(h);
+                            val = h + pp * pp * hb;
+                        } else {
+                            const Q2 lo = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
+                            const Q2 hi = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow + 2);
+                            val = ((hi.y * pp + hi.x) * pp + lo.y) * pp + lo.x;
+                        }
                         Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
                         x = inside ? x : Q(0);
                         s0 += PQ.w0[kp] * x;
@@ -611,12 +626,14 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                     en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v[0];
                 }
                 if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
-                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + prt;
+                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
                     d[0] = (Q)v[1];
                     d[1] = (Q)v[2];
                 }
             }
-            if constexpr (NT == 2) ej += dpp_f64<kDppRowRor8>(ej);  // sum over it (lanes 8 apart)
+            // sum over it: the (0, jt) and (1, jt) lanes of a pair are NT * PARTS apart
+            if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
+            if constexpr (NT == 2 && PARTS == 2) ej += __shfl_xor(ej, 4, 64);
             if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
         }
         }

@@ -305,14 +305,15 @@ class ForceFieldDMD:
 
     def autotune(self, reps=2):
         """time the candidate launch schedules of the force passes; keeps the fastest.
-        Returns rows {waves per block, total waves (0: one per site), pass-1 ms, pass-2 ms}."""
-        rep = np.zeros((64, 4))
+        Returns rows {lanes per pair, waves per block, total waves (0: one per site), pass-1 ms, pass-2 ms}."""
+        rep = np.zeros((64, 5))
         n = C.c_int(0)
         self._call("mapp_dmd_autotune", int(reps), _dp(rep), 64, C.byref(n))
         return rep[: n.value]
 
     def schedule(self):
-        out = np.zeros(4, dtype=np.int32)
+        """[pass-1 lanes per pair, pass-1 waves per block, pass-1 waves, pass-2 waves per block, pass-2 waves]"""
+        out = np.zeros(5, dtype=np.int32)
         self._call("mapp_dmd_get_schedule", _ip(out))
         return out.tolist()
 

@@ -64,13 +64,13 @@ def test_batch_c_d_matches_cells(batch):
 
 def test_autotune_keeps_results(batch):
     lats, ff, orcs = batch
-    ff.set_schedule([4, 0, 4, 0])
+    ff.set_schedule([4, 4, 0, 4, 0])
     ff.force_calc(1)
     f0 = ff.get("f").copy()
     rows = ff.autotune(reps=1)
     print("schedule", ff.schedule(), "candidates", len(rows), flush=True)
     assert len(rows) >= 3
-    for sched in ([1, 64, 2, 128], [2, 0, 1, 0], ff.schedule()):
+    for sched in ([4, 1, 64, 2, 128], [2, 2, 0, 1, 0], [2, 1, 96, 4, 0], ff.schedule()):
         ff.set_schedule(sched)
         ff.force_calc(1)
         assert np.array_equal(ff.get("f").view(np.uint64), f0.view(np.uint64)), sched

@@ -208,3 +208,20 @@ def test_read_fs_matches_set_fs(tmp_path, tables):
     c.init(0.5)
     ec = c.force_calc(1)
     assert abs(ec[0] - ea[0]) <= 1e-12 * abs(ea[0])
+
+
+@pytest.mark.parametrize("precision,no_i,tol", [("fp64", 3, 1e-12), ("mixed", 5, 1e-5)])
+def test_two_lane_pass1_schedule(tables, precision, no_i, tol):
+    """pass 1 with two lanes per (pair, type pair) and a persistent grid: same bars"""
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(6, seed=8)
+    ff, orc = _pair(lat, tables, no_i=no_i, precision=precision)
+    ff.set_schedule([2, 2, 256, 1, 128])
+    en_g = ff.force_calc(1)
+    en_o = orc.force_calc(1)
+    assert rel_inf(ff.get("f"), orc.get("f")) <= tol
+    assert abs(en_g[0] - en_o[0]) <= tol * abs(en_o[0])
+    e_g = ff.energy_calc()
+    e_o = orc.energy_calc()
+    assert abs(e_g - e_o) <= tol * abs(e_o)
+    ff.close()
