@@ -363,7 +363,7 @@ double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
     EamBufs T{ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_F7.p, ctx->nt};
     ctx->d_part.reserve((size_t)nb * 14 + 16);
     ctx->d_out.reserve(16);
-    ctx->d_drho.reserve(ctx->npairs + 1);
+    ctx->d_drho.reserve(2 * ((size_t)ctx->npairs + 1));  // one double2 per pair
     ctx->d_pairF2.reserve((size_t)(ctx->nrev + 1) * 3);
     double* part1 = ctx->d_part.p;
     double* part2 = part1 + (size_t)nb * 7;
