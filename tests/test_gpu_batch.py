@@ -33,7 +33,7 @@ def test_batch_force_matches_cells(batch):
     E = ff.get("E")
     na, ng = ff.cells()
     assert list(na) == [l.natms for l in lats]
-    assert list(ng) == [o.nghost() for o in orcs]
+    assert list(ng) == [o.nghost for o in orcs]
     e_tot = 0.0
     off = 0
     for l, o in zip(lats, orcs):

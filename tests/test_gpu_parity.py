@@ -197,17 +197,24 @@ def test_read_fs_matches_set_fs(tmp_path, tables):
     ea, eb = a.force_calc(1), b.force_calc(1)
     assert np.array_equal(ea, eb)
     assert np.array_equal(a.get("f"), b.get("f"))
-    # element order swapped in the call: types follow the names given
+    # element order swapped in the call: types follow the names given.  The
+    # energy is invariant under the relabelling only with one alpha per site:
+    # the shared phi slot takes alpha from the (larger, smaller) type indices
+    # (src/ff_dmd_I.cpp:213, :227-235), which a relabelling exchanges.
+    lat = W.fcc_alloy(6, seed=22, alpha_per_site=True)
+    a2 = setup(lat, tables)
+    ea = a2.force_calc(1)
     c = ForceFieldDMD(no_types=2, variant="DMD_I")
     c.read_fs(path, ["Cu", "Ni"])
     c.coef(3.0, 3, 25.0, 1000.0, 600.0, tables.masses[::-1])
-    lat2 = W.fcc_alloy(6, seed=22)
+    lat2 = W.fcc_alloy(6, seed=22, alpha_per_site=True)
     lat2.x[:, 3:] = lat.x[:, 4:2:-1]
     lat2.c = np.ascontiguousarray(lat.c[:, ::-1])
     c.set_system(lat2.H, lat2.x, lat2.c)
     c.init(0.5)
     ec = c.force_calc(1)
     assert abs(ec[0] - ea[0]) <= 1e-12 * abs(ea[0])
+    assert rel_inf(c.get("f")[:, :3], a2.get("f")[:, :3]) <= 1e-12
 
 
 @pytest.mark.parametrize("precision,no_i,tol", [("fp64", 3, 1e-12), ("mixed", 5, 1e-5)])
