@@ -5,7 +5,9 @@ whole system with inputs resident in HBM: binary FCC Ni-Cu, 64^3 cells
 (1,048,576 sites), Johnson-style EAM on 10k-point tables, 3-point
 Gauss-Hermite rule, FP64 quadrature.  value = sites / step time.
 
-Also reported on the same system: the mixed path (5-point rule, FP32
+Each path first runs the launch-schedule autotuner on its own system (or
+takes --schedule) and reports the schedule it kept.  Also reported on the
+same system: the mixed path (5-point rule, FP32
 quadrature sums), the learned surrogate (trained here on 4M quadrature
 samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
 with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
@@ -15,6 +17,7 @@ the FP64 CPU oracle timed on a bounded sample (32^3 cells, one thread).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-surrogate] [--no-lbfgs] [--no-batch] [--fp64-only]
+                    [--schedule parts,wpb,waves,wpb2,waves2]
     python bench.py --copy-calibration   (plain copy kernel for HBM counter scaling)
 """
 import argparse
@@ -67,10 +70,16 @@ def time_force(ff, steps, warmup):
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, kernel_ms=(ms_kernels / steps).tolist(), energy=float(en[0]))
 
 
-def run_path(lat, tables, no_i, precision, steps, warmup):
+def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None):
+    """force_calc timing after the launch-schedule autotuner (or with a given schedule)"""
     from mapp_amd.dmd import setup
     ff = setup(lat, tables, no_i=no_i, precision=precision)
+    if schedule is None:
+        ff.autotune(reps=2)
+    else:
+        ff.set_schedule(schedule)
     r = time_force(ff, steps, warmup)
+    r["schedule"] = ff.schedule()
     r["sizes"] = ff.sizes()
     r["f"] = ff.get("f")
     ff.close()
@@ -186,6 +195,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--schedule", type=str, default=None,
+                    help="fixed launch schedule 'parts,wpb,waves,wpb2,waves2' instead of the autotuner (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one, and this path keeps one rank's whole "
@@ -202,7 +213,8 @@ def main():
     natms = lat.natms
     log(f"system: {natms} sites")
 
-    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup)
+    sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
+    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched)
     log("fp64", {k: v for k, v in r64.items() if k != "f"})
     value = natms / (r64["ms_per_step"] / 1e3)
     P = r64["sizes"]["n_inrange"] / natms
@@ -213,7 +225,7 @@ def main():
     f1 = pass1_flops_per_site(P, nt, 3)
     achieved = f1 * natms / (pass1_ms / 1e3) / 1e12
     workload = f"DMD-I force_calc, FCC Ni-Cu {args.cells}^3 cells ({natms} sites), 3-point Gauss-Hermite, FP64"
-    traffic = traffic_for(f"k_pass1 fp64 q3 {args.cells}^3")
+    traffic = traffic_for(f"k_pass1 fp64 q3 {args.cells}^3 parts={r64['schedule'][0]}")
     out = {
         "metric": "DMD energy+grad site-evals/s at 1M sites: surrogate/FP32 vs FP64 quadrature",
         "value": value,
@@ -228,7 +240,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "sites": natms, "pairs_half_list": r64["sizes"]["npairs"],
-                   "pairs_in_quadrature_cut": r64["sizes"]["n_inrange"]},
+                   "pairs_in_quadrature_cut": r64["sizes"]["n_inrange"],
+                   "schedule": r64["schedule"]},
         "roofline": {"bound": "mfma", "pipe": "fp64 vector (78.6 TFLOP/s, same as the fp64 matrix peak)",
                      "kernel": "k_pass1", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
@@ -240,9 +253,8 @@ def main():
         "site_bytes": bytes_per_site(P_list, nt),
     }
     if not args.fp64_only:
-        rmix = run_path(lat, tables, 5, "mixed", args.steps, args.warmup)
-        f_ref = r64["f"]
-        r64q5 = run_path(lat, tables, 5, "fp64", max(2, args.steps // 4), 1)
+        rmix = run_path(lat, tables, 5, "mixed", args.steps, args.warmup, sched)
+        r64q5 = run_path(lat, tables, 5, "fp64", max(2, args.steps // 4), 1, sched)
         paths = {
             "fp64_q3": {"site_evals_per_s": value, "ms_per_step": r64["ms_per_step"]},
             "fp64_q5": {"site_evals_per_s": natms / (r64q5["ms_per_step"] / 1e3), "ms_per_step": r64q5["ms_per_step"],

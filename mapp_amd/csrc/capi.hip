@@ -190,6 +190,25 @@ void build_ghosts(mapp_dmd* ctx) {
 template <int NT>
 void finish_lists(mapp_dmd* ctx);
 
+__global__ void k_quirk_identity(int ntot, int nt, int* __restrict__ qidx) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)ntot * nt) return;
+    qidx[e] = (int)(e / nt + e % nt);
+}
+
+// the same element for the sites of cell k of a batch, numbered within the cell
+// (owned 0..n-1, ghosts n..n+g-1) and mapped back to batch indices
+__global__ void k_quirk_cell(int n, int g, int own_base, int gh_dst, int nt, int* __restrict__ qidx) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n + g) return;
+    const int gs = s < n ? own_base + s : gh_dst + (s - n);
+    for (int t = 0; t < nt; ++t) {
+        const int L = s + t, a = L / nt, tt = L % nt;
+        const int ga = a < n ? own_base + a : gh_dst + (a - n);
+        qidx[(size_t)gs * nt + t] = ga * nt + tt;
+    }
+}
+
 // ghosts + bins + s->x + half list + reverse CSR; positions hold s on entry
 template <int NT>
 void rebuild(mapp_dmd* ctx) {
@@ -259,6 +278,10 @@ void finish_lists(mapp_dmd* ctx) {
     ctx->d_rpos.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL(k_rpos, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff.p, ctx->d_rev.p, natms, ctx->d_rpos.p);
     launch_check("k_rpos");
+    // element read by the reference's dE[jatm + jtype] quirk: jatm + jtype (one cell)
+    ctx->d_qidx.reserve((size_t)ntot * NT);
+    hipLaunchKernelGGL(k_quirk_identity, dim3(nblk((long)ntot * NT)), dim3(kBlock), 0, st, ntot, NT, ctx->d_qidx.p);
+    launch_check("k_quirk_identity");
     // per-atom vectors and pair buffers
     const size_t nv = (size_t)ntot * NT;
     for (auto* b : {&ctx->d_E, &ctx->d_dE, &ctx->d_ddE, &ctx->d_n, &ctx->d_s, &ctx->d_t, &ctx->d_v})
@@ -755,7 +778,7 @@ void run_c_d(mapp_dmd* ctx) {
     ghost_copy(ctx, ctx->d_dE.p, NT);
     constexpr int CLOSE = VAR == 2 ? 1 : 0;
     hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
-                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_n.p);
+                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
     launch_check("k_chem_pot");
     ghost_copy(ctx, ctx->d_n.p, NT);
     hipLaunchKernelGGL((k_c_d<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L2, ctx->d_n.p,
@@ -776,7 +799,7 @@ double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, doubl
     ghost_copy(ctx, ctx->d_dE.p, NT);
     constexpr int CLOSE = VAR == 2 ? 3 : 2;
     hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
-                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_n.p);
+                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
     launch_check("k_chem_pot g");
     ghost_copy(ctx, ctx->d_n.p, NT);
     ctx->d_part.reserve((size_t)nb * 9 + 16);
@@ -1300,6 +1323,13 @@ void init_batch(mapp_dmd* ctx, double skin) {
     ctx->swap_dim.clear();
     ctx->swap_dir.clear();
     finish_lists<NT>(ctx);
+    for (int k = 0; k < ncell; ++k) {
+        const int n = sub[k]->natms, g = sub[k]->nghost;
+        hipLaunchKernelGGL(k_quirk_cell, dim3(nblk(n + g)), dim3(kBlock), 0, st, n, g, own_base[k], natms + gh_base[k],
+                           NT, ctx->d_qidx.p);
+        launch_check("k_quirk_cell");
+    }
+    HIP_OK(hipStreamSynchronize(st));
     ctx->cell_nghost.assign(ncell, 0);
     for (int k = 0; k < ncell; ++k) ctx->cell_nghost[k] = gh_base[k + 1] - gh_base[k];
 }

@@ -68,7 +68,7 @@ struct mapp_dmd {
     mappdmd::DevBuf<unsigned> d_smask;
     mappdmd::SurNorm snorm{};
     bool sur_trained = false;
-    bool energy_full = false;
+    bool energy_full = false;  // energy_calc over every ordered type pair
     // launch shapes of the two force passes (waves per block; total waves, 0 = one per site)
     struct Shape {
         int wpb = 4;
@@ -76,7 +76,8 @@ struct mapp_dmd {
         int parts = 4;  // pass 1: lanes per (pair, type pair)
     } sh1, sh2;
     mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors
-    mappdmd::DevBuf<double> d_bdf;                            // BDF history and solver vectors  // energy_calc over every ordered type pair
+    mappdmd::DevBuf<double> d_bdf;  // BDF history and solver vectors
+    mappdmd::DevBuf<int> d_qidx;    // element read by the dE[jatm + jtype] index quirk
 
     // timing
     bool timing = false;

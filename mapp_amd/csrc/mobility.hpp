@@ -149,7 +149,9 @@ __global__ __launch_bounds__(kBlock) void k_chem_pot(FFParams<double> P, const d
                                                      const double* __restrict__ conc, RevBufs L,
                                                      const double* __restrict__ rhoc,
                                                      const double* __restrict__ phic, const double* __restrict__ dE,
-                                                     double* __restrict__ n) {
+                                                     const int* __restrict__ qidx, double* __restrict__ n) {
+    // qidx[j*NT + t] is the element that the reference's dE[jatm + jtype] reads for
+    // site j (jatm + t within the site's own cell numbering: j + t for one cell)
     constexpr int NPHI = NT * (NT + 1) / 2;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= P.natms) return;
@@ -162,7 +164,8 @@ __global__ __launch_bounds__(kBlock) void k_chem_pot(FFParams<double> P, const d
         for (int it = 0; it < NT; ++it)
 #pragma unroll
             for (int jt = 0; jt < NT; ++jt)
-                acc[jt] += conc[(size_t)I * NT + it] * (rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)] * dE[I + it] +
+                acc[jt] += conc[(size_t)I * NT + it] * (rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)] *
+                                                            dE[qidx[(size_t)I * NT + it]] +
                                                         phic[(size_t)p * NPHI + type2phi(it, jt)]);
     }
     for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
@@ -171,7 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_chem_pot(FFParams<double> P, const d
         for (int it = 0; it < NT; ++it)
 #pragma unroll
             for (int jt = 0; jt < NT; ++jt)
-                acc[it] += conc[(size_t)j * NT + jt] * (rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)] * dE[j + jt] +
+                acc[it] += conc[(size_t)j * NT + jt] * (rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)] *
+                                                            dE[qidx[(size_t)j * NT + jt]] +
                                                         phic[(size_t)p * NPHI + type2phi(it, jt)]);
     }
 #pragma unroll
