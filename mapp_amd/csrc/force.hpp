@@ -410,18 +410,29 @@ struct SiteLoop {
     }
 };
 
-// Pairs inside the cut, packed.  Each lane holds one list entry p of a 64-entry
-// chunk and whether it lies inside the cut; the inside entries move to the low
-// lanes in list order (a permutation through ds_permute: inside entries to
-// their rank, the others after them).  Returns their count; lane l < count of
-// *packed then holds the l-th inside entry.
-__device__ __forceinline__ int pack_in_range(int p, bool in, int* packed) {
+// Pairs inside the cut, packed.  Each lane holds one list entry of a 64-entry
+// chunk and whether it lies inside the cut; pack_dest gives the lane the entry
+// moves to (inside entries to their rank in list order, the others after
+// them), and packed() moves any per-entry value there through ds_permute, so
+// lane l < count then holds the l-th inside entry's values.
+__device__ __forceinline__ int pack_dest(bool in, int* count) {
     const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
     const int cnt = __popcll(m);
     const int lane = threadIdx.x & 63;
     const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
-    const int dest = in ? rank : cnt + (lane - rank);
-    *packed = __builtin_amdgcn_ds_permute(dest << 2, p);
+    *count = cnt;
+    return in ? rank : cnt + (lane - rank);
+}
+__device__ __forceinline__ int packed(int dest, int v) { return __builtin_amdgcn_ds_permute(dest << 2, v); }
+__device__ __forceinline__ double packed(int dest, double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_permute(dest << 2, (int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_ds_permute(dest << 2, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
+    int cnt;
+    *out = packed(pack_dest(in, &cnt), p);
     return cnt;
 }
 
@@ -513,16 +524,18 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
         for (int chunk = p0; chunk < p1; chunk += 64) {
             // one list entry per lane: entries outside the cut get a zero partner
             // share; the entries inside it are packed to the low lanes
-            int packed;
-            int cnt;
+            // entry, partner and squared distance of the inside pairs, packed
+            int pk_p, pk_j, cnt;
+            double pk_r2;
             {
                 const int pc = chunk + lane;
                 bool in = false;
+                int jc = 0;
+                double r2 = 0.0;
                 if (pc < p1) {
-                    const int jc = L.nl[pc];
+                    jc = L.nl[pc];
                     const double4 xj = pos[jc];
                     const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
-                    double r2;
                     {
 #pragma clang fp contract(off)
                         r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
@@ -533,25 +546,19 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                         for (int t = 0; t < NT; ++t) pairE[(size_t)rpos[pc] * NT + t] = 0.0;
                     }
                 }
-                cnt = pack_in_range(pc, in, &packed);
+                const int dest = pack_dest(in, &cnt);
+                pk_p = packed(dest, pc);
+                pk_j = packed(dest, jc);
+                pk_r2 = packed(dest, r2);
             }
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
             const bool valid = slot < cnt;
-            const int p = __shfl(packed, valid ? slot : 0, 64);
-            int j = 0;
-            bool inr = false;
-            double rsq = 0.0;
-            if (valid) {
-                j = L.nl[p];
-                const double4 xj = pos[j];
-                const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
-                {
-#pragma clang fp contract(off)
-                    rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-                }
-                inr = true;  // inside the cut by construction
-            }
+            const int src = valid ? slot : 0;
+            const int p = __shfl(pk_p, src, 64);
+            const int j = __shfl(pk_j, src, 64);
+            const double rsq = __shfl(pk_r2, src, 64);
+            const bool inr = valid;  // inside the cut by construction
             double ej = 0.0;
             if (inr) {
                 if ((lane % LPP) == 0) npair += 1.0;
@@ -715,16 +722,20 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
         double fx = 0.0, fy = 0.0, fz = 0.0, fa = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
-            int packed;
-            int cnt;
+            // entry, partner and separation of the inside pairs, packed
+            int pk_p, pk_j, cnt;
+            double pk_d0, pk_d1, pk_d2, pk_r2;
             {
                 const int pc = chunk + lane;
                 bool in = false;
+                int jc = 0;
+                double e0 = 0.0, e1 = 0.0, e2 = 0.0, r2 = 0.0;
                 if (pc < p1) {
-                    const int jc = L.nl[pc];
+                    jc = L.nl[pc];
                     const double4 xj = pos[jc];
-                    const double e0 = xi.x - xj.x, e1 = xi.y - xj.y, e2 = xi.z - xj.z;
-                    double r2;
+                    e0 = xi.x - xj.x;
+                    e1 = xi.y - xj.y;
+                    e2 = xi.z - xj.z;
                     {
 #pragma clang fp contract(off)
                         r2 = e0 * e0 + e1 * e1 + e2 * e2;
@@ -735,27 +746,23 @@ __global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const doub
                         for (int c = 0; c < XD; ++c) pairF[(size_t)rpos[pc] * XD + c] = 0.0;
                     }
                 }
-                cnt = pack_in_range(pc, in, &packed);
+                const int dest = pack_dest(in, &cnt);
+                pk_p = packed(dest, pc);
+                pk_j = packed(dest, jc);
+                pk_d0 = packed(dest, e0);
+                pk_d1 = packed(dest, e1);
+                pk_d2 = packed(dest, e2);
+                pk_r2 = packed(dest, r2);
             }
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
             const bool valid = slot < cnt;
-            const int p = __shfl(packed, valid ? slot : 0, 64);
-            int j = 0;
-            bool inr = false;
-            double dx0 = 0, dx1 = 0, dx2 = 0, rsq = 0;
-            if (valid) {
-                j = L.nl[p];
-                const double4 xj = pos[j];
-                dx0 = xi.x - xj.x;
-                dx1 = xi.y - xj.y;
-                dx2 = xi.z - xj.z;
-                {
-#pragma clang fp contract(off)
-                    rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-                }
-                inr = true;  // inside the cut by construction
-            }
+            const int src = valid ? slot : 0;
+            const int p = __shfl(pk_p, src, 64);
+            const int j = __shfl(pk_j, src, 64);
+            const double dx0 = __shfl(pk_d0, src, 64), dx1 = __shfl(pk_d1, src, 64), dx2 = __shfl(pk_d2, src, 64);
+            const double rsq = __shfl(pk_r2, src, 64);
+            const bool inr = valid;  // inside the cut by construction
             const bool owned = j < P.natms;
             double S = 0.0, aJ = 0.0;
             if (inr) {
