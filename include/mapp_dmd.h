@@ -191,7 +191,7 @@ int mapp_dmd_set_cells(mapp_dmd_t* ctx, int ncell, const double* H, const int* n
 int mapp_dmd_cells(mapp_dmd_t* ctx, int* ncell, int* natms, int* nghost, int cap);
 
 /* Launch schedules of the two force passes (no counterpart in the reference):
-   {pass-1 lanes per (pair, type pair) (2 or 4), pass-1 waves per block
+   {pass-1 lanes per (pair, type pair) (1, 2 or 4), pass-1 waves per block
    (1, 2, 4), pass-1 total waves (0 = one wavefront per site, otherwise a
    persistent grid whose waves stride through XCD-contiguous site ranges),
    pass-2 waves per block, pass-2 total waves}.  mapp_dmd_autotune times every

@@ -449,6 +449,27 @@ inline int shape_blocks(const mapp_dmd::Shape& sh, int natms) {
     return (int)std::max<long>(1, (w + sh.wpb - 1) / sh.wpb);
 }
 
+// pass 1 with the schedule's lanes per (pair, type pair)
+template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL>
+void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ, const PairBufs& L, Q* deriv,
+                  double* part) {
+    const dim3 blk(64 * ctx->sh1.wpb);
+    hipStream_t st = ctx->stream;
+    if (ctx->sh1.parts == 1)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, deriv, part);
+    else if (ctx->sh1.parts == 2)
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 2>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, deriv, part);
+    else
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 4>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_pairE.p, deriv, part);
+    launch_check("k_pass1");
+}
+
 template <int NT, typename Q, int NQ>
 void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st, bool sur = false) {
     hipStream_t st = ctx->stream;
@@ -490,14 +511,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
         }
     }
     if (!sur) {
-        if (ctx->sh1.parts == 2)
-            hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false, false, 2>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st,
-                               ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p,
-                               ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
-        else
-            hipLaunchKernelGGL((k_pass1<NT, Q, NQ, false, false, 4>), dim3(nb1), dim3(64 * ctx->sh1.wpb), 0, st,
-                               ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p,
-                               ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, deriv, part1);
+        launch_pass1<NT, Q, NQ, false, false>(ctx, dim3(nb1), quad, PQ, L, deriv, part1);
         launch_check("k_pass1");
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
@@ -553,23 +567,8 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     double* part1 = ctx->d_part.p;
     double* part2 = part1 + (size_t)nbw * 2;
     pack_atoms<NT>(ctx);
-    const dim3 g1(nbw), b1(64 * ctx->sh1.wpb);
-    if (ctx->energy_full && ctx->sh1.parts == 2)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true, 2>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p,
-                           L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p,
-                           (Q*)nullptr, part1);
-    else if (ctx->energy_full)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, true, 4>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p, ctx->d_arec.p,
-                           L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p,
-                           (Q*)nullptr, part1);
-    else if (ctx->sh1.parts == 2)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, false, 2>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                           ctx->d_pairE.p, (Q*)nullptr, part1);
-    else
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, true, false, 4>), g1, b1, 0, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
-                           ctx->d_pairE.p, (Q*)nullptr, part1);
+    if (ctx->energy_full) launch_pass1<NT, Q, NQ, true, true>(ctx, dim3(nbw), quad, PQ, L, (Q*)nullptr, part1);
+    else launch_pass1<NT, Q, NQ, true, false>(ctx, dim3(nbw), quad, PQ, L, (Q*)nullptr, part1);
     launch_check("k_pass1 energy");
     hipLaunchKernelGGL((k_embed<NT, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
                        L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p, ctx->d_fa.p,
@@ -2231,7 +2230,7 @@ int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* n
         // candidate schedules: waves per block x total waves (0 = one wavefront per site;
         // otherwise a persistent grid of 1..16 waves per SIMD over the chip's 1024 SIMDs)
         std::vector<mapp_dmd::Shape> cand;
-        for (int parts : {4, 2})
+        for (int parts : {4, 2, 1})
             for (int wpb : {1, 2, 4})
                 for (long waves : {0L, 1024L, 2048L, 4096L, 8192L, 16384L})
                     if (waves == 0 || waves < ctx->natms) cand.push_back({wpb, waves, parts});
@@ -2290,7 +2289,7 @@ int mapp_dmd_get_schedule(mapp_dmd_t* ctx, int* out) {
 int mapp_dmd_set_schedule(mapp_dmd_t* ctx, const int* in) {
     return guarded(ctx, [&]() {
         require(in != nullptr, MAPP_DMD_E_ARG, "null schedule");
-        require(in[0] == 2 || in[0] == 4, MAPP_DMD_E_ARG, "pass-1 lanes per (pair, type pair): 2 or 4");
+        require(in[0] == 1 || in[0] == 2 || in[0] == 4, MAPP_DMD_E_ARG, "pass-1 lanes per (pair, type pair): 1, 2 or 4");
         for (int k : {1, 3}) require(in[k] == 1 || in[k] == 2 || in[k] == 4, MAPP_DMD_E_ARG, "waves per block 1, 2 or 4");
         for (int k : {2, 4}) require(in[k] >= 0, MAPP_DMD_E_ARG, "negative wave count");
         ctx->sh1 = {in[1], (long)in[2], in[0]};

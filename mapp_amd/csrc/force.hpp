@@ -447,6 +447,9 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
 // part 1 the phi half: twice the pairs per wavefront iteration and half the
 // lanes repeating the point set-up, for one more cache-line access per row in
 // FP64 (in FP32 a half row is one 16-byte load).
+// PARTS = 1: one lane loads the whole row and evaluates both cubics.
+// Every variant evaluates a cubic as (c0 + c1 p) + p^2 (c2 + c3 p), so the
+// results do not depend on the variant.
 // The rho and phi lanes then carry the weighted Gauss-Hermite sums of their
 // function.
 //
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                                                   const double* __restrict__ rho7, const double* __restrict__ phi7,
                                                   double* __restrict__ Eown, double* __restrict__ pairE,
                                                   Q* __restrict__ deriv, double* __restrict__ part) {
-    static_assert(PARTS == 2 || PARTS == 4, "two or four lanes per (pair, type pair)");
+    static_assert(PARTS == 1 || PARTS == 2 || PARTS == 4, "one, two or four lanes per (pair, type pair)");
     using Q2 = typename Vec2<Q>::T;
     constexpr int LPP = Pass1Map<NT, PARTS>::LPP, PPW = Pass1Map<NT, PARTS>::PPW;
     const int lane = threadIdx.x & 63;
@@ -506,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
     const bool active = !ENERGY_ONLY || FULL || jt >= it;
     const bool want_phi = it >= jt;
     // four parts: quarters of the row, rho in parts 0/1 and phi in 2/3 (results in 0 and 2);
-    // two parts: part 0 the rho cubic, part 1 the phi cubic
+    // two parts: part 0 the rho cubic, part 1 the phi cubic; one part: both
     const bool rho_lane = prt == 0, phi_lane = prt == PARTS / 2;
     double en = 0.0, npair = 0.0;  // energy, pairs inside the quadrature cut (this lane's share)
     const SiteLoop sl(P.natms);
@@ -518,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
         for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y;
-        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + prt * (8 / PARTS);
+        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS));
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
@@ -570,14 +573,17 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                 const double r_inv = rsqrt(rsq), r = rsq * r_inv;
                 const double ia = iai + iaj, asq = rsqrt(ia);
                 const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
-                double v[3] = {0.0, 0.0, 0.0};  // rho (part 0) or phi (part 2): value, d/dr, d/dalpha
+                // rho (vr) and phi (vp): value, d/dr, d/dalpha.  One lane carries both
+                // (PARTS 1) or its rho / phi lane carries one of them.
+                double vr[3] = {0.0, 0.0, 0.0}, vp[3] = {0.0, 0.0, 0.0};
                 if (br == 1) {
                     const Q rq = (Q)r, iq = (Q)ia, aq = (Q)asq;
                     const Q inv = iq * aq;
                     const Q upper = (rq + PQ.rc) * aq, lower = (rq - PQ.rc) * aq;
                     const int nq = NQ > 0 ? NQ : PQ.no_i;
                     const bool rule = lower < PQ.xi[nq - 1];
-                    Q s0 = 0, s1 = 0, s2 = 0;
+                    Q s0 = 0, s1 = 0, s2 = 0;  // rho sums (PARTS 1) or this lane's function
+                    Q u0 = 0, u1 = 0, u2 = 0;  // phi sums (PARTS 1)
 #pragma unroll
                     for (int kp = 0; kp < (NQ > 0 ? NQ : kMaxPoints); ++kp) {
                         if (NQ == 0 && kp >= nq) break;
@@ -590,57 +596,100 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                         m = m > 0 ? m : 0;
                         pp -= (Q)m;
                         pp = pp < Q(1) ? pp : Q(1);
-                        Q val;
+                        const Q* row = tab + (size_t)m * kQuadRow;
+                        const Q p2 = pp * pp;
                         if constexpr (PARTS == 4) {
-                            const Q2 c = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
+                            const Q2 c = *reinterpret_cast<const Q2*>(row);
                             const Q h = c.x + c.y * pp;
                             const Q hb = dpp_xor1_q<Q>(h);
-                            val = h + pp * pp * hb;
+                            const Q val = h + p2 * hb;
+                            Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
+                            x = inside ? x : Q(0);
+                            s0 += PQ.w0[kp] * x;
+                            if (!ENERGY_ONLY) {
+                                s1 += PQ.w1[kp] * x;
+                                s2 += PQ.w2[kp] * x;
+                            }
+                        } else if constexpr (PARTS == 2) {
+                            const Q2 lo = *reinterpret_cast<const Q2*>(row);
+                            const Q2 hi = *reinterpret_cast<const Q2*>(row + 2);
+                            const Q val = (lo.x + lo.y * pp) + p2 * (hi.x + hi.y * pp);
+                            Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
+                            x = inside ? x : Q(0);
+                            s0 += PQ.w0[kp] * x;
+                            if (!ENERGY_ONLY) {
+                                s1 += PQ.w1[kp] * x;
+                                s2 += PQ.w2[kp] * x;
+                            }
                         } else {
-                            const Q2 lo = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow);
-                            const Q2 hi = *reinterpret_cast<const Q2*>(tab + (size_t)m * kQuadRow + 2);
-                            val = ((hi.y * pp + hi.x) * pp + lo.y) * pp + lo.x;
-                        }
-                        Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
-                        x = inside ? x : Q(0);
-                        s0 += PQ.w0[kp] * x;
-                        if (!ENERGY_ONLY) {
-                            s1 += PQ.w1[kp] * x;
-                            s2 += PQ.w2[kp] * x;
+                            const Q2 a0 = *reinterpret_cast<const Q2*>(row);
+                            const Q2 a1 = *reinterpret_cast<const Q2*>(row + 2);
+                            const Q2 b0 = *reinterpret_cast<const Q2*>(row + 4);
+                            const Q2 b1 = *reinterpret_cast<const Q2*>(row + 6);
+                            const Q vrho = (a0.x + a0.y * pp) + p2 * (a1.x + a1.y * pp);
+                            const Q vphi = (b0.x + b0.y * pp) + p2 * (b1.x + b1.y * pp);
+                            const Q xr = inside ? vrho * t : Q(0);
+                            const Q xp = inside ? (t < Q(0) ? -vphi : vphi) : Q(0);
+                            s0 += PQ.w0[kp] * xr;
+                            u0 += PQ.w0[kp] * xp;
+                            if (!ENERGY_ONLY) {
+                                s1 += PQ.w1[kp] * xr;
+                                s2 += PQ.w2[kp] * xr;
+                                u1 += PQ.w1[kp] * xp;
+                                u2 += PQ.w2[kp] * xp;
+                            }
                         }
                     }
                     if (rule) {
                         const Q sc = Q(kPiInSq) * (Q)r_inv;
-                        const Q v0 = s0 * sc;
-                        v[0] = (double)v0;
-                        if (!ENERGY_ONLY) {
-                            v[1] = (double)(-v0 * (Q)r_inv - Q(2) * aq * (s1 * sc));
-                            v[2] = (double)((Q(0.5) * v0 - s2 * sc) * iq);
+                        auto fin = [&](Q a0, Q a1, Q a2, double* v) {
+                            const Q v0 = a0 * sc;
+                            v[0] = (double)v0;
+                            if (!ENERGY_ONLY) {
+                                v[1] = (double)(-v0 * (Q)r_inv - Q(2) * aq * (a1 * sc));
+                                v[2] = (double)((Q(0.5) * v0 - a2 * sc) * iq);
+                            }
+                        };
+                        if constexpr (PARTS == 1) {
+                            fin(s0, s1, s2, vr);
+                            fin(u0, u1, u2, vp);
+                        } else {
+                            fin(s0, s1, s2, rho_lane ? vr : vp);
                         }
                     }
                 } else if (br == 2 && rsq < P.cut_sq) {
                     double g[3], f[3];
                     classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
                                    phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
-                    for (int c = 0; c < 3; ++c) v[c] = prt == 0 ? g[c] : f[c];
+                    for (int c = 0; c < 3; ++c) {
+                        vr[c] = g[c];
+                        vp[c] = f[c];
+                    }
                 }
                 if (rho_lane) {
-                    Ei += v[0] * cj[jt];
-                    ej = v[0] * ci[it];
+                    Ei += vr[0] * cj[jt];
+                    ej = vr[0] * ci[it];
                 }
                 if (phi_lane && want_phi && active) {
                     const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
-                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v[0];
+                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * vp[0];
                 }
-                if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
-                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
-                    d[0] = (Q)v[1];
-                    d[1] = (Q)v[2];
+                if (!ENERGY_ONLY) {
+                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4;
+                    if (rho_lane) {
+                        d[0] = (Q)vr[1];
+                        d[1] = (Q)vr[2];
+                    }
+                    if (phi_lane) {
+                        d[2] = (Q)vp[1];
+                        d[3] = (Q)vp[2];
+                    }
                 }
             }
             // sum over it: the (0, jt) and (1, jt) lanes of a pair are NT * PARTS apart
             if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
             if constexpr (NT == 2 && PARTS == 2) ej += __shfl_xor(ej, 4, 64);
+            if constexpr (NT == 2 && PARTS == 1) ej += __shfl_xor(ej, 2, 64);
             if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
         }
         }
