@@ -493,7 +493,7 @@ constexpr int kDppRowRor8 = 0x128;  // row_ror:8 (rotate by 8 lanes within a row
 // type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
 // energy equals force_calc's.
 template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false, int PARTS = 4>
-__global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
                                                   const int* __restrict__ rpos, const Q* __restrict__ quad,
@@ -574,8 +574,11 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                 const double ia = iai + iaj, asq = rsqrt(ia);
                 const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
                 // rho (vr) and phi (vp): value, d/dr, d/dalpha.  One lane carries both
-                // (PARTS 1) or its rho / phi lane carries one of them.
-                double vr[3] = {0.0, 0.0, 0.0}, vp[3] = {0.0, 0.0, 0.0};
+                // (PARTS 1); otherwise the lane's own function is in vr (rho lane) or vp
+                // (phi lane), which then share registers.
+                double v1[3] = {0.0, 0.0, 0.0}, v2[PARTS == 1 ? 3 : 1] = {0.0};
+                double* vr = v1;
+                double* vp = PARTS == 1 ? v2 : v1;
                 if (br == 1) {
                     const Q rq = (Q)r, iq = (Q)ia, aq = (Q)asq;
                     const Q inv = iq * aq;
@@ -650,20 +653,20 @@ __global__ __launch_bounds__(kBlock) void k_pass1(FFParams<double> P, FFParams<Q
                                 v[2] = (double)((Q(0.5) * v0 - a2 * sc) * iq);
                             }
                         };
-                        if constexpr (PARTS == 1) {
-                            fin(s0, s1, s2, vr);
-                            fin(u0, u1, u2, vp);
-                        } else {
-                            fin(s0, s1, s2, rho_lane ? vr : vp);
-                        }
+                        fin(s0, s1, s2, v1);
+                        if constexpr (PARTS == 1) fin(u0, u1, u2, v2);
                     }
                 } else if (br == 2 && rsq < P.cut_sq) {
                     double g[3], f[3];
                     classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
                                    phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
                     for (int c = 0; c < 3; ++c) {
-                        vr[c] = g[c];
-                        vp[c] = f[c];
+                        if constexpr (PARTS == 1) {
+                            v1[c] = g[c];
+                            v2[c] = f[c];
+                        } else {
+                            v1[c] = rho_lane ? g[c] : f[c];
+                        }
                     }
                 }
                 if (rho_lane) {
@@ -750,7 +753,7 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
 // lane's record, pairs with dE_i; phi for it < jt lives in the transposed
 // lane's record as in the reference's shared COMP slot).  No table lookups.
 template <int NT, typename Q>
-__global__ __launch_bounds__(kBlock) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
                                                   const int* __restrict__ rpos, const Q* __restrict__ deriv,
                                                   double* __restrict__ fown, double* __restrict__ pairF,
