@@ -554,31 +554,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 pk_j = packed(dest, jc);
                 pk_r2 = packed(dest, r2);
             }
+        // the partner record of the first slot; inside the loop the next slot's
+        // record is requested before the current slot's quadrature, so its latency
+        // hides behind the table loads (slot 0 stands in for slots past the end)
+        int p_n, j_n;
+        double rsq_n;
+        double4 rj_n;
+        double cj_n[NT];
+        {
+            const int src = k < cnt ? k : 0;
+            p_n = __shfl(pk_p, src, 64);
+            j_n = __shfl(pk_j, src, 64);
+            rsq_n = __shfl(pk_r2, src, 64);
+            rj_n = arec[(size_t)j_n * NT + jt];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cj_n[t] = arec[(size_t)j_n * NT + t].z;
+        }
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
             const bool valid = slot < cnt;
-            const int src = valid ? slot : 0;
-            const int p = __shfl(pk_p, src, 64);
-            const int j = __shfl(pk_j, src, 64);
-            const double rsq = __shfl(pk_r2, src, 64);
+            const int p = p_n, j = j_n;
+            const double rsq = rsq_n;
+            const double4 rj = rj_n;
+            double cj[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cj[t] = cj_n[t];
+            if (base + PPW < cnt) {
+                const int nslot = base + PPW + k;
+                const int src = nslot < cnt ? nslot : 0;
+                p_n = __shfl(pk_p, src, 64);
+                j_n = __shfl(pk_j, src, 64);
+                rsq_n = __shfl(pk_r2, src, 64);
+                rj_n = arec[(size_t)j_n * NT + jt];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) cj_n[t] = arec[(size_t)j_n * NT + t].z;
+            }
             const bool inr = valid;  // inside the cut by construction
             double ej = 0.0;
             if (inr) {
                 if ((lane % LPP) == 0) npair += 1.0;
-                const double4 rj = arec[(size_t)j * NT + jt];
-                double cj[NT];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : arec[(size_t)j * NT + t].z;
                 const double aj = rj.x, iaj = rj.y;
                 const double r_inv = rsqrt(rsq), r = rsq * r_inv;
                 const double ia = iai + iaj, asq = rsqrt(ia);
                 const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
-                // rho (vr) and phi (vp): value, d/dr, d/dalpha.  One lane carries both
-                // (PARTS 1); otherwise the lane's own function is in vr (rho lane) or vp
-                // (phi lane), which then share registers.
-                double v1[3] = {0.0, 0.0, 0.0}, v2[PARTS == 1 ? 3 : 1] = {0.0};
-                double* vr = v1;
-                double* vp = PARTS == 1 ? v2 : v1;
+                // v1: the lane's function (rho on the rho lane, phi on the phi lane) or,
+                // with one lane per pair, rho; v2: phi with one lane per pair.
+                // Each holds value, d/dr, d/dalpha.
+                double v1[3] = {0.0, 0.0, 0.0};
+                double v2[3] = {0.0, 0.0, 0.0};
                 if (br == 1) {
                     const Q rq = (Q)r, iq = (Q)ia, aq = (Q)asq;
                     const Q inv = iq * aq;
@@ -669,23 +692,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         }
                     }
                 }
-                if (rho_lane) {
-                    Ei += vr[0] * cj[jt];
-                    ej = vr[0] * ci[it];
-                }
-                if (phi_lane && want_phi && active) {
-                    const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
-                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * vp[0];
-                }
-                if (!ENERGY_ONLY) {
-                    Q* d = deriv + ((size_t)p * NT * NT + q) * 4;
-                    if (rho_lane) {
-                        d[0] = (Q)vr[1];
-                        d[1] = (Q)vr[2];
+                if constexpr (PARTS == 1) {
+                    Ei += v1[0] * cj[jt];
+                    ej = v1[0] * ci[it];
+                    if (want_phi && active) {
+                        const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
+                        en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v2[0];
                     }
-                    if (phi_lane) {
-                        d[2] = (Q)vp[1];
-                        d[3] = (Q)vp[2];
+                    if (!ENERGY_ONLY) {
+                        Q* d = deriv + ((size_t)p * NT * NT + q) * 4;
+                        d[0] = (Q)v1[1];
+                        d[1] = (Q)v1[2];
+                        d[2] = (Q)v2[1];
+                        d[3] = (Q)v2[2];
+                    }
+                } else {
+                    if (rho_lane) {
+                        Ei += v1[0] * cj[jt];
+                        ej = v1[0] * ci[it];
+                    }
+                    if (phi_lane && want_phi && active) {
+                        const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
+                        en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v1[0];
+                    }
+                    if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
+                        Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
+                        d[0] = (Q)v1[1];
+                        d[1] = (Q)v1[2];
                     }
                 }
             }
