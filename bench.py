@@ -70,8 +70,9 @@ def time_force(ff, steps, warmup):
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, kernel_ms=(ms_kernels / steps).tolist(), energy=float(en[0]))
 
 
-def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None):
-    """force_calc timing after the launch-schedule autotuner (or with a given schedule)"""
+def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobility=False):
+    """force_calc timing after the launch-schedule autotuner (or with a given schedule);
+    with mobility, also one c_d_calc and one calc_g (the +dc/dt evaluation)"""
     from mapp_amd.dmd import setup
     ff = setup(lat, tables, no_i=no_i, precision=precision)
     if schedule is None:
@@ -80,6 +81,20 @@ def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None):
         ff.set_schedule(schedule)
     r = time_force(ff, steps, warmup)
     r["schedule"] = ff.schedule()
+    if mobility:
+        ff.create_2nd_neigh_lst()
+        ff.c_d_calc()  # pair-value cache
+        reps = max(3, steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ff.c_d_calc()
+        r["c_d_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+        a = np.zeros_like(ff.get("c"))
+        ff.set_vec("a", a)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ff.calc_g(0, 1e-3)
+        r["calc_g_ms"] = (time.perf_counter() - t0) * 1e3 / reps
     r["sizes"] = ff.sizes()
     r["f"] = ff.get("f")
     ff.close()
@@ -214,7 +229,7 @@ def main():
     log(f"system: {natms} sites")
 
     sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
-    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched)
+    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched, mobility=not args.fp64_only)
     log("fp64", {k: v for k, v in r64.items() if k != "f"})
     value = natms / (r64["ms_per_step"] / 1e3)
     P = r64["sizes"]["n_inrange"] / natms
@@ -249,6 +264,8 @@ def main():
                      "flops_per_site": f1, "kernel_ms": pass1_ms,
                      "hbm_gbs": None if traffic is None else traffic["bytes_per_launch"] / (pass1_ms / 1e3) / 1e9},
         "kernel_ms": kms,
+        "c_d_calc_ms": r64.get("c_d_ms"),
+        "calc_g_ms": r64.get("calc_g_ms"),
         "site_flops": site_flops(P, nt, 3),
         "site_bytes": bytes_per_site(P_list, nt),
     }

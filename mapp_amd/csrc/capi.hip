@@ -476,7 +476,8 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     const int natms = ctx->natms;
     const int nb = nblk(natms);
     const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
-    const int nb1 = sur ? natms : shape_blocks(ctx->sh1, natms);  // the surrogate's pass 1: one wave per block
+    // the surrogate's pass 1: one wavefront per block, persistent over at most 4096 blocks
+    const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);
     ctx->d_out.reserve(16);
@@ -493,17 +494,17 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
             HIP_OK(hipMemsetAsync(ctx->d_nfall.p, 0, sizeof(int) * natms, st));
             const int q = ctx->Pf.no_i;
             if (q == 5)
-                hipLaunchKernelGGL((k_pass1_sur<5>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                hipLaunchKernelGGL((k_pass1_sur<5>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
                                    ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
                                    ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
                                    ctx->d_nfall.p);
             else if (q == 3)
-                hipLaunchKernelGGL((k_pass1_sur<3>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                hipLaunchKernelGGL((k_pass1_sur<3>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
                                    ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
                                    ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
                                    ctx->d_nfall.p);
             else
-                hipLaunchKernelGGL((k_pass1_sur<0>), dim3(natms), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
+                hipLaunchKernelGGL((k_pass1_sur<0>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
                                    ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
                                    ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
                                    ctx->d_nfall.p);

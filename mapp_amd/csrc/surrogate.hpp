@@ -326,11 +326,13 @@ struct SurDev {
     SurNorm N;
 };
 
-// pass 1 with the surrogate (DMD force, NT == 2): one wavefront per block and per
-// site, 16 pairs per chunk.  A block holds a single wavefront, so the LDS
-// hand-offs between the VALU layer and the MFMA layers synchronise with
-// __syncthreads() whatever the site's list length.
+// pass 1 with the surrogate (DMD force, NT == 2): one wavefront per block,
+// persistent over the sites of its XCD range, 16 pairs per chunk.  A block
+// holds a single wavefront, so the LDS hand-offs between the VALU layer and
+// the MFMA layers synchronise with __syncthreads() whatever the site's list
+// length.  Layer-1 weights and all biases sit in LDS for the block's life.
 constexpr int kSurBlock = 64;
+constexpr int kSurBlocks = 4096;  // persistent grid of the surrogate pass
 
 template <int NQ>
 __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
@@ -344,14 +346,26 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
     constexpr int NT = 2;
     __shared__ __attribute__((aligned(16))) __bf16 act[64][kHid + 8];  // 64 samples x 64 features (+pad)
     __shared__ float yout[64][8];
+    __shared__ float4 w1s[4][kHid];     // layer 1 of each network: {w_r, w_ai, w_aj, bias} per output
+    __shared__ float bls[4][3][kHid];   // biases of layers 2, 3 and 4
     const int lane = threadIdx.x;
-    const int i = xcd_block(gridDim.x);
+    for (int e = lane; e < 4 * kHid; e += kSurBlock) {
+        const int m = e / kHid, o = e % kHid;
+        const float* Pm = S.params + (size_t)m * kNParam;
+        w1s[m][o] = make_float4(Pm[kW1 + o], Pm[kW1 + kHid + o], Pm[kW1 + 2 * kHid + o], Pm[kB1 + o]);
+        bls[m][0][o] = Pm[kB2 + o];
+        bls[m][1][o] = Pm[kB3 + o];
+        bls[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
+    }
+    __syncthreads();
     const int tp = lane / 16, k = lane % 16, it = tp / NT, jt = tp % NT;
     const bool want_phi = it >= jt;
-    const float* Pp = S.params + (size_t)tp * kNParam;
     double en = 0.0, npair = 0.0;
-    int fallbacks = 0;
-    if (i < P.natms) {
+    // persistent: the block's single wavefront strides through its XCD's sites
+    const SiteLoop sl(P.natms);
+    for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
+        const int i = __builtin_amdgcn_readfirstlane(i0);
+        int fallbacks = 0;
         const double4 xi = pos[i];
         double ci[NT];
 #pragma unroll
@@ -420,14 +434,20 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 const int per_tp = kCellR * kCellA * kCellA / 32;
                 if ((S.mask[tp * per_tp + cell / 32] >> (cell % 32)) & 1u) use_net = false;
             }
-            // layer 1 on VALU: this lane's sample, 64 features -> LDS (bf16)
+            // layer 1 on VALU: this lane's sample, 64 features -> LDS (bf16, 8 per store)
             {
                 const float u0 = (rf - S.N.r_c) / S.N.r_s, u1 = (aif - S.N.a_c) / S.N.a_s,
                             u2 = (ajf - S.N.a_c) / S.N.a_s;
-                for (int o = 0; o < kHid; ++o) {
-                    const float z = Pp[kB1 + o] + u0 * Pp[kW1 + o] + u1 * Pp[kW1 + kHid + o] +
-                                    u2 * Pp[kW1 + 2 * kHid + o];
-                    act[lane][o] = (__bf16)silu_f(z);
+#pragma unroll
+                for (int o8 = 0; o8 < kHid; o8 += 8) {
+                    bf16x8_t h;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float4 w = w1s[tp][o8 + e];
+                        const float z = w.w + u0 * w.x + u1 * w.y + u2 * w.z;
+                        h[e] = (__bf16)silu_f(z);
+                    }
+                    *reinterpret_cast<bf16x8_t*>(&act[lane][o8]) = h;
                 }
             }
             __syncthreads();
@@ -458,18 +478,18 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 // D: lane holds rows (lane/16)*4 + e of tile m, column n*16 + lane%16
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const float* Pm = S.params + (size_t)m * kNParam;
 #pragma unroll
                     for (int n = 0; n < 4; ++n) {
                         if (n >= nN) break;
+                        const int col = n * 16 + lane % 16;
+                        const float bias = bls[m][layer][col];
 #pragma unroll
                         for (int e = 0; e < 4; ++e) {
-                            const int row = m * 16 + (lane / 16) * 4 + e, col = n * 16 + lane % 16;
+                            const int row = m * 16 + (lane / 16) * 4 + e;
                             if (layer < 2) {
-                                const float z = cacc[m][n][e] + Pm[(layer == 0 ? kB2 : kB3) + col];
-                                act[row][col] = (__bf16)silu_f(z);
+                                act[row][col] = (__bf16)silu_f(cacc[m][n][e] + bias);
                             } else if (col < kOut) {
-                                yout[row][col] = cacc[m][n][e] + Pm[kB4 + col];
+                                yout[row][col] = cacc[m][n][e] + bias;
                             }
                         }
                     }
@@ -520,12 +540,13 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
             const double s = wave_sum(it == t ? Ei : 0.0);
             if (lane == 0) Eown[(size_t)i * NT + t] = s;
         }
+        const double sf = wave_sum((double)fallbacks);
+        if (lane == 0) nfall[i] = (int)sf;
     }
-    const double se = wave_sum(en), sn = wave_sum(npair), sf = wave_sum((double)fallbacks);
+    const double se = wave_sum(en), sn = wave_sum(npair);
     if (lane == 0) {
         part[(size_t)blockIdx.x * 2 + 0] = se;
         part[(size_t)blockIdx.x * 2 + 1] = sn;
-        if (i < P.natms) nfall[i] = (int)sf;
     }
 }
 
