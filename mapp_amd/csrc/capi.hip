@@ -476,7 +476,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     const int natms = ctx->natms;
     const int nb = nblk(natms);
     const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
-    // the surrogate's pass 1: one wavefront per block, persistent over at most 4096 blocks
+    // the surrogate's pass 1: one wavefront per block, persistent over at most kSurBlocks blocks
     const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);

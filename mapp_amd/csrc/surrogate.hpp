@@ -332,7 +332,7 @@ struct SurDev {
 // the MFMA layers synchronise with __syncthreads() whatever the site's list
 // length.  Layer-1 weights and all biases sit in LDS for the block's life.
 constexpr int kSurBlock = 64;
-constexpr int kSurBlocks = 4096;  // persistent grid of the surrogate pass
+constexpr int kSurBlocks = 2048;  // persistent grid of the surrogate pass (8 wavefronts per CU)
 
 template <int NQ>
 __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
