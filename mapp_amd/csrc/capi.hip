@@ -712,10 +712,42 @@ void pair_values_dispatch(mapp_dmd* ctx) {
     }
 }
 
+RevBufs rev_half(mapp_dmd* ctx) { return RevBufs{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p, ctx->d_pair_i.p}; }
+RevBufs rev_second(mapp_dmd* ctx) {
+    return RevBufs{ctx->d_off2.p, ctx->d_nl2.p, ctx->d_roff2.p, ctx->d_rev2.p, ctx->d_pair_i2.p};
+}
+
+// site table of the pair values (both ends of every contributing pair)
+template <int NT>
+void build_site_table(mapp_dmd* ctx) {
+    hipStream_t st = ctx->stream;
+    const int natms = ctx->natms;
+    const RevBufs L = rev_half(ctx);
+    ctx->d_cnt.reserve(natms + 1);
+    hipLaunchKernelGGL((k_table_count<NT>), dim3(nblk(natms)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
+                       ctx->d_phic.p, ctx->d_cnt.p);
+    launch_check("k_table_count");
+    const long ne = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_toff, natms);
+    ctx->d_tidx.reserve(ne + 1);
+    ctx->d_trho.reserve((size_t)(ne + 1) * NT * NT);
+    ctx->d_tphi.reserve((size_t)(ne + 1) * NT * (NT + 1) / 2);
+    hipLaunchKernelGGL((k_table_fill<NT>), dim3(nblk(natms)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
+                       ctx->d_phic.p, ctx->d_toff.p, ctx->d_tidx.p, ctx->d_trho.p, ctx->d_tphi.p);
+    launch_check("k_table_fill");
+    ctx->ntable = ne;
+}
+
+SiteTable site_table(mapp_dmd* ctx) { return SiteTable{ctx->d_toff.p, ctx->d_tidx.p, ctx->d_trho.p, ctx->d_tphi.p}; }
+
 void ensure_pair_cache(mapp_dmd* ctx) {
     if (ctx->cache_valid) return;
-    if (ctx->nt == 1) pair_values_dispatch<1>(ctx);
-    else pair_values_dispatch<2>(ctx);
+    if (ctx->nt == 1) {
+        pair_values_dispatch<1>(ctx);
+        build_site_table<1>(ctx);
+    } else {
+        pair_values_dispatch<2>(ctx);
+        build_site_table<2>(ctx);
+    }
     ctx->cache_valid = true;
 }
 
@@ -747,6 +779,14 @@ void build_list2(mapp_dmd* ctx) {
     ctx->d_pair_i2.reserve(ctx->npairs2 + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, natms, ctx->d_pair_i2.p);
     launch_check("k_pair_owner 2");
+    // both ends of every 2nd-list pair per site
+    const RevBufs L2 = rev_second(ctx);
+    hipLaunchKernelGGL(k_list2_count, dim3(nblk(natms)), dim3(kBlock), 0, st, L2, natms, ctx->d_cnt.p);
+    launch_check("k_list2_count");
+    const long nf = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_foff2, natms);
+    ctx->d_fidx2.reserve(nf + 1);
+    hipLaunchKernelGGL(k_list2_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, L2, natms, ctx->d_foff2.p, ctx->d_fidx2.p);
+    launch_check("k_list2_fill");
     const int ntot = natms + ctx->nghost;
     ctx->d_crd.reserve(ntot);
     hipLaunchKernelGGL(k_crd, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_roff2.p, natms, ctx->d_crd.p);
@@ -760,29 +800,24 @@ void build_list2(mapp_dmd* ctx) {
     ctx->have_list2 = true;
 }
 
-RevBufs rev_half(mapp_dmd* ctx) { return RevBufs{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p, ctx->d_pair_i.p}; }
-RevBufs rev_second(mapp_dmd* ctx) {
-    return RevBufs{ctx->d_off2.p, ctx->d_nl2.p, ctx->d_roff2.p, ctx->d_rev2.p, ctx->d_pair_i2.p};
-}
-
-// c_d_calc: density and n from the pair cache, then rates over the 2nd list
+// c_d_calc: density and n from the site table, then rates over the 2nd list
 template <int NT, int VAR>
 void run_c_d(mapp_dmd* ctx) {
     hipStream_t st = ctx->stream;
-    const int nb = nblk(ctx->natms);
     ensure_pair_cache(ctx);
-    RevBufs L = rev_half(ctx), L2 = rev_second(ctx);
-    hipLaunchKernelGGL((k_density_cache<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L, ctx->d_rhoc.p,
-                       ctx->d_F9.p, ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, (double*)nullptr);
-    launch_check("k_density_cache");
+    const int nbt = nblk((long)ctx->natms * kTableG), nb2 = nblk((long)ctx->natms * kList2G);
+    const SiteTable T = site_table(ctx);
+    hipLaunchKernelGGL((k_density_table<NT>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, T, ctx->d_F9.p,
+                       ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, (double*)nullptr);
+    launch_check("k_density_table");
     ghost_copy(ctx, ctx->d_dE.p, NT);
     constexpr int CLOSE = VAR == 2 ? 1 : 0;
-    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
-                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
+    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
+                       T, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
     launch_check("k_chem_pot");
     ghost_copy(ctx, ctx->d_n.p, NT);
-    hipLaunchKernelGGL((k_c_d<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L2, ctx->d_n.p,
-                       ctx->d_cd.p);
+    hipLaunchKernelGGL((k_c_d<NT, VAR>), dim3(nb2), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, ctx->d_foff2.p,
+                       ctx->d_fidx2.p, ctx->d_n.p, ctx->d_cd.p);
     launch_check("k_c_d");
     HIP_OK(hipStreamSynchronize(st));
 }
@@ -790,39 +825,39 @@ void run_c_d(mapp_dmd* ctx) {
 template <int NT, int VAR>
 double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g) {
     hipStream_t st = ctx->stream;
-    const int nb = nblk(ctx->natms);
     ensure_pair_cache(ctx);
-    RevBufs L = rev_half(ctx), L2 = rev_second(ctx);
-    hipLaunchKernelGGL((k_density_cache<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, L, ctx->d_rhoc.p,
-                       ctx->d_F9.p, ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, ctx->d_ddE.p);
-    launch_check("k_density_cache g");
+    const int nbt = nblk((long)ctx->natms * kTableG), nb2 = nblk((long)ctx->natms * kList2G);
+    const SiteTable T = site_table(ctx);
+    hipLaunchKernelGGL((k_density_table<NT>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, T, ctx->d_F9.p,
+                       ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, ctx->d_ddE.p);
+    launch_check("k_density_table g");
     ghost_copy(ctx, ctx->d_dE.p, NT);
     constexpr int CLOSE = VAR == 2 ? 3 : 2;
-    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p, L,
-                       ctx->d_rhoc.p, ctx->d_phic.p, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
+    hipLaunchKernelGGL((k_chem_pot<NT, CLOSE>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p, ctx->d_conc.p,
+                       T, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
     launch_check("k_chem_pot g");
     ghost_copy(ctx, ctx->d_n.p, NT);
-    ctx->d_part.reserve((size_t)nb * 9 + 16);
+    ctx->d_part.reserve((size_t)nb2 * 9 + 16);
     ctx->d_out.reserve(16);
-    hipLaunchKernelGGL((k_calc_g_s<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, a, L2,
-                       ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, ctx->d_part.p);
+    hipLaunchKernelGGL((k_calc_g_s<NT, VAR>), dim3(nb2), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, a,
+                       ctx->d_foff2.p, ctx->d_fidx2.p, ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, ctx->d_part.p);
     launch_check("k_calc_g_s");
-    reduce_partials(ctx, ctx->d_part.p, nb, 1, 1, 0, ctx->d_out.p);
+    reduce_partials(ctx, ctx->d_part.p, nb2, 1, 1, 0, ctx->d_out.p);
     double inner = 0.0;
     HIP_OK(hipMemcpyAsync(&inner, ctx->d_out.p, sizeof(double), hipMemcpyDeviceToHost, st));
     if (!chk) {
         ghost_copy(ctx, ctx->d_s.p, NT);
-        hipLaunchKernelGGL((k_calc_g_t<NT, VAR>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, L2,
-                           ctx->d_n.p, ctx->d_s.p, ctx->d_t.p, g);
+        hipLaunchKernelGGL((k_calc_g_t<NT, VAR>), dim3(nb2), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p,
+                           ctx->d_foff2.p, ctx->d_fidx2.p, ctx->d_n.p, ctx->d_s.p, ctx->d_t.p, g);
         launch_check("k_calc_g_t");
         ghost_copy(ctx, ctx->d_t.p, NT);
-        hipLaunchKernelGGL((k_calc_g_v<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, L, ctx->d_rhoc.p,
-                           ctx->d_phic.p, ctx->d_dE.p, ctx->d_t.p, ctx->d_v.p, g);
+        hipLaunchKernelGGL((k_calc_g_v<NT>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, alpha_t, T, ctx->d_dE.p,
+                           ctx->d_t.p, ctx->d_v.p, g);
         launch_check("k_calc_g_v");
         ghost_copy(ctx, ctx->d_v.p, NT);
         ghost_copy(ctx, ctx->d_ddE.p, NT);
-        hipLaunchKernelGGL((k_calc_g_w<NT>), dim3(nb), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, L,
-                           ctx->d_rhoc.p, ctx->d_ddE.p, ctx->d_v.p, g);
+        hipLaunchKernelGGL((k_calc_g_w<NT>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, T,
+                           ctx->d_ddE.p, ctx->d_v.p, g);
         launch_check("k_calc_g_w");
     }
     HIP_OK(hipStreamSynchronize(st));

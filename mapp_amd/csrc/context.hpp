@@ -55,7 +55,11 @@ struct mapp_dmd {
     mappdmd::DevBuf<double> d_E, d_dE, d_ddE, d_n, d_s, d_t, d_v, d_cd, d_fa, d_Eown, d_a, d_g;
     mappdmd::DevBuf<double> d_f, d_fown;        // [natms][3+nt]
     mappdmd::DevBuf<double> d_pairE, d_pairF;   // [npairs][nt], [npairs][3+nt]
-    mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair value caches for c_d_calc / calc_g
+    mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair values for c_d_calc / calc_g, per half-list pair
+    mappdmd::DevBuf<int> d_toff, d_tidx;        // site table: both ends of every contributing pair
+    mappdmd::DevBuf<double> d_trho, d_tphi;
+    long ntable = 0;
+    mappdmd::DevBuf<int> d_foff2, d_fidx2;      // 2nd list of both ends (~owner for the partner end)
     bool cache_valid = false;
     mappdmd::DevBuf<double> d_part, d_out, d_xaos, d_stage;
     mappdmd::DevBuf<int> d_iflag;

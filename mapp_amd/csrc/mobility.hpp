@@ -2,17 +2,31 @@
 //   create_2nd_neigh_lst   src/ff_dmd_I.cpp:2138-2217
 //   c_d_calc  DMD_I / DMD_II / DMD   src/ff_dmd_I.cpp:2241-2398, src/ff_dmd_II.cpp:2186-2337, src/ff_dmd.cpp:2241-2398
 //   calc_g    DMD_I / DMD_II / DMD   src/ff_dmd_I.cpp:2402-2674, src/ff_dmd_II.cpp:2341-2630, src/ff_dmd.cpp:2402-2671
-// The reference sweeps the half list and scatters into both ends.  Each
-// kernel here is one lane per owned site that adds (a) its own half-list
-// rows and (b) the rows of lower-index sites that list it, found through the
-// reverse CSR, recomputing the pair term with the same operands in the same
-// roles, so every pair term is bitwise the reference's and only the order of
-// the per-site sums differs.  The index quirk of the chemical-potential sweep
-// (dE[jatm+jtype], dE[iatm+itype] instead of [atom*nt+type]) is kept.
+// The reference sweeps the half list and scatters every pair term into both
+// ends.  Here each site gathers the terms of all its pairs and never writes
+// another site's row, so there are no atomics and every sum has a fixed
+// order.
 //
-// The pair values rho_bar / phi_bar come from a per-pair cache (as the
-// reference's caches filled by force_calc): k_pair_values fills it for the
-// current positions with the same quadrature as force_calc.
+// Pair values.  rho_bar (every ordered type pair) and phi_bar depend on
+// positions and widths only; the reference caches them in force_calc.  Here
+// k_pair_values evaluates them per half-list pair with the same quadrature as
+// force_calc, and k_table_fill turns them into the site table: for each
+// owned site, the entries of its own half-list row, then the entries of the
+// lower-index sites that list it (ascending owner), each holding the partner
+// index and the pair's nt^2 + nt(nt+1)/2 values in the owner's orientation.
+// Pairs whose values are all zero (outside the cut, or the width pair out of
+// range) add nothing to any sum and are left out.  Seen from site i with
+// type m and a partner k of type b, each sweep's term is the same expression
+// whichever end owns the pair (slot(b, m) for the density, slot(m, b) for the
+// chemical potential and the last g sweep), so one loop covers both.
+//
+// Sweeps run G = 16 lanes per site: the lanes stride through the site's
+// entries (coalesced: consecutive lanes read consecutive entries), then a
+// fixed DPP rotation tree sums the lane partials.  The 2nd list (rates) runs
+// 4 lanes per site over its own table, where the entry's sign tells which end
+// owns the pair, since the rate terms are antisymmetric in the two ends.
+// The index quirk of the chemical-potential sweep (dE[jatm+jtype],
+// dE[iatm+itype] instead of [atom*nt+type]) is kept.
 #pragma once
 #include "force.hpp"
 
@@ -97,45 +111,148 @@ __global__ __launch_bounds__(kBlock) void k_pair_values(FFParams<double> P, FFPa
     }
 }
 
-// E = sum c rho_bar from the cache; n = E = F(E), dE = F', ddE = F''
-// (first sweep of c_d_calc, src/ff_dmd_I.cpp:2269-2312, and calc_g, :2447-2493)
+// ---------------------------------------------------------------- site table
+struct SiteTable {
+    const int* off;     // [natms + 1] entries of site i: off[i] .. off[i+1]
+    const int* idx;     // partner
+    const double* rho;  // [entry][nt*nt] rho_bar, slot type2rho of the owner's (it, jt)
+    const double* phi;  // [entry][nt(nt+1)/2] phi_bar
+};
+
 template <int NT>
-__global__ __launch_bounds__(kBlock) void k_density_cache(FFParams<double> P, const double* __restrict__ conc,
-                                                          RevBufs L, const double* __restrict__ rhoc,
-                                                          const double* __restrict__ F9, double* __restrict__ E,
-                                                          double* __restrict__ n, double* __restrict__ dE,
-                                                          double* __restrict__ ddE) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ bool pair_contributes(const double* __restrict__ rhoc, const double* __restrict__ phic,
+                                                 long p) {
+    constexpr int NPHI = NT * (NT + 1) / 2;
+    bool nz = false;
+#pragma unroll
+    for (int k = 0; k < NT * NT; ++k) nz |= rhoc[p * NT * NT + k] != 0.0;
+#pragma unroll
+    for (int k = 0; k < NPHI; ++k) nz |= phic[p * NPHI + k] != 0.0;
+    return nz;
+}
+
+// entries per site: contributing pairs of the own row and of the reverse CSR
+template <int NT>
+__global__ void k_table_count(RevBufs L, int natms, const double* __restrict__ rhoc,
+                              const double* __restrict__ phic, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    int c = 0;
+    for (int p = L.off[i]; p < L.off[i + 1]; ++p) c += pair_contributes<NT>(rhoc, phic, p) ? 1 : 0;
+    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) c += pair_contributes<NT>(rhoc, phic, L.rev[q]) ? 1 : 0;
+    cnt[i] = c;
+}
+
+template <int NT>
+__global__ void k_table_fill(RevBufs L, int natms, const double* __restrict__ rhoc, const double* __restrict__ phic,
+                             const int* __restrict__ toff, int* __restrict__ tidx, double* __restrict__ trho,
+                             double* __restrict__ tphi) {
+    constexpr int NPHI = NT * (NT + 1) / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    long e = toff[i];
+    auto put = [&](long p, int partner) {
+        tidx[e] = partner;
+#pragma unroll
+        for (int k = 0; k < NT * NT; ++k) trho[e * NT * NT + k] = rhoc[p * NT * NT + k];
+#pragma unroll
+        for (int k = 0; k < NPHI; ++k) tphi[e * NPHI + k] = phic[p * NPHI + k];
+        ++e;
+    };
+    for (int p = L.off[i]; p < L.off[i + 1]; ++p)
+        if (pair_contributes<NT>(rhoc, phic, p)) put(p, L.nl[p]);
+    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
+        const int p = L.rev[q];
+        if (pair_contributes<NT>(rhoc, phic, p)) put(p, L.pair_i[p]);
+    }
+}
+
+// 2nd list of both ends: own rows as partner j, reverse entries as ~owner
+__global__ void k_list2_count(RevBufs L2, int natms, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    cnt[i] = (L2.off[i + 1] - L2.off[i]) + (L2.roff[i + 1] - L2.roff[i]);
+}
+__global__ void k_list2_fill(RevBufs L2, int natms, const int* __restrict__ foff, int* __restrict__ fidx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    int e = foff[i];
+    for (int p = L2.off[i]; p < L2.off[i + 1]; ++p) fidx[e++] = L2.nl[p];
+    for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) fidx[e++] = ~L2.pair_i[L2.rev[q]];
+}
+
+// ---------------------------------------------------------------- lane groups
+// G consecutive lanes per site; blocks are remapped so that each XCD takes a
+// contiguous range of sites.
+template <int G>
+struct SiteGroup {
+    int site, l;
+    __device__ __forceinline__ SiteGroup() {
+        const int b = xcd_block(gridDim.x);
+        site = (b * kBlock + (int)threadIdx.x) / G;
+        l = threadIdx.x & (G - 1);
+    }
+};
+
+constexpr int kDppRowRor4 = 0x124;  // row_ror:4 (rotate by 4 lanes within a row of 16)
+
+// sum over the G lanes of a group (fixed tree; every lane of the group gets it)
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+    static_assert(G == 16 || G == 4, "lane groups of 4 or 16");
+    if constexpr (G == 16) {
+        v += dpp_f64<kDppRowRor8>(v);
+        v += dpp_f64<kDppRowRor4>(v);
+    }
+    v += dpp_f64<kDppXor2>(v);
+    v += dpp_f64<kDppXor1>(v);
+    return v;
+}
+
+constexpr int kTableG = 16;  // lanes per site over the pair table
+constexpr int kList2G = 4;   // lanes per site over the 2nd list
+
+// E = sum c rho_bar; n = E = F(E), dE = F', ddE = F''
+// (first sweep of c_d_calc, src/ff_dmd_I.cpp:2269-2312, and calc_g, :2447-2493).
+// Site i of type m, partner k of type b: E[i][m] += c[k][b] rho[slot(b, m)]
+// (own row :2285-2286 and the partner end :2287, with the reference's slot).
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_density_table(FFParams<double> P, const double* __restrict__ conc,
+                                                          SiteTable T, const double* __restrict__ F9,
+                                                          double* __restrict__ E, double* __restrict__ n,
+                                                          double* __restrict__ dE, double* __restrict__ ddE) {
+    const SiteGroup<kTableG> g;
+    const int i = g.site;
     if (i >= P.natms) return;
     double e[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) e[t] = 0.0;
-    // rows of lower-index sites listing i: E[i][jt] += c[I][it] * rho[slot(it,jt)]
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
-        const int p = L.rev[q], I = L.pair_i[p];
+    for (int q = T.off[i] + g.l; q < T.off[i + 1]; q += kTableG) {
+        const int k = T.idx[q];
+        double r[NT * NT], c[NT];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int x = 0; x < NT * NT; ++x) r[x] = T.rho[(size_t)q * NT * NT + x];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                e[jt] += conc[(size_t)I * NT + it] * rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)];
-    }
-    // own row: E[i][it] += c[j][jt] * rho[slot(jt,it)]
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
-        const int j = L.nl[p];
+        for (int b = 0; b < NT; ++b) c[b] = conc[(size_t)k * NT + b];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int m = 0; m < NT; ++m)
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                e[it] += conc[(size_t)j * NT + jt] * rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)];
+            for (int b = 0; b < NT; ++b) e[m] += c[b] * r[type2rho(NT, b, m)];
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < NT; ++t) e[t] = group_sum<kTableG>(e[t]);
+    if (g.l < NT) {
+        double et = e[0];
+#pragma unroll
+        for (int t = 1; t < NT; ++t)
+            if (g.l == t) et = e[t];
         double Fv, dF, ddF;
-        embed_F(P, F9, t, e[t], &Fv, &dF, &ddF);
-        E[(size_t)i * NT + t] = Fv;
-        n[(size_t)i * NT + t] = Fv;
-        dE[(size_t)i * NT + t] = dF;
-        if (ddE) ddE[(size_t)i * NT + t] = ddF;
+        embed_F(P, F9, g.l, et, &Fv, &dF, &ddF);
+        const size_t k = (size_t)i * NT + g.l;
+        E[k] = Fv;
+        n[k] = Fv;
+        dE[k] = dF;
+        if (ddE) ddE[k] = ddF;
     }
 }
 
@@ -144,44 +261,48 @@ __global__ __launch_bounds__(kBlock) void k_density_cache(FFParams<double> P, co
 // CLOSE 1: c_d_calc DMD_II        n =  n + c_0 + 1.5 kT log a
 // CLOSE 2: calc_g   DMD / DMD_I   n = (n + c_0) * beta + 1.5 log a
 // CLOSE 3: calc_g   DMD_II        n = (n + c_0) + 1.5 kT log a
+// Site i of type m, partner k of type b: c[k][b] (rho[slot(m, b)] dE[q(k, b)] + phi[(m, b)])
+// (src/ff_dmd_I.cpp:2336-2339 at both ends).
 template <int NT, int CLOSE>
 __global__ __launch_bounds__(kBlock) void k_chem_pot(FFParams<double> P, const double* __restrict__ alpha,
-                                                     const double* __restrict__ conc, RevBufs L,
-                                                     const double* __restrict__ rhoc,
-                                                     const double* __restrict__ phic, const double* __restrict__ dE,
-                                                     const int* __restrict__ qidx, double* __restrict__ n) {
+                                                     const double* __restrict__ conc, SiteTable T,
+                                                     const double* __restrict__ dE, const int* __restrict__ qidx,
+                                                     double* __restrict__ n) {
     // qidx[j*NT + t] is the element that the reference's dE[jatm + jtype] reads for
     // site j (jatm + t within the site's own cell numbering: j + t for one cell)
     constexpr int NPHI = NT * (NT + 1) / 2;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kTableG> g;
+    const int i = g.site;
     if (i >= P.natms) return;
     double acc[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = n[(size_t)i * NT + t];
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
-        const int p = L.rev[q], I = L.pair_i[p];
+    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+    for (int q = T.off[i] + g.l; q < T.off[i + 1]; q += kTableG) {
+        const int k = T.idx[q];
+        double r[NT * NT], f[NPHI], cd[NT];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int x = 0; x < NT * NT; ++x) r[x] = T.rho[(size_t)q * NT * NT + x];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                acc[jt] += conc[(size_t)I * NT + it] * (rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)] *
-                                                            dE[qidx[(size_t)I * NT + it]] +
-                                                        phic[(size_t)p * NPHI + type2phi(it, jt)]);
-    }
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
-        const int j = L.nl[p];
+        for (int x = 0; x < NPHI; ++x) f[x] = T.phi[(size_t)q * NPHI + x];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int b = 0; b < NT; ++b) cd[b] = dE[qidx[(size_t)k * NT + b]];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt)
-                acc[it] += conc[(size_t)j * NT + jt] * (rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)] *
-                                                            dE[qidx[(size_t)j * NT + jt]] +
-                                                        phic[(size_t)p * NPHI + type2phi(it, jt)]);
+        for (int m = 0; m < NT; ++m)
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+                acc[m] += conc[(size_t)k * NT + b] * (r[type2rho(NT, m, b)] * cd[b] + f[type2phi(m, b)]);
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const double la = log(alpha[(size_t)i * NT + t]);
-        double v = acc[t];
+    for (int t = 0; t < NT; ++t) acc[t] = group_sum<kTableG>(acc[t]);
+    if (g.l < NT) {
+        const int t = g.l;
+        double v = acc[0];
+#pragma unroll
+        for (int x = 1; x < NT; ++x)
+            if (t == x) v = acc[x];
+        const size_t k = (size_t)i * NT + t;
+        v += n[k];
+        const double la = log(alpha[k]);
         if (CLOSE == 0) {
             v += P.c0[t] + 1.5 * P.kbT * la;
             v *= P.beta;
@@ -195,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_chem_pot(FFParams<double> P, const d
             v += P.c0[t];
             v += 1.5 * P.kbT * la;
         }
-        n[(size_t)i * NT + t] = v;
+        n[k] = v;
     }
 }
 
@@ -223,69 +344,86 @@ __device__ __forceinline__ double rate_sij(const FFParams<double>& P, double ni,
     return ((1.0 - ci) * cj * t1 - (1.0 - cj) * ci * t0);
 }
 
-// c_d over the 2nd list
+// c_d over the 2nd list; an entry e >= 0 is the own-row partner j (pair (i, j)),
+// e < 0 the owner ~e of the pair (~e, i)
 template <int NT, int VAR>
-__global__ __launch_bounds__(kBlock) void k_c_d(FFParams<double> P, const double* __restrict__ conc, RevBufs L2,
+__global__ __launch_bounds__(kBlock) void k_c_d(FFParams<double> P, const double* __restrict__ conc,
+                                                const int* __restrict__ foff, const int* __restrict__ fidx,
                                                 const double* __restrict__ n, double* __restrict__ c_d) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kList2G> g;
+    const int i = g.site;
     if (i >= P.natms) return;
+    double acc[NT], ni[NT], ci[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const double ni = n[(size_t)i * NT + t], ci = conc[(size_t)i * NT + t];
-        double acc = 0.0;
-        for (int p = L2.off[i]; p < L2.off[i + 1]; ++p) {
-            const int j = L2.nl[p];
-            acc += rate_sij<VAR>(P, ni, n[(size_t)j * NT + t], ci, conc[(size_t)j * NT + t]);
+        acc[t] = 0.0;
+        ni[t] = n[(size_t)i * NT + t];
+        ci[t] = conc[(size_t)i * NT + t];
+    }
+    for (int q = foff[i] + g.l; q < foff[i + 1]; q += kList2G) {
+        const int e = fidx[q];
+        const bool own = e >= 0;
+        const int k = own ? e : ~e;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double nk = n[(size_t)k * NT + t], ck = conc[(size_t)k * NT + t];
+            const double sij = own ? rate_sij<VAR>(P, ni[t], nk, ci[t], ck) : rate_sij<VAR>(P, nk, ni[t], ck, ci[t]);
+            acc[t] += own ? sij : -sij;
         }
-        for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) {
-            const int I = L2.pair_i[L2.rev[q]];
-            acc -= rate_sij<VAR>(P, n[(size_t)I * NT + t], ni, conc[(size_t)I * NT + t], ci);
-        }
-        c_d[(size_t)i * NT + t] = acc;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = group_sum<kList2G>(acc[t]);
+    if (g.l == 0) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) c_d[(size_t)i * NT + t] = acc[t];
     }
 }
 
-// calc_g residual s = 2(c + a) + rate terms (and c_d for DMD_II); partial sums of s^2
+// calc_g residual s = 2(c + a) - 2 alpha (rate sums) (and c_d for DMD_II); partial sums of s^2
 template <int NT, int VAR>
 __global__ __launch_bounds__(kBlock) void k_calc_g_s(FFParams<double> P, double alpha_t,
                                                      const double* __restrict__ conc, const double* __restrict__ a,
-                                                     RevBufs L2, const double* __restrict__ n, double* __restrict__ s,
+                                                     const int* __restrict__ foff, const int* __restrict__ fidx,
+                                                     const double* __restrict__ n, double* __restrict__ s,
                                                      double* __restrict__ c_d, double* __restrict__ part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double acc2[1] = {0.0};
+    const SiteGroup<kList2G> g;
+    const int i = g.site;
+    double sq = 0.0;
     if (i < P.natms) {
+        double acc[NT], ni[NT], ci[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const size_t k = (size_t)i * NT + t;
-            const double ni = n[k], ci = conc[k];
-            double sv = 2.0 * (ci + a[k]);
-            double cd = 0.0;
-            for (int p = L2.off[i]; p < L2.off[i + 1]; ++p) {
-                const int j = L2.nl[p];
-                const double sij = rate_sij<VAR>(P, ni, n[(size_t)j * NT + t], ci, conc[(size_t)j * NT + t]);
-                if (VAR == 2) {
-                    sv -= 2.0 * alpha_t * sij;
-                    cd += sij;
-                } else {
-                    sv += -2.0 * alpha_t * sij;
-                }
+            acc[t] = 0.0;
+            ni[t] = n[(size_t)i * NT + t];
+            ci[t] = conc[(size_t)i * NT + t];
+        }
+        for (int q = foff[i] + g.l; q < foff[i + 1]; q += kList2G) {
+            const int e = fidx[q];
+            const bool own = e >= 0;
+            const int k = own ? e : ~e;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double nk = n[(size_t)k * NT + t], ck = conc[(size_t)k * NT + t];
+                const double sij =
+                    own ? rate_sij<VAR>(P, ni[t], nk, ci[t], ck) : rate_sij<VAR>(P, nk, ni[t], ck, ci[t]);
+                acc[t] += own ? sij : -sij;
             }
-            for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) {
-                const int I = L2.pair_i[L2.rev[q]];
-                const double sij = rate_sij<VAR>(P, n[(size_t)I * NT + t], ni, conc[(size_t)I * NT + t], ci);
-                if (VAR == 2) {
-                    sv += 2.0 * alpha_t * sij;
-                    cd -= sij;
-                } else {
-                    sv -= -2.0 * alpha_t * sij;
-                }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = group_sum<kList2G>(acc[t]);
+        if (g.l == 0) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const size_t k = (size_t)i * NT + t;
+                const double sv = 2.0 * (ci[t] + a[k]) - 2.0 * alpha_t * acc[t];
+                s[k] = sv;
+                if (VAR == 2) c_d[k] = acc[t];
+                sq += sv * sv;
             }
-            s[k] = sv;
-            if (VAR == 2) c_d[k] = cd;
-            acc2[0] += sv * sv;
         }
     }
-    block_sum_store<1>(acc2, part);
+    double v[1] = {sq};
+    block_sum_store<1>(v, part);
 }
 
 // pair terms of the t / g sweep for the pair (I, J) of type t in the reference's
@@ -332,125 +470,153 @@ __device__ __forceinline__ void tg_pair(const FFParams<double>& P, double alpha_
     *gJ = alpha_t * (t1 * (1.0 - cI) + cI * t0);
 }
 
+// t and the first g term over the 2nd list (src/ff_dmd_I.cpp:2576-2606)
 template <int NT, int VAR>
 __global__ __launch_bounds__(kBlock) void k_calc_g_t(FFParams<double> P, double alpha_t,
-                                                     const double* __restrict__ conc, RevBufs L2,
-                                                     const double* __restrict__ n, const double* __restrict__ s,
-                                                     double* __restrict__ t, double* __restrict__ g) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                     const double* __restrict__ conc, const int* __restrict__ foff,
+                                                     const int* __restrict__ fidx, const double* __restrict__ n,
+                                                     const double* __restrict__ s, double* __restrict__ t,
+                                                     double* __restrict__ g) {
+    const SiteGroup<kList2G> grp;
+    const int i = grp.site;
     if (i >= P.natms) return;
+    double tv[NT], gv[NT], ni[NT], ci[NT], si[NT];
 #pragma unroll
     for (int ty = 0; ty < NT; ++ty) {
         const size_t k = (size_t)i * NT + ty;
-        const double ni = n[k], ci = conc[k], si = s[k];
-        double tv = 0.0, gv = -si;
-        for (int p = L2.off[i]; p < L2.off[i + 1]; ++p) {
-            const size_t kj = (size_t)L2.nl[p] * NT + ty;
+        tv[ty] = 0.0;
+        gv[ty] = 0.0;
+        ni[ty] = n[k];
+        ci[ty] = conc[k];
+        si[ty] = s[k];
+    }
+    for (int q = foff[i] + grp.l; q < foff[i + 1]; q += kList2G) {
+        const int e = fidx[q];
+        const bool own = e >= 0;
+        const int k = own ? e : ~e;
+#pragma unroll
+        for (int ty = 0; ty < NT; ++ty) {
+            const size_t kk = (size_t)k * NT + ty;
+            const double nk = n[kk], ck = conc[kk], sk = s[kk];
             double tI, gI, tJ, gJ;
-            tg_pair<VAR>(P, alpha_t, ni, n[kj], ci, conc[kj], si, s[kj], &tI, &gI, &tJ, &gJ);
-            tv += tI;
-            gv += gI;
+            if (own) {
+                tg_pair<VAR>(P, alpha_t, ni[ty], nk, ci[ty], ck, si[ty], sk, &tI, &gI, &tJ, &gJ);
+                tv[ty] += tI;
+                gv[ty] += gI;
+            } else {
+                tg_pair<VAR>(P, alpha_t, nk, ni[ty], ck, ci[ty], sk, si[ty], &tI, &gI, &tJ, &gJ);
+                tv[ty] += tJ;
+                gv[ty] += gJ;
+            }
         }
-        for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) {
-            const size_t kI = (size_t)L2.pair_i[L2.rev[q]] * NT + ty;
-            double tI, gI, tJ, gJ;
-            tg_pair<VAR>(P, alpha_t, n[kI], ni, conc[kI], ci, s[kI], si, &tI, &gI, &tJ, &gJ);
-            tv += tJ;
-            gv += gJ;
+    }
+#pragma unroll
+    for (int ty = 0; ty < NT; ++ty) {
+        tv[ty] = group_sum<kList2G>(tv[ty]);
+        gv[ty] = group_sum<kList2G>(gv[ty]);
+    }
+    if (grp.l == 0) {
+#pragma unroll
+        for (int ty = 0; ty < NT; ++ty) {
+            const size_t k = (size_t)i * NT + ty;
+            t[k] = tv[ty];
+            g[k] = gv[ty] - si[ty];
         }
-        t[k] = tv;
-        g[k] = gv;
     }
 }
 
-// v and the first g sweep over the half list (src/ff_dmd_I.cpp:2610-2642)
+// v and the next g term over the pair table (src/ff_dmd_I.cpp:2610-2642).
+// Site i of type m, partner k of type b:
+//   v[i][m] += rho[slot(b, m)] t[k][b]
+//   g[i][m] += alpha beta (phi[(m, b)] + rho[slot(b, m)] dE[i][m] + rho[slot(m, b)] dE[k][b]) t[k][b]
 template <int NT>
-__global__ __launch_bounds__(kBlock) void k_calc_g_v(FFParams<double> P, double alpha_t, RevBufs L,
-                                                     const double* __restrict__ rhoc,
-                                                     const double* __restrict__ phic, const double* __restrict__ dE,
-                                                     const double* __restrict__ t, double* __restrict__ v,
-                                                     double* __restrict__ g) {
+__global__ __launch_bounds__(kBlock) void k_calc_g_v(FFParams<double> P, double alpha_t, SiteTable T,
+                                                     const double* __restrict__ dE, const double* __restrict__ t,
+                                                     double* __restrict__ v, double* __restrict__ g) {
     constexpr int NPHI = NT * (NT + 1) / 2;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kTableG> grp;
+    const int i = grp.site;
     if (i >= P.natms) return;
     const double ba = P.beta * alpha_t;
-    double vv[NT], gg[NT];
+    double vv[NT], gg[NT], dEi[NT];
 #pragma unroll
     for (int ty = 0; ty < NT; ++ty) {
         vv[ty] = 0.0;
-        gg[ty] = g[(size_t)i * NT + ty];
+        gg[ty] = 0.0;
+        dEi[ty] = dE[(size_t)i * NT + ty];
     }
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
-        const int j = L.nl[p];
+    for (int q = T.off[i] + grp.l; q < T.off[i + 1]; q += kTableG) {
+        const int k = T.idx[q];
+        double r[NT * NT], f[NPHI], tk[NT], dk[NT];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int x = 0; x < NT * NT; ++x) r[x] = T.rho[(size_t)q * NT * NT + x];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt) {
-                const double rji = rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)];
-                const double rij = rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)];
-                double tmp0 = phic[(size_t)p * NPHI + type2phi(jt, it)] + rji * dE[(size_t)i * NT + it] +
-                              rij * dE[(size_t)j * NT + jt];
-                tmp0 *= ba;
-                vv[it] += rji * t[(size_t)j * NT + jt];
-                gg[it] += tmp0 * t[(size_t)j * NT + jt];
-            }
-    }
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
-        const int p = L.rev[q], I = L.pair_i[p];
+        for (int x = 0; x < NPHI; ++x) f[x] = T.phi[(size_t)q * NPHI + x];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int b = 0; b < NT; ++b) {
+            tk[b] = t[(size_t)k * NT + b];
+            dk[b] = dE[(size_t)k * NT + b];
+        }
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt) {
-                const double rji = rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)];
-                const double rij = rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)];
-                double tmp0 = phic[(size_t)p * NPHI + type2phi(jt, it)] + rji * dE[(size_t)I * NT + it] +
-                              rij * dE[(size_t)i * NT + jt];
-                tmp0 *= ba;
-                vv[jt] += rij * t[(size_t)I * NT + it];
-                gg[jt] += tmp0 * t[(size_t)I * NT + it];
+        for (int m = 0; m < NT; ++m)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                const double rbm = r[type2rho(NT, b, m)];
+                const double tmp0 = (f[type2phi(m, b)] + rbm * dEi[m] + r[type2rho(NT, m, b)] * dk[b]) * ba;
+                vv[m] += rbm * tk[b];
+                gg[m] += tmp0 * tk[b];
             }
     }
 #pragma unroll
     for (int ty = 0; ty < NT; ++ty) {
-        v[(size_t)i * NT + ty] = vv[ty];
-        g[(size_t)i * NT + ty] = gg[ty];
+        vv[ty] = group_sum<kTableG>(vv[ty]);
+        gg[ty] = group_sum<kTableG>(gg[ty]);
+    }
+    if (grp.l == 0) {
+#pragma unroll
+        for (int ty = 0; ty < NT; ++ty) {
+            const size_t k = (size_t)i * NT + ty;
+            v[k] = vv[ty];
+            g[k] += gg[ty];
+        }
     }
 }
 
-// last g sweep: alpha beta rho ddE c v (src/ff_dmd_I.cpp:2647-2671)
+// last g term: alpha beta rho ddE c v (src/ff_dmd_I.cpp:2647-2671).
+// Site i of type m, partner k of type b: alpha beta rho[slot(m, b)] ddE[k][b] c[k][b] v[k][b]
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_calc_g_w(FFParams<double> P, double alpha_t,
-                                                     const double* __restrict__ conc, RevBufs L,
-                                                     const double* __restrict__ rhoc, const double* __restrict__ ddE,
-                                                     const double* __restrict__ v, double* __restrict__ g) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                     const double* __restrict__ conc, SiteTable T,
+                                                     const double* __restrict__ ddE, const double* __restrict__ v,
+                                                     double* __restrict__ g) {
+    const SiteGroup<kTableG> grp;
+    const int i = grp.site;
     if (i >= P.natms) return;
     const double ab = alpha_t * P.beta;
     double gg[NT];
 #pragma unroll
-    for (int ty = 0; ty < NT; ++ty) gg[ty] = g[(size_t)i * NT + ty];
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
-        const int j = L.nl[p];
+    for (int ty = 0; ty < NT; ++ty) gg[ty] = 0.0;
+    for (int q = T.off[i] + grp.l; q < T.off[i + 1]; q += kTableG) {
+        const int k = T.idx[q];
+        double r[NT * NT], w[NT];
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int x = 0; x < NT * NT; ++x) r[x] = T.rho[(size_t)q * NT * NT + x];
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt) {
-                const size_t kj = (size_t)j * NT + jt;
-                gg[it] += ab * rhoc[(size_t)p * NT * NT + type2rho(NT, it, jt)] * ddE[kj] * conc[kj] * v[kj];
-            }
-    }
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
-        const int p = L.rev[q], I = L.pair_i[p];
+        for (int b = 0; b < NT; ++b) {
+            const size_t kk = (size_t)k * NT + b;
+            w[b] = ddE[kk] * conc[kk] * v[kk];
+        }
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
+        for (int m = 0; m < NT; ++m)
 #pragma unroll
-            for (int jt = 0; jt < NT; ++jt) {
-                const size_t kI = (size_t)I * NT + it;
-                gg[jt] += ab * rhoc[(size_t)p * NT * NT + type2rho(NT, jt, it)] * ddE[kI] * conc[kI] * v[kI];
-            }
+            for (int b = 0; b < NT; ++b) gg[m] += ab * r[type2rho(NT, m, b)] * w[b];
     }
 #pragma unroll
-    for (int ty = 0; ty < NT; ++ty) g[(size_t)i * NT + ty] = gg[ty];
+    for (int ty = 0; ty < NT; ++ty) gg[ty] = group_sum<kTableG>(gg[ty]);
+    if (grp.l == 0) {
+#pragma unroll
+        for (int ty = 0; ty < NT; ++ty) g[(size_t)i * NT + ty] += gg[ty];
+    }
 }
 
 }  // namespace mappdmd
