@@ -83,7 +83,9 @@ def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobilit
     r["schedule"] = ff.schedule()
     if mobility:
         ff.create_2nd_neigh_lst()
-        ff.c_d_calc()  # pair-value cache
+        t0 = time.perf_counter()
+        ff.c_d_calc()  # first call: pair values and the site table
+        r["c_d_first_ms"] = (time.perf_counter() - t0) * 1e3
         reps = max(3, steps // 4)
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -209,6 +211,7 @@ def main():
     ap.add_argument("--no-lbfgs", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--schedule", type=str, default=None,
                     help="fixed launch schedule 'parts,wpb,waves,wpb2,waves2' instead of the autotuner (profiling)")
@@ -229,7 +232,7 @@ def main():
     log(f"system: {natms} sites")
 
     sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
-    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched, mobility=not args.fp64_only)
+    r64 = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched, mobility=args.mobility or not args.fp64_only)
     log("fp64", {k: v for k, v in r64.items() if k != "f"})
     value = natms / (r64["ms_per_step"] / 1e3)
     P = r64["sizes"]["n_inrange"] / natms
@@ -266,6 +269,7 @@ def main():
         "kernel_ms": kms,
         "c_d_calc_ms": r64.get("c_d_ms"),
         "calc_g_ms": r64.get("calc_g_ms"),
+        "c_d_calc_first_ms": r64.get("c_d_first_ms"),
         "site_flops": site_flops(P, nt, 3),
         "site_bytes": bytes_per_site(P_list, nt),
     }

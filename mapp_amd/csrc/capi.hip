@@ -474,7 +474,7 @@ template <int NT, typename Q, int NQ>
 void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st, bool sur = false) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
-    const int nb = nblk(natms);
+    const int nb = nblk((long)natms * kGatherG);             // embed and force gather: 16 lanes per site
     const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
     // the surrogate's pass 1: one wavefront per block, persistent over at most kSurBlocks blocks
     const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
@@ -560,7 +560,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
 template <int NT, typename Q, int NQ>
 double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     hipStream_t st = ctx->stream;
-    const int nb = nblk(ctx->natms);
+    const int nb = nblk((long)ctx->natms * kGatherG);
     const int nbw = shape_blocks(ctx->sh1, ctx->natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);

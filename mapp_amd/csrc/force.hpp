@@ -214,6 +214,9 @@ __device__ __forceinline__ double dpp_f64(double v) {
 constexpr int kDppTranspose = 0xD8;  // quad_perm [0,2,1,3]
 constexpr int kDppXor1 = 0xB1;       // quad_perm [1,0,3,2]
 constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kDppRowRor8 = 0x128;   // row_ror:8 (rotate by 8 lanes within a row of 16)
+constexpr int kDppRowRor4 = 0x124;   // row_ror:4
+constexpr int kDppRowRor12 = 0x12C;  // row_ror:12: lane l reads lane (l + 4) mod 16 of its row
 
 template <int NT>
 __device__ __forceinline__ double transpose_tp(double v) {
@@ -358,10 +361,19 @@ __global__ void k_pack_atoms(const double* __restrict__ alpha, const double* __r
     }
 }
 
+// sum over the wavefront, the same value in every lane: a DPP rotation tree
+// inside each row of 16 lanes, then the four row sums in a fixed order
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<kDppRowRor8>(v);
+    v += dpp_f64<kDppRowRor4>(v);
+    v += dpp_f64<kDppXor2>(v);
+    v += dpp_f64<kDppXor1>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // deterministic per-block sum of NV wave partials (lane 0 of each wave holds them)
@@ -487,7 +499,6 @@ __device__ __forceinline__ float dpp_xor1_q<float>(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppXor1, 0xF, 0xF, false));
 }
 
-constexpr int kDppRowRor8 = 0x128;  // row_ror:8 (rotate by 8 lanes within a row of 16)
 
 // ENERGY_ONLY: the loop of ForceField_DMD_I::energy_calc (src/ff_dmd_I.cpp:454-668),
 // type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
@@ -528,15 +539,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             // one list entry per lane: entries outside the cut get a zero partner
             // share; the entries inside it are packed to the low lanes
             // entry, partner and squared distance of the inside pairs, packed
-            int pk_p, pk_j, cnt;
+            int pk_p, pk_j, pk_rp, cnt;
             double pk_r2;
             {
                 const int pc = chunk + lane;
                 bool in = false;
-                int jc = 0;
+                int jc = 0, rp = 0;
                 double r2 = 0.0;
                 if (pc < p1) {
                     jc = L.nl[pc];
+                    rp = rpos[pc];  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
                     {
@@ -546,54 +558,48 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     in = r2 < P.cut_sq_mod;
                     if (!in && jc < P.natms) {
 #pragma unroll
-                        for (int t = 0; t < NT; ++t) pairE[(size_t)rpos[pc] * NT + t] = 0.0;
+                        for (int t = 0; t < NT; ++t) pairE[(size_t)rp * NT + t] = 0.0;
                     }
                 }
                 const int dest = pack_dest(in, &cnt);
                 pk_p = packed(dest, pc);
                 pk_j = packed(dest, jc);
+                pk_rp = packed(dest, rp);
                 pk_r2 = packed(dest, r2);
             }
         // the partner record of the first slot; inside the loop the next slot's
         // record is requested before the current slot's quadrature, so its latency
         // hides behind the table loads (slot 0 stands in for slots past the end)
-        int p_n, j_n;
-        double rsq_n;
-        double4 rj_n;
-        double cj_n[NT];
-        {
-            const int src = k < cnt ? k : 0;
+        int p_n, j_n, rp_n;
+        double rsq_n, aj_n, iaj_n, cj_n, co_n;  // partner alpha, 1/alpha, c of type jt, c of the other type
+        auto request = [&](int src) {
             p_n = __shfl(pk_p, src, 64);
             j_n = __shfl(pk_j, src, 64);
+            rp_n = __shfl(pk_rp, src, 64);
             rsq_n = __shfl(pk_r2, src, 64);
-            rj_n = arec[(size_t)j_n * NT + jt];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) cj_n[t] = arec[(size_t)j_n * NT + t].z;
-        }
+            const double* rj = reinterpret_cast<const double*>(arec + (size_t)j_n * NT + jt);
+            aj_n = rj[0];
+            iaj_n = rj[1];
+            cj_n = rj[2];
+            co_n = NT == 2 ? reinterpret_cast<const double*>(arec + (size_t)j_n * NT + (1 - jt))[2] : 0.0;
+        };
+        request(k < cnt ? k : 0);
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
             const bool valid = slot < cnt;
-            const int p = p_n, j = j_n;
-            const double rsq = rsq_n;
-            const double4 rj = rj_n;
+            const int p = p_n, j = j_n, rp = rp_n;
+            const double rsq = rsq_n, aj = aj_n, iaj = iaj_n;
             double cj[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) cj[t] = cj_n[t];
+            cj[0] = jt == 0 ? cj_n : co_n;
+            if constexpr (NT == 2) cj[1] = jt == 1 ? cj_n : co_n;
             if (base + PPW < cnt) {
                 const int nslot = base + PPW + k;
-                const int src = nslot < cnt ? nslot : 0;
-                p_n = __shfl(pk_p, src, 64);
-                j_n = __shfl(pk_j, src, 64);
-                rsq_n = __shfl(pk_r2, src, 64);
-                rj_n = arec[(size_t)j_n * NT + jt];
-#pragma unroll
-                for (int t = 0; t < NT; ++t) cj_n[t] = arec[(size_t)j_n * NT + t].z;
+                request(nslot < cnt ? nslot : 0);
             }
             const bool inr = valid;  // inside the cut by construction
             double ej = 0.0;
             if (inr) {
                 if ((lane % LPP) == 0) npair += 1.0;
-                const double aj = rj.x, iaj = rj.y;
                 const double r_inv = rsqrt(rsq), r = rsq * r_inv;
                 const double ia = iai + iaj, asq = rsqrt(ia);
                 const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
@@ -610,22 +616,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const bool rule = lower < PQ.xi[nq - 1];
                     Q s0 = 0, s1 = 0, s2 = 0;  // rho sums (PARTS 1) or this lane's function
                     Q u0 = 0, u1 = 0, u2 = 0;  // phi sums (PARTS 1)
+                    // all rows of the rule are requested before any is used, so their
+                    // latencies overlap (a point outside the window reads a clamped row)
+                    constexpr int KP = NQ > 0 ? NQ : kMaxPoints;
+                    constexpr int NV = PARTS == 4 ? 1 : (PARTS == 2 ? 2 : 4);
+                    Q2 rowv[KP][NV];
+                    Q ppv[KP];
 #pragma unroll
-                    for (int kp = 0; kp < (NQ > 0 ? NQ : kMaxPoints); ++kp) {
+                    for (int kp = 0; kp < KP; ++kp) {
                         if (NQ == 0 && kp >= nq) break;
-                        const Q xk = PQ.xi[kp];
-                        const bool inside = rule && xk > lower && xk < upper;
-                        const Q t = rq - xk * inv;
+                        const Q t = rq - PQ.xi[kp] * inv;
                         Q pp = fabs(t) * PQ.dr_inv;
                         int m = (int)pp;
                         m = m < P.nr - 2 ? m : P.nr - 2;
                         m = m > 0 ? m : 0;
                         pp -= (Q)m;
-                        pp = pp < Q(1) ? pp : Q(1);
+                        ppv[kp] = pp < Q(1) ? pp : Q(1);
                         const Q* row = tab + (size_t)m * kQuadRow;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) rowv[kp][v] = *reinterpret_cast<const Q2*>(row + 2 * v);
+                    }
+                    // keep the requests ahead of the arithmetic where the rows fit the
+                    // register budget of four waves per SIMD (at most 24 registers)
+                    if constexpr (KP * NV * sizeof(Q2) <= 96) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int kp = 0; kp < KP; ++kp) {
+                        if (NQ == 0 && kp >= nq) break;
+                        const Q xk = PQ.xi[kp];
+                        const bool inside = rule && xk > lower && xk < upper;
+                        const Q t = rq - xk * inv, pp = ppv[kp];
                         const Q p2 = pp * pp;
                         if constexpr (PARTS == 4) {
-                            const Q2 c = *reinterpret_cast<const Q2*>(row);
+                            const Q2 c = rowv[kp][0];
                             const Q h = c.x + c.y * pp;
                             const Q hb = dpp_xor1_q<Q>(h);
                             const Q val = h + p2 * hb;
@@ -637,8 +659,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                 s2 += PQ.w2[kp] * x;
                             }
                         } else if constexpr (PARTS == 2) {
-                            const Q2 lo = *reinterpret_cast<const Q2*>(row);
-                            const Q2 hi = *reinterpret_cast<const Q2*>(row + 2);
+                            const Q2 lo = rowv[kp][0];
+                            const Q2 hi = rowv[kp][1];
                             const Q val = (lo.x + lo.y * pp) + p2 * (hi.x + hi.y * pp);
                             Q x = prt == 0 ? val * t : (t < Q(0) ? -val : val);
                             x = inside ? x : Q(0);
@@ -648,10 +670,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                 s2 += PQ.w2[kp] * x;
                             }
                         } else {
-                            const Q2 a0 = *reinterpret_cast<const Q2*>(row);
-                            const Q2 a1 = *reinterpret_cast<const Q2*>(row + 2);
-                            const Q2 b0 = *reinterpret_cast<const Q2*>(row + 4);
-                            const Q2 b1 = *reinterpret_cast<const Q2*>(row + 6);
+                            const Q2 a0 = rowv[kp][0];
+                            const Q2 a1 = rowv[kp][1];
+                            const Q2 b0 = rowv[kp][2];
+                            const Q2 b1 = rowv[kp][3];
                             const Q vrho = (a0.x + a0.y * pp) + p2 * (a1.x + a1.y * pp);
                             const Q vphi = (b0.x + b0.y * pp) + p2 * (b1.x + b1.y * pp);
                             const Q xr = inside ? vrho * t : Q(0);
@@ -724,9 +746,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             // sum over it: the (0, jt) and (1, jt) lanes of a pair are NT * PARTS apart
             if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
-            if constexpr (NT == 2 && PARTS == 2) ej += __shfl_xor(ej, 4, 64);
+            if constexpr (NT == 2 && PARTS == 2) ej += dpp_f64<kDppRowRor12>(ej);  // lane l + 4 for l % 8 < 4
             if constexpr (NT == 2 && PARTS == 1) ej += __shfl_xor(ej, 2, 64);
-            if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
+            if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rp * NT + jt] = ej;
         }
         }
 #pragma unroll
@@ -739,8 +761,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     block_store_waves<2>(acc, part);
 }
 
+// ---------------------------------------------------------------- lane groups
+// G consecutive lanes per site (a DPP row for G = 16, a quad for G = 4);
+// blocks are remapped so that each XCD takes a contiguous range of sites.
+template <int G>
+struct SiteGroup {
+    int site, l;
+    __device__ __forceinline__ SiteGroup() {
+        const int b = xcd_block(gridDim.x);
+        site = (b * kBlock + (int)threadIdx.x) / G;
+        l = threadIdx.x & (G - 1);
+    }
+};
+
+// sum over the G lanes of a group (fixed tree; every lane of the group gets it)
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+    static_assert(G == 16 || G == 4, "lane groups of 4 or 16");
+    if constexpr (G == 16) {
+        v += dpp_f64<kDppRowRor8>(v);
+        v += dpp_f64<kDppRowRor4>(v);
+    }
+    v += dpp_f64<kDppXor2>(v);
+    v += dpp_f64<kDppXor1>(v);
+    return v;
+}
+
+// value v[t] of the lane whose group index is t (t < N)
+template <int N>
+__device__ __forceinline__ double pick(const double* v, int t) {
+    double r = v[0];
+#pragma unroll
+    for (int x = 1; x < N; ++x)
+        if (t == x) r = v[x];
+    return r;
+}
+
+constexpr int kGatherG = 16;  // lanes per site of the reverse-CSR gathers
+
 // ---------------------------------------------------------------- embed
-// rho_bar = own share + partner shares; F, F', energy terms, alpha entropic force
+// rho_bar = own share + partner shares (16 lanes per site over the site's
+// contiguous reverse range); F, F', energy terms, alpha entropic force
 template <int NT, bool ENERGY_ONLY>
 __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const double* __restrict__ alpha,
                                                   const double* __restrict__ conc, PairBufs L,
@@ -749,28 +810,33 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
                                                   const double* __restrict__ F9, double* __restrict__ E,
                                                   double* __restrict__ dE, double* __restrict__ fa,
                                                   double4* __restrict__ arec, double* __restrict__ part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kGatherG> g;
+    const int i = g.site;
     double acc[1] = {0.0};
     if (i < P.natms) {
         double e[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) e[t] = Eown[(size_t)i * NT + t];
+        for (int t = 0; t < NT; ++t) e[t] = 0.0;
         const int q1 = L.roff[i + 1];
-        for (int q = L.roff[i]; q < q1; ++q) {
+        for (int q = L.roff[i] + g.l; q < q1; q += kGatherG) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) e[t] += pairE[(size_t)q * NT + t];
         }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+        for (int t = 0; t < NT; ++t) e[t] = group_sum<kGatherG>(e[t]);
+        if (g.l < NT) {
+            const int t = g.l;
+            const size_t k = (size_t)i * NT + t;
+            const double et = Eown[k] + pick<NT>(e, t);
             double Fv, dF;
-            embed_F(P, F9, t, e[t], &Fv, &dF, nullptr);
-            const double c = conc[(size_t)i * NT + t];
-            const double a = alpha[(size_t)i * NT + t];
-            E[(size_t)i * NT + t] = Fv;
+            embed_F(P, F9, t, et, &Fv, &dF, nullptr);
+            const double c = conc[k];
+            const double a = alpha[k];
+            E[k] = Fv;
             if (!ENERGY_ONLY) {
-                dE[(size_t)i * NT + t] = dF;
-                arec[(size_t)i * NT + t].w = dF;
-                fa[(size_t)i * NT + t] = -(1.5 * P.kbT * c / a);
+                dE[k] = dF;
+                arec[k].w = dF;
+                fa[k] = -(1.5 * P.kbT * c / a);
             }
             acc[0] += c * Fv;
             acc[0] += P.kbT * calc_ent(c);
@@ -808,15 +874,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
             // entry, partner and separation of the inside pairs, packed
-            int pk_p, pk_j, cnt;
+            int pk_p, pk_j, pk_rp, cnt;
             double pk_d0, pk_d1, pk_d2, pk_r2;
             {
                 const int pc = chunk + lane;
                 bool in = false;
-                int jc = 0;
+                int jc = 0, rp = 0;
                 double e0 = 0.0, e1 = 0.0, e2 = 0.0, r2 = 0.0;
                 if (pc < p1) {
                     jc = L.nl[pc];
+                    rp = rpos[pc];  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     e0 = xi.x - xj.x;
                     e1 = xi.y - xj.y;
@@ -828,12 +895,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     in = r2 < P.cut_sq_mod;
                     if (!in && jc < P.natms) {
 #pragma unroll
-                        for (int c = 0; c < XD; ++c) pairF[(size_t)rpos[pc] * XD + c] = 0.0;
+                        for (int c = 0; c < XD; ++c) pairF[(size_t)rp * XD + c] = 0.0;
                     }
                 }
                 const int dest = pack_dest(in, &cnt);
                 pk_p = packed(dest, pc);
                 pk_j = packed(dest, jc);
+                pk_rp = packed(dest, rp);
                 pk_d0 = packed(dest, e0);
                 pk_d1 = packed(dest, e1);
                 pk_d2 = packed(dest, e2);
@@ -845,6 +913,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const int src = valid ? slot : 0;
             const int p = __shfl(pk_p, src, 64);
             const int j = __shfl(pk_j, src, 64);
+            const int rp = __shfl(pk_rp, src, 64);
             const double dx0 = __shfl(pk_d0, src, 64), dx1 = __shfl(pk_d1, src, 64), dx2 = __shfl(pk_d2, src, 64);
             const double rsq = __shfl(pk_r2, src, 64);
             const bool inr = valid;  // inside the cut by construction
@@ -887,7 +956,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 vir[5] -= wf * dx0 * dx1;
             }
             if (valid && owned) {
-                double* dst = pairF + (size_t)rpos[p] * XD;
+                double* dst = pairF + (size_t)rp * XD;
                 if (q == 0) {
                     dst[0] = -dx0 * S;
                     dst[1] = -dx1 * S;
@@ -927,20 +996,25 @@ __global__ __launch_bounds__(kBlock) void k_fgather(int natms, PairBufs L, const
                                                     const double* __restrict__ pairF,
                                                     const double* __restrict__ fa, double* __restrict__ f) {
     constexpr int XD = 3 + NT;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kGatherG> g;
+    const int i = g.site;
     if (i >= natms) return;
     double acc[XD];
 #pragma unroll
-    for (int k = 0; k < XD; ++k) acc[k] = fown[(size_t)i * XD + k];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[3 + t] += fa[(size_t)i * NT + t];
+    for (int k = 0; k < XD; ++k) acc[k] = 0.0;
     const int q1 = L.roff[i + 1];
-    for (int q = L.roff[i]; q < q1; ++q) {
+    for (int q = L.roff[i] + g.l; q < q1; q += kGatherG) {
 #pragma unroll
         for (int k = 0; k < XD; ++k) acc[k] += pairF[(size_t)q * XD + k];
     }
 #pragma unroll
-    for (int k = 0; k < XD; ++k) f[(size_t)i * XD + k] = acc[k];
+    for (int k = 0; k < XD; ++k) acc[k] = group_sum<kGatherG>(acc[k]);
+    if (g.l < XD) {
+        const int k = g.l;
+        double v = fown[(size_t)i * XD + k];
+        if (k >= 3) v += fa[(size_t)i * NT + k - 3];
+        f[(size_t)i * XD + k] = v + pick<XD>(acc, k);
+    }
 }
 
 // Two-stage deterministic sum of per-block partials: stage 1 block b sums the

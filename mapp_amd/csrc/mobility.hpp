@@ -181,34 +181,6 @@ __global__ void k_list2_fill(RevBufs L2, int natms, const int* __restrict__ foff
     for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) fidx[e++] = ~L2.pair_i[L2.rev[q]];
 }
 
-// ---------------------------------------------------------------- lane groups
-// G consecutive lanes per site; blocks are remapped so that each XCD takes a
-// contiguous range of sites.
-template <int G>
-struct SiteGroup {
-    int site, l;
-    __device__ __forceinline__ SiteGroup() {
-        const int b = xcd_block(gridDim.x);
-        site = (b * kBlock + (int)threadIdx.x) / G;
-        l = threadIdx.x & (G - 1);
-    }
-};
-
-constexpr int kDppRowRor4 = 0x124;  // row_ror:4 (rotate by 4 lanes within a row of 16)
-
-// sum over the G lanes of a group (fixed tree; every lane of the group gets it)
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-    static_assert(G == 16 || G == 4, "lane groups of 4 or 16");
-    if constexpr (G == 16) {
-        v += dpp_f64<kDppRowRor8>(v);
-        v += dpp_f64<kDppRowRor4>(v);
-    }
-    v += dpp_f64<kDppXor2>(v);
-    v += dpp_f64<kDppXor1>(v);
-    return v;
-}
-
 constexpr int kTableG = 16;  // lanes per site over the pair table
 constexpr int kList2G = 4;   // lanes per site over the 2nd list
 
