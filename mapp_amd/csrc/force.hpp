@@ -465,6 +465,10 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
 // The rho and phi lanes then carry the weighted Gauss-Hermite sums of their
 // function.
 //
+// The derivative records and partner shares stream out with nontemporal
+// stores: they are read back only after the whole pass, so they are kept from
+// displacing the spline tables in L2.
+//
 // ENERGY_ONLY restates energy_calc's triangle (jtype starts at itype, so phi
 // only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
 // goes to pairE in reverse-CSR order (rpos), so the embedding reads it
@@ -723,10 +727,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                     if (!ENERGY_ONLY) {
                         Q* d = deriv + ((size_t)p * NT * NT + q) * 4;
-                        d[0] = (Q)v1[1];
-                        d[1] = (Q)v1[2];
-                        d[2] = (Q)v2[1];
-                        d[3] = (Q)v2[2];
+                        __builtin_nontemporal_store((Q)v1[1], d);
+                        __builtin_nontemporal_store((Q)v1[2], d + 1);
+                        __builtin_nontemporal_store((Q)v2[1], d + 2);
+                        __builtin_nontemporal_store((Q)v2[2], d + 3);
                     }
                 } else {
                     if (rho_lane) {
@@ -739,8 +743,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                     if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
                         Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
-                        d[0] = (Q)v1[1];
-                        d[1] = (Q)v1[2];
+                        __builtin_nontemporal_store((Q)v1[1], d);
+                        __builtin_nontemporal_store((Q)v1[2], d + 1);
                     }
                 }
             }
@@ -748,7 +752,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
             if constexpr (NT == 2 && PARTS == 2) ej += dpp_f64<kDppRowRor12>(ej);  // lane l + 4 for l % 8 < 4
             if constexpr (NT == 2 && PARTS == 1) ej += __shfl_xor(ej, 2, 64);
-            if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rp * NT + jt] = ej;
+            if (valid && rho_lane && it == 0 && j < P.natms) __builtin_nontemporal_store(ej, pairE + (size_t)rp * NT + jt);
         }
         }
 #pragma unroll
@@ -923,11 +927,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const double4 rj = arec[(size_t)j * NT + jt];
                 const double aj = rj.x, iaj = rj.y, cj = rj.z, dEj = rj.w;
                 const double r_inv = rsqrt(rsq);
-                const Q* rec = deriv + (size_t)p * LPP * 4;
-                const double g_dr = (double)rec[q * 4 + 0], g_da = (double)rec[q * 4 + 1];
-                const double gT_dr = (double)rec[qT * 4 + 0], gT_da = (double)rec[qT * 4 + 1];
-                const int qf = want_phi ? q : qT;
-                const double fr_dr = (double)rec[qf * 4 + 2], fr_da = (double)rec[qf * 4 + 3];
+                // the lane's own record; the transposed type pair's comes from its quad lane
+                const Q* rec = deriv + ((size_t)p * LPP + q) * 4;
+                const typename Vec2<Q>::T rg = *reinterpret_cast<const typename Vec2<Q>::T*>(rec);
+                const typename Vec2<Q>::T rf = *reinterpret_cast<const typename Vec2<Q>::T*>(rec + 2);
+                const double g_dr = (double)rg.x, g_da = (double)rg.y;
+                const double f_dr = (double)rf.x, f_da = (double)rf.y;
+                const double gT_dr = transpose_tp<NT>(g_dr), gT_da = transpose_tp<NT>(g_da);
+                const double fT_dr = transpose_tp<NT>(f_dr), fT_da = transpose_tp<NT>(f_da);
+                const double fr_dr = want_phi ? f_dr : fT_dr, fr_da = want_phi ? f_da : fT_da;
                 const double cc = ci * cj;
                 // 1/(alpha_i + alpha_j) = (1/alpha_i)(1/alpha_j) / (1/alpha_i + 1/alpha_j)
                 const double ia = iai + iaj;
