@@ -454,17 +454,18 @@ template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL>
 void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ, const PairBufs& L, Q* deriv,
                   double* part) {
     const dim3 blk(64 * ctx->sh1.wpb);
+    const size_t lds = (size_t)ctx->sh1.wpb * Pass1Stage<NT>::kBytes;  // one stage per wavefront
     hipStream_t st = ctx->stream;
     if (ctx->sh1.parts == 1)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part);
     else if (ctx->sh1.parts == 2)
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 2>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 2>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part);
     else
-        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 4>), grid, blk, 0, st, ctx->P, PQ, ctx->d_pos.p,
+        hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 4>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part);
     launch_check("k_pass1");

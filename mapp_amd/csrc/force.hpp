@@ -504,6 +504,22 @@ __device__ __forceinline__ float dpp_xor1_q<float>(float v) {
 }
 
 
+// Per-wavefront LDS stage of pass 1 (dynamic shared memory, kPass1StageBytes per
+// wavefront): for the inside entries of the current chunk, in rank order,
+// {list entry, partner, reverse position} and per type {alpha, 1/alpha, c, r^2}.
+template <int NT>
+struct Pass1Stage {
+    static constexpr int kBytes = 64 * (16 + 32 * NT);
+    int4* idx;
+    double4* rec;
+    __device__ __forceinline__ explicit Pass1Stage(int wave) {
+        extern __shared__ __attribute__((aligned(32))) unsigned char pass1_lds[];
+        unsigned char* base = pass1_lds + (size_t)wave * kBytes;
+        rec = reinterpret_cast<double4*>(base);
+        idx = reinterpret_cast<int4*>(base + 64 * 32 * NT);
+    }
+};
+
 // ENERGY_ONLY: the loop of ForceField_DMD_I::energy_calc (src/ff_dmd_I.cpp:454-668),
 // type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
 // energy equals force_calc's.
@@ -527,6 +543,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // two parts: part 0 the rho cubic, part 1 the phi cubic; one part: both
     const bool rho_lane = prt == 0, phi_lane = prt == PARTS / 2;
     double en = 0.0, npair = 0.0;  // energy, pairs inside the quadrature cut (this lane's share)
+    const Pass1Stage<NT> st(threadIdx.x >> 6);
     const SiteLoop sl(P.natms);
     for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
         const int i = __builtin_amdgcn_readfirstlane(i0);
@@ -541,10 +558,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
             // one list entry per lane: entries outside the cut get a zero partner
-            // share; the entries inside it are packed to the low lanes
-            // entry, partner and squared distance of the inside pairs, packed
-            int pk_p, pk_j, pk_rp, cnt;
-            double pk_r2;
+            // share; each entry inside it is staged in LDS at its rank among the
+            // inside entries, with its partner's width, inverse width and
+            // concentration of every type, read from HBM once per entry
+            int cnt;
             {
                 const int pc = chunk + lane;
                 bool in = false;
@@ -566,39 +583,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                 }
                 const int dest = pack_dest(in, &cnt);
-                pk_p = packed(dest, pc);
-                pk_j = packed(dest, jc);
-                pk_rp = packed(dest, rp);
-                pk_r2 = packed(dest, r2);
+                if (in) {
+                    st.idx[dest] = make_int4(pc, jc, rp, 0);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const double* rr = reinterpret_cast<const double*>(arec + (size_t)jc * NT + t);
+                        st.rec[dest * NT + t] = make_double4(rr[0], rr[1], rr[2], r2);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-        // the partner record of the first slot; inside the loop the next slot's
-        // record is requested before the current slot's quadrature, so its latency
-        // hides behind the table loads (slot 0 stands in for slots past the end)
-        int p_n, j_n, rp_n;
-        double rsq_n, aj_n, iaj_n, cj_n, co_n;  // partner alpha, 1/alpha, c of type jt, c of the other type
-        auto request = [&](int src) {
-            p_n = __shfl(pk_p, src, 64);
-            j_n = __shfl(pk_j, src, 64);
-            rp_n = __shfl(pk_rp, src, 64);
-            rsq_n = __shfl(pk_r2, src, 64);
-            const double* rj = reinterpret_cast<const double*>(arec + (size_t)j_n * NT + jt);
-            aj_n = rj[0];
-            iaj_n = rj[1];
-            cj_n = rj[2];
-            co_n = NT == 2 ? reinterpret_cast<const double*>(arec + (size_t)j_n * NT + (1 - jt))[2] : 0.0;
-        };
-        request(k < cnt ? k : 0);
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
             const bool valid = slot < cnt;
-            const int p = p_n, j = j_n, rp = rp_n;
-            const double rsq = rsq_n, aj = aj_n, iaj = iaj_n;
+            int p = 0, j = 0, rp = 0;
+            double rsq = 1.0, aj = 1.0, iaj = 1.0;
             double cj[NT];
-            cj[0] = jt == 0 ? cj_n : co_n;
-            if constexpr (NT == 2) cj[1] = jt == 1 ? cj_n : co_n;
-            if (base + PPW < cnt) {
-                const int nslot = base + PPW + k;
-                request(nslot < cnt ? nslot : 0);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cj[t] = 0.0;
+            if (valid) {
+                const int4 e = st.idx[slot];
+                p = e.x;
+                j = e.y;
+                rp = e.z;
+                const double4 rj = st.rec[slot * NT + jt];
+                aj = rj.x;
+                iaj = rj.y;
+                rsq = rj.w;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : st.rec[slot * NT + t].z;
             }
             const bool inr = valid;  // inside the cut by construction
             double ej = 0.0;
