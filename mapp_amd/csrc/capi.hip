@@ -477,7 +477,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     const int natms = ctx->natms;
     const int nb = nblk((long)natms * kGatherG);             // embed and force gather: 16 lanes per site
     const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
-    // the surrogate's pass 1: one wavefront per block, persistent over at most kSurBlocks blocks
+    // the surrogate's pass 1: one block of kSurWaves wavefronts per CU, persistent
     const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);
@@ -494,21 +494,15 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
             ctx->d_nfall.reserve(natms);
             HIP_OK(hipMemsetAsync(ctx->d_nfall.p, 0, sizeof(int) * natms, st));
             const int q = ctx->Pf.no_i;
-            if (q == 5)
-                hipLaunchKernelGGL((k_pass1_sur<5>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
-                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
-                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
-                                   ctx->d_nfall.p);
-            else if (q == 3)
-                hipLaunchKernelGGL((k_pass1_sur<3>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
-                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
-                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
-                                   ctx->d_nfall.p);
-            else
-                hipLaunchKernelGGL((k_pass1_sur<0>), dim3(nb1), dim3(kSurBlock), 0, st, ctx->P, ctx->Pf, S,
-                                   ctx->d_pos.p, ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p,
-                                   ctx->d_phi7.p, ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1,
-                                   ctx->d_nfall.p);
+            auto launch = [&](auto kern) {
+                HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSurLds));
+                hipLaunchKernelGGL(kern, dim3(nb1), dim3(kSurBlock), kSurLds, st, ctx->P, ctx->Pf, S, ctx->d_pos.p,
+                                   ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p, ctx->d_phi7.p,
+                                   ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1, ctx->d_nfall.p);
+            };
+            if (q == 5) launch(k_pass1_sur<5>);
+            else if (q == 3) launch(k_pass1_sur<3>);
+            else launch(k_pass1_sur<0>);
             launch_check("k_pass1_sur");
         }
     }

@@ -377,9 +377,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // deterministic per-block sum of NV wave partials (lane 0 of each wave holds them)
-template <int NV>
+template <int NV, int MAXW = kBlock / 64>
 __device__ __forceinline__ void block_store_waves(const double* v, double* __restrict__ out) {
-    __shared__ double sh[kBlock / 64][NV];
+    __shared__ double sh[MAXW][NV];  // blockDim.x <= 64 * MAXW
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     if (lane == 0) {
 #pragma unroll

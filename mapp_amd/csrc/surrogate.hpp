@@ -43,6 +43,9 @@ constexpr int kCellR = 64, kCellA = 4;  // fallback cells: r x alpha_i x alpha_j
 
 
 __device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU for BF16 activations: the hardware reciprocal (1 ulp) in place of the
+// correctly rounded division, far below the 2^-9 rounding of the BF16 result
+__device__ __forceinline__ float silu_bf16(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu_df(float z) {
     const float s = 1.0f / (1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
@@ -296,25 +299,33 @@ __device__ __forceinline__ int sur_cell(const SurNorm& N, float r, float ai, flo
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-// weight fragments for v_mfma_f32_16x16x32_bf16, B operand (K x N = 32 x 16 per
-// fragment): lane l holds column n*16 + l%16 and rows ks*32 + (l/16)*8 + [0,8).
-// frag[((tp*3 + layer)*4 + n)*2 + ks][lane][8]; layer 2 (64->6) uses n = 0 only,
-// columns >= 6 zero.
-constexpr int kFragPerTp = 3 * 4 * 2;
+// Layers 2-4 chain on v_mfma_f32_16x16x32_bf16 with the weights as the A
+// operand (M = output features, K = input features) and 16 samples as the N
+// columns.  The D layout (lane l: rows (l/16)*4 + e of the M tile, column
+// l%16) is the next layer's B operand (lane l: column l%16, K positions
+// (l/16)*8 + [0,8) of each 32-wide k step) once K position (ks, l/16, e) is
+// read as input feature kfeat(ks, l/16, e) = (2 ks + e/4)*16 + (l/16)*4 + e%4:
+// the activations never leave registers, and the weight fragments carry the
+// same permutation.  frag[tp][f][lane][8], f = frag_index(layer, mt, ks):
+// lane l holds W[kfeat(ks, l/16, e)][mt*16 + l%16], e = 0..7; layer 2 (64->6)
+// has one M tile whose rows >= 6 are zero (layer 0 = W2, 1 = W3, 2 = W4).
+__host__ __device__ constexpr int kfeat(int ks, int g, int e) { return (2 * ks + e / 4) * 16 + g * 4 + e % 4; }
+constexpr int kFragPerTp = 18;  // layers 0 and 1 (64->64): 4 M tiles x 2 k steps; layer 2 (64->6): 2 k steps
+__host__ __device__ constexpr int frag_index(int layer, int mt, int ks) { return layer < 2 ? layer * 8 + mt * 2 + ks : 16 + ks; }
 
 __global__ void k_sur_pack_frags(const float* __restrict__ params, int ntp, __bf16* __restrict__ frag) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (fragment, lane)
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (tp, fragment, lane)
     if (idx >= ntp * kFragPerTp * 64) return;
-    const int lane = idx % 64, fr = idx / 64;
-    const int ks = fr % 2, n = (fr / 2) % 4, layer = (fr / 8) % 3, tp = fr / 24;
+    const int lane = idx % 64, fr = (idx / 64) % kFragPerTp, tp = idx / (64 * kFragPerTp);
+    const int layer = fr < 16 ? fr / 8 : 2, mt = fr < 16 ? (fr % 8) / 2 : 0, ks = fr < 16 ? fr % 2 : fr - 16;
     const float* P = params + (size_t)tp * kNParam;
-    const int col = n * 16 + lane % 16;
+    const int out = mt * 16 + lane % 16;
     for (int e = 0; e < 8; ++e) {
-        const int k = ks * 32 + (lane / 16) * 8 + e;
+        const int in = kfeat(ks, lane / 16, e);
         float w = 0.0f;
-        if (layer == 0) w = P[kW2 + k * kHid + col];
-        else if (layer == 1) w = P[kW3 + k * kHid + col];
-        else w = (n == 0 && col < kOut) ? P[kW4 + k * kOut + col] : 0.0f;
+        if (layer == 0) w = P[kW2 + in * kHid + out];
+        else if (layer == 1) w = P[kW3 + in * kHid + out];
+        else w = out < kOut ? P[kW4 + in * kOut + out] : 0.0f;
         frag[(size_t)idx * 8 + e] = (__bf16)w;
     }
 }
@@ -326,13 +337,28 @@ struct SurDev {
     SurNorm N;
 };
 
-// pass 1 with the surrogate (DMD force, NT == 2): one wavefront per block,
-// persistent over the sites of its XCD range, 16 pairs per chunk.  A block
-// holds a single wavefront, so the LDS hand-offs between the VALU layer and
-// the MFMA layers synchronise with __syncthreads() whatever the site's list
-// length.  Layer-1 weights and all biases sit in LDS for the block's life.
-constexpr int kSurBlock = 64;
-constexpr int kSurBlocks = 2048;  // persistent grid of the surrogate pass (8 wavefronts per CU)
+// pass 1 with the surrogate (DMD force, NT == 2).  One block of kSurWaves
+// wavefronts per CU holds every network's MFMA weight fragments (72 KB), the
+// layer-1 weights and the biases in LDS; its wavefronts are persistent over
+// the sites of their XCD range and never synchronise after the set-up.  Per
+// chunk of 16 pairs, lane tp*16 + k prepares sample (pair k, type pair tp)
+// (partner record, distance, trained box, fallback cell) and writes its
+// standardised input to the wavefront's LDS slot; the wavefront then runs the
+// four networks one after the other, 16 samples per MFMA N tile (lane l:
+// sample l%16, feature group l/16), and lane tp*16 + k reads its six outputs
+// back.  Samples outside the trained box or in a cell whose validation error
+// exceeded the tolerance take the FP32 quadrature instead.
+constexpr int kSurWaves = 16;
+constexpr int kSurBlock = 64 * kSurWaves;
+constexpr int kSurBlocks = 256;  // one block per CU
+struct SurLds {
+    __bf16 frag[4][kFragPerTp][64][8];  // MFMA A operands
+    float4 w1[4][kHid];                 // layer 1: {w_r, w_ai, w_aj, bias} per output
+    float bias[4][3][kHid];             // biases of layers 2, 3 and 4
+    float4 u[kSurWaves][64];            // standardised inputs, lane tp*16 + k
+    float y[kSurWaves][64][8];          // standardised outputs, lane tp*16 + k
+};
+constexpr size_t kSurLds = sizeof(SurLds);
 
 template <int NQ>
 __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
@@ -344,24 +370,27 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                                                          double* __restrict__ pairE, double* __restrict__ deriv,
                                                          double* __restrict__ part, int* __restrict__ nfall) {
     constexpr int NT = 2;
-    __shared__ __attribute__((aligned(16))) __bf16 act[64][kHid + 8];  // 64 samples x 64 features (+pad)
-    __shared__ float yout[64][8];
-    __shared__ float4 w1s[4][kHid];     // layer 1 of each network: {w_r, w_ai, w_aj, bias} per output
-    __shared__ float bls[4][3][kHid];   // biases of layers 2, 3 and 4
-    const int lane = threadIdx.x;
-    for (int e = lane; e < 4 * kHid; e += kSurBlock) {
-        const int m = e / kHid, o = e % kHid;
-        const float* Pm = S.params + (size_t)m * kNParam;
-        w1s[m][o] = make_float4(Pm[kW1 + o], Pm[kW1 + kHid + o], Pm[kW1 + 2 * kHid + o], Pm[kB1 + o]);
-        bls[m][0][o] = Pm[kB2 + o];
-        bls[m][1][o] = Pm[kB3 + o];
-        bls[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sur_lds_raw[];
+    SurLds& sh = *reinterpret_cast<SurLds*>(sur_lds_raw);
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(S.frag);
+        uint4* dst = reinterpret_cast<uint4*>(&sh.frag[0][0][0][0]);
+        for (int e = threadIdx.x; e < 4 * kFragPerTp * 64; e += kSurBlock) dst[e] = src[e];
+        for (int e = threadIdx.x; e < 4 * kHid; e += kSurBlock) {
+            const int m = e / kHid, o = e % kHid;
+            const float* Pm = S.params + (size_t)m * kNParam;
+            sh.w1[m][o] = make_float4(Pm[kW1 + o], Pm[kW1 + kHid + o], Pm[kW1 + 2 * kHid + o], Pm[kB1 + o]);
+            sh.bias[m][0][o] = Pm[kB2 + o];
+            sh.bias[m][1][o] = Pm[kB3 + o];
+            sh.bias[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
+        }
     }
     __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tp = lane / 16, k = lane % 16, it = tp / NT, jt = tp % NT;
+    const int g = lane / 16, n = lane % 16;  // network lanes: feature group, sample column
     const bool want_phi = it >= jt;
     double en = 0.0, npair = 0.0;
-    // persistent: the block's single wavefront strides through its XCD's sites
     const SiteLoop sl(P.natms);
     for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
         const int i = __builtin_amdgcn_readfirstlane(i0);
@@ -402,9 +431,10 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
             const bool valid = slot < cnt;
             const int p = __shfl(packed, valid ? slot : 0, 64);
             int j = 0;
-            bool inr = false;
             double rsq = 0.0;
-            if (valid) {
+            double aj = 1.0, cj[NT] = {0.0, 0.0}, r = 1.0, r_inv = 1.0, ia = 1.0;
+            int br = 0;
+            if (valid) {  // inside the cut by construction
                 j = L.nl[p];
                 const double4 xj = pos[j];
                 const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
@@ -412,11 +442,6 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
 #pragma clang fp contract(off)
                     rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                 }
-                inr = true;  // inside the cut by construction
-            }
-            double aj = 1.0, cj[NT] = {0.0, 0.0}, r = 1.0, r_inv = 1.0, ia = 1.0;
-            int br = 0;
-            if (inr) {
                 const double4 rj = arec[(size_t)j * NT + jt];
 #pragma unroll
                 for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : arec[(size_t)j * NT + t].z;
@@ -427,112 +452,108 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 br = alpha_branch(P, ia, ai, aj);
             }
             const float rf = (float)r, aif = (float)ai, ajf = (float)aj;
-            bool use_net = inr && br == 1 && rf >= S.N.r_lo && rf <= S.N.r_hi && aif >= S.N.a_lo &&
+            bool use_net = valid && br == 1 && rf >= S.N.r_lo && rf <= S.N.r_hi && aif >= S.N.a_lo &&
                            aif <= S.N.a_hi && ajf >= S.N.a_lo && ajf <= S.N.a_hi;
             if (use_net) {
                 const int cell = sur_cell(S.N, rf, aif, ajf);
                 const int per_tp = kCellR * kCellA * kCellA / 32;
                 if ((S.mask[tp * per_tp + cell / 32] >> (cell % 32)) & 1u) use_net = false;
             }
-            // layer 1 on VALU: this lane's sample, 64 features -> LDS (bf16, 8 per store)
-            {
-                const float u0 = (rf - S.N.r_c) / S.N.r_s, u1 = (aif - S.N.a_c) / S.N.a_s,
-                            u2 = (ajf - S.N.a_c) / S.N.a_s;
+            sh.u[w][lane] = make_float4((rf - S.N.r_c) / S.N.r_s, (aif - S.N.a_c) / S.N.a_s,
+                                        (ajf - S.N.a_c) / S.N.a_s, 0.0f);
+            __builtin_amdgcn_wave_barrier();
+            // the four networks, 16 samples each; lane (g, n) holds features kfeat(., g, .) of sample n
+#pragma unroll 1
+            for (int t = 0; t < 4; ++t) {
+                const float4 uu = sh.u[w][t * 16 + n];
+                bf16x8_t b[2];
 #pragma unroll
-                for (int o8 = 0; o8 < kHid; o8 += 8) {
-                    bf16x8_t h;
+                for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
-                        const float4 w = w1s[tp][o8 + e];
-                        const float z = w.w + u0 * w.x + u1 * w.y + u2 * w.z;
-                        h[e] = (__bf16)silu_f(z);
+                        const float4 wt = sh.w1[t][kfeat(ks, g, e)];
+                        b[ks][e] = (__bf16)silu_bf16(wt.w + uu.x * wt.x + uu.y * wt.y + uu.z * wt.z);
                     }
-                    *reinterpret_cast<bf16x8_t*>(&act[lane][o8]) = h;
-                }
-            }
-            __syncthreads();
-            // layers 2, 3 (64 -> 64) and 4 (64 -> 6) on MFMA; M tile m = type pair m
-            for (int layer = 0; layer < 3; ++layer) {
-                const int nN = layer < 2 ? 4 : 1;
-                floatx4_t cacc[4][4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    bf16x8_t a[2];
+                for (int layer = 0; layer < 2; ++layer) {
+                    floatx4_t d[4];
 #pragma unroll
-                    for (int ks = 0; ks < 2; ++ks)
-                        a[ks] = *reinterpret_cast<const bf16x8_t*>(&act[m * 16 + lane % 16][ks * 32 + (lane / 16) * 8]);
-#pragma unroll
-                    for (int n = 0; n < 4; ++n) {
-                        if (n >= nN) break;
+                    for (int mt = 0; mt < 4; ++mt) {
                         floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                         for (int ks = 0; ks < 2; ++ks) {
-                            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(
-                                S.frag + ((size_t)(((m * 3 + layer) * 4 + n) * 2 + ks) * 64 + lane) * 8);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b, c, 0, 0, 0);
+                            const bf16x8_t a =
+                                *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(layer, mt, ks)][lane][0]);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
                         }
-                        cacc[m][n] = c;
+                        d[mt] = c;
+                    }
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt) {
+                        const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][layer][mt * 16 + g * 4]);
+                        b[mt / 2][(mt % 2) * 4 + 0] = (__bf16)silu_bf16(d[mt][0] + bs.x);
+                        b[mt / 2][(mt % 2) * 4 + 1] = (__bf16)silu_bf16(d[mt][1] + bs.y);
+                        b[mt / 2][(mt % 2) * 4 + 2] = (__bf16)silu_bf16(d[mt][2] + bs.z);
+                        b[mt / 2][(mt % 2) * 4 + 3] = (__bf16)silu_bf16(d[mt][3] + bs.w);
                     }
                 }
-                __syncthreads();
-                // D: lane holds rows (lane/16)*4 + e of tile m, column n*16 + lane%16
+                floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-#pragma unroll
-                    for (int n = 0; n < 4; ++n) {
-                        if (n >= nN) break;
-                        const int col = n * 16 + lane % 16;
-                        const float bias = bls[m][layer][col];
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const int row = m * 16 + (lane / 16) * 4 + e;
-                            if (layer < 2) {
-                                act[row][col] = (__bf16)silu_f(cacc[m][n][e] + bias);
-                            } else if (col < kOut) {
-                                yout[row][col] = cacc[m][n][e] + bias;
-                            }
-                        }
+                for (int ks = 0; ks < 2; ++ks) {
+                    const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(2, 0, ks)][lane][0]);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
+                }
+                // rows g*4 + e are outputs 0..3 (g = 0) and 4, 5 (g = 1)
+                if (g < 2) {
+                    const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][2][g * 4]);
+                    float* y = &sh.y[w][t * 16 + n][g * 4];
+                    y[0] = c[0] + bs.x;
+                    y[1] = c[1] + bs.y;
+                    if (g == 0) {
+                        y[2] = c[2] + bs.z;
+                        y[3] = c[3] + bs.w;
                     }
                 }
-                __syncthreads();
             }
-            double g[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+            __builtin_amdgcn_wave_barrier();
+            double gv[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
             if (use_net) {
+                const float* y = sh.y[w][lane];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    g[c] = yout[lane][c] * S.N.y_std[tp][c] + S.N.y_mean[tp][c];
-                    f[c] = yout[lane][3 + c] * S.N.y_std[tp][3 + c] + S.N.y_mean[tp][3 + c];
+                    gv[c] = y[c] * S.N.y_std[tp][c] + S.N.y_mean[tp][c];
+                    fv[c] = y[3 + c] * S.N.y_std[tp][3 + c] + S.N.y_mean[tp][3 + c];
                 }
-            } else if (inr && br == 1) {
+            } else if (valid && br == 1) {
                 ++fallbacks;
                 float gq[3] = {0, 0, 0}, fq[3] = {0, 0, 0};
                 gh_avg<float, NQ, true>(PQ, rf, (float)r_inv, (float)rsqrt(ia), (float)ia,
                                         quad + (size_t)tp * P.nr * kQuadRow, gq, fq);
                 for (int c = 0; c < 3; ++c) {
-                    g[c] = gq[c];
-                    f[c] = fq[c];
+                    gv[c] = gq[c];
+                    fv[c] = fq[c];
                 }
-            } else if (inr && br == 2 && rsq < P.cut_sq) {
+            } else if (valid && br == 2 && rsq < P.cut_sq) {
                 classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
-                               phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, g, f);
+                               phi7 + (size_t)type2phi(it, jt) * P.nr * 7, true, gv, fv);
             }
             double ej = 0.0;
-            if (inr) {
+            if (valid) {
                 if (tp == 0) npair += 1.0;
-                Ei += g[0] * cj[jt];
-                ej = g[0] * ci[it];
+                Ei += gv[0] * cj[jt];
+                ej = gv[0] * ci[it];
                 if (want_phi) {
                     const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
-                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * f[0];
+                    en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * fv[0];
                 }
                 double* d = deriv + ((size_t)p * NT * NT + tp) * 4;
-                d[0] = g[1];
-                d[1] = g[2];
-                d[2] = f[1];
-                d[3] = f[2];
+                __builtin_nontemporal_store(gv[1], d);
+                __builtin_nontemporal_store(gv[2], d + 1);
+                __builtin_nontemporal_store(fv[1], d + 2);
+                __builtin_nontemporal_store(fv[2], d + 3);
             }
             ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
-            if (valid && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;
+            if (valid && it == 0 && j < P.natms) __builtin_nontemporal_store(ej, pairE + (size_t)rpos[p] * NT + jt);
         }
         }
 #pragma unroll
@@ -543,11 +564,8 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
         const double sf = wave_sum((double)fallbacks);
         if (lane == 0) nfall[i] = (int)sf;
     }
-    const double se = wave_sum(en), sn = wave_sum(npair);
-    if (lane == 0) {
-        part[(size_t)blockIdx.x * 2 + 0] = se;
-        part[(size_t)blockIdx.x * 2 + 1] = sn;
-    }
+    double acc[2] = {wave_sum(en), wave_sum(npair)};
+    block_store_waves<2, kSurWaves>(acc, part);
 }
 
 }  // namespace mappdmd
