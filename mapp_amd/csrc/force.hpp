@@ -937,8 +937,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const bool owned = j < P.natms;
             double S = 0.0, aJ = 0.0;
             if (inr) {
-                const double4 rj = arec[(size_t)j * NT + jt];
-                const double aj = rj.x, iaj = rj.y, cj = rj.z, dEj = rj.w;
+                double aj, iaj, cj, dEj;
+                if constexpr (NT == 2) {
+                    // lanes (0, jt) and (1, jt) share the partner's record of type jt:
+                    // each loads one half ({alpha, 1/alpha} or {c, F'}) and the quad
+                    // exchange (xor 2) brings the other
+                    const double2 h = reinterpret_cast<const double2*>(arec + (size_t)j * NT + jt)[it];
+                    const double o0 = dpp_f64<kDppXor2>(h.x), o1 = dpp_f64<kDppXor2>(h.y);
+                    aj = it == 0 ? h.x : o0;
+                    iaj = it == 0 ? h.y : o1;
+                    cj = it == 0 ? o0 : h.x;
+                    dEj = it == 0 ? o1 : h.y;
+                } else {
+                    const double4 rj = arec[(size_t)j * NT + jt];
+                    aj = rj.x;
+                    iaj = rj.y;
+                    cj = rj.z;
+                    dEj = rj.w;
+                }
                 const double r_inv = rsqrt(rsq);
                 // the lane's own record; the transposed type pair's comes from its quad lane
                 const Q* rec = deriv + ((size_t)p * LPP + q) * 4;
