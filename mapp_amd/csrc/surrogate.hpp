@@ -46,6 +46,13 @@ __device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)
 // SiLU for BF16 activations: the hardware reciprocal (1 ulp) in place of the
 // correctly rounded division, far below the 2^-9 rounding of the BF16 result
 __device__ __forceinline__ float silu_bf16(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// Inference carries every hidden pre-activation scaled by kKappa = -log2(e):
+// with z' = kKappa z, SiLU(z) kKappa = z' / (1 + 2^z'), one exp2 and one
+// reciprocal.  The scale is folded into the layer-1 weights and the layer-2/3
+// biases; the 64 -> 64 weights see kKappa-scaled inputs and give
+// kKappa-scaled outputs unchanged, and the linear output layer divides once.
+constexpr float kKappa = -1.44269504088896341f, kInvKappa = -0.693147180559945309f;
+__device__ __forceinline__ float silu_scaled(float zs) { return zs * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(zs)); }
 __device__ __forceinline__ float silu_df(float z) {
     const float s = 1.0f / (1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
@@ -379,9 +386,10 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
         for (int e = threadIdx.x; e < 4 * kHid; e += kSurBlock) {
             const int m = e / kHid, o = e % kHid;
             const float* Pm = S.params + (size_t)m * kNParam;
-            sh.w1[m][o] = make_float4(Pm[kW1 + o], Pm[kW1 + kHid + o], Pm[kW1 + 2 * kHid + o], Pm[kB1 + o]);
-            sh.bias[m][0][o] = Pm[kB2 + o];
-            sh.bias[m][1][o] = Pm[kB3 + o];
+            sh.w1[m][o] = make_float4(kKappa * Pm[kW1 + o], kKappa * Pm[kW1 + kHid + o],
+                                      kKappa * Pm[kW1 + 2 * kHid + o], kKappa * Pm[kB1 + o]);
+            sh.bias[m][0][o] = kKappa * Pm[kB2 + o];
+            sh.bias[m][1][o] = kKappa * Pm[kB3 + o];
             sh.bias[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
         }
     }
@@ -472,14 +480,15 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const float4 wt = sh.w1[t][kfeat(ks, g, e)];
-                        b[ks][e] = (__bf16)silu_bf16(wt.w + uu.x * wt.x + uu.y * wt.y + uu.z * wt.z);
+                        b[ks][e] = (__bf16)silu_scaled(wt.w + uu.x * wt.x + uu.y * wt.y + uu.z * wt.z);
                     }
 #pragma unroll
                 for (int layer = 0; layer < 2; ++layer) {
                     floatx4_t d[4];
 #pragma unroll
                     for (int mt = 0; mt < 4; ++mt) {
-                        floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+                        // the accumulator starts from the (scaled) biases of the lane's rows
+                        floatx4_t c = *reinterpret_cast<const floatx4_t*>(&sh.bias[t][layer][mt * 16 + g * 4]);
 #pragma unroll
                         for (int ks = 0; ks < 2; ++ks) {
                             const bf16x8_t a =
@@ -489,13 +498,9 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                         d[mt] = c;
                     }
 #pragma unroll
-                    for (int mt = 0; mt < 4; ++mt) {
-                        const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][layer][mt * 16 + g * 4]);
-                        b[mt / 2][(mt % 2) * 4 + 0] = (__bf16)silu_bf16(d[mt][0] + bs.x);
-                        b[mt / 2][(mt % 2) * 4 + 1] = (__bf16)silu_bf16(d[mt][1] + bs.y);
-                        b[mt / 2][(mt % 2) * 4 + 2] = (__bf16)silu_bf16(d[mt][2] + bs.z);
-                        b[mt / 2][(mt % 2) * 4 + 3] = (__bf16)silu_bf16(d[mt][3] + bs.w);
-                    }
+                    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) b[mt / 2][(mt % 2) * 4 + e] = (__bf16)silu_scaled(d[mt][e]);
                 }
                 floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -507,11 +512,11 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 if (g < 2) {
                     const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][2][g * 4]);
                     float* y = &sh.y[w][t * 16 + n][g * 4];
-                    y[0] = c[0] + bs.x;
-                    y[1] = c[1] + bs.y;
+                    y[0] = c[0] * kInvKappa + bs.x;
+                    y[1] = c[1] * kInvKappa + bs.y;
                     if (g == 0) {
-                        y[2] = c[2] + bs.z;
-                        y[3] = c[3] + bs.w;
+                        y[2] = c[2] * kInvKappa + bs.z;
+                        y[3] = c[3] * kInvKappa + bs.w;
                     }
                 }
             }
