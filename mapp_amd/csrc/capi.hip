@@ -686,9 +686,9 @@ void run_pair_values(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     const size_t np = (size_t)ctx->npairs + 1;
     ctx->d_rhoc.reserve(np * NT * NT);
     ctx->d_phic.reserve(np * NT * (NT + 1) / 2);
-    hipLaunchKernelGGL((k_pair_values<NT, Q, NQ>), dim3(nblk(ctx->natms)), dim3(kBlock), 0, ctx->stream, ctx->P, PQ,
-                       ctx->d_pos.p, ctx->d_alpha.p, ctx->d_off.p, ctx->d_nl.p, quad, ctx->d_rho7.p, ctx->d_phi7.p,
-                       ctx->d_rhoc.p, ctx->d_phic.p);
+    hipLaunchKernelGGL((k_pair_values<NT, Q, NQ>), dim3(nblk(ctx->npairs)), dim3(kBlock), 0, ctx->stream, ctx->P, PQ,
+                       ctx->d_pos.p, ctx->d_alpha.p, ctx->d_nl.p, ctx->d_pair_i.p, (long)ctx->npairs, quad,
+                       ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_rhoc.p, ctx->d_phic.p);
     launch_check("k_pair_values");
 }
 
@@ -719,14 +719,14 @@ void build_site_table(mapp_dmd* ctx) {
     const int natms = ctx->natms;
     const RevBufs L = rev_half(ctx);
     ctx->d_cnt.reserve(natms + 1);
-    hipLaunchKernelGGL((k_table_count<NT>), dim3(nblk(natms)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
+    hipLaunchKernelGGL((k_table_count<NT>), dim3(nblk((long)natms * kTableG)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
                        ctx->d_phic.p, ctx->d_cnt.p);
     launch_check("k_table_count");
     const long ne = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_toff, natms);
     ctx->d_tidx.reserve(ne + 1);
     ctx->d_trho.reserve((size_t)(ne + 1) * NT * NT);
     ctx->d_tphi.reserve((size_t)(ne + 1) * NT * (NT + 1) / 2);
-    hipLaunchKernelGGL((k_table_fill<NT>), dim3(nblk(natms)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
+    hipLaunchKernelGGL((k_table_fill<NT>), dim3(nblk((long)natms * kTableG)), dim3(kBlock), 0, st, L, natms, ctx->d_rhoc.p,
                        ctx->d_phic.p, ctx->d_toff.p, ctx->d_tidx.p, ctx->d_trho.p, ctx->d_tphi.p);
     launch_check("k_table_fill");
     ctx->ntable = ne;

@@ -50,68 +50,69 @@ __global__ void k_pair_owner(const int* __restrict__ off, int natms, int* __rest
 }
 
 // rho_bar for every ordered type pair and phi_bar for it >= jt, per half-list
-// pair; zeros where the reference computes nothing (outside the cut, alpha out
-// of range, or the rule beyond rc)
+// pair (one thread per pair, owner from pair_i); zeros where the reference
+// computes nothing (outside the cut, alpha out of range, or the rule beyond rc)
 template <int NT, typename Q, int NQ>
 __global__ __launch_bounds__(kBlock) void k_pair_values(FFParams<double> P, FFParams<Q> PQ,
                                                         const double4* __restrict__ pos,
-                                                        const double* __restrict__ alpha, const int* __restrict__ off,
-                                                        const int* __restrict__ nl, const Q* __restrict__ quad,
-                                                        const double* __restrict__ rho7,
+                                                        const double* __restrict__ alpha, const int* __restrict__ nl,
+                                                        const int* __restrict__ pair_i, long npairs,
+                                                        const Q* __restrict__ quad, const double* __restrict__ rho7,
                                                         const double* __restrict__ phi7, double* __restrict__ rhoc,
                                                         double* __restrict__ phic) {
     constexpr int NPHI = NT * (NT + 1) / 2;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.natms) return;
-    const double4 xi = pos[i];
-    double ai[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) ai[t] = alpha[(size_t)i * NT + t];
-    for (int p = off[i]; p < off[i + 1]; ++p) {
-        const int j = nl[p];
-        const double4 xj = pos[j];
-        const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
-        const double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-        double rv[NT * NT], pv[NPHI];
-#pragma unroll
-        for (int k = 0; k < NT * NT; ++k) rv[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < NPHI; ++k) pv[k] = 0.0;
-        if (rsq < P.cut_sq_mod) {
-            const double r = sqrt(rsq), r_inv = 1.0 / r;
-#pragma unroll
-            for (int it = 0; it < NT; ++it)
-#pragma unroll
-                for (int jt = 0; jt < NT; ++jt) {
-                    const bool want_phi = it >= jt;
-                    const double aj = alpha[(size_t)j * NT + jt];
-                    const double a = (ai[it] * aj) / (ai[it] + aj);
-                    if (P.alpha_min < a && a < P.alpha_max) {
-                        Q g[3], f[3];
-                        const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
-                        bool hit = want_phi ? gh_average<Q, NQ, false, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f)
-                                            : gh_average<Q, NQ, false, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f);
-                        if (hit) {
-                            rv[type2rho(NT, it, jt)] = (double)g[0];
-                            if (want_phi) pv[type2phi(it, jt)] = (double)f[0];
-                        }
-                    } else if (P.alpha_max <= a && rsq < P.cut_sq) {
-                        double g[3], f[3];
-                        classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
-                                       phi7 + (size_t)type2phi(it, jt) * P.nr * 7, want_phi, g, f);
-                        rv[type2rho(NT, it, jt)] = g[0];
-                        if (want_phi) pv[type2phi(it, jt)] = f[0];
-                    }
-                }
-        }
-#pragma unroll
-        for (int k = 0; k < NT * NT; ++k) rhoc[(size_t)p * NT * NT + k] = rv[k];
-#pragma unroll
-        for (int k = 0; k < NPHI; ++k) phic[(size_t)p * NPHI + k] = pv[k];
+    const long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npairs) return;
+    const int i = pair_i[p], j = nl[p];
+    const double4 xi = pos[i], xj = pos[j];
+    const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
+    double rsq;
+    {
+#pragma clang fp contract(off)
+        rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;  // as force_calc
     }
+    double rv[NT * NT], pv[NPHI];
+#pragma unroll
+    for (int k = 0; k < NT * NT; ++k) rv[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPHI; ++k) pv[k] = 0.0;
+    if (rsq < P.cut_sq_mod) {
+        const double r = sqrt(rsq), r_inv = 1.0 / r;
+#pragma unroll
+        for (int it = 0; it < NT; ++it)
+#pragma unroll
+            for (int jt = 0; jt < NT; ++jt) {
+                const bool want_phi = it >= jt;
+                const double ai = alpha[(size_t)i * NT + it], aj = alpha[(size_t)j * NT + jt];
+                const double a = (ai * aj) / (ai + aj);
+                if (P.alpha_min < a && a < P.alpha_max) {
+                    Q g[3], f[3];
+                    const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
+                    bool hit = want_phi ? gh_average<Q, NQ, false, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f)
+                                        : gh_average<Q, NQ, false, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f);
+                    if (hit) {
+                        rv[type2rho(NT, it, jt)] = (double)g[0];
+                        if (want_phi) pv[type2phi(it, jt)] = (double)f[0];
+                    }
+                } else if (P.alpha_max <= a && rsq < P.cut_sq) {
+                    double g[3], f[3];
+                    classical_pair(P, rsq, r_inv, rho7 + (size_t)type2rho(NT, it, jt) * P.nr * 7,
+                                   phi7 + (size_t)type2phi(it, jt) * P.nr * 7, want_phi, g, f);
+                    rv[type2rho(NT, it, jt)] = g[0];
+                    if (want_phi) pv[type2phi(it, jt)] = f[0];
+                }
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NT * NT; ++k) rhoc[(size_t)p * NT * NT + k] = rv[k];
+#pragma unroll
+    for (int k = 0; k < NPHI; ++k) phic[(size_t)p * NPHI + k] = pv[k];
 }
 
 // ---------------------------------------------------------------- site table
+constexpr int kTableG = 16;  // lanes per site over the pair table
+constexpr int kList2G = 4;   // lanes per site over the 2nd list
+
 struct SiteTable {
     const int* off;     // [natms + 1] entries of site i: off[i] .. off[i+1]
     const int* idx;     // partner
@@ -132,38 +133,67 @@ __device__ __forceinline__ bool pair_contributes(const double* __restrict__ rhoc
 }
 
 // entries per site: contributing pairs of the own row and of the reverse CSR
+// (16 lanes per site, strided over both ranges)
 template <int NT>
-__global__ void k_table_count(RevBufs L, int natms, const double* __restrict__ rhoc,
-                              const double* __restrict__ phic, int* __restrict__ cnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_table_count(RevBufs L, int natms, const double* __restrict__ rhoc,
+                                                        const double* __restrict__ phic, int* __restrict__ cnt) {
+    const SiteGroup<kTableG> g;
+    const int i = g.site;
     if (i >= natms) return;
     int c = 0;
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p) c += pair_contributes<NT>(rhoc, phic, p) ? 1 : 0;
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) c += pair_contributes<NT>(rhoc, phic, L.rev[q]) ? 1 : 0;
-    cnt[i] = c;
+    for (int p = L.off[i] + g.l; p < L.off[i + 1]; p += kTableG) c += pair_contributes<NT>(rhoc, phic, p) ? 1 : 0;
+    for (int q = L.roff[i] + g.l; q < L.roff[i + 1]; q += kTableG) c += pair_contributes<NT>(rhoc, phic, L.rev[q]) ? 1 : 0;
+    const double s = group_sum<kTableG>((double)c);
+    if (g.l == 0) cnt[i] = (int)s;
 }
 
+// rank of this lane's flag among the set flags of lower lanes of its 16-lane group,
+// and the number of set flags in the group
+__device__ __forceinline__ int group_rank16(bool flag, int l, int* total) {
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(flag);
+    const int gbase = (threadIdx.x & 63) & ~15;
+    const unsigned bits = (unsigned)((m >> gbase) & 0xFFFFull);
+    *total = __popc(bits);
+    return __popc(bits & ((1u << l) - 1u));
+}
+
+// the entries, in list order, own row first; 16 lanes per site, each stride of
+// 16 entries placed by ballot rank
 template <int NT>
-__global__ void k_table_fill(RevBufs L, int natms, const double* __restrict__ rhoc, const double* __restrict__ phic,
-                             const int* __restrict__ toff, int* __restrict__ tidx, double* __restrict__ trho,
-                             double* __restrict__ tphi) {
+__global__ __launch_bounds__(kBlock) void k_table_fill(RevBufs L, int natms, const double* __restrict__ rhoc,
+                                                       const double* __restrict__ phic, const int* __restrict__ toff,
+                                                       int* __restrict__ tidx, double* __restrict__ trho,
+                                                       double* __restrict__ tphi) {
     constexpr int NPHI = NT * (NT + 1) / 2;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const SiteGroup<kTableG> g;
+    const int i = g.site;
     if (i >= natms) return;
     long e = toff[i];
-    auto put = [&](long p, int partner) {
-        tidx[e] = partner;
+    auto put = [&](long dst, long p, int partner) {
+        tidx[dst] = partner;
 #pragma unroll
-        for (int k = 0; k < NT * NT; ++k) trho[e * NT * NT + k] = rhoc[p * NT * NT + k];
+        for (int k = 0; k < NT * NT; ++k) trho[dst * NT * NT + k] = rhoc[p * NT * NT + k];
 #pragma unroll
-        for (int k = 0; k < NPHI; ++k) tphi[e * NPHI + k] = phic[p * NPHI + k];
-        ++e;
+        for (int k = 0; k < NPHI; ++k) tphi[dst * NPHI + k] = phic[p * NPHI + k];
     };
-    for (int p = L.off[i]; p < L.off[i + 1]; ++p)
-        if (pair_contributes<NT>(rhoc, phic, p)) put(p, L.nl[p]);
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
-        const int p = L.rev[q];
-        if (pair_contributes<NT>(rhoc, phic, p)) put(p, L.pair_i[p]);
+    const int o0 = L.off[i], o1 = L.off[i + 1];
+    for (int b = o0; b < o1; b += kTableG) {
+        const int p = b + g.l;
+        const bool in = p < o1 && pair_contributes<NT>(rhoc, phic, p);
+        int tot;
+        const int rk = group_rank16(in, g.l, &tot);
+        if (in) put(e + rk, p, L.nl[p]);
+        e += tot;
+    }
+    const int q0 = L.roff[i], q1 = L.roff[i + 1];
+    for (int b = q0; b < q1; b += kTableG) {
+        const int q = b + g.l;
+        const int p = q < q1 ? L.rev[q] : 0;
+        const bool in = q < q1 && pair_contributes<NT>(rhoc, phic, p);
+        int tot;
+        const int rk = group_rank16(in, g.l, &tot);
+        if (in) put(e + rk, p, L.pair_i[p]);
+        e += tot;
     }
 }
 
@@ -181,8 +211,6 @@ __global__ void k_list2_fill(RevBufs L2, int natms, const int* __restrict__ foff
     for (int q = L2.roff[i]; q < L2.roff[i + 1]; ++q) fidx[e++] = ~L2.pair_i[L2.rev[q]];
 }
 
-constexpr int kTableG = 16;  // lanes per site over the pair table
-constexpr int kList2G = 4;   // lanes per site over the 2nd list
 
 // E = sum c rho_bar; n = E = F(E), dE = F', ddE = F''
 // (first sweep of c_d_calc, src/ff_dmd_I.cpp:2269-2312, and calc_g, :2447-2493).
