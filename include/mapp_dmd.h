@@ -43,7 +43,14 @@ enum {
     /* plain EAM, ForceField_eam (src/ff_eam.cpp): x rows are [3] (positions),
        c holds the type of each site, coef ignores its DMD arguments, and
        force_calc / energy_calc follow src/ff_eam.cpp:44-356 */
-    MAPP_DMD_VARIANT_EAM = 3
+    MAPP_DMD_VARIANT_EAM = 3,
+    /* MD-mode pair potentials with the EAM layout (x rows [3], c = type) and a
+       list cutoff per type pair, cut_sk_sq[COMP(type_i, type_j)]
+       (src/neighbor.cpp:205-208); up to 4 types.  Coefficients come from
+       mapp_dmd_lj_coef / mapp_dmd_fs_coef / mapp_dmd_read_fs_params instead of
+       set_fs + coef. */
+    MAPP_DMD_VARIANT_LJ = 4, /* ForceField_lj (src/ff_lj.cpp), force_calc :160-264, energy_calc :270-341 */
+    MAPP_DMD_VARIANT_FS = 5  /* ForceField_fs (src/ff_fs.cpp), force_calc :379-604, energy_calc :609-686 */
 };
 
 /* arithmetic of the pair quadrature (per-site sums, embedding and the
@@ -94,6 +101,28 @@ int mapp_dmd_read_fs(mapp_dmd_t* ctx, const char* path, const char* const* eleme
 int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min,
                   double alpha_max, double T, const double* mass);
 int mapp_dmd_set_precision(mapp_dmd_t* ctx, int mode);
+
+/* ff_coef "itype jtype epsilon sigma cutoff" of ForceField_lj::coef
+   (src/ff_lj.cpp:72-125), one call per type pair.  Negative values are
+   rejected; a zero epsilon, sigma or cutoff leaves the pair without forces but
+   with the list cutoff (cutoff + skin)^2; a type pair never given a call lists
+   nothing.  The potential is not shifted (the constructor's shift, :35). */
+int mapp_dmd_lj_coef(mapp_dmd_t* ctx, int itype, int jtype, double epsilon, double sigma, double cut);
+/* Finnis-Sinclair coefficients of ForceField_fs::coef (src/ff_fs.cpp:113-335)
+   in type order: A[nt]; t1[nt*nt], t2[nt*nt] as mat_t_1[i][j], mat_t_2[i][j];
+   per type pair in COMP order (i <= j: j(j+1)/2 + i) cut_rho, cut_phi, k1, k2,
+   k3 [nt(nt+1)/2]. */
+int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const double* t2, const double* cut_rho,
+                     const double* cut_phi, const double* k1, const double* k2, const double* k3);
+/* ff_coef FS <file>: the parameter file of ForceField_fs::coef (the element
+   line, nt A lines, nt*nt "t1 t2" lines, nt(nt+1)/2 "cut_rho cut_phi k1 k2 k3"
+   lines, in the file's element order; '#' starts a comment); elements[nt]
+   names the type order.  mapp_dmd_fs_params_read parses it without a context
+   (arrays as for mapp_dmd_fs_coef, in elements' order). */
+int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const* elements);
+int mapp_dmd_fs_params_read(const char* path, int nt, const char* const* elements, double* A, double* t1,
+                            double* t2, double* cut_rho, double* cut_phi, double* k1, double* k2, double* k3,
+                            char* err, int err_cap);
 
 /* System: lower-triangular H (row-major 3x3, rows are the box vectors,
    src/read_cfg.cpp:425-453), natms sites whose x rows hold the reduced
@@ -217,7 +246,7 @@ int mapp_dmd_cfg_read(const char* path, int cap_atoms, int* natms, int* nt, doub
 int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, const double* x, const double* c,
                        int ntypes, const double* mass, const char* const* names, char* err, int err_cap);
 
-/* Learned surrogate (no counterpart in the reference; BASELINE cfg3 workload):
+/* Learned surrogate (no counterpart in the reference):
    one MLP 3->64->64->64->6 (SiLU) per ordered type pair maps (r, alpha_i,
    alpha_j) to {rho_bar, drho_bar/dr, drho_bar/dalpha, phi_bar, dphi_bar/dr,
    dphi_bar/dalpha}.  Trains on the GPU on n_per_tp samples per type pair

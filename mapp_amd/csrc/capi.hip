@@ -239,12 +239,14 @@ void rebuild(mapp_dmd* ctx) {
     // half list
     ctx->d_cnt.reserve(natms + 1);
     hipLaunchKernelGGL((k_half_list<false>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
-                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_cnt.p, (int*)nullptr);
+                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_conc.p, ctx->d_cnt.p,
+                       (int*)nullptr);
     launch_check("k_half_list count");
     ctx->npairs = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_off, natms);
     ctx->d_nl.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL((k_half_list<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
-                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_off.p, ctx->d_nl.p);
+                       ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_conc.p, ctx->d_off.p,
+                       ctx->d_nl.p);
     launch_check("k_half_list fill");
     finish_lists<NT>(ctx);
 }
@@ -305,7 +307,8 @@ void finish_lists(mapp_dmd* ctx) {
 }
 
 // columns of the per-site alpha / c arrays: nt for DMD, one (the type) for EAM
-inline int acols(const mapp_dmd* ctx) { return ctx->variant == MAPP_DMD_VARIANT_EAM ? 1 : ctx->nt; }
+inline bool is_md(const mapp_dmd* ctx) { return ctx->variant >= MAPP_DMD_VARIANT_EAM; }
+inline int acols(const mapp_dmd* ctx) { return is_md(ctx) ? 1 : ctx->nt; }
 
 void rebuild_any(mapp_dmd* ctx) {
     if (acols(ctx) == 1) rebuild<1>(ctx);
@@ -366,10 +369,10 @@ void scatter_x(mapp_dmd* ctx, const double* dev_aos) {
 }
 
 // x columns of the public layout: 3 + nt (DMD) or 3 (EAM)
-inline int xcols(const mapp_dmd* ctx) { return ctx->variant == MAPP_DMD_VARIANT_EAM ? 3 : 3 + ctx->nt; }
+inline int xcols(const mapp_dmd* ctx) { return is_md(ctx) ? 3 : 3 + ctx->nt; }
 
 void gather_x_any(mapp_dmd* ctx, double* dst, int n, const double4* pos, const double* alpha) {
-    if (ctx->variant == MAPP_DMD_VARIANT_EAM)
+    if (is_md(ctx))
         hipLaunchKernelGGL((k_gather_x<1, 0>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
     else if (ctx->nt == 1)
         hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
@@ -378,7 +381,33 @@ void gather_x_any(mapp_dmd* ctx, double* dst, int n, const double4* pos, const d
     launch_check("k_gather_x");
 }
 
-// ------------------------------------------------------------------ plain EAM
+// ------------------------------------------------------------------ MD-mode potentials
+template <int MODEL>
+void md_passes(mapp_dmd* ctx, bool energy_only, int nb, const PairBufs& L, const EamBufs& T, double2* drho,
+               double* part1, double* part2) {
+    hipStream_t st = ctx->stream;
+    if (energy_only)
+        hipLaunchKernelGGL((k_eam_pass1<MODEL, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->md, ctx->d_pos.p,
+                           ctx->d_conc.p, L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p, ctx->d_pairE.p,
+                           ctx->d_pairF.p, drho, part1);
+    else
+        hipLaunchKernelGGL((k_eam_pass1<MODEL, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->md,
+                           ctx->d_pos.p, ctx->d_conc.p, L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p,
+                           ctx->d_pairE.p, ctx->d_pairF.p, drho, part1);
+    launch_check("k_eam_pass1");
+    if constexpr (MODEL != kModelLj) {
+        if (energy_only)
+            hipLaunchKernelGGL((k_eam_embed<MODEL, true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->md,
+                               ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
+        else
+            hipLaunchKernelGGL((k_eam_embed<MODEL, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->md,
+                               ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
+        launch_check("k_eam_embed");
+    } else {
+        HIP_OK(hipMemsetAsync(part2, 0, sizeof(double) * nb, st));  // no embedding term
+    }
+}
+
 double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms, nb = nblk(natms);
@@ -392,32 +421,24 @@ double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
     double* part2 = part1 + (size_t)nb * 7;
     double* part3 = part2 + (size_t)nb;
     double2* drho = reinterpret_cast<double2*>(ctx->d_drho.p);
-    if (energy_only)
-        hipLaunchKernelGGL((k_eam_pass1<true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_pos.p, ctx->d_conc.p,
-                           L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p, ctx->d_pairE.p, ctx->d_pairF.p, drho,
-                           part1);
-    else
-        hipLaunchKernelGGL((k_eam_pass1<false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_pos.p,
-                           ctx->d_conc.p, L, ctx->d_rpos.p, ctx->d_Eown.p, ctx->d_fown.p, ctx->d_pairE.p,
-                           ctx->d_pairF.p, drho, part1);
-    launch_check("k_eam_pass1");
-    if (energy_only)
-        hipLaunchKernelGGL((k_eam_embed<true>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_conc.p, L,
-                           ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
-    else
-        hipLaunchKernelGGL((k_eam_embed<false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, T, ctx->d_conc.p, L,
-                           ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_E.p, ctx->d_dE.p, part2);
-    launch_check("k_eam_embed");
+    const bool lj = ctx->variant == MAPP_DMD_VARIANT_LJ;
+    if (ctx->variant == MAPP_DMD_VARIANT_EAM) md_passes<kModelEam>(ctx, energy_only, nb, L, T, drho, part1, part2);
+    else if (lj) md_passes<kModelLj>(ctx, energy_only, nb, L, T, drho, part1, part2);
+    else md_passes<kModelFs>(ctx, energy_only, nb, L, T, drho, part1, part2);
     double* out = ctx->d_out.p;
     reduce_partials(ctx, part1, nb, 7, 7, 0, out);
     reduce_partials(ctx, part2, nb, 1, 1, 0, out + 7);
     if (!energy_only) {
-        ghost_copy(ctx, ctx->d_dE.p, 1);
-        hipLaunchKernelGGL(k_eam_pass2, dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_pos.p, L, ctx->d_rpos.p, drho,
-                           ctx->d_dE.p, ctx->d_fown.p, ctx->d_pairF2.p, part3);
-        launch_check("k_eam_pass2");
+        if (!lj) {
+            ghost_copy(ctx, ctx->d_dE.p, 1);
+            hipLaunchKernelGGL(k_eam_pass2, dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_pos.p, L, ctx->d_rpos.p,
+                               drho, ctx->d_dE.p, ctx->d_fown.p, ctx->d_pairF2.p, part3);
+            launch_check("k_eam_pass2");
+        } else {
+            HIP_OK(hipMemsetAsync(part3, 0, sizeof(double) * nb * 6, st));
+        }
         hipLaunchKernelGGL(k_eam_gather, dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
-                           ctx->d_pairF2.p, ctx->d_f.p);
+                           lj ? (const double*)nullptr : ctx->d_pairF2.p, ctx->d_f.p);
         launch_check("k_eam_gather");
         reduce_partials(ctx, part3, nb, 6, 6, 0, out + 8);
     }
@@ -924,7 +945,7 @@ void sur_gradient(mapp_dmd* ctx, const SurNorm& N, const float* params, const fl
 // build, then a rebuild (x2s, ghosts, bins, lists) or a forward ghost update.
 int update_x_device(mapp_dmd* ctx) {
     const int nt = acols(ctx), natms = ctx->natms;
-    if (ctx->variant == MAPP_DMD_VARIANT_EAM) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
+    if (is_md(ctx)) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
     else if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
     else scatter_x<2>(ctx, ctx->d_xaos.p);
     HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
@@ -950,12 +971,12 @@ int update_x_device(mapp_dmd* ctx) {
 
 // ------------------------------------------------------------------ minimiser
 void force_any(mapp_dmd* ctx, int st_clc, double* en) {
-    if (ctx->variant == MAPP_DMD_VARIANT_EAM) run_eam(ctx, false, st_clc, en);
+    if (is_md(ctx)) run_eam(ctx, false, st_clc, en);
     else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en);
     else dispatch_force<2>(ctx, st_clc, en);
 }
 double energy_any(mapp_dmd* ctx) {
-    if (ctx->variant == MAPP_DMD_VARIANT_EAM) return run_eam(ctx, true, 0, nullptr);
+    if (is_md(ctx)) return run_eam(ctx, true, 0, nullptr);
     return ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
 }
 
@@ -1003,7 +1024,7 @@ int line_min(mapp_dmd* ctx, long n, double* nrgy, double* alph, long* n_energy) 
         max_h = std::max(max_h, ex[b * 2]);
         min_a = std::min(min_a, ex[b * 2 + 1]);
     }
-    if (ctx->variant != MAPP_DMD_VARIANT_EAM) a_min = std::min(a_min, 0.99 * min_a);
+    if (!is_md(ctx)) a_min = std::min(a_min, 0.99 * min_a);
     *alph = 0.0;
     if (inner <= 0) return 1;
     if (max_h == 0) return 2;
@@ -1371,8 +1392,8 @@ extern "C" {
 int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
     if (!out) return MAPP_DMD_E_ARG;
     *out = nullptr;
-    if (no_types < 1 || no_types > 2) return MAPP_DMD_E_RANGE;
-    if (variant < 0 || variant > 3) return MAPP_DMD_E_ARG;
+    if (variant < 0 || variant > MAPP_DMD_VARIANT_FS) return MAPP_DMD_E_ARG;
+    if (no_types < 1 || no_types > (variant >= MAPP_DMD_VARIANT_LJ ? kMaxTypes : 2)) return MAPP_DMD_E_RANGE;
     std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
     ctx->nt = no_types;
     ctx->variant = variant;
@@ -1466,9 +1487,108 @@ int mapp_dmd_read_fs(mapp_dmd_t* ctx, const char* path, const char* const* eleme
     });
 }
 
+int mapp_dmd_lj_coef(mapp_dmd_t* ctx, int itype, int jtype, double epsilon, double sigma, double cut) {
+    return guarded(ctx, [&]() {
+        // ForceField_lj::coef, src/ff_lj.cpp:72-125
+        require(ctx->variant == MAPP_DMD_VARIANT_LJ, MAPP_DMD_E_STATE, "lj_coef needs the LJ variant");
+        require(itype >= 0 && itype < ctx->nt && jtype >= 0 && jtype < ctx->nt, MAPP_DMD_E_ARG, "no such type");
+        require(epsilon >= 0.0, MAPP_DMD_E_ARG, "LJ Epsilon cannot be smaller than zero");
+        require(sigma >= 0.0, MAPP_DMD_E_ARG, "LJ Sigma cannot be smaller than zero");
+        require(cut >= 0.0, MAPP_DMD_E_ARG, "LJ cutoff cannot be smaller than zero");
+        MdParams& M = ctx->md;
+        const int k = md_comp(itype, jtype);
+        M.nt = ctx->nt;
+        M.eps[k] = epsilon;
+        M.sig[k] = sigma;
+        M.lj_cut[k] = cut;
+        M.offset[k] = 0.0;
+        M.cut_sq[k] = cut * cut;
+        M.chk[k] = (cut == 0.0 || sigma == 0.0 || epsilon == 0.0) ? 0 : 1;
+        M.set[k] = 1;
+        ctx->have_params = true;
+        ctx->inited = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const double* t2, const double* cut_rho,
+                     const double* cut_phi, const double* k1, const double* k2, const double* k3) {
+    return guarded(ctx, [&]() {
+        // ForceField_fs::coef (src/ff_fs.cpp:113-335); cut_sq at init (:346)
+        require(ctx->variant == MAPP_DMD_VARIANT_FS, MAPP_DMD_E_STATE, "fs_coef needs the FS variant");
+        require(A && t1 && t2 && cut_rho && cut_phi && k1 && k2 && k3, MAPP_DMD_E_ARG, "null coefficients");
+        MdParams& M = ctx->md;
+        const int nt = ctx->nt, nc = nt * (nt + 1) / 2;
+        M.nt = nt;
+        for (int i = 0; i < nt; ++i) {
+            M.A[i] = A[i];
+            for (int j = 0; j < nt; ++j) {
+                M.t1[i][j] = t1[i * nt + j];
+                M.t2[i][j] = t2[i * nt + j];
+            }
+        }
+        for (int k = 0; k < nc; ++k) {
+            M.cut_rho[k] = cut_rho[k];
+            M.cut_phi[k] = cut_phi[k];
+            M.k1[k] = k1[k];
+            M.k2[k] = k2[k];
+            M.k3[k] = k3[k];
+            M.cut_sq[k] = std::max(cut_phi[k] * cut_phi[k], cut_rho[k] * cut_rho[k]);
+            M.chk[k] = 1;
+            M.set[k] = 1;
+        }
+        ctx->have_params = true;
+        ctx->inited = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_fs_params_read(const char* path, int nt, const char* const* elements, double* A, double* t1, double* t2,
+                            double* cut_rho, double* cut_phi, double* k1, double* k2, double* k3, char* err,
+                            int err_cap) {
+    auto fail = [&](const std::string& m, int code) {
+        if (err && err_cap > 0) {
+            std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
+            err[err_cap - 1] = 0;
+        }
+        return code;
+    };
+    if (!path || !elements || nt < 1 || nt > kMaxTypes) return fail("bad arguments", MAPP_DMD_E_ARG);
+    std::vector<std::string> names(elements, elements + nt);
+    FsParams fp;
+    const std::string e = read_fs_params(path, names, fp);
+    if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
+    const int nc = nt * (nt + 1) / 2;
+    std::copy(fp.A.begin(), fp.A.end(), A);
+    std::copy(fp.t1.begin(), fp.t1.end(), t1);
+    std::copy(fp.t2.begin(), fp.t2.end(), t2);
+    for (int k = 0; k < nc; ++k) {
+        cut_rho[k] = fp.cut_rho[k];
+        cut_phi[k] = fp.cut_phi[k];
+        k1[k] = fp.k1[k];
+        k2[k] = fp.k2[k];
+        k3[k] = fp.k3[k];
+    }
+    return 0;
+}
+
+int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const* elements) {
+    return guarded(ctx, [&]() {
+        require(path && elements, MAPP_DMD_E_ARG, "null path or elements");
+        std::vector<std::string> names(elements, elements + ctx->nt);
+        FsParams fp;
+        const std::string e = read_fs_params(path, names, fp);
+        if (!e.empty()) throw ApiError(MAPP_DMD_E_IO, e);
+        return mapp_dmd_fs_coef(ctx, fp.A.data(), fp.t1.data(), fp.t2.data(), fp.cut_rho.data(), fp.cut_phi.data(),
+                                fp.k1.data(), fp.k2.data(), fp.k3.data());
+    });
+}
+
 int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, double alpha_max, double T,
                   const double* mass) {
     return guarded(ctx, [&]() {
+        require(ctx->variant < MAPP_DMD_VARIANT_LJ, MAPP_DMD_E_STATE,
+                "LJ and FS take mapp_dmd_lj_coef / mapp_dmd_fs_coef / mapp_dmd_read_fs_params");
         require(ctx->have_tables, MAPP_DMD_E_STATE, "tables must be set before coef");
         if (ctx->variant == MAPP_DMD_VARIANT_EAM) {
             // ForceField_eam::coef, src/ff_eam.cpp:424-425
@@ -1571,10 +1691,10 @@ int mapp_dmd_set_system(mapp_dmd_t* ctx, const double* H, int natms, const doubl
             ctx->h_s[(size_t)i * 4 + 3] = 0.0;
             for (int t = 0; t < nt; ++t) ctx->h_alpha[(size_t)i * nt + t] = xc > 3 ? r[3 + t] : 1.0;
         }
-        if (ctx->variant == MAPP_DMD_VARIANT_EAM)
+        if (is_md(ctx))
             for (int i = 0; i < natms; ++i)
                 require(c[i] >= 0.0 && c[i] < ctx->nt && c[i] == (double)(int)c[i], MAPP_DMD_E_ARG,
-                        "EAM types must be integers in [0, nt)");
+                        "MD-mode types must be integers in [0, nt)");
         ctx->have_system = true;
         ctx->inited = false;
         return 0;
@@ -1589,10 +1709,32 @@ int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
         const int nt = acols(ctx), natms = ctx->natms;
         ctx->skin = skin;
         const FFParams<double>& P = ctx->P;
-        // ForceField_DMD_I::init, src/ff_dmd_I.cpp:675-685; ForceField_eam::init, src/ff_eam.cpp:362-370
-        const double base = ctx->variant == MAPP_DMD_VARIANT_EAM ? P.cut_sq : P.cut_sq_mod;
-        ctx->bp.cut_sk_sq = base + skin * skin + 2.0 * std::sqrt(base) * skin;
-        set_ph(ctx, std::sqrt(base));
+        if (ctx->variant >= MAPP_DMD_VARIANT_LJ) {
+            // ForceField_lj::init (src/ff_lj.cpp:129-153), ForceField_fs::init
+            // (src/ff_fs.cpp:339-360): a cutoff+skin per type pair with coefficients,
+            // ghosts for the largest cutoff
+            MdParams& M = ctx->md;
+            double ph_cut = 0.0;
+            ctx->bp.pairwise = 1;
+            ctx->bp.cut_sk_sq = 0.0;
+            for (int k = 0; k < kMaxComp; ++k) {
+                ctx->bp.cut_sk_sq_tp[k] = 0.0;
+                if (M.chk[k]) {
+                    ctx->bp.cut_sk_sq_tp[k] = M.cut_sq[k] + skin * skin + 2 * std::sqrt(M.cut_sq[k]) * skin;
+                    ph_cut = std::max(ph_cut, std::sqrt(M.cut_sq[k]));
+                } else if (M.set[k] && ctx->variant == MAPP_DMD_VARIANT_LJ) {
+                    ctx->bp.cut_sk_sq_tp[k] = (M.lj_cut[k] + skin) * (M.lj_cut[k] + skin);  // coef-time value
+                }
+            }
+            require(ph_cut > 0.0, MAPP_DMD_E_STATE, "no type pair has coefficients");
+            set_ph(ctx, ph_cut);
+        } else {
+            // ForceField_DMD_I::init, src/ff_dmd_I.cpp:675-685; ForceField_eam::init, src/ff_eam.cpp:362-370
+            const double base = ctx->variant == MAPP_DMD_VARIANT_EAM ? P.cut_sq : P.cut_sq_mod;
+            ctx->bp.pairwise = 0;
+            ctx->bp.cut_sk_sq = base + skin * skin + 2.0 * std::sqrt(base) * skin;
+            set_ph(ctx, std::sqrt(base));
+        }
         for (int d = 0; d < 3; ++d)
             require(ctx->bp.cut_ph_s[d] < 1.0, MAPP_DMD_E_RANGE,
                     "box thinner than cutoff+skin: needs more than one ghost swap per face");
@@ -1659,7 +1801,7 @@ int mapp_dmd_force_calc(mapp_dmd_t* ctx, int st_clc, double* en_st) {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede force_calc");
         require(en_st != nullptr, MAPP_DMD_E_ARG, "null en_st");
         HIP_OK(hipSetDevice(ctx->device));
-        if (ctx->variant == MAPP_DMD_VARIANT_EAM) run_eam(ctx, false, st_clc, en_st);
+        if (is_md(ctx)) run_eam(ctx, false, st_clc, en_st);
         else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en_st);
         else dispatch_force<2>(ctx, st_clc, en_st);
         return 0;
@@ -1671,7 +1813,7 @@ int mapp_dmd_energy_calc(mapp_dmd_t* ctx, double* energy) {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede energy_calc");
         require(energy != nullptr, MAPP_DMD_E_ARG, "null energy");
         HIP_OK(hipSetDevice(ctx->device));
-        if (ctx->variant == MAPP_DMD_VARIANT_EAM) *energy = run_eam(ctx, true, 0, nullptr);
+        if (is_md(ctx)) *energy = run_eam(ctx, true, 0, nullptr);
         else *energy = ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
         return 0;
     });
@@ -1690,7 +1832,7 @@ int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx) {
 
 int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
     return guarded(ctx, [&]() {
-        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE, "c_d_calc is a DMD entry point");
+        require(!is_md(ctx), MAPP_DMD_E_STATE, "c_d_calc is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede c_d_calc");
         HIP_OK(hipSetDevice(ctx->device));
         if (ctx->nt == 1) c_d_variant<1>(ctx);
@@ -1701,7 +1843,7 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
 
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev, double* g_dev, double* cost) {
     return guarded(ctx, [&]() {
-        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE, "calc_g is a DMD entry point");
+        require(!is_md(ctx), MAPP_DMD_E_STATE, "calc_g is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede calc_g");
         require(cost != nullptr, MAPP_DMD_E_ARG, "null cost");
         HIP_OK(hipSetDevice(ctx->device));
@@ -1717,7 +1859,7 @@ int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
 int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int batch_per_tp, double lr,
                              unsigned long long seed, double tol, const double* box, double* report) {
     return guarded(ctx, [&]() {
-        require(ctx->have_params && ctx->nt == 2 && ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE,
+        require(ctx->have_params && ctx->nt == 2 && !is_md(ctx), MAPP_DMD_E_STATE,
                 "the surrogate covers the two-type DMD force field after coef");
         require(n_per_tp >= kMB && batch_per_tp >= kMB && batch_per_tp % kMB == 0 && n_per_tp >= batch_per_tp &&
                     epochs >= 1 && lr > 0,
@@ -2106,7 +2248,7 @@ int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, con
 int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_del_t, int max_retry, int flags,
                      double* report, long* counts) {
     return guarded(ctx, [&]() {
-        require(ctx->variant != MAPP_DMD_VARIANT_EAM && ctx->inited, MAPP_DMD_E_STATE,
+        require(!is_md(ctx) && ctx->inited, MAPP_DMD_E_STATE,
                 "bdf_run needs an initialised DMD context");
         require(nsteps >= 0 && max_order >= 1 && max_order <= kBdfMaxOrder && max_retry >= 0 && report,
                 MAPP_DMD_E_ARG, "bad bdf_run arguments");
@@ -2252,7 +2394,7 @@ int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_
 
 int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* ncand) {
     return guarded(ctx, [&]() {
-        require(ctx->inited && ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_STATE,
+        require(ctx->inited && !is_md(ctx), MAPP_DMD_E_STATE,
                 "autotune needs an initialised DMD context");
         require(reps >= 1, MAPP_DMD_E_ARG, "reps >= 1");
         HIP_OK(hipSetDevice(ctx->device));
@@ -2334,7 +2476,7 @@ int mapp_dmd_set_cells(mapp_dmd_t* ctx, int ncell, const double* H, const int* n
                        const double* c) {
     return guarded(ctx, [&]() {
         require(ncell >= 1 && H && natms && x && c, MAPP_DMD_E_ARG, "bad cells");
-        require(ctx->variant != MAPP_DMD_VARIANT_EAM, MAPP_DMD_E_ARG, "batches hold DMD cells");
+        require(!is_md(ctx), MAPP_DMD_E_ARG, "batches hold DMD cells");
         const int nt = ctx->nt, xc = xcols(ctx);
         long tot = 0;
         for (int k = 0; k < ncell; ++k) {

@@ -19,6 +19,7 @@ struct mapp_dmd {
 
     // parameters
     mappdmd::FFParams<double> P{};
+    mappdmd::MdParams md{};  // LJ / FS coefficients per type pair
     mappdmd::FFParams<float> Pf{};
     double rsq_crd = 0, mod_rc = 0;
     double wi1_ref[mappdmd::kMaxPoints]{}, wi2_ref[mappdmd::kMaxPoints]{};

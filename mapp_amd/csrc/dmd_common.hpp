@@ -26,13 +26,35 @@ struct FFParams {
     T c0[kMaxTypes];
 };
 
+constexpr int kMaxComp = kMaxTypes * (kMaxTypes + 1) / 2;
+
+// COMP(i, j) of the reference (src/init.h:114): the unordered type pair
+__host__ __device__ constexpr int md_comp(int i, int j) { return i < j ? (j + 1) * j / 2 + i : (i + 1) * i / 2 + j; }
+
 // Geometry of the single-rank periodic domain (reduced coordinates in [0,1)).
+// pairwise: the list cutoff is cut_sk_sq_tp[COMP(type_i, type_j)] (MD-mode
+// potentials with a cutoff per type pair), else cut_sk_sq.
 struct BoxParams {
     double H[3][3], B[3][3];
     double cut_ph_s[3], s_ph_lo[3], s_ph_hi[3];
     double bin_size[3];
     int grid[3], denom[3], tot_bin;
     double cut_sk_sq;
+    int pairwise;
+    double cut_sk_sq_tp[kMaxComp];
+};
+
+// Coefficients of the MD-mode pair potentials per type pair (COMP order):
+// Lennard-Jones (src/ff_lj.cpp) and Finnis-Sinclair (src/ff_fs.cpp).
+struct MdParams {
+    int nt;
+    int chk[kMaxComp];         // the pair has coefficients
+    double cut_sq[kMaxComp];   // force cutoff squared
+    double eps[kMaxComp], sig[kMaxComp], offset[kMaxComp];  // LJ
+    double A[kMaxTypes], t1[kMaxTypes][kMaxTypes], t2[kMaxTypes][kMaxTypes];  // FS
+    double cut_rho[kMaxComp], cut_phi[kMaxComp], k1[kMaxComp], k2[kMaxComp], k3[kMaxComp];
+    double lj_cut[kMaxComp];   // LJ cutoff of the coef call (list cutoff of pairs without coefficients)
+    int set[kMaxComp];         // the pair was given coefficients
 };
 
 #define HIP_OK(expr)                                                          \

@@ -217,9 +217,11 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
                                                    const int* __restrict__ bin_of,
                                                    const int* __restrict__ bin_start,
                                                    const int* __restrict__ bin_atoms, BoxParams bp,
+                                                   const double* __restrict__ type,
                                                    int* __restrict__ cnt_or_off, int* __restrict__ nl) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natms) return;
+    const int ti = bp.pairwise ? (int)type[i] : 0;
     int tmp = bin_of[i];
     int loc[3];
     for (int d = 2; d > -1; --d) {
@@ -248,7 +250,8 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
                     rsq += d0 * d0;
                     rsq += d1 * d1;
                     rsq += d2 * d2;
-                    if (rsq < bp.cut_sk_sq) {
+                    const double cut = bp.pairwise ? bp.cut_sk_sq_tp[md_comp(ti, (int)type[j])] : bp.cut_sk_sq;
+                    if (rsq < cut) {
                         if (FILL) nl[w] = j;
                         ++w;
                     }

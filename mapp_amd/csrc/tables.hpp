@@ -174,4 +174,77 @@ inline std::string read_fs_file(const char* path, const std::vector<std::string>
     return "";
 }
 
+// Finnis-Sinclair parameter file of ForceField_fs::coef (src/ff_fs.cpp:113-335):
+// the element line, one A per element, nt*nt "t1 t2" lines (row element,
+// column element), nt(nt+1)/2 "cut_rho cut_phi k1 k2 k3" lines (i <= j), in
+// the file's element order; blank lines and '#' comments are skipped.  names
+// gives the type order of the outputs (A[nt], t1/t2[nt*nt], per-pair arrays
+// in COMP order).  Returns "" or an error message.
+struct FsParams {
+    std::vector<double> A, t1, t2, cut_rho, cut_phi, k1, k2, k3;
+};
+inline std::string read_fs_params(const char* path, const std::vector<std::string>& names, FsParams& out) {
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return std::string("FS file ") + path + " not found";
+    struct Closer {
+        FILE* f;
+        ~Closer() { std::fclose(f); }
+    } closer{fp};
+    std::vector<char> buf(1 << 16);
+    auto next = [&](std::vector<std::string>& t) -> bool {
+        while (std::fgets(buf.data(), (int)buf.size(), fp)) {
+            t = split_tokens(buf.data());
+            if (!t.empty()) return true;
+        }
+        return false;
+    };
+    const int nt = (int)names.size();
+    std::vector<std::string> t;
+    if (!next(t)) return "FS file: missing element line";
+    if ((int)t.size() != nt) return "number of types represented and forcefield do not match";
+    std::vector<int> type(nt, -1);
+    for (int i = 0; i < nt; ++i) {
+        for (int k = 0; k < nt; ++k)
+            if (names[k] == t[i]) type[i] = k;
+        if (type[i] < 0) return "Atom type " + t[i] + " not found";
+        for (int k = 0; k < i; ++k)
+            if (type[k] == type[i]) return "Atom type " + t[i] + " duplicated in the force field file";
+    }
+    const int nc = nt * (nt + 1) / 2;
+    out.A.assign(nt, 0.0);
+    out.t1.assign((size_t)nt * nt, 0.0);
+    out.t2.assign((size_t)nt * nt, 0.0);
+    for (auto* v : {&out.cut_rho, &out.cut_phi, &out.k1, &out.k2, &out.k3}) v->assign(nc, 0.0);
+    auto num = [](const std::string& s, double* v) {
+        char* end = nullptr;
+        *v = std::strtod(s.c_str(), &end);
+        return end && end != s.c_str();
+    };
+    for (int i = 0; i < nt; ++i) {
+        if (!next(t) || t.size() != 1 || !num(t[0], &out.A[type[i]])) return "Wrong format in the force filed file";
+    }
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j) {
+            double a, b;
+            if (!next(t) || t.size() != 2 || !num(t[0], &a) || !num(t[1], &b))
+                return "Wrong format in the force filed file";
+            out.t1[(size_t)type[i] * nt + type[j]] = a;
+            out.t2[(size_t)type[i] * nt + type[j]] = b;
+        }
+    for (int i = 0; i < nt; ++i)
+        for (int j = i; j < nt; ++j) {
+            double v[5];
+            if (!next(t) || t.size() != 5) return "Wrong format in the force filed file";
+            for (int k = 0; k < 5; ++k)
+                if (!num(t[k], &v[k])) return "Wrong format in the force filed file";
+            const int c = md_comp(type[i], type[j]);
+            out.cut_rho[c] = v[0];
+            out.cut_phi[c] = v[1];
+            out.k1[c] = v[2];
+            out.k2[c] = v[3];
+            out.k3[c] = v[4];
+        }
+    return "";
+}
+
 }  // namespace mappdmd

@@ -14,7 +14,8 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmapp_dmd.so")
 
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3}
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5}
+MD_VARIANTS = ("EAM", "LJ", "FS")  # x rows [3], c = type
 PREC = {"fp64": 0, "mixed": 1, "surrogate": 2}
 VEC = {"x": 0, "c": 1, "f": 2, "E": 3, "dE": 4, "n": 5, "c_d": 6, "ddE": 7, "s": 8, "t": 9, "v": 10, "a": 11, "g": 12}
 
@@ -68,6 +69,9 @@ _SIGS = {
     "mapp_dmd_surrogate_gradient": [CTX, FP, FP, C.c_long, FP, FP, FP, FP],
     "mapp_dmd_surrogate_params": [CTX, FP, FP],
     "mapp_dmd_surrogate_fallbacks": [CTX, LP],
+    "mapp_dmd_lj_coef": [CTX, C.c_int, C.c_int, D, D, D],
+    "mapp_dmd_fs_coef": [CTX, DP, DP, DP, DP, DP, DP, DP, DP],
+    "mapp_dmd_read_fs_params": [CTX, C.c_char_p, C.POINTER(C.c_char_p)],
 }
 
 
@@ -75,6 +79,8 @@ _NOCTX_SIGS = {
     "mapp_dmd_cfg_read": [C.c_char_p, C.c_int, IP, IP, DP, DP, DP, DP, C.c_char_p, C.c_int, C.c_char_p, C.c_int],
     "mapp_dmd_cfg_write": [C.c_char_p, DP, C.c_int, C.c_int, DP, DP, C.c_int, DP, C.POINTER(C.c_char_p),
                            C.c_char_p, C.c_int],
+    "mapp_dmd_fs_params_read": [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), DP, DP, DP, DP, DP, DP, DP, DP,
+                                C.c_char_p, C.c_int],
 }
 
 
@@ -166,6 +172,21 @@ class ForceFieldDMD:
         m = np.ascontiguousarray(masses, dtype=np.float64)
         self._call("mapp_dmd_coef", r_crd, no_i, alpha_min, alpha_max, T, _dp(m))
 
+    def lj_coef(self, itype, jtype, epsilon, sigma, cut):
+        """ForceField_lj::coef: itype jtype epsilon sigma cutoff"""
+        self._call("mapp_dmd_lj_coef", int(itype), int(jtype), float(epsilon), float(sigma), float(cut))
+
+    def fs_coef(self, p):
+        """ForceField_fs::coef with the parameter file's values (FSParams, type order)"""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
+        self._call("mapp_dmd_fs_coef", *[_dp(a) for a in arrs])
+
+    def read_fs_params(self, path, elements):
+        """ff_coef FS <file>: the parameter file of ForceField_fs::coef"""
+        arr = (C.c_char_p * len(elements))(*[e.encode() for e in elements])
+        self._call("mapp_dmd_read_fs_params", path.encode(), arr)
+
     def set_precision(self, mode):
         self._call("mapp_dmd_set_precision", PREC[mode])
 
@@ -230,8 +251,8 @@ class ForceFieldDMD:
 
     def get(self, name):
         s = self.sizes()
-        eam = self.variant == "EAM"
-        width = (3 if eam else 3 + self.nt) if name in ("x", "f") else (1 if eam else self.nt)
+        md = self.variant in MD_VARIANTS
+        width = (3 if md else 3 + self.nt) if name in ("x", "f") else (1 if md else self.nt)
         out = np.zeros((s["natms"], width))
         self._call("mapp_dmd_get_vec", VEC[name], _dp(out), out.size)
         return out
@@ -257,7 +278,7 @@ class ForceFieldDMD:
 
     def ghosts(self):
         s = self.sizes()
-        x = np.zeros((max(s["nghost"], 1), 3 if self.variant == "EAM" else 3 + self.nt))
+        x = np.zeros((max(s["nghost"], 1), 3 if self.variant in MD_VARIANTS else 3 + self.nt))
         src = np.zeros(max(s["nghost"], 1), dtype=np.int32)
         self._call("mapp_dmd_get_ghosts", _dp(x), _ip(src))
         return x[: s["nghost"]], src[: s["nghost"]]
@@ -421,6 +442,43 @@ def setup_eam(H, s3, types, tables, skin=0.5, device=0):
     ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
     ff.init(skin)
     return ff
+
+
+def setup_lj(H, s3, types, coefs, nt, skin=0.5, device=0):
+    """Lennard-Jones (ForceField_lj): coefs = [(itype, jtype, epsilon, sigma, cutoff), ...]"""
+    ff = ForceFieldDMD(no_types=nt, variant="LJ", device=device)
+    for c in coefs:
+        ff.lj_coef(*c)
+    ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    ff.init(skin)
+    return ff
+
+
+def setup_fs(H, s3, types, params, skin=0.5, device=0):
+    """Finnis-Sinclair (ForceField_fs) with FSParams"""
+    ff = ForceFieldDMD(no_types=len(params.A), variant="FS", device=device)
+    ff.fs_coef(params)
+    ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    ff.init(skin)
+    return ff
+
+
+def read_fs_params(path, elements):
+    """the FS parameter file (ForceField_fs::coef layout) in elements' type order, without a GPU"""
+    from .workload import FSParams
+    lib = load_library()
+    nt = len(elements)
+    nc = nt * (nt + 1) // 2
+    A, t1, t2 = np.zeros(nt), np.zeros(nt * nt), np.zeros(nt * nt)
+    per = [np.zeros(nc) for _ in range(5)]
+    err = C.create_string_buffer(256)
+    arr = (C.c_char_p * nt)(*[e.encode() for e in elements])
+    rc = lib.mapp_dmd_fs_params_read(path.encode(), nt, arr, _dp(A), _dp(t1), _dp(t2), *[_dp(a) for a in per],
+                                     err, 256)
+    if rc != 0:
+        raise DMDError(rc, err.value.decode())
+    return FSParams(elements=list(elements), A=A, t1=t1.reshape(nt, nt), t2=t2.reshape(nt, nt), cut_rho=per[0],
+                    cut_phi=per[1], k1=per[2], k2=per[3], k3=per[4])
 
 
 def setup(lat, tables, no_i=3, variant="DMD_I", precision="fp64", skin=0.5, r_crd=3.0,

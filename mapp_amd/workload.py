@@ -177,5 +177,92 @@ def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0
     return Lattice(H=H, x=np.ascontiguousarray(x), c=np.ascontiguousarray(c), cart=cart)
 
 
+@dataclass
+class FSParams:
+    """Finnis-Sinclair coefficients of ForceField_fs (type order): A[nt],
+    t1[nt][nt], t2[nt][nt] (mat_t_1[i][j]), and per type pair in COMP order
+    (i <= j: j(j+1)/2 + i) cut_rho, cut_phi, k1, k2, k3."""
+    elements: list
+    A: np.ndarray
+    t1: np.ndarray
+    t2: np.ndarray
+    cut_rho: np.ndarray
+    cut_phi: np.ndarray
+    k1: np.ndarray
+    k2: np.ndarray
+    k3: np.ndarray
+
+
+def comp(i, j):
+    """COMP(i, j): the unordered type pair index"""
+    return (j + 1) * j // 2 + i if i < j else (i + 1) * i // 2 + j
+
+
+def fs_params(elements=("A", "B")):
+    """synthetic two-type Finnis-Sinclair coefficients for a BCC lattice of
+    a = 2.87: densities d^2 (t1 + t2 d), d = r - cut_rho, positive over the
+    first two shells; pair terms d^2 (k1 + k2 r + k3 r^2) with a minimum near
+    the first shell"""
+    nt = len(elements)
+    A = np.array([1.8, 2.3, 2.0, 1.6][:nt])
+    t1 = np.array([[1.0, 0.8, 0.9, 0.7], [0.6, 1.2, 0.5, 0.8], [0.9, 0.7, 1.1, 0.6], [0.8, 0.6, 0.7, 1.0]])[:nt, :nt]
+    t2 = np.array([[0.3, -0.2, 0.1, 0.2], [0.25, -0.1, 0.2, 0.1], [0.2, 0.1, -0.3, 0.1], [0.1, 0.2, 0.1, -0.2]])[:nt, :nt]
+    nc = nt * (nt + 1) // 2
+    cut_rho = np.array([3.5, 3.3, 3.1, 3.4, 3.2, 3.0, 3.3, 3.2, 3.1, 3.0])[:nc]
+    cut_phi = np.array([3.4, 3.2, 3.0, 3.3, 3.1, 2.9, 3.2, 3.1, 3.0, 2.9])[:nc]
+    k1 = np.array([1.3, 1.1, 0.9, 1.2, 1.0, 0.8, 1.1, 1.0, 0.9, 0.8])[:nc]
+    k2 = np.array([-0.45, -0.38, -0.30, -0.40, -0.35, -0.28, -0.4, -0.36, -0.3, -0.27])[:nc]
+    k3 = np.array([0.035, 0.03, 0.025, 0.032, 0.028, 0.022, 0.03, 0.028, 0.025, 0.022])[:nc]
+    return FSParams(elements=list(elements), A=A, t1=t1, t2=t2, cut_rho=cut_rho, cut_phi=cut_phi, k1=k1, k2=k2, k3=k3)
+
+
+def write_fs_params(path, p: FSParams, order=None):
+    """the parameter file of ForceField_fs::coef (element line, A lines, t1 t2
+    lines, cut_rho cut_phi k1 k2 k3 lines) with %.17g, in the element order
+    `order` (indices into p.elements; default the type order)"""
+    nt = len(p.elements)
+    order = list(range(nt)) if order is None else list(order)
+    with open(path, "w") as fh:
+        fh.write("# Finnis-Sinclair parameters\n")
+        fh.write(" ".join(p.elements[k] for k in order) + "\n")
+        fh.write("# A\n")
+        for k in order:
+            fh.write("%.17g\n" % p.A[k])
+        fh.write("# t1 t2\n")
+        for a in order:
+            for b in order:
+                fh.write("%.17g %.17g\n" % (p.t1[a][b], p.t2[a][b]))
+        fh.write("# cut_rho cut_phi k1 k2 k3\n")
+        for ia in range(nt):
+            for ib in range(ia, nt):
+                c = comp(order[ia], order[ib])
+                fh.write("%.17g %.17g %.17g %.17g %.17g\n" % (p.cut_rho[c], p.cut_phi[c], p.k1[c], p.k2[c], p.k3[c]))
+
+
+# Lennard-Jones coefficients of a two-type test alloy on the FCC a = 3.6
+# lattice: (itype, jtype, epsilon, sigma, cutoff), one cutoff per type pair,
+# each between two neighbour shells (3.6, 4.41, 5.09, 5.69)
+LJ_COEFS = [(0, 0, 0.50, 2.30, 4.75), (0, 1, 0.40, 2.35, 4.0), (1, 1, 0.60, 2.40, 5.4)]
+
+
+def md_lattice(n, a, kind="fcc", nt=2, frac=0.3, seed=1, jitter=0.03):
+    """n^3 cubic cells of an FCC or BCC lattice, jittered N(0, jitter); type 1
+    on a fraction frac of the sites (nt = 2).  Returns H, reduced positions
+    [natms][3] and the types."""
+    rng = np.random.default_rng(seed)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) if kind == "fcc" else \
+        np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    g = np.arange(n)
+    cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    s0 = ((cells[:, None, :] + basis[None, :, :]) / n).reshape(-1, 3)
+    H = np.diag([a * n] * 3).astype(np.float64)
+    cart = s0 @ H + rng.normal(0.0, jitter, size=s0.shape)
+    s = x2s(cart, m3inv(H))
+    types = np.zeros(s.shape[0])
+    if nt > 1:
+        types[rng.random(s.shape[0]) < frac] = 1.0
+    return H, np.ascontiguousarray(s), types
+
+
 # defaults of the DMD workloads (no_i = 3 or 5 points, alpha_min/max, T, skin)
 DEFAULTS = dict(r_crd=3.0, alpha_min=25.0, alpha_max=1000.0, T=600.0, skin=0.5)

@@ -90,6 +90,12 @@ struct orc {
     double *rho, *drho_dr, *drho_dalpha, *phi, *dphi_dr, *dphi_dalpha;
     int fresh;
     int fix_dE_index; /* 1: dE[atom*nt+type] in the chemical-potential sweep */
+    /* MD-mode potentials: per type pair (COMP order) list cutoffs and coefficients */
+    double md_cut_sq[36], md_cut_sk_sq[36];
+    int md_chk[36], md_set[36];
+    double lj_eps[36], lj_sig[36], lj_cut[36], lj_offset[36];
+    double fs_A[8], fs_t1[8][8], fs_t2[8][8];
+    double fs_cut_rho[36], fs_cut_phi[36], fs_k1[36], fs_k2[36], fs_k3[36];
     int energy_full;  /* 1: consistent energy: energy_calc over all type pairs, cross-type
                          pair terms weighted as in the forces (force_calc's energy too) */
     /* 2nd list */
@@ -105,12 +111,16 @@ static int type2rho(int nt, int i, int j) { return j * nt + i; }
    forces are (the force loop weights each ordered type pair by c_i c_j, :375-380) */
 #define XW(o, it, jt) (((o)->energy_full && (it) != (jt)) ? 2.0 : 1.0)
 static int comp(int i, int j) { return i < j ? (j + 1) * j / 2 + i : (i + 1) * i / 2 + j; }
+/* MD layout: x rows [3], c holds the type (plain EAM, LJ, FS) */
+#define ORC_MD(o) ((o)->variant >= ORC_EAM)
+/* list cutoff per type pair (LJ, FS) */
+#define ORC_PAIRWISE(o) ((o)->variant == ORC_LJ || (o)->variant == ORC_FS)
 
 orc_t* orc_create(int nt, int variant) {
     orc_t* o = (orc_t*)calloc(1, sizeof(orc_t));
     o->nt = nt;
     o->variant = variant;
-    o->xdim = variant == ORC_EAM ? 3 : 3 + nt;
+    o->xdim = variant >= ORC_EAM ? 3 : 3 + nt;
     o->fresh = 0;
     return o;
 }
@@ -332,7 +342,7 @@ int orc_set_system(orc_t* o, const double* Hf, int natms, const double* x, const
     o->cap = natms > 16 ? natms : 16;
     o->x = (double*)malloc((size_t)o->cap * o->xdim * sizeof(double));
     o->c = (double*)malloc((size_t)o->cap * o->nt * sizeof(double));
-    int cd = o->variant == ORC_EAM ? 1 : o->nt;
+    int cd = ORC_MD(o) ? 1 : o->nt;
     memcpy(o->x, x, (size_t)natms * o->xdim * sizeof(double));
     memcpy(o->c, c, (size_t)natms * cd * sizeof(double));
     o->x0 = (double*)calloc((size_t)natms * 3, sizeof(double));
@@ -368,7 +378,7 @@ static void set_ph(orc_t* o, double cut) {
    ph_xchng, :464-497.  x holds reduced coordinates on entry. */
 static void ph_setup(orc_t* o) {
     int natms = o->natms, xd = o->xdim;
-    int cd = o->variant == ORC_EAM ? 1 : o->nt;
+    int cd = ORC_MD(o) ? 1 : o->nt;
     o->nghost = 0;
     o->nswap = 0;
     free(o->ghost_src);
@@ -514,6 +524,7 @@ static void create_list(orc_t* o) {
                         const double* xj = o->x + (long)j * xd;
                         double rsq = 0.0;
                         for (int d = 0; d < 3; d++) rsq += (xi[d] - xj[d]) * (xi[d] - xj[d]);
+                        if (ORC_PAIRWISE(o)) cut = o->md_cut_sk_sq[comp((int)o->c[i], (int)o->c[j])];
                         if (rsq < cut) {
                             if (np == cap) {
                                 cap = cap * 3 / 2;
@@ -560,10 +571,75 @@ static void rebuild(orc_t* o) {
 int orc_init(orc_t* o, double skin) {
     if (!o->have_params || !o->x) return -1;
     o->skin = skin;
+    if (ORC_PAIRWISE(o)) {
+        /* ForceField_lj::init (src/ff_lj.cpp:129-153), ForceField_fs::init
+           (src/ff_fs.cpp:339-360): cutoff+skin of every type pair with
+           coefficients, ghosts for the largest cutoff */
+        double ph_cut = 0.0;
+        int ncomp = o->nt * (o->nt + 1) / 2;
+        for (int k = 0; k < ncomp; k++)
+            if (o->md_chk[k]) {
+                o->md_cut_sk_sq[k] = o->md_cut_sq[k] + skin * skin + 2 * sqrt(o->md_cut_sq[k]) * skin;
+                ph_cut = MAXV(ph_cut, sqrt(o->md_cut_sq[k]));
+            } else if (o->md_set[k] && o->variant == ORC_LJ) {
+                o->md_cut_sk_sq[k] = (o->lj_cut[k] + skin) * (o->lj_cut[k] + skin);  /* coef time value */
+            }
+        o->cut_sk_sq = 0.0;
+        set_ph(o, ph_cut);
+        rebuild(o);
+        return 0;
+    }
     double base = o->variant == ORC_EAM ? o->cut_sq : o->cut_sq_mod;
     o->cut_sk_sq = base + skin * skin + 2.0 * sqrt(base) * skin;
     set_ph(o, sqrt(base));
     rebuild(o);
+    return 0;
+}
+
+/* ForceField_lj::coef (src/ff_lj.cpp:72-125); the shift of the constructor is
+   off (:35), so the offset is zero.  A type pair with a zero cutoff, sigma or
+   epsilon has no coefficients but keeps the list cutoff (cut+skin)^2 of the
+   coef call (taken here with the skin of init); a type pair never given
+   coefficients, whose cutoff the reference leaves uninitialised, lists nothing. */
+int orc_lj_coef(orc_t* o, int itype, int jtype, double eps, double sig, double cut) {
+    if (o->variant != ORC_LJ || itype < 0 || jtype < 0 || itype >= o->nt || jtype >= o->nt) return -1;
+    if (eps < 0.0 || sig < 0.0 || cut < 0.0) return -2;
+    int k = comp(itype, jtype);
+    o->lj_eps[k] = eps;
+    o->lj_sig[k] = sig;
+    o->lj_cut[k] = cut;
+    o->lj_offset[k] = 0.0;
+    o->md_cut_sq[k] = cut * cut;
+    o->md_chk[k] = (cut == 0.0 || sig == 0.0 || eps == 0.0) ? 0 : 1;
+    o->md_set[k] = 1;
+    o->have_params = 1;
+    return 0;
+}
+
+/* ForceField_fs::coef (src/ff_fs.cpp:113-335) with the file's values in type
+   order; cut_sq = max(cut_phi^2, cut_rho^2) at init (src/ff_fs.cpp:346) */
+int orc_fs_coef(orc_t* o, const double* A, const double* t1, const double* t2, const double* cut_rho,
+                const double* cut_phi, const double* k1, const double* k2, const double* k3) {
+    if (o->variant != ORC_FS) return -1;
+    int nt = o->nt, ncomp = nt * (nt + 1) / 2;
+    for (int i = 0; i < nt; i++) {
+        o->fs_A[i] = A[i];
+        for (int j = 0; j < nt; j++) {
+            o->fs_t1[i][j] = t1[i * nt + j];
+            o->fs_t2[i][j] = t2[i * nt + j];
+        }
+    }
+    for (int k = 0; k < ncomp; k++) {
+        o->fs_cut_rho[k] = cut_rho[k];
+        o->fs_cut_phi[k] = cut_phi[k];
+        o->fs_k1[k] = k1[k];
+        o->fs_k2[k] = k2[k];
+        o->fs_k3[k] = k3[k];
+        o->md_cut_sq[k] = MAXV(cut_phi[k] * cut_phi[k], cut_rho[k] * cut_rho[k]);
+        o->md_chk[k] = 1;
+        o->md_set[k] = 1;
+    }
+    o->have_params = 1;
     return 0;
 }
 
@@ -593,7 +669,7 @@ int orc_set_x_update(orc_t* o, const double* x_owned, int* rebuilt) {
 
 /* new owned c, forwarded to the ghosts (Atoms::update of c) */
 void orc_set_c(orc_t* o, const double* c_owned) {
-    int cd = o->variant == ORC_EAM ? 1 : o->nt;
+    int cd = ORC_MD(o) ? 1 : o->nt;
     memcpy(o->c, c_owned, (size_t)o->natms * cd * sizeof(double));
     update_vec(o, o->c, cd);
 }
@@ -708,10 +784,16 @@ int orc_pair_quadrature(const orc_t* o, double r, double alpha, int rho_slot, in
 
 static void force_calc_eam(orc_t* o, int st_clc, double* en_st);
 static double energy_calc_eam(orc_t* o);
+static void force_calc_lj(orc_t* o, int st_clc, double* en_st);
+static double energy_calc_lj(orc_t* o);
+static void force_calc_fs(orc_t* o, int st_clc, double* en_st);
+static double energy_calc_fs(orc_t* o);
 
 /* ForceField_DMD_I::force_calc, src/ff_dmd_I.cpp:88-450 */
 void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
     if (o->variant == ORC_EAM) { force_calc_eam(o, st_clc, en_st); return; }
+    if (o->variant == ORC_LJ) { force_calc_lj(o, st_clc, en_st); return; }
+    if (o->variant == ORC_FS) { force_calc_fs(o, st_clc, en_st); return; }
     int nt = o->nt, xd = o->xdim, natms = o->natms, nphi = nt * (nt + 1) / 2;
     double* x = o->x;
     double* c = o->c;
@@ -874,6 +956,8 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
 /* ForceField_DMD_I::energy_calc, src/ff_dmd_I.cpp:454-668 (jtype from itype) */
 double orc_energy_calc(orc_t* o) {
     if (o->variant == ORC_EAM) return energy_calc_eam(o);
+    if (o->variant == ORC_LJ) return energy_calc_lj(o);
+    if (o->variant == ORC_FS) return energy_calc_fs(o);
     int nt = o->nt, xd = o->xdim, natms = o->natms;
     double* x = o->x;
     double* c = o->c;
@@ -1496,6 +1580,267 @@ static double energy_calc_eam(orc_t* o) {
    x = x_prev + a h over all (3+nt) columns, Atoms::update_0, energy_calc. */
 enum { ORC_LS_S = 0, ORC_LS_F_DOWNHILL = 1, ORC_LS_F_GRAD0 = 2, ORC_LS_F_ALPHAMIN = 3,
        ORC_MIN_F_TOLERANCE = 4, ORC_MIN_F_MAX_ITER = 5 };
+
+/* ForceField_lj::force_calc (src/ff_lj.cpp:160-264); f is zeroed first, as the
+   MD drivers do (zero_f before every force_calc, src/md_nh.cpp:512,605) */
+static void force_calc_lj(orc_t* o, int st_clc, double* en_st) {
+    int natms = o->natms;
+    double* x = o->x;
+    double* f = o->f;
+    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long i = 0; i < (long)natms * 3; i++) f[i] = 0.0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            if (!o->md_chk[curs]) continue;
+            long icomp = 3L * iatm, jcomp = 3L * jatm;
+            double dx0 = x[icomp] - x[jcomp];
+            double dx1 = x[icomp + 1] - x[jcomp + 1];
+            double dx2 = x[icomp + 2] - x[jcomp + 2];
+            double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+            double csq = o->md_cut_sq[curs];
+            if (rsq < csq) {
+                double sig = o->lj_sig[curs], eps = o->lj_eps[curs];
+                double sig2 = sig * sig / rsq;
+                double sig6 = sig2 * sig2 * sig2;
+                double sig12 = sig6 * sig6;
+                double ft = 24.0 * eps * (2.0 * sig12 - sig6) / rsq;
+                f[icomp] += ft * dx0;
+                f[icomp + 1] += ft * dx1;
+                f[icomp + 2] += ft * dx2;
+                if (jatm < natms) {
+                    ns[0] += 4.0 * eps * (sig12 - sig6) + o->lj_offset[curs];
+                    f[jcomp] -= ft * dx0;
+                    f[jcomp + 1] -= ft * dx1;
+                    f[jcomp + 2] -= ft * dx2;
+                    if (st_clc) {
+                        ns[1] -= ft * dx0 * dx0;
+                        ns[2] -= ft * dx1 * dx1;
+                        ns[3] -= ft * dx2 * dx2;
+                        ns[4] -= ft * dx1 * dx2;
+                        ns[5] -= ft * dx2 * dx0;
+                        ns[6] -= ft * dx0 * dx1;
+                    }
+                } else {
+                    ns[0] += 2.0 * eps * (sig12 - sig6) + o->lj_offset[curs] * 0.5;
+                    if (st_clc) {
+                        ns[1] -= 0.5 * ft * dx0 * dx0;
+                        ns[2] -= 0.5 * ft * dx1 * dx1;
+                        ns[3] -= 0.5 * ft * dx2 * dx2;
+                        ns[4] -= 0.5 * ft * dx1 * dx2;
+                        ns[5] -= 0.5 * ft * dx2 * dx0;
+                        ns[6] -= 0.5 * ft * dx0 * dx1;
+                    }
+                }
+            }
+        }
+    }
+    en_st[0] = ns[0];
+    if (st_clc)
+        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+}
+
+/* ForceField_lj::energy_calc (src/ff_lj.cpp:270-341) */
+static double energy_calc_lj(orc_t* o) {
+    int natms = o->natms;
+    double* x = o->x;
+    double en = 0.0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            if (!o->md_chk[curs]) continue;
+            long icomp = 3L * iatm, jcomp = 3L * jatm;
+            double dx0 = x[icomp] - x[jcomp];
+            double dx1 = x[icomp + 1] - x[jcomp + 1];
+            double dx2 = x[icomp + 2] - x[jcomp + 2];
+            double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+            if (rsq < o->md_cut_sq[curs]) {
+                double sig = o->lj_sig[curs], eps = o->lj_eps[curs];
+                double sig2 = sig * sig / rsq;
+                double sig6 = sig2 * sig2 * sig2;
+                double sig12 = sig6 * sig6;
+                if (jatm < natms) en += 4.0 * eps * (sig12 - sig6) + o->lj_offset[curs];
+                else en += 2.0 * eps * (sig12 - sig6) + o->lj_offset[curs] * 0.5;
+            }
+        }
+    }
+    return en;
+}
+
+/* ForceField_fs::force_calc (src/ff_fs.cpp:379-604): rho[i] is complete when
+   i's own row ends (lower sites added their shares before), so the embedding
+   energy and rho[i] = -A/sqrt(rho) are taken there; atoms->update(rho_n)
+   forwards it to the ghosts before the density-force sweep */
+static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
+    int natms = o->natms;
+    double* x = o->x;
+    double* f = o->f;
+    double* rho = o->E;
+    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    long np = o->no_pairs;
+    double* drhoi = (double*)calloc((size_t)np + 1, sizeof(double));
+    double* drhoj = (double*)calloc((size_t)np + 1, sizeof(double));
+    for (long i = 0; i < (long)natms * 3; i++) f[i] = 0.0;
+    for (int iatm = 0; iatm < natms; iatm++) rho[iatm] = 0.0;
+    long istart = 0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        long icomp = 3L * iatm;
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            drhoi[istart] = drhoj[istart] = 0.0;
+            if (o->md_chk[curs]) {
+                long jcomp = 3L * jatm;
+                double dx0 = x[icomp] - x[jcomp];
+                double dx1 = x[icomp + 1] - x[jcomp + 1];
+                double dx2 = x[icomp + 2] - x[jcomp + 2];
+                double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+                if (rsq < o->md_cut_sq[curs]) {
+                    double r = sqrt(rsq);
+                    double r_inv = 1.0 / r;
+                    if (r < o->fs_cut_rho[curs]) {
+                        double dr_rho = r - o->fs_cut_rho[curs];
+                        rho[iatm] += dr_rho * dr_rho * (o->fs_t1[jtype][itype] + o->fs_t2[jtype][itype] * dr_rho);
+                        drhoi[istart] = -r_inv * dr_rho * (o->fs_t1[jtype][itype] + 1.5 * o->fs_t2[jtype][itype] * dr_rho);
+                        drhoj[istart] = -r_inv * dr_rho * (o->fs_t1[itype][jtype] + 1.5 * o->fs_t2[itype][jtype] * dr_rho);
+                        if (jatm < natms)
+                            rho[jatm] += dr_rho * dr_rho * (o->fs_t1[itype][jtype] + o->fs_t2[itype][jtype] * dr_rho);
+                    }
+                    if (r < o->fs_cut_phi[curs]) {
+                        double dr_phi = r - o->fs_cut_phi[curs];
+                        double k1 = o->fs_k1[curs], k2 = o->fs_k2[curs], k3 = o->fs_k3[curs];
+                        double phi_coef = 2.0 * dr_phi * (k1 + k2 * r + k3 * rsq) + dr_phi * dr_phi * (k2 + 2.0 * k3 * r);
+                        phi_coef *= -r_inv;
+                        f[icomp] += dx0 * phi_coef;
+                        f[icomp + 1] += dx1 * phi_coef;
+                        f[icomp + 2] += dx2 * phi_coef;
+                        if (jatm < natms) {
+                            f[jcomp] -= dx0 * phi_coef;
+                            f[jcomp + 1] -= dx1 * phi_coef;
+                            f[jcomp + 2] -= dx2 * phi_coef;
+                            ns[0] += dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                            if (st_clc) {
+                                ns[1] -= phi_coef * dx0 * dx0;
+                                ns[2] -= phi_coef * dx1 * dx1;
+                                ns[3] -= phi_coef * dx2 * dx2;
+                                ns[4] -= phi_coef * dx1 * dx2;
+                                ns[5] -= phi_coef * dx2 * dx0;
+                                ns[6] -= phi_coef * dx0 * dx1;
+                            }
+                        } else {
+                            ns[0] += 0.5 * dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                            if (st_clc) {
+                                ns[1] -= 0.5 * phi_coef * dx0 * dx0;
+                                ns[2] -= 0.5 * phi_coef * dx1 * dx1;
+                                ns[3] -= 0.5 * phi_coef * dx2 * dx2;
+                                ns[4] -= 0.5 * phi_coef * dx1 * dx2;
+                                ns[5] -= 0.5 * phi_coef * dx2 * dx0;
+                                ns[6] -= 0.5 * phi_coef * dx0 * dx1;
+                            }
+                        }
+                    }
+                }
+            }
+            istart++;
+        }
+        double rho_sqd = sqrt(rho[iatm]);
+        ns[0] -= o->fs_A[itype] * rho_sqd;
+        rho[iatm] = -o->fs_A[itype] / rho_sqd;
+    }
+    update_vec(o, rho, 1);
+    istart = 0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        long icomp = 3L * iatm;
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            if (drhoi[istart] != 0.0 || drhoj[istart] != 0.0) {
+                int jatm = o->nl[jj];
+                long jcomp = 3L * jatm;
+                double dx0 = x[icomp] - x[jcomp];
+                double dx1 = x[icomp + 1] - x[jcomp + 1];
+                double dx2 = x[icomp + 2] - x[jcomp + 2];
+                double rho_coef = drhoi[istart] * rho[iatm] + drhoj[istart] * rho[jatm];
+                f[icomp] += dx0 * rho_coef;
+                f[icomp + 1] += dx1 * rho_coef;
+                f[icomp + 2] += dx2 * rho_coef;
+                if (jatm < natms) {
+                    f[jcomp] -= dx0 * rho_coef;
+                    f[jcomp + 1] -= dx1 * rho_coef;
+                    f[jcomp + 2] -= dx2 * rho_coef;
+                    if (st_clc) {
+                        ns[1] -= rho_coef * dx0 * dx0;
+                        ns[2] -= rho_coef * dx1 * dx1;
+                        ns[3] -= rho_coef * dx2 * dx2;
+                        ns[4] -= rho_coef * dx1 * dx2;
+                        ns[5] -= rho_coef * dx2 * dx0;
+                        ns[6] -= rho_coef * dx0 * dx1;
+                    }
+                } else if (st_clc) {
+                    ns[1] -= 0.5 * rho_coef * dx0 * dx0;
+                    ns[2] -= 0.5 * rho_coef * dx1 * dx1;
+                    ns[3] -= 0.5 * rho_coef * dx2 * dx2;
+                    ns[4] -= 0.5 * rho_coef * dx1 * dx2;
+                    ns[5] -= 0.5 * rho_coef * dx2 * dx0;
+                    ns[6] -= 0.5 * rho_coef * dx0 * dx1;
+                }
+            }
+            istart++;
+        }
+    }
+    free(drhoi);
+    free(drhoj);
+    en_st[0] = ns[0];
+    if (st_clc)
+        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+}
+
+/* ForceField_fs::energy_calc (src/ff_fs.cpp:609-686) */
+static double energy_calc_fs(orc_t* o) {
+    int natms = o->natms;
+    double* x = o->x;
+    double* rho = o->E;
+    double en = 0.0;
+    for (int i = 0; i < natms; i++) rho[i] = 0.0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        long icomp = 3L * iatm;
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            if (!o->md_chk[curs]) continue;
+            long jcomp = 3L * jatm;
+            double dx0 = x[icomp] - x[jcomp];
+            double dx1 = x[icomp + 1] - x[jcomp + 1];
+            double dx2 = x[icomp + 2] - x[jcomp + 2];
+            double rsq = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+            if (rsq < o->md_cut_sq[curs]) {
+                double r = sqrt(rsq);
+                if (r < o->fs_cut_rho[curs]) {
+                    double dr_rho = r - o->fs_cut_rho[curs];
+                    rho[iatm] += dr_rho * dr_rho * (o->fs_t1[jtype][itype] + o->fs_t2[jtype][itype] * dr_rho);
+                    if (jatm < natms)
+                        rho[jatm] += dr_rho * dr_rho * (o->fs_t1[itype][jtype] + o->fs_t2[itype][jtype] * dr_rho);
+                }
+                if (r < o->fs_cut_phi[curs]) {
+                    double dr_phi = r - o->fs_cut_phi[curs];
+                    double k1 = o->fs_k1[curs], k2 = o->fs_k2[curs], k3 = o->fs_k3[curs];
+                    if (jatm < natms) en += dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                    else en += 0.5 * dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                }
+            }
+        }
+        en -= o->fs_A[itype] * sqrt(rho[iatm]);
+    }
+    return en;
+}
 
 static double ls_energy(orc_t* o, const double* x_prev, const double* h, double a, double* xbuf, long n) {
     for (long i = 0; i < n; i++) xbuf[i] = x_prev[i] + a * h[i];

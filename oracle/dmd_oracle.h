@@ -22,7 +22,19 @@ extern "C" {
 typedef struct orc orc_t;
 
 /* variants of the DMD force field (they differ only in c_d_calc / calc_g) */
-enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3 };
+enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3, ORC_LJ = 4, ORC_FS = 5 };
+
+/* MD-mode potentials (x rows [3], c holds the type): Lennard-Jones
+   (ForceField_lj, src/ff_lj.cpp) and Finnis-Sinclair (ForceField_fs,
+   src/ff_fs.cpp).  Their neighbour lists use a cutoff per type pair,
+   cut_sk_sq[COMP(type_i, type_j)] (src/neighbor.cpp:205-208). */
+/* ForceField_lj::coef "itype jtype epsilon sigma cutoff" (src/ff_lj.cpp:72-125) */
+int orc_lj_coef(orc_t* o, int itype, int jtype, double epsilon, double sigma, double cut);
+/* ForceField_fs::coef after reading its file (src/ff_fs.cpp:113-335): A[nt],
+   t1[nt*nt], t2[nt*nt] as mat_t_1[i][j], and per type pair in COMP order
+   cut_rho, cut_phi, k1, k2, k3 (nt(nt+1)/2 each) */
+int orc_fs_coef(orc_t* o, const double* A, const double* t1, const double* t2, const double* cut_rho,
+                const double* cut_phi, const double* k1, const double* k2, const double* k3);
 
 orc_t* orc_create(int nt, int variant);
 void orc_destroy(orc_t* o);

@@ -10,7 +10,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_dmd.so")
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3}
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5}
+MD = ("EAM", "LJ", "FS")
 
 D = C.c_double
 DP = C.POINTER(C.c_double)
@@ -74,6 +75,8 @@ def lib():
         L.orc_min_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, D, DP, IP, IP, IP]
         L.orc_bdf_run.argtypes = [C.c_void_p, C.c_int, C.c_int, D, C.c_int, C.c_int, DP, IP]
         L.orc_bdf_coef.argtypes = [C.c_int, D, DP, C.c_int, DP]
+        L.orc_lj_coef.argtypes = [C.c_void_p, C.c_int, C.c_int, D, D, D]
+        L.orc_fs_coef.argtypes = [C.c_void_p] + [DP] * 8
         _lib = L
     return _lib
 
@@ -89,14 +92,14 @@ def _view(ptr, n, dtype):
 
 
 class Oracle:
-    """CPU restatement of ForceField_DMD* / ForceField_eam on one rank."""
+    """CPU restatement of ForceField_DMD* / ForceField_eam / _lj / _fs on one rank."""
 
     def __init__(self, nt=2, variant="DMD_I"):
         self.L = lib()
         self.nt = nt
         self.variant = variant
         self.o = self.L.orc_create(nt, VARIANT[variant])
-        self.xdim = 3 if variant == "EAM" else 3 + nt
+        self.xdim = 3 if variant in MD else 3 + nt
 
     def __del__(self):
         try:
@@ -113,6 +116,16 @@ class Oracle:
     def set_params(self, r_crd=3.0, no_i=3, alpha_min=25.0, alpha_max=1000.0, T=600.0, masses=(58.6934, 63.546)):
         m = np.ascontiguousarray(masses[: self.nt], dtype=np.float64)
         assert self.L.orc_set_params(self.o, r_crd, no_i, alpha_min, alpha_max, T, _dp(m)) == 0
+
+    def lj_coef(self, itype, jtype, epsilon, sigma, cut):
+        """ForceField_lj::coef: itype jtype epsilon sigma cutoff"""
+        assert self.L.orc_lj_coef(self.o, itype, jtype, epsilon, sigma, cut) == 0
+
+    def fs_coef(self, p):
+        """ForceField_fs::coef with the file's values: p = FSParams"""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
+        assert self.L.orc_fs_coef(self.o, *[_dp(a) for a in arrs]) == 0
 
     def set_system(self, H, x, c):
         H = np.ascontiguousarray(H, dtype=np.float64)
@@ -197,7 +210,7 @@ class Oracle:
         if name == "f":
             w = self.xdim
         else:
-            w = 1 if self.variant == "EAM" else self.nt
+            w = 1 if self.variant in MD else self.nt
         ptr = self.L.orc_vec(self.o, name.encode())
         return _view(ptr, n * w, np.float64).reshape(n, w)
 

@@ -21,15 +21,15 @@ def _pair(lat, tables, no_i=3, variant="DMD_I", precision="fp64", fresh=True):
 
 
 @pytest.fixture(scope="module")
-def cfg0(tables):
+def cell10(tables):
     from mapp_amd import workload as W
     lat = W.fcc_alloy(10, seed=1)
     ff, orc = _pair(lat, tables)
     return lat, ff, orc
 
 
-def test_ghosts_bit_exact(cfg0):
-    lat, ff, orc = cfg0
+def test_ghosts_bit_exact(cell10):
+    lat, ff, orc = cell10
     gx, gsrc = ff.ghosts()
     ox, osrc = orc.ghosts()
     print("nghost", len(gsrc), len(osrc), flush=True)
@@ -37,8 +37,8 @@ def test_ghosts_bit_exact(cfg0):
     assert np.array_equal(gx.view(np.uint64), ox.view(np.uint64))
 
 
-def test_neighbor_list_bit_exact(cfg0):
-    lat, ff, orc = cfg0
+def test_neighbor_list_bit_exact(cell10):
+    lat, ff, orc = cell10
     goff, gnl = ff.neighbor_list()
     ooff, onl = orc.neighbor_list()
     print("pairs", len(gnl), len(onl), flush=True)
@@ -46,8 +46,8 @@ def test_neighbor_list_bit_exact(cfg0):
     assert np.array_equal(gnl, onl)
 
 
-def test_force_calc_fp64(cfg0):
-    lat, ff, orc = cfg0
+def test_force_calc_fp64(cell10):
+    lat, ff, orc = cell10
     en_g = ff.force_calc(1)
     en_o = orc.force_calc(1)
     print("energy", en_g[0], en_o[0], flush=True)
@@ -59,15 +59,15 @@ def test_force_calc_fp64(cfg0):
         assert err <= 1e-12, name
 
 
-def test_energy_calc_fp64(cfg0):
-    lat, ff, orc = cfg0
+def test_energy_calc_fp64(cell10):
+    lat, ff, orc = cell10
     eg = ff.energy_calc()
     eo = orc.energy_calc()
     assert abs(eg - eo) <= 1e-12 * abs(eo)
 
 
-def test_deterministic_repeat(cfg0):
-    lat, ff, orc = cfg0
+def test_deterministic_repeat(cell10):
+    lat, ff, orc = cell10
     e1 = ff.force_calc(1)
     f1 = ff.get("f")
     e2 = ff.force_calc(1)
