@@ -50,7 +50,11 @@ enum {
        mapp_dmd_lj_coef / mapp_dmd_fs_coef / mapp_dmd_read_fs_params instead of
        set_fs + coef. */
     MAPP_DMD_VARIANT_LJ = 4, /* ForceField_lj (src/ff_lj.cpp), force_calc :160-264, energy_calc :270-341 */
-    MAPP_DMD_VARIANT_FS = 5  /* ForceField_fs (src/ff_fs.cpp), force_calc :379-604, energy_calc :609-686 */
+    MAPP_DMD_VARIANT_FS = 5, /* ForceField_fs (src/ff_fs.cpp), force_calc :379-604, energy_calc :609-686 */
+    /* ForceField_fsm (src/ff_fsm.cpp), force_calc :371-576, energy_calc
+       :582-664: FS with the cutoffs kept squared (:281-282) and compared as
+       rsq < cut^2; same coefficient entry points and file format as FS */
+    MAPP_DMD_VARIANT_FSM = 6
 };
 
 /* arithmetic of the pair quadrature (per-site sums, embedding and the
@@ -111,7 +115,8 @@ int mapp_dmd_lj_coef(mapp_dmd_t* ctx, int itype, int jtype, double epsilon, doub
 /* Finnis-Sinclair coefficients of ForceField_fs::coef (src/ff_fs.cpp:113-335)
    in type order: A[nt]; t1[nt*nt], t2[nt*nt] as mat_t_1[i][j], mat_t_2[i][j];
    per type pair in COMP order (i <= j: j(j+1)/2 + i) cut_rho, cut_phi, k1, k2,
-   k3 [nt(nt+1)/2]. */
+   k3 [nt(nt+1)/2].  Also for the FSM variant (ForceField_fsm::coef,
+   src/ff_fsm.cpp:111-333), which keeps cut_rho^2 and cut_phi^2. */
 int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const double* t2, const double* cut_rho,
                      const double* cut_phi, const double* k1, const double* k2, const double* k3);
 /* ff_coef FS <file>: the parameter file of ForceField_fs::coef (the element

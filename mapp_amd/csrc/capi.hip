@@ -424,7 +424,8 @@ double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
     const bool lj = ctx->variant == MAPP_DMD_VARIANT_LJ;
     if (ctx->variant == MAPP_DMD_VARIANT_EAM) md_passes<kModelEam>(ctx, energy_only, nb, L, T, drho, part1, part2);
     else if (lj) md_passes<kModelLj>(ctx, energy_only, nb, L, T, drho, part1, part2);
-    else md_passes<kModelFs>(ctx, energy_only, nb, L, T, drho, part1, part2);
+    else if (ctx->variant == MAPP_DMD_VARIANT_FS) md_passes<kModelFs>(ctx, energy_only, nb, L, T, drho, part1, part2);
+    else md_passes<kModelFsm>(ctx, energy_only, nb, L, T, drho, part1, part2);
     double* out = ctx->d_out.p;
     reduce_partials(ctx, part1, nb, 7, 7, 0, out);
     reduce_partials(ctx, part2, nb, 1, 1, 0, out + 7);
@@ -1392,7 +1393,7 @@ extern "C" {
 int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
     if (!out) return MAPP_DMD_E_ARG;
     *out = nullptr;
-    if (variant < 0 || variant > MAPP_DMD_VARIANT_FS) return MAPP_DMD_E_ARG;
+    if (variant < 0 || variant > MAPP_DMD_VARIANT_FSM) return MAPP_DMD_E_ARG;
     if (no_types < 1 || no_types > (variant >= MAPP_DMD_VARIANT_LJ ? kMaxTypes : 2)) return MAPP_DMD_E_RANGE;
     std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
     ctx->nt = no_types;
@@ -1514,8 +1515,11 @@ int mapp_dmd_lj_coef(mapp_dmd_t* ctx, int itype, int jtype, double epsilon, doub
 int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const double* t2, const double* cut_rho,
                      const double* cut_phi, const double* k1, const double* k2, const double* k3) {
     return guarded(ctx, [&]() {
-        // ForceField_fs::coef (src/ff_fs.cpp:113-335); cut_sq at init (:346)
-        require(ctx->variant == MAPP_DMD_VARIANT_FS, MAPP_DMD_E_STATE, "fs_coef needs the FS variant");
+        // ForceField_fs::coef (src/ff_fs.cpp:113-335); cut_sq at init (:346).
+        // ForceField_fsm::coef (src/ff_fsm.cpp:281-282) keeps the cutoffs
+        // squared and takes sqrt(cut^2) as the cutoff distance (:424-426)
+        const bool fsm = ctx->variant == MAPP_DMD_VARIANT_FSM;
+        require(ctx->variant == MAPP_DMD_VARIANT_FS || fsm, MAPP_DMD_E_STATE, "fs_coef needs the FS or FSM variant");
         require(A && t1 && t2 && cut_rho && cut_phi && k1 && k2 && k3, MAPP_DMD_E_ARG, "null coefficients");
         MdParams& M = ctx->md;
         const int nt = ctx->nt, nc = nt * (nt + 1) / 2;
@@ -1528,12 +1532,14 @@ int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const d
             }
         }
         for (int k = 0; k < nc; ++k) {
-            M.cut_rho[k] = cut_rho[k];
-            M.cut_phi[k] = cut_phi[k];
+            M.cut_sq_rho[k] = cut_rho[k] * cut_rho[k];
+            M.cut_sq_phi[k] = cut_phi[k] * cut_phi[k];
+            M.cut_rho[k] = fsm ? std::sqrt(M.cut_sq_rho[k]) : cut_rho[k];
+            M.cut_phi[k] = fsm ? std::sqrt(M.cut_sq_phi[k]) : cut_phi[k];
             M.k1[k] = k1[k];
             M.k2[k] = k2[k];
             M.k3[k] = k3[k];
-            M.cut_sq[k] = std::max(cut_phi[k] * cut_phi[k], cut_rho[k] * cut_rho[k]);
+            M.cut_sq[k] = std::max(M.cut_sq_phi[k], M.cut_sq_rho[k]);
             M.chk[k] = 1;
             M.set[k] = 1;
         }
@@ -1588,7 +1594,7 @@ int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, dou
                   const double* mass) {
     return guarded(ctx, [&]() {
         require(ctx->variant < MAPP_DMD_VARIANT_LJ, MAPP_DMD_E_STATE,
-                "LJ and FS take mapp_dmd_lj_coef / mapp_dmd_fs_coef / mapp_dmd_read_fs_params");
+                "LJ, FS and FSM take mapp_dmd_lj_coef / mapp_dmd_fs_coef / mapp_dmd_read_fs_params");
         require(ctx->have_tables, MAPP_DMD_E_STATE, "tables must be set before coef");
         if (ctx->variant == MAPP_DMD_VARIANT_EAM) {
             // ForceField_eam::coef, src/ff_eam.cpp:424-425

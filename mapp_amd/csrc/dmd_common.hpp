@@ -53,6 +53,7 @@ struct MdParams {
     double eps[kMaxComp], sig[kMaxComp], offset[kMaxComp];  // LJ
     double A[kMaxTypes], t1[kMaxTypes][kMaxTypes], t2[kMaxTypes][kMaxTypes];  // FS
     double cut_rho[kMaxComp], cut_phi[kMaxComp], k1[kMaxComp], k2[kMaxComp], k3[kMaxComp];
+    double cut_sq_rho[kMaxComp], cut_sq_phi[kMaxComp];  // FSM: the cutoffs squared, tested as rsq < cut^2
     double lj_cut[kMaxComp];   // LJ cutoff of the coef call (list cutoff of pairs without coefficients)
     int set[kMaxComp];         // the pair was given coefficients
 };

@@ -24,7 +24,10 @@ struct EamBufs {
     int nt;
 };
 
-enum { kModelEam = 0, kModelLj = 1, kModelFs = 2 };
+// FSM is FS with the cutoff tests on squares (rsq < cut^2, src/ff_fsm.cpp:424,
+// :463, :518); its one-sweep density force (:466-476) is the same sum as FS's
+// two factors (-A/sqrt(rho) kept per site) in another rounding order
+enum { kModelEam = 0, kModelLj = 1, kModelFs = 2, kModelFsm = 3 };
 
 __device__ __forceinline__ void eam_row(const FFParams<double>& P, double r, int* m_out, double* p_out) {
     double p = r * P.dr_inv;
@@ -84,7 +87,8 @@ __device__ __forceinline__ bool md_pair(const FFParams<double>& P, const EamBufs
         if (!M.chk[c] || !(rsq < M.cut_sq[c])) return false;
         const double r = sqrt(rsq);
         const double r_inv = 1.0 / r;
-        if (r < M.cut_rho[c]) {
+        constexpr bool kSq = MODEL == kModelFsm;
+        if (kSq ? rsq < M.cut_sq_rho[c] : r < M.cut_rho[c]) {
             const double d = r - M.cut_rho[c];
             *rho_i = d * d * (M.t1[jt][it] + M.t2[jt][it] * d);
             *rho_j = d * d * (M.t1[it][jt] + M.t2[it][jt] * d);
@@ -92,7 +96,7 @@ __device__ __forceinline__ bool md_pair(const FFParams<double>& P, const EamBufs
                 *dr = make_double2(-r_inv * d * (M.t1[jt][it] + 1.5 * M.t2[jt][it] * d),
                                    -r_inv * d * (M.t1[it][jt] + 1.5 * M.t2[it][jt] * d));
         }
-        if (r < M.cut_phi[c]) {
+        if (kSq ? rsq < M.cut_sq_phi[c] : r < M.cut_phi[c]) {
             const double d = r - M.cut_phi[c];
             const double k1 = M.k1[c], k2 = M.k2[c], k3 = M.k3[c];
             *phi = d * d * (k1 + k2 * r + k3 * rsq);
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_embed(FFParams<double> P, EamBuf
         double r = rho_own[i];
         for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) r += pairE[q];
         const int it = (int)type[i];
-        if constexpr (MODEL == kModelFs) {
+        if constexpr (MODEL == kModelFs || MODEL == kModelFsm) {
             const double sq = sqrt(r);
             acc[0] = -(M.A[it] * sq);
             rho[i] = r;

@@ -14,8 +14,8 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmapp_dmd.so")
 
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5}
-MD_VARIANTS = ("EAM", "LJ", "FS")  # x rows [3], c = type
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6}
+MD_VARIANTS = ("EAM", "LJ", "FS", "FSM")  # x rows [3], c = type
 PREC = {"fp64": 0, "mixed": 1, "surrogate": 2}
 VEC = {"x": 0, "c": 1, "f": 2, "E": 3, "dE": 4, "n": 5, "c_d": 6, "ddE": 7, "s": 8, "t": 9, "v": 10, "a": 11, "g": 12}
 
@@ -177,7 +177,7 @@ class ForceFieldDMD:
         self._call("mapp_dmd_lj_coef", int(itype), int(jtype), float(epsilon), float(sigma), float(cut))
 
     def fs_coef(self, p):
-        """ForceField_fs::coef with the parameter file's values (FSParams, type order)"""
+        """ForceField_fs::coef (ForceField_fsm::coef for FSM) with the parameter file's values (FSParams, type order)"""
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
                 for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
         self._call("mapp_dmd_fs_coef", *[_dp(a) for a in arrs])
@@ -454,9 +454,10 @@ def setup_lj(H, s3, types, coefs, nt, skin=0.5, device=0):
     return ff
 
 
-def setup_fs(H, s3, types, params, skin=0.5, device=0):
-    """Finnis-Sinclair (ForceField_fs) with FSParams"""
-    ff = ForceFieldDMD(no_types=len(params.A), variant="FS", device=device)
+def setup_fs(H, s3, types, params, skin=0.5, device=0, variant="FS"):
+    """Finnis-Sinclair with FSParams: ForceField_fs, or ForceField_fsm with variant FSM"""
+    assert variant in ("FS", "FSM")
+    ff = ForceFieldDMD(no_types=len(params.A), variant=variant, device=device)
     ff.fs_coef(params)
     ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
     ff.init(skin)

@@ -96,6 +96,7 @@ struct orc {
     double lj_eps[36], lj_sig[36], lj_cut[36], lj_offset[36];
     double fs_A[8], fs_t1[8][8], fs_t2[8][8];
     double fs_cut_rho[36], fs_cut_phi[36], fs_k1[36], fs_k2[36], fs_k3[36];
+    double fs_cut_sq_rho[36], fs_cut_sq_phi[36];  /* FSM keeps the squares */
     int energy_full;  /* 1: consistent energy: energy_calc over all type pairs, cross-type
                          pair terms weighted as in the forces (force_calc's energy too) */
     /* 2nd list */
@@ -113,8 +114,8 @@ static int type2rho(int nt, int i, int j) { return j * nt + i; }
 static int comp(int i, int j) { return i < j ? (j + 1) * j / 2 + i : (i + 1) * i / 2 + j; }
 /* MD layout: x rows [3], c holds the type (plain EAM, LJ, FS) */
 #define ORC_MD(o) ((o)->variant >= ORC_EAM)
-/* list cutoff per type pair (LJ, FS) */
-#define ORC_PAIRWISE(o) ((o)->variant == ORC_LJ || (o)->variant == ORC_FS)
+/* list cutoff per type pair (LJ, FS, FSM) */
+#define ORC_PAIRWISE(o) ((o)->variant == ORC_LJ || (o)->variant == ORC_FS || (o)->variant == ORC_FSM)
 
 orc_t* orc_create(int nt, int variant) {
     orc_t* o = (orc_t*)calloc(1, sizeof(orc_t));
@@ -617,10 +618,12 @@ int orc_lj_coef(orc_t* o, int itype, int jtype, double eps, double sig, double c
 }
 
 /* ForceField_fs::coef (src/ff_fs.cpp:113-335) with the file's values in type
-   order; cut_sq = max(cut_phi^2, cut_rho^2) at init (src/ff_fs.cpp:346) */
+   order; cut_sq = max(cut_phi^2, cut_rho^2) at init (src/ff_fs.cpp:346).
+   ForceField_fsm::coef (src/ff_fsm.cpp:111-333) reads the same file and keeps
+   cut_rho^2, cut_phi^2 (:281-282); cut_sq = max of the two at init (:345) */
 int orc_fs_coef(orc_t* o, const double* A, const double* t1, const double* t2, const double* cut_rho,
                 const double* cut_phi, const double* k1, const double* k2, const double* k3) {
-    if (o->variant != ORC_FS) return -1;
+    if (o->variant != ORC_FS && o->variant != ORC_FSM) return -1;
     int nt = o->nt, ncomp = nt * (nt + 1) / 2;
     for (int i = 0; i < nt; i++) {
         o->fs_A[i] = A[i];
@@ -635,7 +638,9 @@ int orc_fs_coef(orc_t* o, const double* A, const double* t1, const double* t2, c
         o->fs_k1[k] = k1[k];
         o->fs_k2[k] = k2[k];
         o->fs_k3[k] = k3[k];
-        o->md_cut_sq[k] = MAXV(cut_phi[k] * cut_phi[k], cut_rho[k] * cut_rho[k]);
+        o->fs_cut_sq_rho[k] = cut_rho[k] * cut_rho[k];
+        o->fs_cut_sq_phi[k] = cut_phi[k] * cut_phi[k];
+        o->md_cut_sq[k] = MAXV(o->fs_cut_sq_phi[k], o->fs_cut_sq_rho[k]);
         o->md_chk[k] = 1;
         o->md_set[k] = 1;
     }
@@ -788,12 +793,15 @@ static void force_calc_lj(orc_t* o, int st_clc, double* en_st);
 static double energy_calc_lj(orc_t* o);
 static void force_calc_fs(orc_t* o, int st_clc, double* en_st);
 static double energy_calc_fs(orc_t* o);
+static void force_calc_fsm(orc_t* o, int st_clc, double* en_st);
+static double energy_calc_fsm(orc_t* o);
 
 /* ForceField_DMD_I::force_calc, src/ff_dmd_I.cpp:88-450 */
 void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
     if (o->variant == ORC_EAM) { force_calc_eam(o, st_clc, en_st); return; }
     if (o->variant == ORC_LJ) { force_calc_lj(o, st_clc, en_st); return; }
     if (o->variant == ORC_FS) { force_calc_fs(o, st_clc, en_st); return; }
+    if (o->variant == ORC_FSM) { force_calc_fsm(o, st_clc, en_st); return; }
     int nt = o->nt, xd = o->xdim, natms = o->natms, nphi = nt * (nt + 1) / 2;
     double* x = o->x;
     double* c = o->c;
@@ -958,6 +966,7 @@ double orc_energy_calc(orc_t* o) {
     if (o->variant == ORC_EAM) return energy_calc_eam(o);
     if (o->variant == ORC_LJ) return energy_calc_lj(o);
     if (o->variant == ORC_FS) return energy_calc_fs(o);
+    if (o->variant == ORC_FSM) return energy_calc_fsm(o);
     int nt = o->nt, xd = o->xdim, natms = o->natms;
     double* x = o->x;
     double* c = o->c;
@@ -1839,6 +1848,128 @@ static double energy_calc_fs(orc_t* o) {
         }
         en -= o->fs_A[itype] * sqrt(rho[iatm]);
     }
+    return en;
+}
+
+/* virial share of a pair force coefficient c along d (w: 1, or 0.5 for a
+   ghost partner) */
+static void pair_virial(double* ns, double w, double c, double d0, double d1, double d2) {
+    ns[1] -= w * c * d0 * d0;
+    ns[2] -= w * c * d1 * d1;
+    ns[3] -= w * c * d2 * d2;
+    ns[4] -= w * c * d1 * d2;
+    ns[5] -= w * c * d2 * d0;
+    ns[6] -= w * c * d0 * d1;
+}
+
+/* the density sweep of ForceField_fsm (src/ff_fsm.cpp:407-434 in force_calc,
+   :616-643 in energy_calc): shares inside cut_rho^2 with the half list's
+   partner shares to local sites only; with en != NULL (energy_calc) also the
+   pair energy inside cut_phi^2 (ghost partners at half weight) and the
+   embedding energy -A sqrt(rho) once a site's row ends, when rho is complete
+   (lower sites added their shares before) */
+static void fsm_density(orc_t* o, double* rho, double* en) {
+    int natms = o->natms;
+    const double* x = o->x;
+    for (int i = 0; i < natms; i++) rho[i] = 0.0;
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            if (!o->md_chk[curs]) continue;
+            double d[3];
+            for (int k = 0; k < 3; k++) d[k] = x[3L * iatm + k] - x[3L * jatm + k];
+            double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (en && !(rsq < o->md_cut_sq[curs])) continue;
+            double r = sqrt(rsq);
+            if (rsq < o->fs_cut_sq_rho[curs]) {
+                double dr = r - sqrt(o->fs_cut_sq_rho[curs]);
+                rho[iatm] += dr * dr * (o->fs_t1[jtype][itype] + o->fs_t2[jtype][itype] * dr);
+                if (jatm < natms) rho[jatm] += dr * dr * (o->fs_t1[itype][jtype] + o->fs_t2[itype][jtype] * dr);
+            }
+            if (en && rsq < o->fs_cut_sq_phi[curs]) {
+                double dp = r - sqrt(o->fs_cut_sq_phi[curs]);
+                double e = dp * dp * (o->fs_k1[curs] + o->fs_k2[curs] * r + o->fs_k3[curs] * rsq);
+                *en += jatm < natms ? e : 0.5 * e;
+            }
+        }
+        if (en) *en += -o->fs_A[itype] * sqrt(rho[iatm]);
+    }
+}
+
+/* ForceField_fsm::force_calc (src/ff_fsm.cpp:371-576): the density sweep,
+   atoms->update(rho_n) (:436) forwards the raw densities to the ghosts, then
+   one sweep takes the embedding energy of each site (:448) and per pair the
+   density force A_i g_ji/sqrt(rho_i) + A_j g_ij/sqrt(rho_j) over r, with
+   g = dr (t1 + 1.5 t2 dr) (:466-476), and the pair force and energy inside
+   cut_phi^2 (:518-558); ghost partners take half the virial and energy */
+static void force_calc_fsm(orc_t* o, int st_clc, double* en_st) {
+    int natms = o->natms;
+    const double* x = o->x;
+    double* f = o->f;
+    double* rho = o->E;
+    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long i = 0; i < (long)natms * 3; i++) f[i] = 0.0;
+    fsm_density(o, rho, NULL);
+    update_vec(o, rho, 1);
+    for (int iatm = 0; iatm < natms; iatm++) {
+        int itype = (int)o->c[iatm];
+        ns[0] += -o->fs_A[itype] * sqrt(rho[iatm]);
+        for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
+            int jatm = o->nl[jj];
+            int jtype = (int)o->c[jatm];
+            int curs = comp(itype, jtype);
+            if (!o->md_chk[curs]) continue;
+            double d[3];
+            for (int k = 0; k < 3; k++) d[k] = x[3L * iatm + k] - x[3L * jatm + k];
+            double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(rsq < o->md_cut_sq[curs])) continue;
+            double r = sqrt(rsq);
+            int local = jatm < natms;
+            double w = local ? 1.0 : 0.5;
+            double coef[2] = {0.0, 0.0};
+            int have[2] = {0, 0};
+            if (rsq < o->fs_cut_sq_rho[curs]) {
+                double dr = r - sqrt(o->fs_cut_sq_rho[curs]);
+                double c = o->fs_A[itype] * (dr * (1.0 * o->fs_t1[jtype][itype] + 1.5 * o->fs_t2[jtype][itype] * dr)) /
+                           sqrt(rho[iatm]);
+                c += o->fs_A[jtype] * (dr * (1.0 * o->fs_t1[itype][jtype] + 1.5 * o->fs_t2[itype][jtype] * dr)) /
+                     sqrt(rho[jatm]);
+                c *= 1.0 / r;
+                coef[0] = c;
+                have[0] = 1;
+            }
+            if (rsq < o->fs_cut_sq_phi[curs]) {
+                double dp = r - sqrt(o->fs_cut_sq_phi[curs]);
+                double k1 = o->fs_k1[curs], k2 = o->fs_k2[curs], k3 = o->fs_k3[curs];
+                double c = 2.0 * dp * (k1 + k2 * r + k3 * rsq) + dp * dp * (k2 + 2.0 * k3 * r);
+                c *= -1.0 / r;
+                coef[1] = c;
+                have[1] = 1;
+                double e = dp * dp * (k1 + k2 * r + k3 * rsq);
+                ns[0] += local ? e : 0.5 * e;
+            }
+            for (int t = 0; t < 2; t++) {
+                if (!have[t]) continue;
+                for (int k = 0; k < 3; k++) {
+                    f[3L * iatm + k] += d[k] * coef[t];
+                    if (local) f[3L * jatm + k] -= d[k] * coef[t];
+                }
+                if (st_clc) pair_virial(ns, w, coef[t], d[0], d[1], d[2]);
+            }
+        }
+    }
+    en_st[0] = ns[0];
+    if (st_clc)
+        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+}
+
+/* ForceField_fsm::energy_calc (src/ff_fsm.cpp:582-664) */
+static double energy_calc_fsm(orc_t* o) {
+    double en = 0.0;
+    fsm_density(o, o->E, &en);
     return en;
 }
 

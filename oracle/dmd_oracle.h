@@ -22,7 +22,7 @@ extern "C" {
 typedef struct orc orc_t;
 
 /* variants of the DMD force field (they differ only in c_d_calc / calc_g) */
-enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3, ORC_LJ = 4, ORC_FS = 5 };
+enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3, ORC_LJ = 4, ORC_FS = 5, ORC_FSM = 6 };
 
 /* MD-mode potentials (x rows [3], c holds the type): Lennard-Jones
    (ForceField_lj, src/ff_lj.cpp) and Finnis-Sinclair (ForceField_fs,

@@ -10,8 +10,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_dmd.so")
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5}
-MD = ("EAM", "LJ", "FS")
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6}
+MD = ("EAM", "LJ", "FS", "FSM")
 
 D = C.c_double
 DP = C.POINTER(C.c_double)
@@ -122,7 +122,7 @@ class Oracle:
         assert self.L.orc_lj_coef(self.o, itype, jtype, epsilon, sigma, cut) == 0
 
     def fs_coef(self, p):
-        """ForceField_fs::coef with the file's values: p = FSParams"""
+        """ForceField_fs::coef / ForceField_fsm::coef with the file's values: p = FSParams"""
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
                 for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
         assert self.L.orc_fs_coef(self.o, *[_dp(a) for a in arrs]) == 0

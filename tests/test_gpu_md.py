@@ -1,7 +1,8 @@
 """GPU parity of the MD-mode pair potentials through the C ABI against the
 oracle on the same seeded inputs: Lennard-Jones (ForceField_lj,
 src/ff_lj.cpp) with a different cutoff per type pair on a two-type FCC
-lattice, and Finnis-Sinclair (ForceField_fs, src/ff_fs.cpp) on a two-type BCC
+lattice, and Finnis-Sinclair (ForceField_fs, src/ff_fs.cpp, and its
+squared-cutoff form ForceField_fsm, src/ff_fsm.cpp) on a two-type BCC
 lattice.  Bars: neighbour lists (per-type-pair cutoffs) bit-exact; energy,
 virial and forces <= 1e-12 relative (infinity norm); energy_calc <= 1e-12."""
 import numpy as np
@@ -42,19 +43,19 @@ def test_lj_parity(n, seed):
     ff.close()
 
 
-@pytest.mark.parametrize("n,seed", [(6, 5), (8, 6)])
-def test_fs_parity(n, seed):
+@pytest.mark.parametrize("variant,n,seed", [("FS", 6, 5), ("FS", 8, 6), ("FSM", 6, 9), ("FSM", 8, 10)])
+def test_fs_parity(variant, n, seed):
     from mapp_amd import workload as W
     from mapp_amd.dmd import setup_fs
     from oracle import oracle as O
     p = W.fs_params()
     H, s, t = W.md_lattice(n, 2.87, "bcc", seed=seed)
-    ff = setup_fs(H, s, t, p)
-    orc = O.Oracle(nt=2, variant="FS")
+    ff = setup_fs(H, s, t, p, variant=variant)
+    orc = O.Oracle(nt=2, variant=variant)
     orc.fs_coef(p)
     orc.set_system(H, s.copy(), t.reshape(-1, 1))
     orc.init(0.5)
-    _compare(ff, orc, f"fs {n}")
+    _compare(ff, orc, f"{variant} {n}")
     ff.close()
 
 

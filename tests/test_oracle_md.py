@@ -7,6 +7,11 @@
     of a site against a brute-force sum over its periodic neighbours with
     mat_t[j][i] (density at i from j); f = -grad U by central differences;
     total force zero; energy_calc = force_calc's energy
+  - FSM (ForceField_fsm, src/ff_fsm.cpp): raw densities against the same
+    brute-force sum; f = -grad U by central differences; energy_calc =
+    force_calc's energy; energy, virial and forces equal FS's on the same
+    parameters to 1e-13 (the two differ only in rounding order and in the
+    cutoff test on squares)
   - the FS parameter file (ForceField_fs::coef layout) read back exactly,
     also with the file's elements in another order
   - the per-type-pair list cutoffs: the list equals a brute-force search"""
@@ -32,6 +37,28 @@ def _fs(H, s, types, p, skin=0.5):
     o.set_system(H, s.copy(), np.asarray(types, dtype=np.float64).reshape(-1, 1))
     o.init(skin)
     return o
+
+
+def _fsm(H, s, types, p, skin=0.5):
+    o = O.Oracle(nt=len(p.A), variant="FSM")
+    o.fs_coef(p)
+    o.set_system(H, s.copy(), np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    o.init(skin)
+    return o
+
+
+def _brute_rho(H, s, t, p, i):
+    r, _ = _pairs(H, s)
+    ti = t.astype(int)
+    rho = 0.0
+    for j in range(len(ti)):
+        if j == i:
+            continue
+        c = W.comp(ti[i], ti[j])
+        if r[i, j] < p.cut_rho[c]:
+            d = r[i, j] - p.cut_rho[c]
+            rho += d * d * (p.t1[ti[j]][ti[i]] + p.t2[ti[j]][ti[i]] * d)
+    return rho
 
 
 def _pairs(H, s):
@@ -128,19 +155,45 @@ def test_fs_density_against_brute_force():
     o = _fs(H, s, t, p)
     o.force_calc(1)
     stored = o.get("E")[:, 0]  # -A/sqrt(rho) after force_calc
-    r, _ = _pairs(H, s)
     ti = t.astype(int)
     for i in (0, 17, 101):
-        rho = 0.0
-        for j in range(len(ti)):
-            if j == i:
-                continue
-            c = W.comp(ti[i], ti[j])
-            if r[i, j] < p.cut_rho[c]:
-                d = r[i, j] - p.cut_rho[c]
-                rho += d * d * (p.t1[ti[j]][ti[i]] + p.t2[ti[j]][ti[i]] * d)
-        want = -p.A[ti[i]] / np.sqrt(rho)
+        want = -p.A[ti[i]] / np.sqrt(_brute_rho(H, s, t, p, i))
         assert abs(stored[i] - want) <= 1e-12 * abs(want)
+
+
+def test_fsm_density_against_brute_force():
+    p = W.fs_params()
+    H, s, t = W.md_lattice(4, 2.87, "bcc", seed=11)
+    o = _fsm(H, s, t, p)
+    o.force_calc(1)
+    stored = o.get("E")[:, 0]  # the raw density after force_calc
+    for i in (3, 40, 120):
+        want = _brute_rho(H, s, t, p, i)
+        assert abs(stored[i] - want) <= 1e-12 * abs(want)
+
+
+@pytest.mark.parametrize("seed", [12, 13])
+def test_fsm_forces_are_minus_grad(seed):
+    p = W.fs_params()
+    H, s, t = W.md_lattice(4, 2.87, "bcc", seed=seed)
+    make = lambda H_, s_, t_: _fsm(H_, s_, t_, p)
+    assert _fd_check(make, H, s, t, atoms=[1, 30, 77, 126]) < 1e-7
+    o = make(H, s, t)
+    en = o.force_calc(1)
+    f = o.get("f")
+    assert np.abs(f.sum(axis=0)).max() <= 1e-11 * np.abs(f).sum()
+    assert abs(o.energy_calc() - en[0]) <= 1e-12 * abs(en[0])
+
+
+def test_fsm_equals_fs():
+    p = W.fs_params()
+    H, s, t = W.md_lattice(5, 2.87, "bcc", seed=14)
+    a, b = _fs(H, s, t, p), _fsm(H, s, t, p)
+    ea, eb = np.asarray(a.force_calc(1)), np.asarray(b.force_calc(1))
+    assert np.abs(ea - eb).max() <= 1e-13 * np.abs(ea).max()
+    fa, fb = a.get("f"), b.get("f")
+    assert np.abs(fa - fb).max() <= 1e-13 * np.abs(fa).max()
+    assert abs(a.energy_calc() - b.energy_calc()) <= 1e-13 * abs(ea[0])
 
 
 @pytest.mark.parametrize("order", [[0, 1], [1, 0]])
