@@ -13,7 +13,8 @@ samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
 with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
 (surrogate force in one launch per kernel, autotuned, 20 BDF steps), the roofline of the
 dominant kernel (pass 1, timed with HIP events on the library's stream) and
-the FP64 CPU oracle timed on a bounded sample (32^3 cells, one thread).
+the FP64 CPU oracle timed on a bounded sample (32^3 cells per thread, one
+thread and then one box per thread of the process's CPU share, at most 16).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-surrogate] [--no-lbfgs] [--no-batch] [--fp64-only]
@@ -162,7 +163,25 @@ def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
                 bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [])
 
 
-def cpu_baseline(tables, n=32, reps=3):
+def cpu_threads():
+    """host threads for the multi-rank CPU baseline: the process's CPU share
+    (OMP_NUM_THREADS where set), at most 16"""
+    k = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        k = min(k, int(env))
+    return max(1, min(k, 16))
+
+
+def cpu_baseline(tables, n=32, reps=3, threads=None):
+    """The FP64 oracle's force_calc on the host.  One thread on one n^3 box
+    (the reference's one rank), then K threads each with its own n^3 box run
+    together, which models K ranks of the reference with no communication
+    (each box periodic, so its ghosts cost what a rank's halo would);
+    value = K * sites / wall time of the K concurrent calls, median of reps.
+    ctypes releases the GIL for the oracle's calls, so the threads run in
+    parallel."""
+    import threading
     from mapp_amd import workload as W
     from oracle import oracle as O
     lat = W.fcc_alloy(n, seed=7)
@@ -172,10 +191,42 @@ def cpu_baseline(tables, n=32, reps=3):
         t0 = time.perf_counter()
         orc.force_calc(1)
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return dict(value=lat.natms / med, unit="site-evals/s", cores=1, kind="port",
-                sample=f"FP64 oracle force_calc(1), {n}^3 FCC cells ({lat.natms} sites), 3-point rule, "
-                       f"median of {reps}, 1 thread")
+    del orc
+    one = lat.natms / float(np.median(times))
+    k = threads or cpu_threads()
+    bar = threading.Barrier(k + 1)
+    errs = []
+
+    def rank(seed):
+        try:
+            o = O.make(W.fcc_alloy(n, seed=seed), tables, no_i=3, variant="DMD_I")
+        except Exception as e:  # keep the barrier count intact
+            errs.append(e)
+            o = None
+        for _ in range(reps):
+            bar.wait()
+            if o is not None:
+                o.force_calc(1)
+            bar.wait()
+
+    ts = [threading.Thread(target=rank, args=(7 + r,)) for r in range(k)]
+    for t in ts:
+        t.start()
+    walls = []
+    for _ in range(reps):
+        bar.wait()
+        t0 = time.perf_counter()
+        bar.wait()
+        walls.append(time.perf_counter() - t0)
+    for t in ts:
+        t.join()
+    if errs:
+        raise errs[0]
+    return dict(value=k * lat.natms / float(np.median(walls)), unit="site-evals/s", cores=k, kind="port",
+                sample=f"FP64 oracle force_calc(1) on {k} threads at once, each on its own {n}^3 FCC box "
+                       f"({lat.natms} sites), 3-point rule, median of {reps} concurrent calls",
+                single_thread=dict(value=one, cores=1,
+                                   sample=f"the same call on one {n}^3 box, 1 thread, median of {reps}"))
 
 
 def copy_calibration(mib=1024, reps=5):

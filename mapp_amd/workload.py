@@ -266,3 +266,145 @@ def md_lattice(n, a, kind="fcc", nt=2, frac=0.3, seed=1, jitter=0.03):
 
 # defaults of the DMD workloads (no_i = 3 or 5 points, alpha_min/max, T, skin)
 DEFAULTS = dict(r_crd=3.0, alpha_min=25.0, alpha_max=1000.0, T=600.0, skin=0.5)
+
+
+# ---------------------------------------------------------------- MEAM
+# Synthetic two-element MEAM parameters (values of the usual magnitude for fcc
+# metals, made up for the tests): the library-file entry of each element in
+# the order of ForceField_meam::read_global (src/ff_meam.cpp:2745-2790) and
+# the local-file settings of read_local (:2858-3191).
+MEAM_LIB_KEYS = ("lat", "z", "ielt", "atwt", "alpha", "b0", "b1", "b2", "b3", "alat", "esub", "asub",
+                 "t0", "t1", "t2", "t3", "rozero", "ibar")
+MEAM_LATTICES = ("fcc", "bcc", "hcp", "dim", "dia", "b1", "c11", "l12", "b2")
+
+MEAM_LIB = {
+    "Aa": dict(lat="fcc", z=12.0, ielt=28, atwt=58.7, alpha=5.0, b0=2.5, b1=2.1, b2=5.0, b3=2.5, alat=3.55,
+               esub=4.4, asub=1.0, t0=1.0, t1=3.0, t2=1.5, t3=3.0, rozero=1.0, ibar=3),
+    "Bb": dict(lat="fcc", z=12.0, ielt=29, atwt=63.5, alpha=5.1, b0=3.4, b1=2.0, b2=4.2, b3=3.0, alat=3.62,
+               esub=3.5, asub=0.95, t0=1.0, t1=2.6, t2=2.8, t3=2.1, rozero=1.1, ibar=3),
+    "Cc": dict(lat="bcc", z=8.0, ielt=26, atwt=55.8, alpha=5.2, b0=3.6, b1=1.0, b2=1.0, b3=1.0, alat=2.86,
+               esub=4.3, asub=0.56, t0=1.0, t1=1.5, t2=2.0, t3=-1.2, rozero=1.0, ibar=3),
+}
+
+
+def meam_local(nt, **over):
+    """local-file settings as (keyword, indices..., value) tuples, 1-based
+    element indices of the library order"""
+    loc = [("rc", 4.5), ("delr", 0.1), ("augt1", 1), ("ialloy", 0), ("emb_lin_neg", 0), ("erose_form", 0)]
+    if nt > 1:
+        loc += [("lattce", 1, 2, "l12"), ("Ec", 1, 2, 4.15), ("re", 1, 2, 2.52), ("alpha", 1, 2, 5.05),
+                ("Cmin", 1, 1, 2, 1.8), ("Cmin", 2, 2, 1, 1.9), ("Cmax", 1, 2, 1, 2.75)]
+    keyed = {}
+    for e in loc:
+        keyed[e[:-1]] = e
+    for k, v in over.items():
+        keyed[(k,)] = (k, v)
+    return list(keyed.values())
+
+
+def write_meam_lib(path, elements, lib=None):
+    """a library file: per element the quoted name and lattice, then 16 numbers"""
+    lib = lib or MEAM_LIB
+    with open(path, "w") as fh:
+        fh.write("# MEAM library (synthetic parameters)\n")
+        fh.write("# elt lat z ielement atwt\n# alpha b0 b1 b2 b3 alat esub asub\n# t0 t1 t2 t3 rozero ibar\n")
+        for el in elements:
+            e = lib[el]
+            fh.write(f"'{el}' '{e['lat']}' {e['z']!r} {e['ielt']} {e['atwt']!r}\n")
+            fh.write(" ".join(repr(float(e[k])) for k in ("alpha", "b0", "b1", "b2", "b3", "alat", "esub",
+                                                          "asub")) + "\n")
+            fh.write(" ".join(repr(float(e[k])) for k in ("t0", "t1", "t2", "t3", "rozero")) + f" {e['ibar']}\n")
+
+
+def write_meam_local(path, local):
+    """a local parameter file in the keyword(i,j) = value form"""
+    with open(path, "w") as fh:
+        fh.write("# local MEAM settings\n")
+        for e in local:
+            if len(e) == 2:
+                fh.write(f"{e[0]} = {e[1]}\n")
+            else:
+                idx = ",".join(str(i) for i in e[1:-1])
+                v = e[-1]
+                fh.write(f"{e[0]}({idx}) = " + (f"'{v}'" if isinstance(v, str) else repr(v)) + "\n")
+
+
+def meam_state(types, lib_order=None, local=(), lib=None):
+    """The members ForceField_meam holds after read_global and read_local for
+    atom types `types` (element names), a library listing `lib_order` (default:
+    types) and local settings `local`: a dict of numpy arrays in type order,
+    the layout of the parameter structs of the product and the oracle."""
+    lib = lib or MEAM_LIB
+    lib_order = list(lib_order or types)
+    nt = len(types)
+    M = 4
+    st = dict(nt=nt)
+    for k in ("Z", "A", "rho0", "beta0", "beta1", "beta2", "beta3", "t0", "t1", "t2", "t3"):
+        st[k] = np.zeros(M)
+    st["ielt"] = np.zeros(M, dtype=np.int32)
+    st["ibar"] = np.zeros(M, dtype=np.int32)
+    st["lattice"] = -np.ones((M, M), dtype=np.int32)
+    for k in ("nn2", "zbl"):
+        st[k] = np.zeros((M, M), dtype=np.int32)
+    for k in ("Ec", "alpha", "re", "delta", "attrac", "repuls", "ebound"):
+        st[k] = np.zeros((M, M))
+    st["cmin"] = np.zeros((M, M, M))
+    st["cmax"] = np.zeros((M, M, M))
+    type_ref = []  # library position -> type
+    for el in lib_order:
+        if el not in types:
+            continue
+        t = types.index(el)
+        type_ref.append(t)
+        e = lib[el]
+        lat = MEAM_LATTICES.index(e["lat"])
+        st["lattice"][t, t] = lat
+        st["Z"][t] = e["z"]
+        st["ielt"][t] = e["ielt"]
+        st["alpha"][t, t] = e["alpha"]
+        for q in range(4):
+            st[f"beta{q}"][t] = e[f"b{q}"]
+        al = float(e["alat"])  # nearest-neighbour distance of the element's lattice (:2766-2776)
+        st["re"][t, t] = {0: lambda: al / np.sqrt(2.0), 1: lambda: al * 0.5 * np.sqrt(3.0), 2: lambda: al,
+                          3: lambda: al, 4: lambda: al * 0.25 * np.sqrt(3.0)}[lat]()
+        st["Ec"][t, t] = e["esub"]
+        st["A"][t] = e["asub"]
+        for q in range(4):
+            st[f"t{q}"][t] = e[f"t{q}"]
+        st["rho0"][t] = e["rozero"]
+        st["ibar"][t] = e["ibar"]
+    st["cmin"][:] = 2.0
+    st["cmax"][:] = 2.8
+    st["ebound"][:] = 1.96 / 1.8
+    st["zbl"][:] = 1
+    sc = dict(rc=4.0, delr=0.1, delr_inv=10.0, gsmooth=99.0, augt1=1, ialloy=0, mix_ref_t=0, erose_form=0,
+              emb_lin_neg=0, bkgd_dyn=0)
+    for e in local:
+        key, vals = e[0], e[1:]
+        ti = [type_ref[int(v) - 1] for v in vals[:-1]]
+        v = vals[-1]
+        if key in ("Ec", "alpha", "re"):
+            st[key][ti[0], ti[1]] = st[key][ti[1], ti[0]] = v
+        elif key == "rho0":
+            st["rho0"][ti[0]] = v
+        elif key in ("delta", "attrac", "repuls"):
+            st[key][ti[0], ti[1]] = v
+        elif key == "lattce":
+            st["lattice"][ti[0], ti[1]] = st["lattice"][ti[1], ti[0]] = MEAM_LATTICES.index(v)
+        elif key == "nn2":
+            st["nn2"][ti[0], ti[1]] = st["nn2"][ti[1], ti[0]] = int(v)
+        elif key in ("Cmin", "Cmax"):
+            a = st["cmin" if key == "Cmin" else "cmax"]
+            a[ti[0], ti[1], ti[2]] = a[ti[1], ti[0], ti[2]] = v
+        elif key in ("rc", "delr", "gsmooth_factor"):
+            sc["gsmooth" if key == "gsmooth_factor" else key] = float(v)
+        elif key in ("augt1", "ialloy", "erose_form", "emb_lin_neg"):
+            sc[key] = int(v)
+        elif key == "mixture_ref_t":
+            sc["mix_ref_t"] = int(v)
+        elif key == "bkgd_dyn":  # the reader stores it in emb_lin_neg
+            sc["emb_lin_neg"] = int(v)
+        else:
+            raise ValueError(key)
+    st.update(sc)
+    return st

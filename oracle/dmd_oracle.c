@@ -35,6 +35,7 @@
  */
 #include "dmd_oracle.h"
 #include "gh_table.h"
+#include "meam_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -99,6 +100,9 @@ struct orc {
     double fs_cut_sq_rho[36], fs_cut_sq_phi[36];  /* FSM keeps the squares */
     int energy_full;  /* 1: consistent energy: energy_calc over all type pairs, cross-type
                          pair terms weighted as in the forces (force_calc's energy too) */
+    /* MEAM state (meam_oracle.c) */
+    void* ext;
+    void (*free_ext)(void*);
     /* 2nd list */
     int* nl2_off;
     int* nl2;
@@ -149,7 +153,21 @@ void orc_destroy(orc_t* o) {
     free_atoms(o);
     free_caches(o);
     free(o->F_arr); free(o->rho_arr); free(o->phi_arr);
+    if (o->ext && o->free_ext) o->free_ext(o->ext);
     free(o);
+}
+
+void* orc_ext(orc_t* o) { return o->ext; }
+void orc_set_ext(orc_t* o, void* ext, void (*free_ext)(void*)) {
+    if (o->ext && o->free_ext) o->free_ext(o->ext);
+    o->ext = ext;
+    o->free_ext = free_ext;
+}
+/* MEAM: coefficients are in; rc is its one cutoff */
+void orc_mark_params(orc_t* o, double rc) {
+    o->rc = rc;
+    o->cut_sq = rc * rc;
+    o->have_params = 1;
 }
 
 void orc_set_fresh_caches(orc_t* o, int fresh) { o->fresh = fresh; }
@@ -439,6 +457,7 @@ static void update_vec(orc_t* o, double* v, int d) {
     for (int g = 0; g < o->nghost; g++)
         memcpy(v + (long)(o->natms + g) * d, v + (long)o->ghost_src[g] * d, d * sizeof(double));
 }
+void orc_update_vec(orc_t* o, double* v, int d) { update_vec(o, v, d); }
 static void update_int(orc_t* o, int* v) {
     for (int g = 0; g < o->nghost; g++) v[o->natms + g] = v[o->ghost_src[g]];
 }
@@ -481,7 +500,9 @@ static int s2bin(const orc_t* o, const double* s) {
 }
 
 /* bin_atoms_s (src/neighbor.cpp:467-503), s2x of every atom, then the pairwise
-   branch of create_list (src/neighbor.cpp:191-238) */
+   branch of create_list (src/neighbor.cpp:191-238), or for MEAM
+   (pair_wise = 0, src/ff_meam.cpp:675) the full branch (:240-284): every
+   j != i in the same stencil order, ghosts included */
 static void create_list(orc_t* o) {
     int ntot = o->natms + o->nghost, xd = o->xdim;
     int* first = (int*)malloc((size_t)o->tot_bin * sizeof(int));
@@ -521,7 +542,7 @@ static void create_list(orc_t* o) {
                     }
                     if (bad) continue;
                     for (int j = first[no]; j != -1; j = next[j]) {
-                        if (j <= i) continue;
+                        if (o->variant == ORC_MEAM ? j == i : j <= i) continue;
                         const double* xj = o->x + (long)j * xd;
                         double rsq = 0.0;
                         for (int d = 0; d < 3; d++) rsq += (xi[d] - xj[d]) * (xi[d] - xj[d]);
@@ -538,6 +559,12 @@ static void create_list(orc_t* o) {
     }
     o->nl_off[o->natms] = (int)np;
     o->no_pairs = np;
+    if (o->variant == ORC_MEAM) { /* no_pairs counts j > i only (:272) */
+        long nh = 0;
+        for (int i = 0; i < o->natms; i++)
+            for (int q = o->nl_off[i]; q < o->nl_off[i + 1]; q++) nh += o->nl[q] > i;
+        o->no_pairs = nh;
+    }
     free(first); free(next); free(atm_bin);
 }
 
@@ -572,6 +599,14 @@ static void rebuild(orc_t* o) {
 int orc_init(orc_t* o, double skin) {
     if (!o->have_params || !o->x) return -1;
     o->skin = skin;
+    if (o->variant == ORC_MEAM) {
+        /* ForceField_meam::init (src/ff_meam.cpp:654-681): cutoff+skin of rc_meam
+           for every type pair, ghosts for rc_meam */
+        o->cut_sk_sq = o->cut_sq + skin * skin + 2 * sqrt(o->cut_sq) * skin;
+        set_ph(o, o->rc);
+        rebuild(o);
+        return 0;
+    }
     if (ORC_PAIRWISE(o)) {
         /* ForceField_lj::init (src/ff_lj.cpp:129-153), ForceField_fs::init
            (src/ff_fs.cpp:339-360): cutoff+skin of every type pair with
@@ -802,6 +837,7 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
     if (o->variant == ORC_LJ) { force_calc_lj(o, st_clc, en_st); return; }
     if (o->variant == ORC_FS) { force_calc_fs(o, st_clc, en_st); return; }
     if (o->variant == ORC_FSM) { force_calc_fsm(o, st_clc, en_st); return; }
+    if (o->variant == ORC_MEAM) { meam_force_calc(o, st_clc, en_st); return; }
     int nt = o->nt, xd = o->xdim, natms = o->natms, nphi = nt * (nt + 1) / 2;
     double* x = o->x;
     double* c = o->c;
@@ -967,6 +1003,7 @@ double orc_energy_calc(orc_t* o) {
     if (o->variant == ORC_LJ) return energy_calc_lj(o);
     if (o->variant == ORC_FS) return energy_calc_fs(o);
     if (o->variant == ORC_FSM) return energy_calc_fsm(o);
+    if (o->variant == ORC_MEAM) return meam_energy_calc(o);
     int nt = o->nt, xd = o->xdim, natms = o->natms;
     double* x = o->x;
     double* c = o->c;

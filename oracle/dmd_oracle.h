@@ -22,7 +22,7 @@ extern "C" {
 typedef struct orc orc_t;
 
 /* variants of the DMD force field (they differ only in c_d_calc / calc_g) */
-enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3, ORC_LJ = 4, ORC_FS = 5, ORC_FSM = 6 };
+enum { ORC_DMD = 0, ORC_DMD_I = 1, ORC_DMD_II = 2, ORC_EAM = 3, ORC_LJ = 4, ORC_FS = 5, ORC_FSM = 6, ORC_MEAM = 7 };
 
 /* MD-mode potentials (x rows [3], c holds the type): Lennard-Jones
    (ForceField_lj, src/ff_lj.cpp) and Finnis-Sinclair (ForceField_fs,
@@ -70,6 +70,12 @@ void orc_set_fix_dE_index(orc_t* o, int fix);
 void orc_set_energy_full(orc_t* o, int full);
 
 void orc_force_calc(orc_t* o, int st_clc, double* en_st);
+/* for the MEAM restatement (meam_oracle.c): its state hangs on the oracle;
+   orc_update_vec is Atoms::update of a per-atom vector of dimension d */
+void* orc_ext(orc_t* o);
+void orc_set_ext(orc_t* o, void* ext, void (*free_ext)(void*));
+void orc_update_vec(orc_t* o, double* v, int d);
+void orc_mark_params(orc_t* o, double rc);
 double orc_energy_calc(orc_t* o);
 void orc_create_2nd_neigh_lst(orc_t* o);
 void orc_c_d_calc(orc_t* o);

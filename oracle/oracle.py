@@ -10,8 +10,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_dmd.so")
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6}
-MD = ("EAM", "LJ", "FS", "FSM")
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6, "MEAM": 7}
+MD = ("EAM", "LJ", "FS", "FSM", "MEAM")
 
 D = C.c_double
 DP = C.POINTER(C.c_double)
@@ -77,8 +77,48 @@ def lib():
         L.orc_bdf_coef.argtypes = [C.c_int, D, DP, C.c_int, DP]
         L.orc_lj_coef.argtypes = [C.c_void_p, C.c_int, C.c_int, D, D, D]
         L.orc_fs_coef.argtypes = [C.c_void_p] + [DP] * 8
+        L.orc_meam_coef.argtypes = [C.c_void_p, C.POINTER(MeamParams)]
+        L.orc_meam_set_fix.argtypes = [C.c_void_p, C.c_int]
+        L.orc_meam_table.argtypes = [C.c_void_p, IP, DP]
+        L.orc_meam_table.restype = DP
+        L.orc_meam_records.argtypes = [C.c_void_p]
+        L.orc_meam_records.restype = DP
+        L.orc_meam_rho_ref.argtypes = [C.c_void_p]
+        L.orc_meam_rho_ref.restype = DP
+        L.orc_meam_phi.argtypes = [C.c_void_p, D, C.c_int, C.c_int]
+        L.orc_meam_phi.restype = D
         _lib = L
     return _lib
+
+
+class MeamParams(C.Structure):
+    """orc_meam_params (meam_oracle.h): ForceField_meam's members after its readers"""
+    _T = 4
+    _fields_ = ([("nt", C.c_int)] +
+                [(k, D * 4) for k in ("Z", "A", "rho0", "beta0", "beta1", "beta2", "beta3", "t0", "t1", "t2",
+                                      "t3")] +
+                [(k, C.c_int * 4) for k in ("ielt", "ibar")] +
+                [(k, (C.c_int * 4) * 4) for k in ("lattice", "nn2", "zbl")] +
+                [(k, (D * 4) * 4) for k in ("Ec", "alpha", "re", "delta", "attrac", "repuls", "ebound")] +
+                [(k, ((D * 4) * 4) * 4) for k in ("cmin", "cmax")] +
+                [(k, D) for k in ("rc", "delr", "delr_inv", "gsmooth")] +
+                [(k, C.c_int) for k in ("augt1", "ialloy", "mix_ref_t", "erose_form", "emb_lin_neg",
+                                        "bkgd_dyn")])
+
+    @classmethod
+    def from_state(cls, st):
+        """from the dict of arrays of mapp_amd.workload.meam_state"""
+        p = cls()
+        for name, ctype in cls._fields_:
+            v = st[name]
+            if isinstance(v, np.ndarray):
+                flat = np.asarray(v).reshape(-1)
+                base = getattr(p, name)
+                buf = np.ctypeslib.as_array(base).reshape(-1)
+                buf[:] = flat
+            else:
+                setattr(p, name, v)
+        return p
 
 
 def _dp(a):
@@ -126,6 +166,30 @@ class Oracle:
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
                 for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
         assert self.L.orc_fs_coef(self.o, *[_dp(a) for a in arrs]) == 0
+
+    def meam_coef(self, state):
+        """ForceField_meam::coef after its readers: state as mapp_amd.workload.meam_state"""
+        self._meam = MeamParams.from_state(state)
+        assert self.L.orc_meam_coef(self.o, C.byref(self._meam)) == 0
+
+    def meam_set_fix(self, fix):
+        self.L.orc_meam_set_fix(self.o, 1 if fix else 0)
+
+    def meam_table(self):
+        nr, dr = C.c_int(0), C.c_double(0.0)
+        p = self.L.orc_meam_table(self.o, C.byref(nr), C.byref(dr))
+        nc = self.nt * (self.nt + 1) // 2
+        return np.ctypeslib.as_array(p, shape=(nc, nr.value, 7)).copy(), dr.value
+
+    def meam_records(self):
+        n = self.L.orc_natms(self.o) + self.L.orc_nghost(self.o)
+        return np.ctypeslib.as_array(self.L.orc_meam_records(self.o), shape=(n, 38)).copy()
+
+    def meam_rho_ref(self):
+        return np.ctypeslib.as_array(self.L.orc_meam_rho_ref(self.o), shape=(self.nt,)).copy()
+
+    def meam_phi(self, r, a, b):
+        return self.L.orc_meam_phi(self.o, r, a, b)
 
     def set_system(self, H, x, c):
         H = np.ascontiguousarray(H, dtype=np.float64)

@@ -1,0 +1,150 @@
+"""Oracle pins of the MEAM restatement (ForceField_meam, src/ff_meam.cpp), CPU, FP64.
+
+  - MEAM's pair potential is built so that the reference structure has the
+    Rose energy: a perfect FCC crystal of one element at a nearest-neighbour
+    distance on a node of the phi table has E/atom = E_rose(r) and zero
+    forces; a perfect L12 A3B crystal of two elements has E/atom = E_rose of
+    the A-B pair (Ec mixing, t averaging, the angular rho2 term of the A
+    sites and the L12 pair construction all enter)
+  - with the corrected force loop (fix = 1), f = -grad U by central
+    differences on an isolated cluster with partially screened pairs (every
+    three-body screening force present) and on jittered periodic crystals,
+    one and two elements, ialloy 0 and 1
+  - the reference's force loop (fix = 0) differs from the corrected one only
+    in the documented terms: with t2 = 0, ialloy = 0, no pair in the cutoff
+    taper and no partial screening both give the same forces
+  - energy_calc = force_calc's energy; total force zero where no screening
+    atom is a ghost"""
+import numpy as np
+import pytest
+
+from mapp_amd import workload as W
+from oracle import oracle as O
+
+
+def _make(st, H, s, t, fix=0, skin=0.3):
+    o = O.Oracle(nt=st["nt"], variant="MEAM")
+    o.meam_coef(st)
+    o.meam_set_fix(fix)
+    o.set_system(H, s.copy(), np.asarray(t, dtype=np.float64).reshape(-1, 1))
+    o.init(skin)
+    return o
+
+
+def _erose(st, a, b, r):
+    astar = st["alpha"][a, b] * (r / st["re"][a, b] - 1.0)
+    return -st["Ec"][a, b] * (1.0 + astar) * np.exp(-astar)
+
+
+def _fd(st, H, s, t, atoms, fix=1, h=1e-5):
+    o = _make(st, H, s, t, fix)
+    o.force_calc(1)
+    f = o.get("f")
+    L = np.diag(H)
+    worst = 0.0
+    for a in atoms:
+        for k in range(3):
+            sp, sm = s.copy(), s.copy()
+            sp[a, k] += h / L[k]
+            sm[a, k] -= h / L[k]
+            up = _make(st, H, sp % 1.0, t, fix).energy_calc()
+            um = _make(st, H, sm % 1.0, t, fix).energy_calc()
+            worst = max(worst, abs(-(up - um) / (2 * h) - f[a, k]))
+    return worst / np.abs(f).max()
+
+
+def _cluster(seed=6, jitter=0.08):
+    Hs, s0, t = W.md_lattice(2, 3.58, "fcc", nt=2, seed=seed, jitter=jitter)
+    x = s0 @ Hs + 9.0
+    H = np.diag([30.0] * 3)
+    return H, x / 30.0, t
+
+
+def _screening_local(ialloy=0):
+    # Cmin below the second-neighbour C = 1 of fcc: second neighbours partially screened
+    return W.meam_local(2, ialloy=ialloy) + [("Cmin", 1, 1, 1, 0.5), ("Cmin", 2, 2, 2, 0.6), ("Cmin", 1, 2, 1, 0.7),
+                              ("Cmin", 1, 2, 2, 0.8), ("Cmin", 1, 1, 2, 0.55), ("Cmin", 2, 2, 1, 0.65)]
+
+
+@pytest.mark.parametrize("node", [500, 507, 515])
+def test_fcc_reference_structure_has_rose_energy(node):
+    st = W.meam_state(["Aa"], local=[("rc", 4.5)])
+    o = O.Oracle(nt=1, variant="MEAM")
+    o.meam_coef(st)
+    _, dr = o.meam_table()
+    r1 = node * dr
+    H, s, t = W.md_lattice(4, r1 * np.sqrt(2.0), "fcc", nt=1, jitter=0.0)
+    o = _make(st, H, s, t)
+    en = o.force_calc(1)
+    want = _erose(st, 0, 0, r1)
+    assert abs(en[0] / len(t) - want) <= 1e-12 * abs(want)
+    assert np.abs(o.get("f")).max() <= 1e-12 * abs(want)
+    assert abs(o.energy_calc() - en[0]) <= 1e-12 * abs(en[0])
+    assert abs(o.meam_rho_ref()[0] - 12.0 * st["rho0"][0]) <= 1e-15 * 12.0
+
+
+@pytest.mark.parametrize("node", [505, 512])
+def test_l12_reference_structure_has_rose_energy(node):
+    st = W.meam_state(["Aa", "Bb"], local=W.meam_local(2))
+    o = O.Oracle(nt=2, variant="MEAM")
+    o.meam_coef(st)
+    _, dr = o.meam_table()
+    r1 = node * dr
+    H, s, _ = W.md_lattice(4, r1 * np.sqrt(2.0), "fcc", nt=1, jitter=0.0)
+    t = np.zeros(len(s))
+    t[0::4] = 1.0  # the cube corners: B; faces: A (A3B)
+    o = _make(st, H, s, t)
+    en = o.force_calc(1)
+    want = _erose(st, 0, 1, r1)
+    assert abs(en[0] / len(t) - want) <= 1e-12 * abs(want)
+    assert np.abs(o.get("f")).max() <= 1e-11 * abs(want)
+
+
+@pytest.mark.parametrize("ialloy", [0, 1, 2])
+def test_fd_cluster_with_partial_screening(ialloy):
+    st = W.meam_state(["Aa", "Bb"], local=_screening_local(ialloy))
+    H, s, t = _cluster()
+    assert _fd(st, H, s, t, [0, 5, 17, 31]) < 1e-7
+    for fix in (0, 1):
+        o = _make(st, H, s, t, fix)
+        en = o.force_calc(1)
+        f = o.get("f")
+        assert np.abs(f.sum(axis=0)).max() <= 1e-13 * np.abs(f).sum()
+        assert abs(o.energy_calc() - en[0]) <= 1e-12 * abs(en[0])
+
+
+def _no_partial(nt):
+    # rc between the second and third fcc shells and Cmax below the
+    # nearest-neighbour C (about 3): every pair of a jittered crystal is either
+    # unscreened or fully screened, so no screening atom is dropped
+    return [("rc", 4.0)] + [("Cmax", i, j, k, 2.3) for i in range(1, nt + 1) for j in range(1, nt + 1)
+                            for k in range(1, nt + 1)]
+
+
+@pytest.mark.parametrize("case", ["one", "two", "two_ialloy1"])
+def test_fd_periodic(case):
+    if case == "one":
+        st = W.meam_state(["Aa"], local=_no_partial(1))
+        H, s, t = W.md_lattice(3, 3.55, "fcc", nt=1, seed=3, jitter=0.03)
+    else:
+        loc = W.meam_local(2, ialloy=1 if case == "two_ialloy1" else 0) + _no_partial(2)
+        st = W.meam_state(["Aa", "Bb"], local=loc)
+        H, s, t = W.md_lattice(3, 3.58, "fcc", nt=2, seed=4, jitter=0.03)
+    assert _fd(st, H, s, t, [0, 9, 50, len(t) - 1]) < 1e-7
+    o = _make(st, H, s, t, 1)
+    o.force_calc(1)
+    f = o.get("f")
+    assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).sum()
+
+
+def test_quirks_confined_to_documented_terms():
+    lib = {k: dict(v) for k, v in W.MEAM_LIB.items()}
+    for el in ("Aa", "Bb"):
+        lib[el]["t2"] = 0.0
+    st = W.meam_state(["Aa", "Bb"], local=W.meam_local(2) + _no_partial(2), lib=lib)
+    H, s, t = _cluster(seed=8, jitter=0.03)
+    a, b = _make(st, H, s, t, 0), _make(st, H, s, t, 1)
+    ea, eb = a.force_calc(1), b.force_calc(1)
+    assert np.array_equal(ea, eb)
+    assert np.array_equal(a.get("f"), b.get("f"))
+    assert _fd(st, H, s, t, [0, 7], fix=0) < 1e-7
