@@ -54,7 +54,14 @@ enum {
     /* ForceField_fsm (src/ff_fsm.cpp), force_calc :371-576, energy_calc
        :582-664: FS with the cutoffs kept squared (:281-282) and compared as
        rsq < cut^2; same coefficient entry points and file format as FS */
-    MAPP_DMD_VARIANT_FSM = 6
+    MAPP_DMD_VARIANT_FSM = 6,
+    /* ForceField_meam (src/ff_meam.cpp): modified EAM with angular partial
+       densities and three-body screening over the FULL neighbour list
+       (pair_wise = 0, src/ff_meam.cpp:675; src/neighbor.cpp:240-284), one
+       cutoff rc for every type pair; force_calc :699-2142, energy_calc
+       :2148-2589.  Coefficients from mapp_dmd_meam_coef /
+       mapp_dmd_read_meam_params. */
+    MAPP_DMD_VARIANT_MEAM = 7
 };
 
 /* arithmetic of the pair quadrature (per-site sums, embedding and the
@@ -128,6 +135,60 @@ int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const
 int mapp_dmd_fs_params_read(const char* path, int nt, const char* const* elements, double* A, double* t1,
                             double* t2, double* cut_rho, double* cut_phi, double* k1, double* k2, double* k3,
                             char* err, int err_cap);
+
+/* MEAM parameters: the members ForceField_meam holds after read_global and
+   read_local (src/ff_meam.cpp:2660-3191), before coef applies augt1, the
+   alloy rules, the reference densities and the pair table (:2593-2627).
+   Per-element values in type order; lattice codes 0 fcc, 1 bcc, 2 hcp,
+   3 dim, 4 dia, 5 b1, 6 c11, 7 l12, 8 b2 (src/ff_meam.cpp:13), -1 unset.
+   delr_inv stays 10 whatever delr the local file gives (the reader sets delr
+   only, :3090). */
+#define MAPP_MEAM_MAXT 4
+typedef struct {
+    int nt;
+    double Z[MAPP_MEAM_MAXT], A[MAPP_MEAM_MAXT], rho0[MAPP_MEAM_MAXT];
+    double beta0[MAPP_MEAM_MAXT], beta1[MAPP_MEAM_MAXT], beta2[MAPP_MEAM_MAXT], beta3[MAPP_MEAM_MAXT];
+    double t0[MAPP_MEAM_MAXT], t1[MAPP_MEAM_MAXT], t2[MAPP_MEAM_MAXT], t3[MAPP_MEAM_MAXT];
+    int ielt[MAPP_MEAM_MAXT], ibar[MAPP_MEAM_MAXT];
+    int lattice[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT], nn2[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    int zbl[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double Ec[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT], alpha[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double re[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT], delta[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double attrac[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT], repuls[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double ebound[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double cmin[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double cmax[MAPP_MEAM_MAXT][MAPP_MEAM_MAXT][MAPP_MEAM_MAXT];
+    double rc, delr, delr_inv, gsmooth;
+    int augt1, ialloy, mix_ref_t, erose_form, emb_lin_neg, bkgd_dyn;
+} mapp_meam_params;
+
+/* ff_coef MEAM <library> <local>: ForceField_meam::coef (src/ff_meam.cpp:
+   2593-2627).  mapp_dmd_meam_params_read parses the library file (per element
+   the quoted name and lattice, z, ielement, atwt, alpha, b0..b3, alat, esub,
+   asub, t0..t3, rozero, ibar; quotes read as blanks; elements not among the
+   types are skipped, :2660-2857) and the local file ("keyword(i,j,k) = value"
+   with quotes, commas, parentheses and '=' read as blanks; indices 1-based in
+   library order; "NULL" skips it, :2858-3191) into p, for the types named by
+   elements[nt].  The reader's own behaviour is kept: "delr" leaves delr_inv at
+   10, "bkgd_dyn" sets emb_lin_neg, "zbl" is an error.  mapp_dmd_meam_coef
+   derives the tables; it rejects mix_ref_t = 1 (the reference's pair potential
+   then reads an uninitialised background density, :3747-3748), an unset pair
+   lattice and an unknown ibar. */
+int mapp_dmd_meam_params_read(const char* library, const char* local, int nt, const char* const* elements,
+                              mapp_meam_params* p, char* err, int err_cap);
+int mapp_dmd_meam_coef(mapp_dmd_t* ctx, const mapp_meam_params* p);
+int mapp_dmd_read_meam_params(mapp_dmd_t* ctx, const char* library, const char* local,
+                              const char* const* elements);
+/* 1: a corrected MEAM force loop -- the rho2 angular argument carries its
+   direction factor (the reference drops it, src/ff_meam.cpp:1692-1693), the
+   screening derivative keeps the cutoff taper when the screening product is 1
+   (:1262-1263 drop it), the ialloy = 1 screening derivatives of the t
+   averages take the atomic density (:1896-1906 take its r-derivative) and
+   square t2, t3 (:1910-1913 use t1), and the two three-body derivatives of
+   the screening go to their own atoms (the dCfunc2 call swaps them, :2052).
+   0 (default): the reference.  Forces on a screening atom that is a ghost in
+   the pair owner's list are dropped in both (:2076). */
+int mapp_dmd_meam_set_fix(mapp_dmd_t* ctx, int fix);
 
 /* System: lower-triangular H (row-major 3x3, rows are the box vectors,
    src/read_cfg.cpp:425-453), natms sites whose x rows hold the reduced

@@ -12,6 +12,7 @@
 #include "force.hpp"
 #include "mobility.hpp"
 #include "eam.hpp"
+#include "meam.hpp"
 #include "surrogate.hpp"
 #include "lbfgs.hpp"
 #include "cfg.hpp"
@@ -189,6 +190,7 @@ void build_ghosts(mapp_dmd* ctx) {
 
 template <int NT>
 void finish_lists(mapp_dmd* ctx);
+void finish_meam(mapp_dmd* ctx);
 
 __global__ void k_quirk_identity(int ntot, int nt, int* __restrict__ qidx) {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,7 +250,8 @@ void rebuild(mapp_dmd* ctx) {
                        ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_conc.p, ctx->d_off.p,
                        ctx->d_nl.p);
     launch_check("k_half_list fill");
-    finish_lists<NT>(ctx);
+    if (ctx->variant == MAPP_DMD_VARIANT_MEAM) finish_meam(ctx);
+    else finish_lists<NT>(ctx);
 }
 
 // reverse CSR, pair owners, reverse positions and per-site / per-pair buffers
@@ -408,7 +411,10 @@ void md_passes(mapp_dmd* ctx, bool energy_only, int nb, const PairBufs& L, const
     }
 }
 
+double run_meam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st);
+
 double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
+    if (ctx->variant == MAPP_DMD_VARIANT_MEAM) return run_meam(ctx, energy_only, st_clc, en_st);
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms, nb = nblk(natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
@@ -451,6 +457,67 @@ double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
         en_st[0] = energy;
         if (st_clc)
             for (int k = 0; k < 6; ++k) en_st[1 + k] = h[1 + k] + h[8 + k];
+    }
+    return energy;
+}
+
+// ------------------------------------------------------------------ MEAM
+// buffers of the full list: per-entry screening and dU/dS, per-site records
+void finish_meam(mapp_dmd* ctx) {
+    const int natms = ctx->natms, ntot = natms + ctx->nghost;
+    hipStream_t st = ctx->stream;
+    ctx->d_iflag.reserve(4);
+    HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_meam_maxlist, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_iflag.p);
+    launch_check("k_meam_maxlist");
+    int mx = 0;
+    HIP_OK(hipMemcpyAsync(&mx, ctx->d_iflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    require(mx <= kMeamMaxNb, MAPP_DMD_E_RANGE, "MEAM: a site has more list entries than the kernels stage (192)");
+    ctx->d_mpairs.reserve((size_t)ctx->npairs + 1);
+    ctx->d_mrec.reserve((size_t)ntot * kMeamRec);
+    ctx->d_f.reserve((size_t)natms * 3);
+}
+
+// force_calc (src/ff_meam.cpp:699-2142) or energy_calc (:2148-2589)
+double run_meam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
+    hipStream_t st = ctx->stream;
+    const int natms = ctx->natms;
+    const int nb = (natms + kMeamWaves - 1) / kMeamWaves;
+    const MeamDev M = ctx->meam.device(ctx->meam_fix);
+    ctx->d_part.reserve((size_t)nb * 14 + 16);
+    ctx->d_out.reserve(16);
+    double* part1 = ctx->d_part.p;
+    double* part2 = part1 + nb;
+    double* part3 = part2 + (size_t)nb * 7;
+    double* out = ctx->d_out.p;
+    if (energy_only)
+        hipLaunchKernelGGL((k_meam_density<true>), dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p,
+                           ctx->d_off.p, ctx->d_nl.p, ctx->d_mtab.p, natms, ctx->d_mpairs.p, ctx->d_mrec.p, part1);
+    else
+        hipLaunchKernelGGL((k_meam_density<false>), dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p,
+                           ctx->d_off.p, ctx->d_nl.p, ctx->d_mtab.p, natms, ctx->d_mpairs.p, ctx->d_mrec.p, part1);
+    launch_check("k_meam_density");
+    reduce_partials(ctx, part1, nb, 1, 1, 0, out);
+    if (!energy_only) {
+        ghost_copy(ctx, ctx->d_mrec.p, kMeamRec);
+        hipLaunchKernelGGL(k_meam_force, dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p, ctx->d_off.p,
+                           ctx->d_nl.p, ctx->d_mtab.p, natms, ctx->d_mpairs.p, ctx->d_mrec.p, ctx->d_f.p, part2);
+        launch_check("k_meam_force");
+        hipLaunchKernelGGL(k_meam_kforce, dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p, ctx->d_off.p,
+                           ctx->d_nl.p, natms, ctx->d_mpairs.p, ctx->d_f.p, part3);
+        launch_check("k_meam_kforce");
+        reduce_partials(ctx, part2, nb, 7, 7, 0, out + 1);
+        reduce_partials(ctx, part3, nb, 6, 6, 0, out + 8);
+    }
+    double h[14];
+    HIP_OK(hipMemcpyAsync(h, out, sizeof(double) * (energy_only ? 1 : 14), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    const double energy = energy_only ? h[0] : h[0] + h[1];
+    if (en_st) {
+        en_st[0] = energy;
+        if (st_clc)
+            for (int k = 0; k < 6; ++k) en_st[1 + k] = h[2 + k] + h[8 + k];
     }
     return energy;
 }
@@ -1393,7 +1460,7 @@ extern "C" {
 int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
     if (!out) return MAPP_DMD_E_ARG;
     *out = nullptr;
-    if (variant < 0 || variant > MAPP_DMD_VARIANT_FSM) return MAPP_DMD_E_ARG;
+    if (variant < 0 || variant > MAPP_DMD_VARIANT_MEAM) return MAPP_DMD_E_ARG;
     if (no_types < 1 || no_types > (variant >= MAPP_DMD_VARIANT_LJ ? kMaxTypes : 2)) return MAPP_DMD_E_RANGE;
     std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
     ctx->nt = no_types;
@@ -1590,6 +1657,61 @@ int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const
     });
 }
 
+int mapp_dmd_meam_params_read(const char* library, const char* local, int nt, const char* const* elements,
+                              mapp_meam_params* p, char* err, int err_cap) {
+    auto fail = [&](const std::string& m, int code) {
+        if (err && err_cap > 0) {
+            std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
+            err[err_cap - 1] = 0;
+        }
+        return code;
+    };
+    if (!library || !elements || !p || nt < 1 || nt > kMeamT) return fail("bad arguments", MAPP_DMD_E_ARG);
+    std::vector<int> type_ref;
+    std::string e = meam_host::read_library(library, nt, elements, p, &type_ref);
+    if (e.empty() && local) e = meam_host::read_local(local, type_ref, p);
+    if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
+    return 0;
+}
+
+int mapp_dmd_meam_coef(mapp_dmd_t* ctx, const mapp_meam_params* p) {
+    return guarded(ctx, [&]() {
+        // ForceField_meam::coef after its readers (src/ff_meam.cpp:2593-2627)
+        require(ctx->variant == MAPP_DMD_VARIANT_MEAM, MAPP_DMD_E_STATE, "meam_coef needs the MEAM variant");
+        require(p != nullptr && p->nt == ctx->nt, MAPP_DMD_E_ARG, "parameters for another number of types");
+        meam_host::Setup su;
+        const std::string e = su.build(*p);
+        if (!e.empty()) throw ApiError(MAPP_DMD_E_ARG, e);
+        HIP_OK(hipSetDevice(ctx->device));
+        ctx->d_mtab.reserve(su.table.size());
+        HIP_OK(hipMemcpy(ctx->d_mtab.p, su.table.data(), sizeof(double) * su.table.size(), hipMemcpyHostToDevice));
+        ctx->meam = std::move(su);
+        ctx->have_params = true;
+        ctx->inited = false;
+        return 0;
+    });
+}
+
+int mapp_dmd_read_meam_params(mapp_dmd_t* ctx, const char* library, const char* local,
+                              const char* const* elements) {
+    return guarded(ctx, [&]() {
+        require(library && elements, MAPP_DMD_E_ARG, "null library or elements");
+        mapp_meam_params p;
+        char err[256] = {0};
+        const int rc = mapp_dmd_meam_params_read(library, local, ctx->nt, elements, &p, err, sizeof err);
+        if (rc != 0) throw ApiError(rc, err);
+        return mapp_dmd_meam_coef(ctx, &p);
+    });
+}
+
+int mapp_dmd_meam_set_fix(mapp_dmd_t* ctx, int fix) {
+    return guarded(ctx, [&]() {
+        require(ctx->variant == MAPP_DMD_VARIANT_MEAM, MAPP_DMD_E_STATE, "meam_set_fix needs the MEAM variant");
+        ctx->meam_fix = fix ? 1 : 0;
+        return 0;
+    });
+}
+
 int mapp_dmd_coef(mapp_dmd_t* ctx, double r_crd, int no_i, double alpha_min, double alpha_max, double T,
                   const double* mass) {
     return guarded(ctx, [&]() {
@@ -1715,7 +1837,16 @@ int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
         const int nt = acols(ctx), natms = ctx->natms;
         ctx->skin = skin;
         const FFParams<double>& P = ctx->P;
-        if (ctx->variant >= MAPP_DMD_VARIANT_LJ) {
+        ctx->bp.full = 0;
+        if (ctx->variant == MAPP_DMD_VARIANT_MEAM) {
+            // ForceField_meam::init (src/ff_meam.cpp:654-681): rc_meam + skin for
+            // every type pair, ghosts for rc_meam, the full list (pair_wise = 0)
+            const double rc = ctx->meam.p.rc;
+            ctx->bp.pairwise = 0;
+            ctx->bp.full = 1;
+            ctx->bp.cut_sk_sq = rc * rc + skin * skin + 2 * std::sqrt(rc * rc) * skin;
+            set_ph(ctx, rc);
+        } else if (ctx->variant >= MAPP_DMD_VARIANT_LJ) {
             // ForceField_lj::init (src/ff_lj.cpp:129-153), ForceField_fs::init
             // (src/ff_fs.cpp:339-360): a cutoff+skin per type pair with coefficients,
             // ghosts for the largest cutoff

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "dmd_common.hpp"
+#include "meam_setup.hpp"
 #include "surrogate_types.hpp"
 
 struct mapp_dmd {
@@ -20,6 +21,10 @@ struct mapp_dmd {
     // parameters
     mappdmd::FFParams<double> P{};
     mappdmd::MdParams md{};  // LJ / FS coefficients per type pair
+    mappdmd::meam_host::Setup meam;  // MEAM: parameters after coef, pair table (host)
+    int meam_fix = 0;
+    mappdmd::DevBuf<double> d_mtab, d_mrec;  // MEAM pair table, per-site records [ntot][38]
+    mappdmd::DevBuf<double4> d_mpairs;       // MEAM per list entry {fc, S, dS, dU/dS}
     mappdmd::FFParams<float> Pf{};
     double rsq_crd = 0, mod_rc = 0;
     double wi1_ref[mappdmd::kMaxPoints]{}, wi2_ref[mappdmd::kMaxPoints]{};

@@ -41,6 +41,7 @@ struct BoxParams {
     int grid[3], denom[3], tot_bin;
     double cut_sk_sq;
     int pairwise;
+    int full;  // every j != i (MEAM's full list) instead of j > i
     double cut_sk_sq_tp[kMaxComp];
 };
 

@@ -211,7 +211,8 @@ __global__ void k_segment_sort(const int* __restrict__ start, int nseg, int* __r
 
 // ---- half neighbor list ----------------------------------------------------
 // stencil order of find_bin_no: offset of dimension 0 outermost, 2 innermost;
-// inside a bin ascending index; j > i; rsq = ((d0*d0) + d1*d1) + d2*d2 < cut
+// inside a bin ascending index; j > i (every j != i with bp.full, the full list
+// of src/neighbor.cpp:240-284); rsq = ((d0*d0) + d1*d1) + d2*d2 < cut
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ pos, int natms,
                                                    const int* __restrict__ bin_of,
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
                 int q1 = bin_start[no + 1];
                 for (int q = bin_start[no]; q < q1; ++q) {
                     int j = bin_atoms[q];
-                    if (j <= i) continue;
+                    if (bp.full ? j == i : j <= i) continue;
                     double4 xj = pos[j];
                     double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
                     double rsq = 0.0;

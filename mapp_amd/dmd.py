@@ -14,8 +14,8 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmapp_dmd.so")
 
-VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6}
-MD_VARIANTS = ("EAM", "LJ", "FS", "FSM")  # x rows [3], c = type
+VARIANT = {"DMD": 0, "DMD_I": 1, "DMD_II": 2, "EAM": 3, "LJ": 4, "FS": 5, "FSM": 6, "MEAM": 7}
+MD_VARIANTS = ("EAM", "LJ", "FS", "FSM", "MEAM")  # x rows [3], c = type
 PREC = {"fp64": 0, "mixed": 1, "surrogate": 2}
 VEC = {"x": 0, "c": 1, "f": 2, "E": 3, "dE": 4, "n": 5, "c_d": 6, "ddE": 7, "s": 8, "t": 9, "v": 10, "a": 11, "g": 12}
 
@@ -71,6 +71,9 @@ _SIGS = {
     "mapp_dmd_surrogate_fallbacks": [CTX, LP],
     "mapp_dmd_lj_coef": [CTX, C.c_int, C.c_int, D, D, D],
     "mapp_dmd_fs_coef": [CTX, DP, DP, DP, DP, DP, DP, DP, DP],
+    "mapp_dmd_meam_coef": [CTX, C.c_void_p],
+    "mapp_dmd_read_meam_params": [CTX, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)],
+    "mapp_dmd_meam_set_fix": [CTX, C.c_int],
     "mapp_dmd_read_fs_params": [CTX, C.c_char_p, C.POINTER(C.c_char_p)],
 }
 
@@ -79,6 +82,8 @@ _NOCTX_SIGS = {
     "mapp_dmd_cfg_read": [C.c_char_p, C.c_int, IP, IP, DP, DP, DP, DP, C.c_char_p, C.c_int, C.c_char_p, C.c_int],
     "mapp_dmd_cfg_write": [C.c_char_p, DP, C.c_int, C.c_int, DP, DP, C.c_int, DP, C.POINTER(C.c_char_p),
                            C.c_char_p, C.c_int],
+    "mapp_dmd_meam_params_read": [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.c_void_p,
+                                  C.c_char_p, C.c_int],
     "mapp_dmd_fs_params_read": [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), DP, DP, DP, DP, DP, DP, DP, DP,
                                 C.c_char_p, C.c_int],
 }
@@ -127,6 +132,39 @@ def _fp(a):
 SUR_NPARAM = 8966
 SUR_NNORM = 56
 SUR_REPORT = ("loss", "val_mean_err", "val_max_err", "masked_frac", "steps", "seconds", "n_train", "n_val")
+
+
+class MeamParams(C.Structure):
+    """mapp_meam_params (include/mapp_dmd.h): ForceField_meam's members after its readers"""
+    _fields_ = ([("nt", C.c_int)] +
+                [(k, D * 4) for k in ("Z", "A", "rho0", "beta0", "beta1", "beta2", "beta3", "t0", "t1", "t2",
+                                      "t3")] +
+                [(k, C.c_int * 4) for k in ("ielt", "ibar")] +
+                [(k, (C.c_int * 4) * 4) for k in ("lattice", "nn2", "zbl")] +
+                [(k, (D * 4) * 4) for k in ("Ec", "alpha", "re", "delta", "attrac", "repuls", "ebound")] +
+                [(k, ((D * 4) * 4) * 4) for k in ("cmin", "cmax")] +
+                [(k, D) for k in ("rc", "delr", "delr_inv", "gsmooth")] +
+                [(k, C.c_int) for k in ("augt1", "ialloy", "mix_ref_t", "erose_form", "emb_lin_neg",
+                                        "bkgd_dyn")])
+
+    @classmethod
+    def from_state(cls, st):
+        """from a dict of arrays with the struct's field names"""
+        p = cls()
+        for name, _ in cls._fields_:
+            v = st[name]
+            if isinstance(v, np.ndarray):
+                np.ctypeslib.as_array(getattr(p, name)).reshape(-1)[:] = np.asarray(v).reshape(-1)
+            else:
+                setattr(p, name, v)
+        return p
+
+    def to_state(self):
+        st = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            st[name] = np.ctypeslib.as_array(v).copy() if hasattr(v, "_length_") else v
+        return st
 
 
 class ForceFieldDMD:
@@ -181,6 +219,19 @@ class ForceFieldDMD:
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
                 for a in (p.A, p.t1, p.t2, p.cut_rho, p.cut_phi, p.k1, p.k2, p.k3)]
         self._call("mapp_dmd_fs_coef", *[_dp(a) for a in arrs])
+
+    def meam_coef(self, state):
+        """ForceField_meam::coef after its readers: state as mapp_amd.workload.meam_state"""
+        self._meam = MeamParams.from_state(state)
+        self._call("mapp_dmd_meam_coef", C.cast(C.pointer(self._meam), C.c_void_p))
+
+    def read_meam_params(self, library, local, elements):
+        """ff_coef MEAM <library> <local>"""
+        arr = (C.c_char_p * len(elements))(*[e.encode() for e in elements])
+        self._call("mapp_dmd_read_meam_params", library.encode(), None if local is None else local.encode(), arr)
+
+    def meam_set_fix(self, fix):
+        self._call("mapp_dmd_meam_set_fix", 1 if fix else 0)
 
     def read_fs_params(self, path, elements):
         """ff_coef FS <file>: the parameter file of ForceField_fs::coef"""
@@ -462,6 +513,30 @@ def setup_fs(H, s3, types, params, skin=0.5, device=0, variant="FS"):
     ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
     ff.init(skin)
     return ff
+
+
+def setup_meam(H, s3, types, state, skin=0.3, fix=False, device=0):
+    """MEAM (ForceField_meam) with the parameters of mapp_amd.workload.meam_state"""
+    ff = ForceFieldDMD(no_types=int(state["nt"]), variant="MEAM", device=device)
+    ff.meam_coef(state)
+    ff.meam_set_fix(fix)
+    ff.set_system(H, s3, np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    ff.init(skin)
+    return ff
+
+
+def meam_params_read(library, local, elements):
+    """the library and local files of ForceField_meam::coef for the types
+    `elements`, without a GPU: a dict of arrays (mapp_meam_params fields)"""
+    lib = load_library()
+    p = MeamParams()
+    err = C.create_string_buffer(256)
+    arr = (C.c_char_p * len(elements))(*[e.encode() for e in elements])
+    rc = lib.mapp_dmd_meam_params_read(library.encode(), None if local is None else local.encode(), len(elements),
+                                       arr, C.cast(C.pointer(p), C.c_void_p), err, 256)
+    if rc != 0:
+        raise DMDError(rc, err.value.decode())
+    return p.to_state()
 
 
 def read_fs_params(path, elements):

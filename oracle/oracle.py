@@ -250,7 +250,7 @@ class Oracle:
     def neighbor_list(self):
         n = self.natms
         off = _view(self.L.orc_list_offsets(self.o), n + 1, np.int32)
-        nl = _view(self.L.orc_list(self.o), int(self.L.orc_npairs(self.o)), np.int32)
+        nl = _view(self.L.orc_list(self.o), int(off[n]), np.int32)  # MEAM: the full list
         return off, nl
 
     def neighbor_list2(self):

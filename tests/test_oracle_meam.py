@@ -148,3 +148,35 @@ def test_quirks_confined_to_documented_terms():
     assert np.array_equal(ea, eb)
     assert np.array_equal(a.get("f"), b.get("f"))
     assert _fd(st, H, s, t, [0, 7], fix=0) < 1e-7
+
+
+# ---------------------------------------------------------------- the product's readers (no GPU)
+def test_reader_round_trip(tmp_path):
+    """ff_coef MEAM <library> <local>: a library listing another element and
+    the types in another order, and a local file, read back as the members
+    the reference holds (indices of the local file in library order)"""
+    from mapp_amd.dmd import meam_params_read
+    lib, loc = str(tmp_path / "lib.meam"), str(tmp_path / "loc.meam")
+    W.write_meam_lib(lib, ["Cc", "Bb", "Aa"])
+    local = W.meam_local(2, ialloy=1) + [("nn2", 1, 2, 1), ("delta", 1, 2, 0.05), ("attrac", 2, 1, 0.01),
+                                         ("repuls", 1, 1, 0.02), ("gsmooth_factor", 20.0), ("bkgd_dyn", 1)]
+    W.write_meam_local(loc, local)
+    got = meam_params_read(lib, loc, ["Aa", "Bb"])
+    want = W.meam_state(["Aa", "Bb"], lib_order=["Cc", "Bb", "Aa"], local=local)
+    for k, v in want.items():
+        assert np.array_equal(np.asarray(got[k]), np.asarray(v)), k
+    assert got["delr_inv"] == 10.0 and got["emb_lin_neg"] == 1  # the reader's own behaviour
+
+
+@pytest.mark.parametrize("line,msg", [("zbl(1,2) = 0", "wrong input"), ("Ec(1,3) = 2.0", "wrong command"),
+                                      ("foo = 1", "wrong input"), ("rc 4.0 5.0", "wrong command")])
+def test_reader_rejects(tmp_path, line, msg):
+    from mapp_amd.dmd import DMDError, meam_params_read
+    lib, loc = str(tmp_path / "lib.meam"), str(tmp_path / "loc.meam")
+    W.write_meam_lib(lib, ["Aa", "Bb"])
+    with open(loc, "w") as fh:
+        fh.write(line + "\n")
+    with pytest.raises(DMDError, match=msg):
+        meam_params_read(lib, loc, ["Aa", "Bb"])
+    with pytest.raises(DMDError, match="not in the library"):
+        meam_params_read(lib, "NULL", ["Aa", "Zz"])
