@@ -11,15 +11,17 @@ same system: the mixed path (5-point rule, FP32
 quadrature sums), the learned surrogate (trained here on 4M quadrature
 samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
 with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
-(surrogate force in one launch per kernel, autotuned, 20 BDF steps), the roofline of the
+(surrogate force in one launch per kernel, autotuned, 20 BDF steps), MEAM
+force_calc on the same lattice size (two-element synthetic alloy), the roofline of the
 dominant kernel (pass 1, timed with HIP events on the library's stream) and
 the FP64 CPU oracle timed on a bounded sample (32^3 cells per thread, one
 thread and then one box per thread of the process's CPU share, at most 16).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--no-surrogate] [--no-lbfgs] [--no-batch] [--fp64-only]
+                    [--no-surrogate] [--no-lbfgs] [--no-batch] [--no-meam] [--fp64-only]
                     [--schedule parts,wpb,waves,wpb2,waves2]
     python bench.py --copy-calibration   (plain copy kernel for HBM counter scaling)
+    python bench.py --meam-only          (MEAM force_calc alone, for profiling)
 """
 import argparse
 import json
@@ -163,6 +165,44 @@ def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
                 bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [])
 
 
+def run_meam(n, steps, warmup, cpu_n=8):
+    """MEAM (ForceField_meam) force_calc on n^3 FCC cells of the two-element
+    synthetic alloy (jittered, L12 cross lattice, default screening), wall
+    time per call with the result on the host; the FP64 oracle on an cpu_n^3
+    sample, one thread"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_meam
+    from oracle import oracle as O
+    st = W.meam_state(["Aa", "Bb"], local=W.meam_local(2))
+    H, s, t = W.md_lattice(n, 3.58, "fcc", nt=2, seed=41, jitter=0.05)
+    t0 = time.perf_counter()
+    ff = setup_meam(H, s, t, st)
+    build_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        ff.force_calc(1)
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ff.force_calc(1)
+        times.append(time.perf_counter() - t0)
+    entries = int(ff.neighbor_list()[0][-1])
+    ff.close()
+    ms = float(np.median(times)) * 1e3
+    Hc, sc, tc = W.md_lattice(cpu_n, 3.58, "fcc", nt=2, seed=41, jitter=0.05)
+    o = O.Oracle(nt=2, variant="MEAM")
+    o.meam_coef(st)
+    o.set_system(Hc, sc.copy(), tc.reshape(-1, 1))
+    o.init(0.3)
+    ct = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.force_calc(1)
+        ct.append(time.perf_counter() - t0)
+    return dict(sites=len(t), list_entries=entries, build_seconds=build_s, ms_per_force=ms,
+                site_evals_per_s=len(t) / (ms / 1e3),
+                cpu_oracle=dict(sites=len(tc), site_evals_per_s=len(tc) / float(np.median(ct)), cores=1))
+
+
 def cpu_threads():
     """host threads for the multi-rank CPU baseline: the process's CPU share
     (OMP_NUM_THREADS where set), at most 16"""
@@ -261,6 +301,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-meam", action="store_true")
+    ap.add_argument("--meam-only", action="store_true", help="time MEAM force_calc alone (profiling)")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -273,6 +315,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.meam_only:
+        print(json.dumps({"meam": run_meam(args.cells, args.steps, args.warmup)}), flush=True)
         return
 
     from mapp_amd import workload as W
@@ -352,6 +397,9 @@ def main():
         if not args.no_batch:
             out["batch"] = run_batch(tables, max(3, args.steps // 4), 1)
             log("batch", out["batch"])
+        if not args.no_meam:
+            out["meam"] = run_meam(args.cells, max(3, args.steps // 4), 1)
+            log("meam", out["meam"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tables)
     print(json.dumps(out), flush=True)

@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
                                                        const double* __restrict__ tab, int natms,
                                                        double4* __restrict__ pairs,
                                                        const double* __restrict__ rec, double* __restrict__ f,
-                                                       double* __restrict__ part) {
+                                                       int* __restrict__ active, double* __restrict__ part) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kMeamWaves + w;
     const MeamStage S(w);
@@ -352,6 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
         const double4 xi = pos[i];
         const int it = (int)type[i];
         const double v2[6] = {1, 2, 2, 1, 2, 1}, v3[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+        bool any = false;  // an entry whose screening moves other atoms
         for (int jn = lane; jn < n; jn += 64) {
             const double4 pv = pairs[p0 + jn];
             double dUds_store = 0.0;
@@ -637,12 +638,15 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
                 dUds_store = (sij != 0.0 && sij != 1.0) ? dUdsij : 0.0;
             }
             pairs[p0 + jn].w = dUds_store;
+            any = any || dUds_store != 0.0;
         }
+        const bool wave_any = __ballot(any) != 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) fi[a] = wave_sum(fi[a]);
         if (lane == 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) f[(size_t)i * 3 + a] = fi[a];
+            active[i] = wave_any ? 1 : 0;
         }
     }
     double acc[7] = {wave_sum(en)};
@@ -658,7 +662,8 @@ __global__ __launch_bounds__(kBlock) void k_meam_kforce(MeamDev M, const double4
                                                         const double* __restrict__ type,
                                                         const int* __restrict__ off, const int* __restrict__ nl,
                                                         int natms, const double4* __restrict__ pairs,
-                                                        double* __restrict__ f, double* __restrict__ part) {
+                                                        const int* __restrict__ active, double* __restrict__ f,
+                                                        double* __restrict__ part) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.x * kMeamWaves + w;
     double vir[6] = {0, 0, 0, 0, 0, 0}, fk[3] = {0, 0, 0};
@@ -668,7 +673,9 @@ __global__ __launch_bounds__(kBlock) void k_meam_kforce(MeamDev M, const double4
         const int q0 = off[k], q1 = off[k + 1];
         for (int qo = q0; qo < q1; ++qo) {
             const int o = nl[qo];
-            if (o >= natms) continue;  // k local in o's list <=> o local in k's list
+            // k local in o's list <=> o local in k's list; o without a pair whose
+            // screening moves other atoms adds nothing
+            if (o >= natms || !active[o]) continue;
             const double4 xo = pos[o];
             const int ot = (int)type[o];
             const int p0 = off[o], np = off[o + 1] - p0;
