@@ -246,12 +246,13 @@ LJ_COEFS = [(0, 0, 0.50, 2.30, 4.75), (0, 1, 0.40, 2.35, 4.0), (1, 1, 0.60, 2.40
 
 
 def md_lattice(n, a, kind="fcc", nt=2, frac=0.3, seed=1, jitter=0.03):
-    """n^3 cubic cells of an FCC or BCC lattice, jittered N(0, jitter); type 1
+    """n^3 cubic cells of an FCC, BCC or diamond lattice, jittered N(0, jitter); type 1
     on a fraction frac of the sites (nt = 2).  Returns H, reduced positions
     [natms][3] and the types."""
     rng = np.random.default_rng(seed)
-    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) if kind == "fcc" else \
-        np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    fcc = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    basis = {"fcc": fcc, "bcc": np.array([[0, 0, 0], [0.5, 0.5, 0.5]]),
+             "dia": np.concatenate([fcc, fcc + 0.25])}[kind]
     g = np.arange(n)
     cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
     s0 = ((cells[:, None, :] + basis[None, :, :]) / n).reshape(-1, 3)
@@ -284,6 +285,8 @@ MEAM_LIB = {
                esub=3.5, asub=0.95, t0=1.0, t1=2.6, t2=2.8, t3=2.1, rozero=1.1, ibar=3),
     "Cc": dict(lat="bcc", z=8.0, ielt=26, atwt=55.8, alpha=5.2, b0=3.6, b1=1.0, b2=1.0, b3=1.0, alat=2.86,
                esub=4.3, asub=0.56, t0=1.0, t1=1.5, t2=2.0, t3=-1.2, rozero=1.0, ibar=3),
+    "Dd": dict(lat="dia", z=4.0, ielt=14, atwt=28.1, alpha=4.9, b0=4.4, b1=5.5, b2=5.5, b3=5.5, alat=5.43,
+               esub=4.6, asub=1.0, t0=1.0, t1=3.3, t2=5.1, t3=-0.8, rozero=1.0, ibar=1),
 }
 
 
@@ -408,3 +411,27 @@ def meam_state(types, lib_order=None, local=(), lib=None):
             raise ValueError(key)
     st.update(sc)
     return st
+
+
+def meam_variants():
+    """MEAM settings that reach every branch of the set-up and force loop:
+    name -> (state, lattice kind, lattice constant, number of types)"""
+    A = MEAM_LIB["Aa"]
+    return {
+        "bcc_nn2": (meam_state(["Cc"], local=[("rc", 4.0), ("nn2", 1, 1, 1), ("Cmin", 1, 1, 1, 0.36)]), "bcc", 2.86, 1),
+        "dia": (meam_state(["Dd"], local=[("rc", 4.5)]), "dia", 5.43, 1),
+        "ibar0": (meam_state(["Aa"], local=[("rc", 4.5)], lib={"Aa": dict(A, ibar=0)}), "fcc", 3.55, 1),
+        "ibar4_gsmooth": (meam_state(["Aa"], local=[("rc", 4.5), ("gsmooth_factor", 10.0)],
+                                     lib={"Aa": dict(A, ibar=4)}), "fcc", 3.55, 1),
+        "ibar_m5": (meam_state(["Aa"], local=[("rc", 4.5)], lib={"Aa": dict(A, ibar=-5)}), "fcc", 3.55, 1),
+        "erose1": (meam_state(["Aa", "Bb"], local=meam_local(2, erose_form=1) +
+                              [("attrac", 1, 1, 0.05), ("repuls", 1, 1, 0.03), ("attrac", 1, 2, 0.02),
+                               ("repuls", 2, 2, 0.04)]), "fcc", 3.58, 2),
+        "erose2_emb_lin_neg": (meam_state(["Aa", "Bb"], local=meam_local(2, erose_form=2, emb_lin_neg=1) +
+                                          [("attrac", 1, 1, 0.05), ("repuls", 1, 2, 0.03)]), "fcc", 3.58, 2),
+        "b2_nn2": (meam_state(["Aa", "Cc"], local=[("rc", 4.5), ("lattce", 1, 2, "b2"), ("nn2", 1, 2, 1),
+                                                   ("Cmin", 1, 1, 2, 0.8), ("Cmin", 2, 2, 1, 0.9)]), "fcc", 3.5, 2),
+        "c11": (meam_state(["Dd", "Aa"], local=[("rc", 4.5), ("lattce", 1, 2, "c11")]), "fcc", 3.6, 2),
+        "b1_bkgd_dyn": (meam_state(["Aa", "Bb"], local=[("rc", 4.5), ("lattce", 1, 2, "b1"), ("bkgd_dyn", 1),
+                                                         ("augt1", 0)]), "fcc", 3.58, 2),
+    }

@@ -6,8 +6,10 @@ relative (infinity norm over the array; the GPU sums each site's terms in
 another order than the reference's sequential loops); energy_calc <= 1e-12.
 Cases: one element, two elements (L12 cross lattice, ialloy 0, 1, 2),
 partially screened pairs with the reference's force loop and the corrected
-one, and a perfect fcc crystal whose energy is the Rose energy (the pair
-potential's construction, through the product's own tables)."""
+one, every lattice / ibar / erose form / second-neighbour series / background
+option (bcc with nn2, diamond, B1, B2, C11), and a perfect fcc crystal whose
+energy is the Rose energy (the pair potential's construction, through the
+product's own tables)."""
 import numpy as np
 import pytest
 
@@ -79,6 +81,19 @@ def test_partial_screening_periodic(fix, ialloy):
     H, s, t = W.md_lattice(4, 3.58, "fcc", nt=2, seed=20 + ialloy, jitter=0.06)
     ff, orc = _pair(st, H, s, t, fix=fix)
     _compare(ff, orc, f"screened fix {fix} ialloy {ialloy}")
+    ff.close()
+
+
+@pytest.mark.parametrize("name", ["bcc_nn2", "dia", "ibar0", "ibar4_gsmooth", "ibar_m5", "erose1",
+                                  "erose2_emb_lin_neg", "b2_nn2", "c11", "b1_bkgd_dyn"])
+def test_variants(name):
+    """every lattice, ibar, erose form, nn2 series and background option"""
+    from mapp_amd import workload as W
+    st, kind, a, nt = W.meam_variants()[name]
+    n = {"bcc": 4, "dia": 2, "fcc": 4}[kind]
+    H, s, t = W.md_lattice(n, a, kind, nt=nt, seed=7, jitter=0.05)
+    ff, orc = _pair(st, H, s, t)
+    _compare(ff, orc, name)
     ff.close()
 
 

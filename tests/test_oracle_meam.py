@@ -150,6 +150,17 @@ def test_quirks_confined_to_documented_terms():
     assert _fd(st, H, s, t, [0, 7], fix=0) < 1e-7
 
 
+@pytest.mark.parametrize("name", sorted(W.meam_variants()))
+def test_fd_variants(name):
+    """every lattice, ibar, erose form, nn2 series and background option: the
+    corrected loop's forces are -grad U on an isolated cluster"""
+    st, kind, a, nt = W.meam_variants()[name]
+    Hs, s0, t = W.md_lattice(2, a, kind, nt=nt, seed=3, jitter=0.06)
+    H = np.diag([30.0] * 3)
+    s = (s0 @ Hs + 9.0) / 30.0
+    assert _fd(st, H, s, t, [0, 5, len(t) - 1]) < 1e-7
+
+
 # ---------------------------------------------------------------- the product's readers (no GPU)
 def test_reader_round_trip(tmp_path):
     """ff_coef MEAM <library> <local>: a library listing another element and
