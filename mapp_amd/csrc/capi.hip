@@ -477,6 +477,7 @@ void finish_meam(mapp_dmd* ctx) {
     ctx->d_mpairs.reserve((size_t)ctx->npairs + 1);
     ctx->d_mrec.reserve((size_t)ntot * kMeamRec);
     ctx->d_mactive.reserve((size_t)natms + 1);
+    ctx->d_mact.reserve((size_t)ctx->npairs + 1);
     ctx->d_f.reserve((size_t)natms * 3);
 }
 
@@ -504,10 +505,10 @@ double run_meam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
         ghost_copy(ctx, ctx->d_mrec.p, kMeamRec);
         hipLaunchKernelGGL(k_meam_force, dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p, ctx->d_off.p,
                            ctx->d_nl.p, ctx->d_mtab.p, natms, ctx->d_mpairs.p, ctx->d_mrec.p, ctx->d_f.p,
-                           ctx->d_mactive.p, part2);
+                           ctx->d_mactive.p, ctx->d_mact.p, part2);
         launch_check("k_meam_force");
         hipLaunchKernelGGL(k_meam_kforce, dim3(nb), dim3(kBlock), 0, st, M, ctx->d_pos.p, ctx->d_conc.p, ctx->d_off.p,
-                           ctx->d_nl.p, natms, ctx->d_mpairs.p, ctx->d_mactive.p, ctx->d_f.p, part3);
+                           ctx->d_nl.p, natms, ctx->d_mpairs.p, ctx->d_mactive.p, ctx->d_mact.p, ctx->d_f.p, part3);
         launch_check("k_meam_kforce");
         reduce_partials(ctx, part2, nb, 7, 7, 0, out + 1);
         reduce_partials(ctx, part3, nb, 6, 6, 0, out + 8);

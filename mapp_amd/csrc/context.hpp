@@ -25,7 +25,8 @@ struct mapp_dmd {
     int meam_fix = 0;
     mappdmd::DevBuf<double> d_mtab, d_mrec;  // MEAM pair table, per-site records [ntot][38]
     mappdmd::DevBuf<double4> d_mpairs;       // MEAM per list entry {fc, S, dS, dU/dS}
-    mappdmd::DevBuf<int> d_mactive;          // MEAM: the site has a pair with dU/dS != 0 and 0 < S fc < 1
+    mappdmd::DevBuf<int> d_mactive, d_mact;  // MEAM: per site the count and list positions of its own
+                                             // entries (j > i) with dU/dS != 0 and 0 < S fc < 1
     mappdmd::FFParams<float> Pf{};
     double rsq_crd = 0, mod_rc = 0;
     double wi1_ref[mappdmd::kMaxPoints]{}, wi2_ref[mappdmd::kMaxPoints]{};

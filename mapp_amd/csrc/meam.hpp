@@ -338,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
                                                        const double* __restrict__ tab, int natms,
                                                        double4* __restrict__ pairs,
                                                        const double* __restrict__ rec, double* __restrict__ f,
-                                                       int* __restrict__ active, double* __restrict__ part) {
+                                                       int* __restrict__ nact, int* __restrict__ act,
+                                                       double* __restrict__ part) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kMeamWaves + w;
     const MeamStage S(w);
@@ -352,7 +353,9 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
         const double4 xi = pos[i];
         const int it = (int)type[i];
         const double v2[6] = {1, 2, 2, 1, 2, 1}, v3[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
-        bool any = false;  // an entry whose screening moves other atoms
+        // the entries this site owns (j > i) whose screening moves other atoms,
+        // compacted in list order for k_meam_kforce
+        int na = 0;
         for (int jn = lane; jn < n; jn += 64) {
             const double4 pv = pairs[p0 + jn];
             double dUds_store = 0.0;
@@ -638,15 +641,18 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
                 dUds_store = (sij != 0.0 && sij != 1.0) ? dUdsij : 0.0;
             }
             pairs[p0 + jn].w = dUds_store;
-            any = any || dUds_store != 0.0;
+            const bool own = j > i && dUds_store != 0.0;
+            int cnt;
+            const int dest = pack_dest(own, &cnt);
+            if (own) act[p0 + na + dest] = jn;
+            na += cnt;
         }
-        const bool wave_any = __ballot(any) != 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) fi[a] = wave_sum(fi[a]);
         if (lane == 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) f[(size_t)i * 3 + a] = fi[a];
-            active[i] = wave_any ? 1 : 0;
+            nact[i] = na;
         }
     }
     double acc[7] = {wave_sum(en)};
@@ -662,26 +668,60 @@ __global__ __launch_bounds__(kBlock) void k_meam_kforce(MeamDev M, const double4
                                                         const double* __restrict__ type,
                                                         const int* __restrict__ off, const int* __restrict__ nl,
                                                         int natms, const double4* __restrict__ pairs,
-                                                        const int* __restrict__ active, double* __restrict__ f,
-                                                        double* __restrict__ part) {
+                                                        const int* __restrict__ nact, const int* __restrict__ act,
+                                                        double* __restrict__ f, double* __restrict__ part) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.x * kMeamWaves + w;
     double vir[6] = {0, 0, 0, 0, 0, 0}, fk[3] = {0, 0, 0};
+    // per wave: the local neighbours o of k and the running count of their
+    // compacted entries, so the (o, entry) candidates spread over the lanes
+    __shared__ int s_o[kMeamWaves][kMeamMaxNb];
+    __shared__ int s_pre[kMeamWaves][kMeamMaxNb + 1];
     if (k < natms) {
         const double4 xk = pos[k];
         const int kt = (int)type[k];
-        const int q0 = off[k], q1 = off[k + 1];
-        for (int qo = q0; qo < q1; ++qo) {
-            const int o = nl[qo];
-            // k local in o's list <=> o local in k's list; o without a pair whose
-            // screening moves other atoms adds nothing
-            if (o >= natms || !active[o]) continue;
+        const int q0 = off[k], nk = off[k + 1] - q0;
+        int* so = s_o[w];
+        int* pre = s_pre[w];
+        int carry = 0;
+        for (int c = 0; c < nk; c += 64) {
+            const int e = c + lane;
+            int o = -1, cnt = 0;
+            if (e < nk) {
+                o = nl[q0 + e];
+                // k local in o's list <=> o local in k's list; only o's own pairs
+                // (p > o) whose screening moves other atoms contribute
+                cnt = o < natms ? nact[o] : 0;
+                so[e] = o;
+            }
+            int incl = cnt;  // inclusive scan over the chunk
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += v;
+            }
+            if (e < nk) pre[e + 1] = carry + incl;
+            if (c == 0 && lane == 0) pre[0] = 0;
+            carry += __shfl(incl, 63, 64);
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int total = carry;
+        for (int t = lane; t < total; t += 64) {
+            // the entry e with pre[e] <= t < pre[e + 1]
+            int lo = 0, hi = nk;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (pre[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const int o = so[lo];
+            const int p0 = off[o];
+            const int pn = act[p0 + (t - pre[lo])];
             const double4 xo = pos[o];
             const int ot = (int)type[o];
-            const int p0 = off[o], np = off[o + 1] - p0;
-            for (int pn = lane; pn < np; pn += 64) {
+            {
                 const int p = nl[p0 + pn];
-                if (p <= o || p == k) continue;
+                if (p == k) continue;
                 const double4 pv = pairs[p0 + pn];
                 const double dUds = pv.w;
                 if (dUds == 0.0) continue;
