@@ -106,29 +106,41 @@ __device__ __forceinline__ double meam_phi(const double* __restrict__ tab, const
     return ((q[3] * pp + q[2]) * pp + q[1]) * pp + q[0];
 }
 
-// per-wave LDS: the site's list entries {x, y, z, type} and indices, its record
+// per-wave LDS: the site's list entries {x, y, z, type}, indices and squared
+// distances to the site, its record
 struct MeamStage {
     double4* nb;
     int* idx;
+    double* r2;
     double* rec;
     __device__ __forceinline__ explicit MeamStage(int w) {
         __shared__ double4 s_nb[kMeamWaves][kMeamMaxNb];
         __shared__ int s_idx[kMeamWaves][kMeamMaxNb];
+        __shared__ double s_r2[kMeamWaves][kMeamMaxNb];
         __shared__ double s_rec[kMeamWaves][kMeamRec];
         nb = s_nb[w];
         idx = s_idx[w];
+        r2 = s_r2[w];
         rec = s_rec[w];
     }
 };
 
 __device__ __forceinline__ void meam_stage(const MeamStage& S, const double4* __restrict__ pos,
                                            const double* __restrict__ type, const int* __restrict__ nl, int p0,
-                                           int n, int lane) {
+                                           int n, int lane, double4 xi) {
     for (int q = lane; q < n; q += 64) {
         const int j = nl[p0 + q];
         const double4 x = pos[j];
         S.nb[q] = make_double4(x.x, x.y, x.z, type[j]);
         S.idx[q] = j;
+        // |x_k - x_i|^2 as the reference forms it for a screening atom k (:1198-1201)
+        const double xki = x.x - xi.x, yki = x.y - xi.y, zki = x.z - xi.z;
+        double r2;
+        {
+#pragma clang fp contract(off)
+            r2 = xki * xki + yki * yki + zki * zki;
+        }
+        S.r2[q] = r2;
     }
     __builtin_amdgcn_wave_barrier();
 }
@@ -148,6 +160,9 @@ __device__ __forceinline__ void meam_screen(const MeamDev& M, const MeamStage& S
     dfcij *= M.delr_inv;
     for (int kn = 0; kn < n && sij != 0.0; ++kn) {
         if (S.idx[kn] == j) continue;
+        // both distances must lie inside the ellipse bound; the staged one first
+        const double rki2 = S.r2[kn];
+        if (!(rki2 < rbound)) continue;
         const double4 xk = S.nb[kn];
         const int kt = (int)xk.w;
         const double xjk = xj.x - xk.x, yjk = xj.y - xk.y, zjk = xj.z - xk.z;
@@ -157,13 +172,6 @@ __device__ __forceinline__ void meam_screen(const MeamDev& M, const MeamStage& S
             rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
         }
         if (!(rjk2 < rbound)) continue;
-        const double xki = xk.x - xi0, yki = xk.y - xi1, zki = xk.z - xi2;
-        double rki2;
-        {
-#pragma clang fp contract(off)
-            rki2 = xki * xki + yki * yki + zki * zki;
-        }
-        if (!(rki2 < rbound)) continue;
         const double x_ki = rki2 / rij2, x_jk = rjk2 / rij2;
         double a = 1.0 - (x_ki - x_jk) * (x_ki - x_jk);
         if (!(a > 0)) continue;
@@ -210,8 +218,8 @@ __global__ __launch_bounds__(kBlock) void k_meam_density(MeamDev M, const double
     double en = 0.0;
     if (i < natms) {
         const int p0 = off[i], n = off[i + 1] - p0;
-        meam_stage(S, pos, type, nl, p0, n, lane);
         const double4 xi = pos[i];
+        meam_stage(S, pos, type, nl, p0, n, lane, xi);
         const int it = (int)type[i];
         double acc[kMeamSums];
 #pragma unroll
@@ -346,24 +354,37 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
     double en = 0.0, vir[6] = {0, 0, 0, 0, 0, 0}, fi[3] = {0, 0, 0};
     if (i < natms) {
         const int p0 = off[i], n = off[i + 1] - p0;
-        meam_stage(S, pos, type, nl, p0, n, lane);
+        const double4 xi = pos[i];
+        meam_stage(S, pos, type, nl, p0, n, lane, xi);
         if (lane < kMeamRec) S.rec[lane] = rec[(size_t)i * kMeamRec + lane];
         __builtin_amdgcn_wave_barrier();
         const double* Ri = S.rec;
-        const double4 xi = pos[i];
         const int it = (int)type[i];
         const double v2[6] = {1, 2, 2, 1, 2, 1}, v3[10] = {1, 3, 3, 3, 6, 3, 1, 3, 3, 1};
+        // per wave: the entries of the current chunk with three-body screening
+        // forces, so that their (entry, k) candidates spread over the lanes
+        __shared__ double4 s3x[kMeamWaves][64], s3p[kMeamWaves][64];
+        __shared__ int s3j[kMeamWaves][64];
+        double4* e3x = s3x[w];
+        double4* e3p = s3p[w];
+        int* e3j = s3j[w];
         // the entries this site owns (j > i) whose screening moves other atoms,
         // compacted in list order for k_meam_kforce
         int na = 0;
-        for (int jn = lane; jn < n; jn += 64) {
-            const double4 pv = pairs[p0 + jn];
-            double dUds_store = 0.0;
-            const double4 xj = S.nb[jn];
-            const int j = S.idx[jn], jt = (int)xj.w;
+        for (int c = 0; c < n; c += 64) {
+            const int jn = c + lane;
+            const bool valid = jn < n;
+            bool three = false;
+            double4 pv = make_double4(0.0, 0.0, 0.0, 0.0), xj = make_double4(0.0, 0.0, 0.0, 0.0);
+            double dUds_store = 0.0, rij2 = 1.0, sij3 = 0.0, dUds3 = 0.0;
+            int j = -1;
+            if (valid) {
+            pv = pairs[p0 + jn];
+            xj = S.nb[jn];
+            j = S.idx[jn];
+            const int jt = (int)xj.w;
             const double xij = xi.x - xj.x, yij = xi.y - xj.y, zij = xi.z - xj.z;
             const double d[3] = {-xij, -yij, -zij};
-            double rij2;
             {
 #pragma clang fp contract(off)
                 rij2 = xij * xij + yij * yij + zij * zij;
@@ -586,66 +607,81 @@ __global__ __launch_bounds__(kBlock) void k_meam_force(MeamDev M, const double4*
                 vir[3] += 0.25 * (d[1] * fm[2] + d[2] * fm[1]);
                 vir[4] += 0.25 * (d[0] * fm[2] + d[2] * fm[0]);
                 vir[5] += 0.25 * (d[0] * fm[1] + d[1] * fm[0]);
-                // the site's own three-body screening forces (:2033-2072)
-                if (sij != 0.0 && sij != 1.0 && dUdsij != 0.0) {
-                    const double rbound = rij2 * M.ebound[it][jt];
-                    for (int kn = 0; kn < n; ++kn) {
-                        if (S.idx[kn] == j) continue;
-                        const double4 xk = S.nb[kn];
-                        const int kt = (int)xk.w;
-                        const double xjk = xj.x - xk.x, yjk = xj.y - xk.y, zjk = xj.z - xk.z;
-                        double rjk2, rki2;
-                        {
-#pragma clang fp contract(off)
-                            rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
-                        }
-                        if (!(rjk2 < rbound)) continue;
-                        const double xki = xk.x - xi.x, yki = xk.y - xi.y, zki = xk.z - xi.z;
-                        {
-#pragma clang fp contract(off)
-                            rki2 = xki * xki + yki * yki + zki * zki;
-                        }
-                        if (!(rki2 < rbound)) continue;
-                        const double x_ki = rki2 / rij2, x_jk = rjk2 / rij2;
-                        const double a = 1.0 - (x_ki - x_jk) * (x_ki - x_jk);
-                        if (a == 0.0) continue;
-                        double cikj = (2.0 * (x_ki + x_jk) + a - 2.0) / a;
-                        const double cmax = M.cmax[it][jt][kt], cmin = M.cmin[it][jt][kt];
-                        if (!(cikj < cmax && cikj > cmin)) continue;
-                        cikj = (cikj - cmin) / (cmax - cmin);
-                        double dfc;
-                        const double sikj = meam_dfcut(cikj, &dfc);
-                        double dc1, dc2;
-                        if (M.fix) meam_dc2(rij2, rjk2, rki2, &dc1, &dc2);
-                        else meam_dc2(rij2, rki2, rjk2, &dc1, &dc2);
-                        const double aa = (sij / (cmax - cmin)) * (dfc / sikj);
-                        const double f1 = dUdsij * (aa * dc1), f2 = dUdsij * (aa * dc2);
-                        if (aa * dc1 == 0.0 && aa * dc2 == 0.0) continue;
-                        const double dik[3] = {xki, yki, zki}, djk[3] = {-xjk, -yjk, -zjk};
-                        double fa[3], fb[3];
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            fa[q] = f1 * dik[q];
-                            fb[q] = f2 * djk[q];
-                            fi[q] += fa[q];
-                        }
-                        const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
-                        vir[0] += third * (dik[0] * fa[0] + djk[0] * fb[0]);
-                        vir[1] += third * (dik[1] * fa[1] + djk[1] * fb[1]);
-                        vir[2] += third * (dik[2] * fa[2] + djk[2] * fb[2]);
-                        vir[3] += sixth * (dik[1] * fa[2] + djk[1] * fb[2] + dik[2] * fa[1] + djk[2] * fb[1]);
-                        vir[4] += sixth * (dik[0] * fa[2] + djk[0] * fb[2] + dik[2] * fa[0] + djk[2] * fb[0]);
-                        vir[5] += sixth * (dik[0] * fa[1] + djk[0] * fb[1] + dik[1] * fa[0] + djk[1] * fb[0]);
-                    }
-                }
+                // the site's own three-body screening forces (:2033-2072) follow
+                // for the flagged entries, below
+                three = sij != 0.0 && sij != 1.0 && dUdsij != 0.0;
+                sij3 = sij;
+                dUds3 = dUdsij;
                 dUds_store = (sij != 0.0 && sij != 1.0) ? dUdsij : 0.0;
             }
             pairs[p0 + jn].w = dUds_store;
-            const bool own = j > i && dUds_store != 0.0;
+            }
+            const bool own = valid && j > i && dUds_store != 0.0;
             int cnt;
             const int dest = pack_dest(own, &cnt);
             if (own) act[p0 + na + dest] = jn;
             na += cnt;
+            // three-body screening forces on the site: (entry, k) over the lanes
+            int n3;
+            const int d3 = pack_dest(three, &n3);
+            if (three) {
+                e3x[d3] = xj;
+                e3p[d3] = make_double4(rij2, sij3, dUds3, 0.0);
+                e3j[d3] = j;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int tq = lane; tq < n3 * n; tq += 64) {
+                const int e = tq / n, kn = tq - e * n;
+                if (S.idx[kn] == e3j[e]) continue;
+                const double4 xj3 = e3x[e];
+                const double4 q3 = e3p[e];
+                const int jt = (int)xj3.w;
+                const double r2 = q3.x, sij = q3.y, dUdsij = q3.z;
+                const double rbound = r2 * M.ebound[it][jt];
+                const double rki2 = S.r2[kn];
+                if (!(rki2 < rbound)) continue;
+                const double4 xk = S.nb[kn];
+                const int kt = (int)xk.w;
+                const double xjk = xj3.x - xk.x, yjk = xj3.y - xk.y, zjk = xj3.z - xk.z;
+                double rjk2;
+                {
+#pragma clang fp contract(off)
+                    rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
+                }
+                if (!(rjk2 < rbound)) continue;
+                const double xki = xk.x - xi.x, yki = xk.y - xi.y, zki = xk.z - xi.z;
+                const double x_ki = rki2 / r2, x_jk = rjk2 / r2;
+                const double a = 1.0 - (x_ki - x_jk) * (x_ki - x_jk);
+                if (a == 0.0) continue;
+                double cikj = (2.0 * (x_ki + x_jk) + a - 2.0) / a;
+                const double cmax = M.cmax[it][jt][kt], cmin = M.cmin[it][jt][kt];
+                if (!(cikj < cmax && cikj > cmin)) continue;
+                cikj = (cikj - cmin) / (cmax - cmin);
+                double dfc;
+                const double sikj = meam_dfcut(cikj, &dfc);
+                double dc1, dc2;
+                if (M.fix) meam_dc2(r2, rjk2, rki2, &dc1, &dc2);
+                else meam_dc2(r2, rki2, rjk2, &dc1, &dc2);
+                const double aa = (sij / (cmax - cmin)) * (dfc / sikj);
+                if (aa * dc1 == 0.0 && aa * dc2 == 0.0) continue;
+                const double f1 = dUdsij * (aa * dc1), f2 = dUdsij * (aa * dc2);
+                const double dik[3] = {xki, yki, zki}, djk[3] = {-xjk, -yjk, -zjk};
+                double fa[3], fb[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    fa[q] = f1 * dik[q];
+                    fb[q] = f2 * djk[q];
+                    fi[q] += fa[q];
+                }
+                const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+                vir[0] += third * (dik[0] * fa[0] + djk[0] * fb[0]);
+                vir[1] += third * (dik[1] * fa[1] + djk[1] * fb[1]);
+                vir[2] += third * (dik[2] * fa[2] + djk[2] * fb[2]);
+                vir[3] += sixth * (dik[1] * fa[2] + djk[1] * fb[2] + dik[2] * fa[1] + djk[2] * fb[1]);
+                vir[4] += sixth * (dik[0] * fa[2] + djk[0] * fb[2] + dik[2] * fa[0] + djk[2] * fb[0]);
+                vir[5] += sixth * (dik[0] * fa[1] + djk[0] * fb[1] + dik[1] * fa[0] + djk[1] * fb[0]);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) fi[a] = wave_sum(fi[a]);
