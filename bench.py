@@ -11,8 +11,9 @@ same system: the mixed path (5-point rule, FP32
 quadrature sums), the learned surrogate (trained here on 4M quadrature
 samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
 with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
-(surrogate force in one launch per kernel, autotuned, 20 BDF steps), MEAM
-force_calc on the same lattice size (two-element synthetic alloy), the roofline of the
+(surrogate force in one launch per kernel, autotuned, 20 BDF steps), MEAM,
+plain EAM, Lennard-Jones, Finnis-Sinclair and FSM force_calc on the same
+lattice size, the roofline of the
 dominant kernel (pass 1, timed with HIP events on the library's stream) and
 the FP64 CPU oracle timed on a bounded sample (32^3 cells per thread, one
 thread and then one box per thread of the process's CPU share, at most 16).
@@ -22,6 +23,7 @@ thread and then one box per thread of the process's CPU share, at most 16).
                     [--schedule parts,wpb,waves,wpb2,waves2]
     python bench.py --copy-calibration   (plain copy kernel for HBM counter scaling)
     python bench.py --meam-only          (MEAM force_calc alone, for profiling)
+    python bench.py --md-only            (plain EAM, LJ, FS, FSM force_calc alone)
 """
 import argparse
 import json
@@ -203,6 +205,78 @@ def run_meam(n, steps, warmup, cpu_n=8):
                 cpu_oracle=dict(sites=len(tc), site_evals_per_s=len(tc) / float(np.median(ct)), cores=1))
 
 
+def run_md(n, tables, steps, warmup, cpu_n=8):
+    """The MD-mode potentials on n^3 cells (1,048,576 sites at n = 64 fcc):
+    plain EAM (Johnson tables, FCC a = 3.6, one site in three of type 1),
+    Lennard-Jones (two types, FCC a = 3.6), Finnis-Sinclair and its
+    squared-cutoff form (two types, BCC a = 2.87, 2 n^3 sites); force_calc wall
+    time per call, and the FP64 oracle on a cpu_n^3 sample, one thread"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_eam, setup_fs, setup_lj
+    from oracle import oracle as O
+
+    def eam(nn):
+        lat = W.fcc_alloy(nn, seed=21)
+        t = (np.arange(lat.natms) % 3 == 0).astype(np.float64)
+        return lat.H, np.ascontiguousarray(lat.x[:, :3]), t
+
+    cases = {
+        "eam": (lambda nn: eam(nn), lambda H, s, t: setup_eam(H, s, t, tables),
+                lambda: _orc_eam(O, tables)),
+        "lj": (lambda nn: W.md_lattice(nn, 3.6, "fcc", seed=3), lambda H, s, t: setup_lj(H, s, t, W.LJ_COEFS, nt=2),
+               lambda: _orc_lj(O, W)),
+        "fs": (lambda nn: W.md_lattice(nn, 2.87, "bcc", seed=5), lambda H, s, t: setup_fs(H, s, t, W.fs_params()),
+               lambda: _orc_fs(O, W, "FS")),
+        "fsm": (lambda nn: W.md_lattice(nn, 2.87, "bcc", seed=5),
+                lambda H, s, t: setup_fs(H, s, t, W.fs_params(), variant="FSM"), lambda: _orc_fs(O, W, "FSM")),
+    }
+    out = {}
+    for name, (lat, make, orc) in cases.items():
+        H, s, t = lat(n)
+        ff = make(H, s, t)
+        for _ in range(warmup):
+            ff.force_calc(1)
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ff.force_calc(1)
+            times.append(time.perf_counter() - t0)
+        ff.close()
+        ms = float(np.median(times)) * 1e3
+        Hc, sc, tc = lat(cpu_n)
+        o = orc()
+        o.set_system(Hc, sc.copy(), np.asarray(tc, dtype=np.float64).reshape(-1, 1))
+        o.init(0.5)
+        ct = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            o.force_calc(1)
+            ct.append(time.perf_counter() - t0)
+        out[name] = dict(sites=len(t), ms_per_force=ms, site_evals_per_s=len(t) / (ms / 1e3),
+                         cpu_oracle=dict(sites=len(tc), site_evals_per_s=len(tc) / float(np.median(ct)), cores=1))
+    return out
+
+
+def _orc_eam(O, tables):
+    o = O.Oracle(nt=2, variant="EAM")
+    o.set_fs(tables)
+    o.set_params(masses=tables.masses)
+    return o
+
+
+def _orc_lj(O, W):
+    o = O.Oracle(nt=2, variant="LJ")
+    for c in W.LJ_COEFS:
+        o.lj_coef(*c)
+    return o
+
+
+def _orc_fs(O, W, variant):
+    o = O.Oracle(nt=2, variant=variant)
+    o.fs_coef(W.fs_params())
+    return o
+
+
 def cpu_threads():
     """host threads for the multi-rank CPU baseline: the process's CPU share
     (OMP_NUM_THREADS where set), at most 16"""
@@ -303,6 +377,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-meam", action="store_true")
     ap.add_argument("--meam-only", action="store_true", help="time MEAM force_calc alone (profiling)")
+    ap.add_argument("--md-only", action="store_true", help="time the EAM, LJ, FS and FSM force_calc alone")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -318,6 +393,10 @@ def main():
         return
     if args.meam_only:
         print(json.dumps({"meam": run_meam(args.cells, args.steps, args.warmup)}), flush=True)
+        return
+    if args.md_only:
+        from mapp_amd import workload as W
+        print(json.dumps({"md": run_md(args.cells, W.johnson_tables(), args.steps, args.warmup)}), flush=True)
         return
 
     from mapp_amd import workload as W
@@ -400,6 +479,8 @@ def main():
         if not args.no_meam:
             out["meam"] = run_meam(args.cells, max(3, args.steps // 4), 1)
             log("meam", out["meam"])
+            out["md"] = run_md(args.cells, tables, max(3, args.steps // 4), 1)
+            log("md", out["md"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tables)
     print(json.dumps(out), flush=True)

@@ -416,7 +416,7 @@ double run_meam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st);
 double run_eam(mapp_dmd* ctx, bool energy_only, int st_clc, double* en_st) {
     if (ctx->variant == MAPP_DMD_VARIANT_MEAM) return run_meam(ctx, energy_only, st_clc, en_st);
     hipStream_t st = ctx->stream;
-    const int natms = ctx->natms, nb = nblk(natms);
+    const int natms = ctx->natms, nb = nblk((long)natms * kEamG);  // 16-lane site groups
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     EamBufs T{ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_F7.p, ctx->nt};
     ctx->d_part.reserve((size_t)nb * 14 + 16);

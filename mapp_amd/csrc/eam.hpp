@@ -29,6 +29,10 @@ struct EamBufs {
 // two factors (-A/sqrt(rho) kept per site) in another rounding order
 enum { kModelEam = 0, kModelLj = 1, kModelFs = 2, kModelFsm = 3 };
 
+// 16 lanes (one DPP row) per site in every pass: each lane takes every 16th
+// list entry, the group sums per-site values in a fixed tree
+constexpr int kEamG = 16;
+
 __device__ __forceinline__ void eam_row(const FFParams<double>& P, double r, int* m_out, double* p_out) {
     double p = r * P.dr_inv;
     int m = (int)p;
@@ -121,13 +125,15 @@ __global__ __launch_bounds__(kBlock) void k_eam_pass1(FFParams<double> P, EamBuf
                                                       double* __restrict__ fown, double* __restrict__ pairE,
                                                       double* __restrict__ pairF, double2* __restrict__ drho,
                                                       double* __restrict__ part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kEamG> g;
+    const int i = g.site;
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i < P.natms) {
         const int it = (int)type[i];
         const double4 xi = pos[i];
         double rho = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
+        const int p1 = L.off[i + 1];
+        for (int p = L.off[i] + g.l; p < p1; p += kEamG) {
             const int j = L.nl[p];
             const double4 xj = pos[j];
             const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
@@ -166,11 +172,19 @@ __global__ __launch_bounds__(kBlock) void k_eam_pass1(FFParams<double> P, EamBuf
                 }
             }
         }
-        rho_own[i] = rho;
+        rho = group_sum<kEamG>(rho);
         if (!ENERGY_ONLY) {
-            fown[(size_t)i * 3 + 0] = fx;
-            fown[(size_t)i * 3 + 1] = fy;
-            fown[(size_t)i * 3 + 2] = fz;
+            fx = group_sum<kEamG>(fx);
+            fy = group_sum<kEamG>(fy);
+            fz = group_sum<kEamG>(fz);
+        }
+        if (g.l == 0) {
+            rho_own[i] = rho;
+            if (!ENERGY_ONLY) {
+                fown[(size_t)i * 3 + 0] = fx;
+                fown[(size_t)i * 3 + 1] = fy;
+                fown[(size_t)i * 3 + 2] = fz;
+            }
         }
     }
     block_sum_store<7>(acc, part);
@@ -185,13 +199,18 @@ __global__ __launch_bounds__(kBlock) void k_eam_embed(FFParams<double> P, EamBuf
                                                       const double* __restrict__ rho_own,
                                                       const double* __restrict__ pairE, double* __restrict__ rho,
                                                       double* __restrict__ dF, double* __restrict__ part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kEamG> g;
+    const int i = g.site;
     double acc[1] = {0.0};
     if (i < P.natms) {
-        double r = rho_own[i];
-        for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) r += pairE[q];
+        double part_r = 0.0;
+        const int q1 = L.roff[i + 1];
+        for (int q = L.roff[i] + g.l; q < q1; q += kEamG) part_r += pairE[q];
+        double r = rho_own[i] + group_sum<kEamG>(part_r);
         const int it = (int)type[i];
-        if constexpr (MODEL == kModelFs || MODEL == kModelFsm) {
+        if (g.l != 0) {
+            // one lane per site evaluates the embedding
+        } else if constexpr (MODEL == kModelFs || MODEL == kModelFsm) {
             const double sq = sqrt(r);
             acc[0] = -(M.A[it] * sq);
             rho[i] = r;
@@ -220,13 +239,15 @@ __global__ __launch_bounds__(kBlock) void k_eam_pass2(FFParams<double> P, const 
                                                       const int* __restrict__ rpos, const double2* __restrict__ drho,
                                                       const double* __restrict__ dF, double* __restrict__ fown,
                                                       double* __restrict__ pairF2, double* __restrict__ part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kEamG> g;
+    const int i = g.site;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (i < P.natms) {
         const double4 xi = pos[i];
         const double dFi = dF[i];
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int p = L.off[i]; p < L.off[i + 1]; ++p) {
+        const int p1 = L.off[i + 1];
+        for (int p = L.off[i] + g.l; p < p1; p += kEamG) {
             const int j = L.nl[p];
             const double2 dr = drho[p];
             const bool owned = j < P.natms;
@@ -255,9 +276,14 @@ __global__ __launch_bounds__(kBlock) void k_eam_pass2(FFParams<double> P, const 
                 pairF2[q * 3 + 2] = -dx2 * fpair;
             }
         }
-        fown[(size_t)i * 3 + 0] += fx;
-        fown[(size_t)i * 3 + 1] += fy;
-        fown[(size_t)i * 3 + 2] += fz;
+        fx = group_sum<kEamG>(fx);
+        fy = group_sum<kEamG>(fy);
+        fz = group_sum<kEamG>(fz);
+        if (g.l == 0) {
+            fown[(size_t)i * 3 + 0] += fx;
+            fown[(size_t)i * 3 + 1] += fy;
+            fown[(size_t)i * 3 + 2] += fz;
+        }
     }
     block_sum_store<6>(acc, part);
 }
@@ -265,10 +291,12 @@ __global__ __launch_bounds__(kBlock) void k_eam_pass2(FFParams<double> P, const 
 __global__ __launch_bounds__(kBlock) void k_eam_gather(int natms, PairBufs L, const double* __restrict__ fown,
                                                        const double* __restrict__ pairF,
                                                        const double* __restrict__ pairF2, double* __restrict__ f) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const SiteGroup<kEamG> g;
+    const int i = g.site;
     if (i >= natms) return;
-    double a0 = fown[(size_t)i * 3], a1 = fown[(size_t)i * 3 + 1], a2 = fown[(size_t)i * 3 + 2];
-    for (int q = L.roff[i]; q < L.roff[i + 1]; ++q) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int q1 = L.roff[i + 1];
+    for (int q = L.roff[i] + g.l; q < q1; q += kEamG) {
         if (pairF2) {
             a0 += pairF[(size_t)q * 3] + pairF2[(size_t)q * 3];
             a1 += pairF[(size_t)q * 3 + 1] + pairF2[(size_t)q * 3 + 1];
@@ -279,9 +307,14 @@ __global__ __launch_bounds__(kBlock) void k_eam_gather(int natms, PairBufs L, co
             a2 += pairF[(size_t)q * 3 + 2];
         }
     }
-    f[(size_t)i * 3] = a0;
-    f[(size_t)i * 3 + 1] = a1;
-    f[(size_t)i * 3 + 2] = a2;
+    a0 = group_sum<kEamG>(a0);
+    a1 = group_sum<kEamG>(a1);
+    a2 = group_sum<kEamG>(a2);
+    if (g.l == 0) {
+        f[(size_t)i * 3] = fown[(size_t)i * 3] + a0;
+        f[(size_t)i * 3 + 1] = fown[(size_t)i * 3 + 1] + a1;
+        f[(size_t)i * 3 + 2] = fown[(size_t)i * 3 + 2] + a2;
+    }
 }
 
 }  // namespace mappdmd
