@@ -15,8 +15,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib", "libmapp_dmd.so")
 SOURCES = ["capi.hip"]
-HEADERS = ["dmd_common.hpp", "context.hpp", "geometry.hpp", "force.hpp",
-           "mobility.hpp", "tables.hpp", "gh_table.h"]
 
 
 def hipcc_cmd(out, sources):
