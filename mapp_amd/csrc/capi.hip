@@ -711,11 +711,11 @@ void upload_tables(mapp_dmd* ctx) {
         for (int jt = 0; jt < nt; ++jt) {
             const double* rs = ctx->rho7.data() + (size_t)type2rho(nt, it, jt) * nr * 7;
             const double* ps = ctx->phi7.data() + (size_t)type2phi(it, jt) * nr * 7;
-            double* d = q64.data() + (size_t)(it * nt + jt) * nr * kQuadRow;
+            double* d = q64.data() + (size_t)(it * nt + jt) * kQuadRow;
             for (int m = 0; m < nr; ++m)
                 for (int k = 0; k < 4; ++k) {
-                    d[(size_t)m * kQuadRow + k] = rs[(size_t)m * 7 + k];
-                    d[(size_t)m * kQuadRow + 4 + k] = ps[(size_t)m * 7 + k];
+                    d[(size_t)m * quad_stride(nt) + k] = rs[(size_t)m * 7 + k];
+                    d[(size_t)m * quad_stride(nt) + 4 + k] = ps[(size_t)m * 7 + k];
                 }
         }
     for (size_t k = 0; k < q64.size(); ++k) q32[k] = (float)q64[k];

@@ -33,6 +33,12 @@ namespace mappdmd {
 
 constexpr int kBlock = 256;
 constexpr int kQuadRow = 8;  // rho c0..c3, r*phi c0..c3 of one ordered type pair
+// The quadrature table is bin-major, quad[m][tp][kQuadRow]: the rows of all
+// type pairs at one grid row are adjacent, so the type pairs of a pair that
+// evaluate at the same distance (the middle node of an odd rule, t = r) share
+// cache lines.  Type pair tp starts at quad + tp * kQuadRow; consecutive rows
+// of one type pair are quad_stride(nt) elements apart.
+__host__ __device__ constexpr int quad_stride(int nt) { return nt * nt * kQuadRow; }
 
 __host__ __device__ constexpr int type2rho(int nt, int i, int j) { return j * nt + i; }
 __host__ __device__ constexpr int type2phi(int i, int j) {
@@ -50,7 +56,7 @@ __device__ __forceinline__ T horner3(const T* c, T p) {
 // Returns false when the whole rule lies beyond the cutoff (lower >= xi_max).
 template <typename Q, int NQ, bool DERIV, bool PHI>
 __device__ __forceinline__ bool gh_average(const FFParams<Q>& P, Q r, Q r_inv, Q alpha,
-                                           const Q* __restrict__ tab, Q* g, Q* f) {
+                                           const Q* __restrict__ tab, int stride, Q* g, Q* f) {
     const int nq = NQ > 0 ? NQ : P.no_i;
     const Q asq = sqrt(alpha);
     const Q inv = Q(1) / asq;
@@ -69,7 +75,7 @@ __device__ __forceinline__ bool gh_average(const FFParams<Q>& P, Q r, Q r_inv, Q
             m = m < P.nr - 2 ? m : P.nr - 2;
             p -= (Q)m;
             p = p < Q(1) ? p : Q(1);
-            const Q* row = tab + (size_t)m * kQuadRow;
+            const Q* row = tab + (size_t)m * stride;
             const Q rv = horner3(row, p) * t;
             g0 = fma(P.w0[k], rv, g0);
             if (DERIV) {
@@ -255,7 +261,7 @@ __device__ __forceinline__ int alpha_branch(const FFParams<double>& P, double ia
 // g, f = {value, d/dr, d/dalpha}; returns whether the rule is inside.
 template <typename Q, int NQ, bool DERIV>
 __device__ __forceinline__ bool gh_avg(const FFParams<Q>& P, Q r, Q r_inv, Q asq, Q ia, const Q* __restrict__ tab,
-                                       Q* g, Q* f) {
+                                       int stride, Q* g, Q* f) {
     const int nq = NQ > 0 ? NQ : P.no_i;
     const Q inv = ia * asq;
     const Q upper = (r + P.rc) * asq;
@@ -274,7 +280,7 @@ __device__ __forceinline__ bool gh_avg(const FFParams<Q>& P, Q r, Q r_inv, Q asq
         m = m > 0 ? m : 0;
         p -= (Q)m;
         p = p < Q(1) ? p : Q(1);
-        const Q* row = tab + (size_t)m * kQuadRow;
+        const Q* row = tab + (size_t)m * stride;
         Q rv = horner3(row, p) * t;
         Q pv = horner3(row + 4, p);
         pv = t < Q(0) ? -pv : pv;
@@ -322,8 +328,8 @@ __device__ __forceinline__ void lane_pair(const FFParams<double>& P, const FFPar
     const int br = alpha_branch(P, ia, ai, aj);
     if (br == 1) {
         Q gq[3], fq[3];
-        const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
-        if (gh_avg<Q, NQ, DERIV>(PQ, (Q)r, (Q)r_inv, (Q)asq, (Q)ia, tab, gq, fq)) {
+        const Q* tab = quad + (size_t)(it * NT + jt) * kQuadRow;
+        if (gh_avg<Q, NQ, DERIV>(PQ, (Q)r, (Q)r_inv, (Q)asq, (Q)ia, tab, quad_stride(NT), gq, fq)) {
             g[0] = (double)gq[0];
             f[0] = (double)fq[0];
             if (DERIV) {
@@ -553,7 +559,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y;
-        const Q* tab = quad + (size_t)q * P.nr * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS));
+        const Q* tab = quad + (size_t)q * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS));
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         for (int chunk = p0; chunk < p1; chunk += 64) {
@@ -649,7 +655,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         m = m > 0 ? m : 0;
                         pp -= (Q)m;
                         ppv[kp] = pp < Q(1) ? pp : Q(1);
-                        const Q* row = tab + (size_t)m * kQuadRow;
+                        const Q* row = tab + (size_t)m * quad_stride(NT);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) rowv[kp][v] = *reinterpret_cast<const Q2*>(row + 2 * v);
                     }

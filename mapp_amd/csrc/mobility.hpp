@@ -87,9 +87,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_values(FFParams<double> P, FFPa
                 const double a = (ai * aj) / (ai + aj);
                 if (P.alpha_min < a && a < P.alpha_max) {
                     Q g[3], f[3];
-                    const Q* tab = quad + (size_t)(it * NT + jt) * P.nr * kQuadRow;
-                    bool hit = want_phi ? gh_average<Q, NQ, false, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f)
-                                        : gh_average<Q, NQ, false, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, g, f);
+                    const Q* tab = quad + (size_t)(it * NT + jt) * kQuadRow;
+                    bool hit = want_phi ? gh_average<Q, NQ, false, true>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, quad_stride(NT), g, f)
+                                        : gh_average<Q, NQ, false, false>(PQ, (Q)r, (Q)r_inv, (Q)a, tab, quad_stride(NT), g, f);
                     if (hit) {
                         rv[type2rho(NT, it, jt)] = (double)g[0];
                         if (want_phi) pv[type2phi(it, jt)] = (double)f[0];

@@ -80,8 +80,8 @@ __global__ void k_sur_samples(FFParams<float> PQ, SurNorm N, const float* __rest
     const double ia = 1.0 / ai + 1.0 / aj;
     const float asq = (float)rsqrt(ia);
     float g[3] = {0, 0, 0}, f[3] = {0, 0, 0};
-    const float* tab = quad + (size_t)tp * PQ.nr * kQuadRow;
-    gh_avg<float, NQ, true>(PQ, (float)r, (float)(1.0 / r), asq, (float)ia, tab, g, f);
+    const float* tab = quad + (size_t)tp * kQuadRow;
+    gh_avg<float, NQ, true>(PQ, (float)r, (float)(1.0 / r), asq, (float)ia, tab, quad_stride(NT), g, f);
     X[s * 3 + 0] = (float)r;
     X[s * 3 + 1] = (float)ai;
     X[s * 3 + 2] = (float)aj;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 ++fallbacks;
                 float gq[3] = {0, 0, 0}, fq[3] = {0, 0, 0};
                 gh_avg<float, NQ, true>(PQ, rf, (float)r_inv, (float)rsqrt(ia), (float)ia,
-                                        quad + (size_t)tp * P.nr * kQuadRow, gq, fq);
+                                        quad + (size_t)tp * kQuadRow, quad_stride(NT), gq, fq);
                 for (int c = 0; c < 3; ++c) {
                     gv[c] = gq[c];
                     fv[c] = fq[c];
