@@ -32,7 +32,8 @@ enum {
     MAPP_DMD_E_STATE = -2, /* call out of order (e.g. force before init) */
     MAPP_DMD_E_HIP = -3,   /* HIP runtime error */
     MAPP_DMD_E_RANGE = -4, /* value outside what the tables / kernels cover */
-    MAPP_DMD_E_IO = -5     /* file could not be read or parsed */
+    MAPP_DMD_E_IO = -5,    /* file could not be read or parsed */
+    MAPP_DMD_E_INTERNAL = -6 /* unexpected host-side failure (message in mapp_dmd_last_error / err) */
 };
 
 /* variants differ in the mobility of c_d_calc / calc_g only */

@@ -52,6 +52,31 @@ int guarded(mapp_dmd* ctx, F&& fn) {
     } catch (const std::bad_alloc&) {
         ctx->err = "host allocation failed";
         return MAPP_DMD_E_HIP;
+    } catch (const std::exception& e) {
+        ctx->err = std::string("unexpected host error: ") + e.what();
+        return MAPP_DMD_E_INTERNAL;
+    } catch (...) {
+        ctx->err = "unexpected host error";
+        return MAPP_DMD_E_INTERNAL;
+    }
+}
+
+// the same for the entry points without a context: no exception crosses the C ABI,
+// the message goes to the caller's err buffer
+void put_err(char* err, int cap, const std::string& m);
+template <typename F>
+int guarded_noctx(char* err, int err_cap, F&& fn) {
+    try {
+        return fn();
+    } catch (const std::bad_alloc&) {
+        put_err(err, err_cap, "host allocation failed");
+        return MAPP_DMD_E_INTERNAL;
+    } catch (const std::exception& e) {
+        put_err(err, err_cap, std::string("unexpected host error: ") + e.what());
+        return MAPP_DMD_E_INTERNAL;
+    } catch (...) {
+        put_err(err, err_cap, "unexpected host error");
+        return MAPP_DMD_E_INTERNAL;
     }
 }
 
@@ -1622,30 +1647,32 @@ int mapp_dmd_fs_coef(mapp_dmd_t* ctx, const double* A, const double* t1, const d
 int mapp_dmd_fs_params_read(const char* path, int nt, const char* const* elements, double* A, double* t1, double* t2,
                             double* cut_rho, double* cut_phi, double* k1, double* k2, double* k3, char* err,
                             int err_cap) {
-    auto fail = [&](const std::string& m, int code) {
-        if (err && err_cap > 0) {
-            std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
-            err[err_cap - 1] = 0;
+    return guarded_noctx(err, err_cap, [&]() -> int {
+        auto fail = [&](const std::string& m, int code) {
+            if (err && err_cap > 0) {
+                std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
+                err[err_cap - 1] = 0;
+            }
+            return code;
+        };
+        if (!path || !elements || nt < 1 || nt > kMaxTypes) return fail("bad arguments", MAPP_DMD_E_ARG);
+        std::vector<std::string> names(elements, elements + nt);
+        FsParams fp;
+        const std::string e = read_fs_params(path, names, fp);
+        if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
+        const int nc = nt * (nt + 1) / 2;
+        std::copy(fp.A.begin(), fp.A.end(), A);
+        std::copy(fp.t1.begin(), fp.t1.end(), t1);
+        std::copy(fp.t2.begin(), fp.t2.end(), t2);
+        for (int k = 0; k < nc; ++k) {
+            cut_rho[k] = fp.cut_rho[k];
+            cut_phi[k] = fp.cut_phi[k];
+            k1[k] = fp.k1[k];
+            k2[k] = fp.k2[k];
+            k3[k] = fp.k3[k];
         }
-        return code;
-    };
-    if (!path || !elements || nt < 1 || nt > kMaxTypes) return fail("bad arguments", MAPP_DMD_E_ARG);
-    std::vector<std::string> names(elements, elements + nt);
-    FsParams fp;
-    const std::string e = read_fs_params(path, names, fp);
-    if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
-    const int nc = nt * (nt + 1) / 2;
-    std::copy(fp.A.begin(), fp.A.end(), A);
-    std::copy(fp.t1.begin(), fp.t1.end(), t1);
-    std::copy(fp.t2.begin(), fp.t2.end(), t2);
-    for (int k = 0; k < nc; ++k) {
-        cut_rho[k] = fp.cut_rho[k];
-        cut_phi[k] = fp.cut_phi[k];
-        k1[k] = fp.k1[k];
-        k2[k] = fp.k2[k];
-        k3[k] = fp.k3[k];
-    }
-    return 0;
+        return 0;
+    });
 }
 
 int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const* elements) {
@@ -1662,19 +1689,21 @@ int mapp_dmd_read_fs_params(mapp_dmd_t* ctx, const char* path, const char* const
 
 int mapp_dmd_meam_params_read(const char* library, const char* local, int nt, const char* const* elements,
                               mapp_meam_params* p, char* err, int err_cap) {
-    auto fail = [&](const std::string& m, int code) {
-        if (err && err_cap > 0) {
-            std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
-            err[err_cap - 1] = 0;
-        }
-        return code;
-    };
-    if (!library || !elements || !p || nt < 1 || nt > kMeamT) return fail("bad arguments", MAPP_DMD_E_ARG);
-    std::vector<int> type_ref;
-    std::string e = meam_host::read_library(library, nt, elements, p, &type_ref);
-    if (e.empty() && local) e = meam_host::read_local(local, type_ref, p);
-    if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
-    return 0;
+    return guarded_noctx(err, err_cap, [&]() -> int {
+        auto fail = [&](const std::string& m, int code) {
+            if (err && err_cap > 0) {
+                std::strncpy(err, m.c_str(), (size_t)err_cap - 1);
+                err[err_cap - 1] = 0;
+            }
+            return code;
+        };
+        if (!library || !elements || !p || nt < 1 || nt > kMeamT) return fail("bad arguments", MAPP_DMD_E_ARG);
+        std::vector<int> type_ref;
+        std::string e = meam_host::read_library(library, nt, elements, p, &type_ref);
+        if (e.empty() && local) e = meam_host::read_local(local, type_ref, p);
+        if (!e.empty()) return fail(e, MAPP_DMD_E_IO);
+        return 0;
+    });
 }
 
 int mapp_dmd_meam_coef(mapp_dmd_t* ctx, const mapp_meam_params* p) {
@@ -2335,53 +2364,57 @@ void put_err(char* err, int cap, const std::string& m) {
 
 int mapp_dmd_cfg_read(const char* path, int cap_atoms, int* natms, int* nt, double* H, double* x, double* c,
                       double* mass, char* names, int names_cap, char* err, int err_cap) {
-    if (!path || !natms || !nt) return MAPP_DMD_E_ARG;
-    CfgData d;
-    const std::string e = read_cfg_file(path, d);
-    if (!e.empty()) {
-        put_err(err, err_cap, e);
-        return MAPP_DMD_E_IO;
-    }
-    *natms = d.natms;
-    *nt = d.nt;
-    if (!x) return MAPP_DMD_OK;  // sizes only
-    if (cap_atoms < d.natms) {
-        put_err(err, err_cap, "buffer holds fewer atoms than the file");
-        return MAPP_DMD_E_RANGE;
-    }
-    if (H) std::memcpy(H, d.H, sizeof(d.H));
-    std::memcpy(x, d.x.data(), d.x.size() * sizeof(double));
-    if (c) std::memcpy(c, d.c.data(), d.c.size() * sizeof(double));
-    if (mass)
-        for (int k = 0; k < d.nt; ++k) mass[k] = k < (int)d.mass.size() ? d.mass[k] : 0.0;
-    if (names && names_cap > 0) {
-        std::string all;
-        for (size_t k = 0; k < d.names.size(); ++k) all += (k ? "\n" : "") + d.names[k];
-        put_err(names, names_cap, all);
-    }
-    return MAPP_DMD_OK;
+    return guarded_noctx(err, err_cap, [&]() -> int {
+        if (!path || !natms || !nt) return MAPP_DMD_E_ARG;
+        CfgData d;
+        const std::string e = read_cfg_file(path, d);
+        if (!e.empty()) {
+            put_err(err, err_cap, e);
+            return MAPP_DMD_E_IO;
+        }
+        *natms = d.natms;
+        *nt = d.nt;
+        if (!x) return MAPP_DMD_OK;  // sizes only
+        if (cap_atoms < d.natms) {
+            put_err(err, err_cap, "buffer holds fewer atoms than the file");
+            return MAPP_DMD_E_RANGE;
+        }
+        if (H) std::memcpy(H, d.H, sizeof(d.H));
+        std::memcpy(x, d.x.data(), d.x.size() * sizeof(double));
+        if (c) std::memcpy(c, d.c.data(), d.c.size() * sizeof(double));
+        if (mass)
+            for (int k = 0; k < d.nt; ++k) mass[k] = k < (int)d.mass.size() ? d.mass[k] : 0.0;
+        if (names && names_cap > 0) {
+            std::string all;
+            for (size_t k = 0; k < d.names.size(); ++k) all += (k ? "\n" : "") + d.names[k];
+            put_err(names, names_cap, all);
+        }
+        return MAPP_DMD_OK;
+    });
 }
 
 int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, const double* x, const double* c,
                        int ntypes, const double* mass, const char* const* names, char* err, int err_cap) {
-    if (!path || !H || !x || !c || natms < 0 || nt < 1 || ntypes < 0 || (ntypes && (!mass || !names)))
-        return MAPP_DMD_E_ARG;
-    CfgData d;
-    d.natms = natms;
-    d.nt = nt;
-    std::memcpy(d.H, H, sizeof(d.H));
-    d.x.assign(x, x + (size_t)natms * (3 + nt));
-    d.c.assign(c, c + (size_t)natms * nt);
-    for (int k = 0; k < ntypes; ++k) {
-        d.mass.push_back(mass[k]);
-        d.names.emplace_back(names[k]);
-    }
-    const std::string e = write_cfg_file(path, d);
-    if (!e.empty()) {
-        put_err(err, err_cap, e);
-        return MAPP_DMD_E_IO;
-    }
-    return MAPP_DMD_OK;
+    return guarded_noctx(err, err_cap, [&]() -> int {
+        if (!path || !H || !x || !c || natms < 0 || nt < 1 || ntypes < 0 || (ntypes && (!mass || !names)))
+            return MAPP_DMD_E_ARG;
+        CfgData d;
+        d.natms = natms;
+        d.nt = nt;
+        std::memcpy(d.H, H, sizeof(d.H));
+        d.x.assign(x, x + (size_t)natms * (3 + nt));
+        d.c.assign(c, c + (size_t)natms * nt);
+        for (int k = 0; k < ntypes; ++k) {
+            d.mass.push_back(mass[k]);
+            d.names.emplace_back(names[k]);
+        }
+        const std::string e = write_cfg_file(path, d);
+        if (!e.empty()) {
+            put_err(err, err_cap, e);
+            return MAPP_DMD_E_IO;
+        }
+        return MAPP_DMD_OK;
+    });
 }
 
 
