@@ -471,9 +471,10 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
 // The rho and phi lanes then carry the weighted Gauss-Hermite sums of their
 // function.
 //
-// The derivative records and partner shares stream out with nontemporal
-// stores: they are read back only after the whole pass, so they are kept from
-// displacing the spline tables in L2.
+// The derivative records stream out with nontemporal stores (whole lines, read
+// back only after the whole pass, so they are kept from displacing the spline
+// tables in L2); the partner shares, 16 bytes at scattered reverse positions,
+// use ordinary stores so that L2 merges them into lines before they leave.
 //
 // ENERGY_ONLY restates energy_calc's triangle (jtype starts at itype, so phi
 // only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
@@ -771,7 +772,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
             if constexpr (NT == 2 && PARTS == 2) ej += dpp_f64<kDppRowRor12>(ej);  // lane l + 4 for l % 8 < 4
             if constexpr (NT == 2 && PARTS == 1) ej += __shfl_xor(ej, 2, 64);
-            if (valid && rho_lane && it == 0 && j < P.natms) __builtin_nontemporal_store(ej, pairE + (size_t)rp * NT + jt);
+            // an ordinary store: the 16-byte shares land scattered, and L2 merges them into lines
+            if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rp * NT + jt] = ej;
         }
         }
 #pragma unroll

@@ -558,7 +558,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 __builtin_nontemporal_store(fv[2], d + 3);
             }
             ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
-            if (valid && it == 0 && j < P.natms) __builtin_nontemporal_store(ej, pairE + (size_t)rpos[p] * NT + jt);
+            if (valid && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;  // scattered: L2 merges lines
         }
         }
 #pragma unroll
