@@ -104,6 +104,26 @@ def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobilit
         r["calc_g_ms"] = (time.perf_counter() - t0) * 1e3 / reps
     r["sizes"] = ff.sizes()
     r["f"] = ff.get("f")
+    if mobility:
+        # Atoms::update_0 on device-resident positions: one site moved past the
+        # skin (ghosts, bins, half list, reverse list rebuilt), then small moves
+        # only (ghost positions forwarded)
+        x = ff.get("x")
+        x[7, 0] += 0.8
+        ff.set_vec("x", x)
+        t0 = time.perf_counter()
+        rebuilt = ff.set_x(None)
+        ff.sync()
+        r["list_rebuild_ms"] = (time.perf_counter() - t0) * 1e3
+        assert rebuilt
+        x = ff.get("x")  # the rebuild may have wrapped the moved site into the box
+        x[:, :3] += 1e-3
+        ff.set_vec("x", x)
+        t0 = time.perf_counter()
+        rebuilt = ff.set_x(None)
+        ff.sync()
+        r["x_update_ms"] = (time.perf_counter() - t0) * 1e3
+        assert not rebuilt
     ff.close()
     return r
 
@@ -445,6 +465,8 @@ def main():
         "c_d_calc_ms": r64.get("c_d_ms"),
         "calc_g_ms": r64.get("calc_g_ms"),
         "c_d_calc_first_ms": r64.get("c_d_first_ms"),
+        "list_rebuild_ms": r64.get("list_rebuild_ms"),
+        "x_update_ms": r64.get("x_update_ms"),
         "site_flops": site_flops(P, nt, 3),
         "site_bytes": bytes_per_site(P_list, nt),
     }

@@ -184,6 +184,10 @@ class ForceFieldDMD:
             msg = self.lib.mapp_dmd_last_error(self.ctx).decode() if not noctx and self.ctx else "create failed"
             raise DMDError(rc, msg)
 
+    def sync(self):
+        """wait for the context's stream"""
+        self._call("mapp_dmd_sync")
+
     def close(self):
         if self.ctx:
             self.lib.mapp_dmd_destroy(self.ctx)
@@ -254,9 +258,14 @@ class ForceFieldDMD:
         self._call("mapp_dmd_fin")
 
     def set_x(self, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        """Atoms::update_0 after a position change; x = None takes the positions
+        already on the device (set_vec("x", ...) or a device-resident driver)"""
         flag = C.c_int(0)
-        self._call("mapp_dmd_set_x", _dp(x), C.byref(flag))
+        if x is None:
+            self._call("mapp_dmd_set_x", None, C.byref(flag))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            self._call("mapp_dmd_set_x", _dp(x), C.byref(flag))
         return bool(flag.value)
 
     def set_c(self, c):

@@ -156,6 +156,30 @@ def test_set_x_update_and_rebuild(tables):
     assert rel_inf(ff.get("f"), orc.get("f")) <= 1e-12
 
 
+def test_set_x_device_resident(tables):
+    """update_0 on positions already in HBM (set_vec then set_x(None)): same
+    rebuild decision, ghosts and lists as the host-buffer call"""
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(6, seed=6)
+    ff, orc = _pair(lat, tables)
+    x = orc.x_all()[: lat.natms].copy()
+    x[:, :3] += np.random.default_rng(2).normal(0, 0.05, size=(lat.natms, 3))
+    ff.set_vec("x", x)
+    assert ff.set_x(None) == orc.set_x(x) == False
+    gx, _ = ff.ghosts()
+    ox, _ = orc.ghosts()
+    assert np.array_equal(gx.view(np.uint64), ox.view(np.uint64))
+    x[11, 2] -= 0.8
+    ff.set_vec("x", x)
+    assert ff.set_x(None) == orc.set_x(x) == True
+    goff, gnl = ff.neighbor_list()
+    ooff, onl = orc.neighbor_list()
+    assert np.array_equal(goff, ooff) and np.array_equal(gnl, onl)
+    assert rel_inf(ff.force_calc(1), orc.force_calc(1)) <= 1e-12
+    assert rel_inf(ff.get("f"), orc.get("f")) <= 1e-12
+    ff.close()
+
+
 def test_plain_eam_parity(tables):
     """ForceField_eam (src/ff_eam.cpp:44-356) on the GPU against the oracle's EAM:
     list bit-exact, energy / virial / forces <= 1e-12 relative"""
