@@ -85,6 +85,17 @@ void launch_check(const char* what) {
     if (e != hipSuccess) throw HipError(e, what, 0);
 }
 
+// ascending order inside every segment of v[0..n) (segments start[0..nseg]),
+// key[value] = the segment that holds the value
+void sort_segments(mapp_dmd* ctx, const int* start, const int* key, int* v, long n) {
+    if (n <= 0) return;
+    ctx->d_sort_tmp.reserve(n);
+    hipLaunchKernelGGL(k_rank_place, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, v, n, key, start,
+                       ctx->d_sort_tmp.p);
+    launch_check("k_rank_place");
+    HIP_OK(hipMemcpyAsync(v, ctx->d_sort_tmp.p, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
 // exclusive scan of n ints; returns the total (synchronizes the stream)
 long exclusive_scan(mapp_dmd* ctx, const int* in, int* out, long n, int depth = 0) {
     if (n <= 0) return 0;
@@ -257,9 +268,7 @@ void rebuild(mapp_dmd* ctx) {
     hipLaunchKernelGGL(k_bin_fill, dim3(nblk(ntot)), dim3(kBlock), 0, st, ctx->d_bin_of.p, ntot,
                        ctx->d_bin_start.p, ctx->d_bin_cnt.p, ctx->d_bin_atoms.p);
     launch_check("k_bin_fill");
-    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(bp.tot_bin)), dim3(kBlock), 0, st, ctx->d_bin_start.p,
-                       bp.tot_bin, ctx->d_bin_atoms.p);
-    launch_check("k_segment_sort");
+    sort_segments(ctx, ctx->d_bin_start.p, ctx->d_bin_of.p, ctx->d_bin_atoms.p, ntot);
     // Neighbor::bin_atoms_s ends with s2x of every atom
     hipLaunchKernelGGL(k_s2x, dim3(nblk(ntot)), dim3(kBlock), 0, st, ctx->d_pos.p, ntot, bp);
     launch_check("k_s2x");
@@ -298,9 +307,7 @@ void finish_lists(mapp_dmd* ctx) {
     hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
                        ctx->d_roff.p, ctx->d_cnt.p, ctx->d_rev.p);
     launch_check("k_rev_fill");
-    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff.p, natms,
-                       ctx->d_rev.p);
-    launch_check("k_segment_sort rev");
+    sort_segments(ctx, ctx->d_roff.p, ctx->d_nl.p, ctx->d_rev.p, nrev);
     ctx->d_pair_i.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_pair_i.p);
     launch_check("k_pair_owner");
@@ -886,8 +893,7 @@ void build_list2(mapp_dmd* ctx) {
     hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
                        ctx->d_roff2.p, ctx->d_cnt.p, ctx->d_rev2.p);
     launch_check("k_rev_fill 2");
-    hipLaunchKernelGGL(k_segment_sort, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff2.p, natms, ctx->d_rev2.p);
-    launch_check("k_segment_sort 2");
+    sort_segments(ctx, ctx->d_roff2.p, ctx->d_nl2.p, ctx->d_rev2.p, nrev);
     ctx->d_pair_i2.reserve(ctx->npairs2 + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, natms, ctx->d_pair_i2.p);
     launch_check("k_pair_owner 2");

@@ -52,6 +52,7 @@ struct mapp_dmd {
     mappdmd::DevBuf<int> d_flag, d_offs, d_bin_of, d_bin_cnt, d_bin_start, d_bin_atoms;
     mappdmd::DevBuf<int> d_cnt, d_off, d_nl, d_roff, d_rev;
     mappdmd::DevBuf<int> d_off2, d_nl2, d_roff2, d_rev2, d_crd;
+    mappdmd::DevBuf<int> d_sort_tmp;  // out-of-place segment sort
     mappdmd::DevBuf<int> d_pair_i, d_pair_i2, d_rpos;
     mappdmd::DevBuf<double> d_inva, d_deriv;
     mappdmd::DevBuf<double4> d_arec;  // [ntot][nt] {alpha, 1/alpha, c, dE}  // deriv: [npairs][nt*nt][4] of the quadrature type

@@ -192,21 +192,24 @@ __global__ void k_bin_fill(const int* __restrict__ bin_of, int n, const int* __r
     bin_atoms[bin_start[b] + k] = i;
 }
 
-// ascending atom index inside each bin, as the reference's linked bins
-// (built by pushing atoms from the highest index down, src/neighbor.cpp:428-434)
-__global__ void k_segment_sort(const int* __restrict__ start, int nseg, int* __restrict__ v) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nseg) return;
-    int lo = start[b], hi = start[b + 1];
-    for (int q = lo + 1; q < hi; ++q) {
-        int key = v[q];
-        int w = q - 1;
-        while (w >= lo && v[w] > key) {
-            v[w + 1] = v[w];
-            --w;
-        }
-        v[w + 1] = key;
-    }
+// Ascending order inside every segment (atom indices inside each bin, as the
+// reference's linked bins built by pushing atoms from the highest index down,
+// src/neighbor.cpp:428-434; pair indices inside each reverse-list segment).
+// One thread per value: value x of segment s = key[x] goes to
+// out[start[s] + r], r = the number of the segment's values below x.  The
+// values of a segment are distinct, so the ranks are a permutation of the
+// segment and every write stays inside it; the segment is read through the
+// caches instead of being shifted entry by entry in memory.
+__global__ void k_rank_place(const int* __restrict__ v, long n, const int* __restrict__ key,
+                             const int* __restrict__ start, int* __restrict__ out) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int x = v[q];
+    const int s = key[x];
+    const int lo = start[s], hi = start[s + 1];
+    int r = 0;
+    for (int t = lo; t < hi; ++t) r += v[t] < x ? 1 : 0;
+    out[lo + r] = x;
 }
 
 // ---- half neighbor list ----------------------------------------------------
