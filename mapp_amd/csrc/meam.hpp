@@ -145,10 +145,21 @@ __device__ __forceinline__ void meam_stage(const MeamStage& S, const double4* __
     __builtin_amdgcn_wave_barrier();
 }
 
-// screening S_ij over the site's list, with the cutoff taper (:1171-1266)
+// screening candidates of one lane per round of meam_screen (bytes in LDS,
+// lane-interleaved: slot c of lane l at c * 64 + l)
+constexpr int kMeamCand = 32;
+
+// screening S_ij over the site's list, with the cutoff taper (:1171-1266).
+// Each round first walks the candidates k with the two cheap ellipse tests and
+// keeps the list positions of those inside both (up to kMeamCand, in list
+// order, in this lane's column of cand), then runs the screening arithmetic
+// on those only: the lanes of a wavefront no longer execute the divisions and
+// the taper for every candidate that any one of them needs.  The candidates
+// reach the product and the sum in the same order as in a single walk.
 __device__ __forceinline__ void meam_screen(const MeamDev& M, const MeamStage& S, int n, int j, int it, int jt,
                                             double xi0, double xi1, double xi2, double4 xj, double rij2, double rij,
-                                            double* fc_o, double* s_o, double* ds_o) {
+                                            unsigned char* __restrict__ cand, double* fc_o, double* s_o,
+                                            double* ds_o) {
     if (!(rij < M.rc)) {
         *fc_o = *s_o = *ds_o = 0.0;
         return;
@@ -158,39 +169,52 @@ __device__ __forceinline__ void meam_screen(const MeamDev& M, const MeamStage& S
     const double rbound = M.ebound[it][jt] * rij2, rij4 = rij2 * rij2;
     const double fcij = meam_dfcut(rnorm, &dfcij);
     dfcij *= M.delr_inv;
-    for (int kn = 0; kn < n && sij != 0.0; ++kn) {
-        if (S.idx[kn] == j) continue;
-        // both distances must lie inside the ellipse bound; the staged one first
-        const double rki2 = S.r2[kn];
-        if (!(rki2 < rbound)) continue;
-        const double4 xk = S.nb[kn];
-        const int kt = (int)xk.w;
-        const double xjk = xj.x - xk.x, yjk = xj.y - xk.y, zjk = xj.z - xk.z;
-        double rjk2;
-        {
+    for (int kn0 = 0; kn0 < n && sij != 0.0;) {
+        int nc = 0;
+        for (; kn0 < n && nc < kMeamCand; ++kn0) {
+            if (S.idx[kn0] == j) continue;
+            // both distances must lie inside the ellipse bound; the staged one first
+            if (!(S.r2[kn0] < rbound)) continue;
+            const double4 xk = S.nb[kn0];
+            const double xjk = xj.x - xk.x, yjk = xj.y - xk.y, zjk = xj.z - xk.z;
+            double rjk2;
+            {
 #pragma clang fp contract(off)
-            rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
+                rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
+            }
+            if (rjk2 < rbound) cand[(nc++) * 64] = (unsigned char)kn0;
         }
-        if (!(rjk2 < rbound)) continue;
-        const double x_ki = rki2 / rij2, x_jk = rjk2 / rij2;
-        double a = 1.0 - (x_ki - x_jk) * (x_ki - x_jk);
-        if (!(a > 0)) continue;
-        double cikj = (2.0 * (x_ki + x_jk) + a - 2.0) / a;
-        const double cmax = M.cmax[it][jt][kt], cmin = M.cmin[it][jt][kt];
-        if (cikj < cmax && cikj > cmin) {
-            cikj = (cikj - cmin) / (cmax - cmin);
-            double dfikj;
-            const double sikj = meam_dfcut(cikj, &dfikj);
-            sij *= sikj;
-            const double coef1 = dfikj / ((cmax - cmin) * sikj);
-            a = rjk2 - rki2;
-            const double b = rki2 + rjk2;
-            double den = rij4 - a * a;
-            den *= den;
-            dsij += coef1 * (-4.0 * (-2.0 * rij2 * a * a + rij4 * b + a * a * b) / den);
-        } else if (cikj < cmin) {
-            sij = 0.0;
-            dsij = 0.0;
+        for (int c = 0; c < nc && sij != 0.0; ++c) {
+            const int kn = cand[c * 64];
+            const double rki2 = S.r2[kn];
+            const double4 xk = S.nb[kn];
+            const int kt = (int)xk.w;
+            const double xjk = xj.x - xk.x, yjk = xj.y - xk.y, zjk = xj.z - xk.z;
+            double rjk2;
+            {
+#pragma clang fp contract(off)
+                rjk2 = xjk * xjk + yjk * yjk + zjk * zjk;
+            }
+            const double x_ki = rki2 / rij2, x_jk = rjk2 / rij2;
+            double a = 1.0 - (x_ki - x_jk) * (x_ki - x_jk);
+            if (!(a > 0)) continue;
+            double cikj = (2.0 * (x_ki + x_jk) + a - 2.0) / a;
+            const double cmax = M.cmax[it][jt][kt], cmin = M.cmin[it][jt][kt];
+            if (cikj < cmax && cikj > cmin) {
+                cikj = (cikj - cmin) / (cmax - cmin);
+                double dfikj;
+                const double sikj = meam_dfcut(cikj, &dfikj);
+                sij *= sikj;
+                const double coef1 = dfikj / ((cmax - cmin) * sikj);
+                a = rjk2 - rki2;
+                const double b = rki2 + rjk2;
+                double den = rij4 - a * a;
+                den *= den;
+                dsij += coef1 * (-4.0 * (-2.0 * rij2 * a * a + rij4 * b + a * a * b) / den);
+            } else if (cikj < cmin) {
+                sij = 0.0;
+                dsij = 0.0;
+            }
         }
     }
     const double coef1 = sij * fcij, coef2 = sij * dfcij / rij;
@@ -215,6 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_meam_density(MeamDev M, const double
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * kMeamWaves + w;
     const MeamStage S(w);
+    __shared__ unsigned char s_cand[kMeamWaves][kMeamCand * 64];
     double en = 0.0;
     if (i < natms) {
         const int p0 = off[i], n = off[i + 1] - p0;
@@ -236,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_meam_density(MeamDev M, const double
             }
             const double rij = sqrt(rij2);
             double fc, s, ds;
-            meam_screen(M, S, n, j, it, jt, xi.x, xi.y, xi.z, xj, rij2, rij, &fc, &s, &ds);
+            meam_screen(M, S, n, j, it, jt, xi.x, xi.y, xi.z, xj, rij2, rij, &s_cand[w][lane], &fc, &s, &ds);
             pairs[p0 + jn] = make_double4(fc, s, ds, 0.0);
             if (s * fc != 0.0) {
                 const double sij = s * fc;
