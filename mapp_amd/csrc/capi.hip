@@ -274,13 +274,13 @@ void rebuild(mapp_dmd* ctx) {
     launch_check("k_s2x");
     // half list
     ctx->d_cnt.reserve(natms + 1);
-    hipLaunchKernelGGL((k_half_list<false>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
+    hipLaunchKernelGGL((k_half_list<false>), dim3(nblk((long)natms * 64)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
                        ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_conc.p, ctx->d_cnt.p,
                        (int*)nullptr);
     launch_check("k_half_list count");
     ctx->npairs = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_off, natms);
     ctx->d_nl.reserve(ctx->npairs + 1);
-    hipLaunchKernelGGL((k_half_list<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
+    hipLaunchKernelGGL((k_half_list<true>), dim3(nblk((long)natms * 64)), dim3(kBlock), 0, st, ctx->d_pos.p, natms,
                        ctx->d_bin_of.p, ctx->d_bin_start.p, ctx->d_bin_atoms.p, bp, ctx->d_conc.p, ctx->d_off.p,
                        ctx->d_nl.p);
     launch_check("k_half_list fill");

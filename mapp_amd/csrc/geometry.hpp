@@ -215,7 +215,12 @@ __global__ void k_rank_place(const int* __restrict__ v, long n, const int* __res
 // ---- half neighbor list ----------------------------------------------------
 // stencil order of find_bin_no: offset of dimension 0 outermost, 2 innermost;
 // inside a bin ascending index; j > i (every j != i with bp.full, the full list
-// of src/neighbor.cpp:240-284); rsq = ((d0*d0) + d1*d1) + d2*d2 < cut
+// of src/neighbor.cpp:240-284); rsq = ((d0*d0) + d1*d1) + d2*d2 < cut.
+// One wavefront per site: lane s < 27 takes stencil bin s (s = 9(a+1) + 3(b+1)
+// + c+1 for offsets a, b, c), a prefix sum of the bin sizes concatenates the 27
+// bins in stencil order, and 64 consecutive candidates are tested per round;
+// ballot ranks write the accepted ones in candidate order, which is the list
+// order of a single thread walking the stencil.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ pos, int natms,
                                                    const int* __restrict__ bin_of,
@@ -223,8 +228,10 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
                                                    const int* __restrict__ bin_atoms, BoxParams bp,
                                                    const double* __restrict__ type,
                                                    int* __restrict__ cnt_or_off, int* __restrict__ nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natms) return;
+    const int lane = threadIdx.x & 63;
+    const long site = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (site >= natms) return;  // the whole wavefront
+    const int i = (int)site;
     const int ti = bp.pairwise ? (int)type[i] : 0;
     int tmp = bin_of[i];
     int loc[3];
@@ -232,38 +239,57 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
         loc[d] = tmp / bp.denom[d];
         tmp -= loc[d] * bp.denom[d];
     }
-    const double4 xi = pos[i];
-    int w = FILL ? cnt_or_off[i] : 0;
-    for (int a = -1; a <= 1; ++a) {
-        int t0 = loc[0] + a;
-        if (t0 < 0 || t0 > bp.grid[0] - 1) continue;
-        for (int b = -1; b <= 1; ++b) {
-            int t1 = loc[1] + b;
-            if (t1 < 0 || t1 > bp.grid[1] - 1) continue;
-            for (int c = -1; c <= 1; ++c) {
-                int t2 = loc[2] + c;
-                if (t2 < 0 || t2 > bp.grid[2] - 1) continue;
-                int no = t0 * bp.denom[0] + t1 * bp.denom[1] + t2 * bp.denom[2];
-                int q1 = bin_start[no + 1];
-                for (int q = bin_start[no]; q < q1; ++q) {
-                    int j = bin_atoms[q];
-                    if (bp.full ? j == i : j <= i) continue;
-                    double4 xj = pos[j];
-                    double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-                    double rsq = 0.0;
-                    rsq += d0 * d0;
-                    rsq += d1 * d1;
-                    rsq += d2 * d2;
-                    const double cut = bp.pairwise ? bp.cut_sk_sq_tp[md_comp(ti, (int)type[j])] : bp.cut_sk_sq;
-                    if (rsq < cut) {
-                        if (FILL) nl[w] = j;
-                        ++w;
-                    }
-                }
-            }
+    int bstart = 0, bsize = 0;
+    if (lane < 27) {
+        const int t0 = loc[0] + lane / 9 - 1, t1 = loc[1] + (lane / 3) % 3 - 1, t2 = loc[2] + lane % 3 - 1;
+        if (t0 >= 0 && t0 < bp.grid[0] && t1 >= 0 && t1 < bp.grid[1] && t2 >= 0 && t2 < bp.grid[2]) {
+            const int no = t0 * bp.denom[0] + t1 * bp.denom[1] + t2 * bp.denom[2];
+            bstart = bin_start[no];
+            bsize = bin_start[no + 1] - bstart;
         }
     }
-    if (!FILL) cnt_or_off[i] = w;
+    int incl = bsize;  // inclusive prefix over the stencil bins
+    for (int d = 1; d < 32; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int excl = incl - bsize;
+    const int total = __shfl(incl, 26, 64);
+    const double4 xi = pos[i];
+    int w = FILL ? cnt_or_off[i] : 0;
+    for (int base = 0; base < total; base += 64) {
+        const int q = base + lane;
+        // stencil bin of candidate q: the number of bins whose inclusive prefix is <= q
+        int s = 0;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1) {
+            const int probe = s + step - 1;
+            const int v = __shfl(incl, probe < 27 ? probe : 26, 64);
+            if (probe < 27 && v <= q) s += step;
+        }
+        s = s < 26 ? s : 26;
+        const int bs = __shfl(bstart, s, 64), ex = __shfl(excl, s, 64);
+        bool take = false;
+        int j = 0;
+        if (q < total) {
+            j = bin_atoms[bs + (q - ex)];
+            if (bp.full ? j != i : j > i) {
+                const double4 xj = pos[j];
+                double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+                double rsq = 0.0;
+                rsq += d0 * d0;
+                rsq += d1 * d1;
+                rsq += d2 * d2;
+                const double cut = bp.pairwise ? bp.cut_sk_sq_tp[md_comp(ti, (int)type[j])] : bp.cut_sk_sq;
+                take = rsq < cut;
+            }
+        }
+        const unsigned long long m = __ballot(take);
+        if (FILL && take)
+            nl[w + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = j;
+        w += __popcll(m);
+    }
+    if (!FILL && lane == 0) cnt_or_off[i] = w;
 }
 
 // ---- reverse CSR: for owned j, the list positions p whose partner is j ----
