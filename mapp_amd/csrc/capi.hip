@@ -86,12 +86,13 @@ void launch_check(const char* what) {
 }
 
 // ascending order inside every segment of v[0..n) (segments start[0..nseg]),
-// key[value] = the segment that holds the value
-void sort_segments(mapp_dmd* ctx, const int* start, const int* key, int* v, long n) {
+// key[value] = the segment that holds the value; pos_of (optional) receives the
+// final position of every value
+void sort_segments(mapp_dmd* ctx, const int* start, const int* key, int* v, long n, int* pos_of = nullptr) {
     if (n <= 0) return;
     ctx->d_sort_tmp.reserve(n);
     hipLaunchKernelGGL(k_rank_place, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, v, n, key, start,
-                       ctx->d_sort_tmp.p);
+                       ctx->d_sort_tmp.p, pos_of);
     launch_check("k_rank_place");
     HIP_OK(hipMemcpyAsync(v, ctx->d_sort_tmp.p, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
 }
@@ -307,14 +308,13 @@ void finish_lists(mapp_dmd* ctx) {
     hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
                        ctx->d_roff.p, ctx->d_cnt.p, ctx->d_rev.p);
     launch_check("k_rev_fill");
-    sort_segments(ctx, ctx->d_roff.p, ctx->d_nl.p, ctx->d_rev.p, nrev);
+    // reverse positions of the pairs with an owned partner (rpos) come with the ordering
+    ctx->d_rpos.reserve(ctx->npairs + 1);
+    sort_segments(ctx, ctx->d_roff.p, ctx->d_nl.p, ctx->d_rev.p, nrev, ctx->d_rpos.p);
     ctx->d_pair_i.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_pair_i.p);
     launch_check("k_pair_owner");
     ctx->nrev = nrev;
-    ctx->d_rpos.reserve(ctx->npairs + 1);
-    hipLaunchKernelGGL(k_rpos, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_roff.p, ctx->d_rev.p, natms, ctx->d_rpos.p);
-    launch_check("k_rpos");
     // element read by the reference's dE[jatm + jtype] quirk: jatm + jtype (one cell)
     ctx->d_qidx.reserve((size_t)ntot * NT);
     hipLaunchKernelGGL(k_quirk_identity, dim3(nblk((long)ntot * NT)), dim3(kBlock), 0, st, ntot, NT, ctx->d_qidx.p);

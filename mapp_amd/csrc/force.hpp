@@ -1028,12 +1028,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     block_store_waves<6>(vir, part);
 }
 
-// reverse-CSR position of each pair whose partner is owned
-__global__ void k_rpos(const int* __restrict__ roff, const int* __restrict__ rev, int natms, int* __restrict__ rpos) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= natms) return;
-    for (int q = roff[j]; q < roff[j + 1]; ++q) rpos[rev[q]] = q;
-}
 
 // ---------------------------------------------------------------- force gather
 template <int NT>

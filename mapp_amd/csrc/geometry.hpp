@@ -199,9 +199,10 @@ __global__ void k_bin_fill(const int* __restrict__ bin_of, int n, const int* __r
 // out[start[s] + r], r = the number of the segment's values below x.  The
 // values of a segment are distinct, so the ranks are a permutation of the
 // segment and every write stays inside it; the segment is read through the
-// caches instead of being shifted entry by entry in memory.
+// caches instead of being shifted entry by entry in memory.  With pos_of, also
+// pos_of[x] = the position x lands at (the reverse position of each pair).
 __global__ void k_rank_place(const int* __restrict__ v, long n, const int* __restrict__ key,
-                             const int* __restrict__ start, int* __restrict__ out) {
+                             const int* __restrict__ start, int* __restrict__ out, int* __restrict__ pos_of) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int x = v[q];
@@ -210,6 +211,7 @@ __global__ void k_rank_place(const int* __restrict__ v, long n, const int* __res
     int r = 0;
     for (int t = lo; t < hi; ++t) r += v[t] < x ? 1 : 0;
     out[lo + r] = x;
+    if (pos_of) pos_of[x] = lo + r;
 }
 
 // ---- half neighbor list ----------------------------------------------------
