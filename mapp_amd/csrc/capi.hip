@@ -299,13 +299,13 @@ void finish_lists(mapp_dmd* ctx) {
     ctx->d_cnt.reserve(natms + 1);
     // reverse CSR (partner j owned)
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
-    hipLaunchKernelGGL(k_rev_count, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
+    hipLaunchKernelGGL(k_rev_count, dim3(nblk((long)natms * kRevG)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
                        ctx->d_cnt.p);
     launch_check("k_rev_count");
     long nrev = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_roff, natms);
     ctx->d_rev.reserve(nrev + 1);
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
-    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
+    hipLaunchKernelGGL(k_rev_fill, dim3(nblk((long)natms * kRevG)), dim3(kBlock), 0, st, ctx->d_off.p, ctx->d_nl.p, natms,
                        ctx->d_roff.p, ctx->d_cnt.p, ctx->d_rev.p);
     launch_check("k_rev_fill");
     // reverse positions of the pairs with an owned partner (rpos) come with the ordering
@@ -884,13 +884,13 @@ void build_list2(mapp_dmd* ctx) {
                        ctx->d_nl.p, ctx->rsq_crd, ctx->d_off2.p, ctx->d_nl2.p);
     launch_check("k_list2 fill");
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
-    hipLaunchKernelGGL(k_rev_count, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
+    hipLaunchKernelGGL(k_rev_count, dim3(nblk((long)natms * kRevG)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
                        ctx->d_cnt.p);
     launch_check("k_rev_count 2");
     long nrev = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_roff2, natms);
     ctx->d_rev2.reserve(nrev + 1);
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));
-    hipLaunchKernelGGL(k_rev_fill, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
+    hipLaunchKernelGGL(k_rev_fill, dim3(nblk((long)natms * kRevG)), dim3(kBlock), 0, st, ctx->d_off2.p, ctx->d_nl2.p, natms,
                        ctx->d_roff2.p, ctx->d_cnt.p, ctx->d_rev2.p);
     launch_check("k_rev_fill 2");
     sort_segments(ctx, ctx->d_roff2.p, ctx->d_nl2.p, ctx->d_rev2.p, nrev);

@@ -295,24 +295,29 @@ __global__ __launch_bounds__(256) void k_half_list(const double4* __restrict__ p
 }
 
 // ---- reverse CSR: for owned j, the list positions p whose partner is j ----
+// kRevG lanes per site over its list entries; the fill order inside a segment
+// is arbitrary (atomics) and is put in ascending pair order by k_rank_place
+constexpr int kRevG = 16;
 __global__ void k_rev_count(const int* __restrict__ off, const int* __restrict__ nl, int natms,
                             int* __restrict__ rcnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kRevG), l = (int)(t % kRevG);
     if (i >= natms) return;
-    for (int p = off[i]; p < off[i + 1]; ++p) {
-        int j = nl[p];
+    for (int p = off[i] + l; p < off[i + 1]; p += kRevG) {
+        const int j = nl[p];
         if (j < natms) atomicAdd(&rcnt[j], 1);
     }
 }
 
 __global__ void k_rev_fill(const int* __restrict__ off, const int* __restrict__ nl, int natms,
                            const int* __restrict__ rstart, int* __restrict__ rcur, int* __restrict__ rev) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kRevG), l = (int)(t % kRevG);
     if (i >= natms) return;
-    for (int p = off[i]; p < off[i + 1]; ++p) {
-        int j = nl[p];
+    for (int p = off[i] + l; p < off[i + 1]; p += kRevG) {
+        const int j = nl[p];
         if (j < natms) {
-            int k = atomicAdd(&rcur[j], 1);
+            const int k = atomicAdd(&rcur[j], 1);
             rev[rstart[j] + k] = p;
         }
     }
