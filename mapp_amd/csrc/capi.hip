@@ -875,12 +875,12 @@ void build_list2(mapp_dmd* ctx) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
     ctx->d_cnt.reserve(natms + 1);
-    hipLaunchKernelGGL((k_list2<false>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
+    hipLaunchKernelGGL((k_list2<false>), dim3(nblk((long)natms * 64)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
                        ctx->d_nl.p, ctx->rsq_crd, ctx->d_cnt.p, (int*)nullptr);
     launch_check("k_list2 count");
     ctx->npairs2 = counts_to_offsets(ctx, ctx->d_cnt.p, ctx->d_off2, natms);
     ctx->d_nl2.reserve(ctx->npairs2 + 1);
-    hipLaunchKernelGGL((k_list2<true>), dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
+    hipLaunchKernelGGL((k_list2<true>), dim3(nblk((long)natms * 64)), dim3(kBlock), 0, st, ctx->d_pos.p, natms, ctx->d_off.p,
                        ctx->d_nl.p, ctx->rsq_crd, ctx->d_off2.p, ctx->d_nl2.p);
     launch_check("k_list2 fill");
     HIP_OK(hipMemsetAsync(ctx->d_cnt.p, 0, sizeof(int) * (natms + 1), st));

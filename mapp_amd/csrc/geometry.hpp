@@ -323,26 +323,37 @@ __global__ void k_rev_fill(const int* __restrict__ off, const int* __restrict__ 
     }
 }
 
-// sub-list of the half list with rsq < rsq_crd (create_2nd_neigh_lst)
+// sub-list of the half list with rsq < rsq_crd (create_2nd_neigh_lst); one
+// wavefront per site, 64 list entries per round, ballot ranks keep the list order
 template <bool FILL>
-__global__ void k_list2(const double4* __restrict__ pos, int natms, const int* __restrict__ off,
-                        const int* __restrict__ nl, double rsq_crd, int* __restrict__ cnt_or_off,
-                        int* __restrict__ nl2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natms) return;
+__global__ __launch_bounds__(256) void k_list2(const double4* __restrict__ pos, int natms,
+                                               const int* __restrict__ off, const int* __restrict__ nl,
+                                               double rsq_crd, int* __restrict__ cnt_or_off,
+                                               int* __restrict__ nl2) {
+    const int lane = threadIdx.x & 63;
+    const long site = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (site >= natms) return;  // the whole wavefront
+    const int i = (int)site;
     const double4 xi = pos[i];
     int w = FILL ? cnt_or_off[i] : 0;
-    for (int p = off[i]; p < off[i + 1]; ++p) {
-        int j = nl[p];
-        double4 xj = pos[j];
-        double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-        double rsq = d0 * d0 + d1 * d1 + d2 * d2;
-        if (rsq < rsq_crd) {
-            if (FILL) nl2[w] = j;
-            ++w;
+    const int p1 = off[i + 1];
+    for (int p0 = off[i]; p0 < p1; p0 += 64) {
+        const int p = p0 + lane;
+        bool take = false;
+        int j = 0;
+        if (p < p1) {
+            j = nl[p];
+            const double4 xj = pos[j];
+            double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
+            double rsq = d0 * d0 + d1 * d1 + d2 * d2;
+            take = rsq < rsq_crd;
         }
+        const unsigned long long m = __ballot(take);
+        if (FILL && take)
+            nl2[w + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = j;
+        w += __popcll(m);
     }
-    if (!FILL) cnt_or_off[i] = w;
+    if (!FILL && lane == 0) cnt_or_off[i] = w;
 }
 
 // coordination: own 2nd-list size + number of lower-index owners listing i
