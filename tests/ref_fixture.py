@@ -1,0 +1,49 @@
+"""Loading the golden outputs of the reference's own force field
+(tests/golden/ref_dmd_*.npz, written by tests/golden/make_ref_fixtures.py from
+the reference compiled by oracle/Makefile.ref) and rebuilding their inputs.
+
+Each fixture holds the tables, box, positions / widths / concentrations as the
+reference read them, calc_g's a and beta, then per rule q (3, 5) the geometry
+(q{q}_x_all, q{q}_list_off, q{q}_list, q{q}_list2_off, q{q}_list2, q{q}_crd)
+and per variant and rule the results ({variant}_q{q}_en_st, _f, _E_force,
+_dE_force, _energy_calc, _c_d, _n_cd, _E_cd, _dE_cd, _cost0, _g, _cost1)."""
+import glob
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# (rule, variant) pairs each fixture holds
+RUNS = ((3, "DMD"), (3, "DMD_I"), (3, "DMD_II"), (5, "DMD_II"))
+
+
+def fixture_paths():
+    return sorted(glob.glob(os.path.join(GOLDEN, "ref_dmd_*.npz")))
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def tables(d):
+    from mapp_amd.workload import FSTables
+    nrho, drho, nr, dr = d["grid"]
+    return FSTables(elements=["Ni", "Cu"], nrho=int(nrho), drho=float(drho), nr=int(nr), dr=float(dr),
+                    F=d["F"], rho=d["rho"], rphi=d["rphi"], masses=list(d["masses"]))
+
+
+def lattice(d):
+    from mapp_amd.workload import Lattice
+    x = np.ascontiguousarray(d["x_read"])
+    return Lattice(H=np.ascontiguousarray(d["H"]), x=x, c=np.ascontiguousarray(d["c_read"]), cart=None)
+
+
+def coef(d):
+    r_crd, alpha_min, alpha_max, T, skin = (float(v) for v in d["coef"])
+    return dict(r_crd=r_crd, alpha_min=alpha_min, alpha_max=alpha_max, T=T), skin
+
+
+def csr(d, q, name):
+    off = d[f"q{q}_{name}_off"]
+    return off, d[f"q{q}_{name}"]

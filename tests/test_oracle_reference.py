@@ -1,0 +1,167 @@
+"""The oracle against the reference's own force field, bit for bit.
+
+The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
+_DMD_I and _DMD_II, compiled from the reference's sources, produced for two
+500-site binary systems with two types: per-(site, type) widths (the
+density-slot transposition of src/ff_dmd_I.cpp:206 vs :376 and the shared phi
+slot do not cancel), asymmetric density tables rho_ab != rho_ba, the
+dE[jatm+jtype] index of :2336 / :2515, energy_calc's type triangle (:508), and
+in the second system widths in all three regimes of :178-180 / :261-310 with
+concentrations on the edges of calc_ent's margin (:2230-2237).
+
+Ghost positions, both neighbour lists and crd must be identical, and so must
+every array of the force field: the oracle restates the reference's operation
+order and both are built with -ffp-contract=off against the same libm, so
+they agree bit for bit.  A dropped term, wrong index or transposed slot moves
+them by far more than rounding (checked below by perturbing the oracle's
+inputs in ways those mistakes would).
+
+Where the reference build is present (this container), the last test also
+runs it live on a fresh system and compares again."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from tests import ref_fixture as R
+from tests.conftest import rel_inf
+
+PATHS = R.fixture_paths()
+
+
+def _oracle(d, q, variant):
+    from oracle import oracle as O
+    t = R.tables(d)
+    lat = R.lattice(d)
+    kw, skin = R.coef(d)
+    o = O.Oracle(nt=2, variant=variant)
+    o.set_fs(t)
+    o.set_params(no_i=q, masses=t.masses, **kw)
+    o.set_system(lat.H, lat.x, lat.c)
+    o.init(skin)
+    o.set_fresh_caches(True)
+    return o
+
+
+def _run_sequence(o, d):
+    """the harness's sequence: 2nd list, force_calc(1), energy_calc, c_d_calc, calc_g(0), calc_g(1)"""
+    out = {}
+    o.create_2nd_neigh_lst()
+    out["en_st"] = o.force_calc(1)
+    out["f"] = o.get("f")
+    out["E_force"] = o.get("E")
+    out["dE_force"] = o.get("dE")
+    out["energy_calc"] = o.energy_calc()
+    o.c_d_calc()
+    for k, v in (("c_d", "c_d"), ("n_cd", "n"), ("E_cd", "E"), ("dE_cd", "dE")):
+        out[k] = o.get(v)
+    beta = float(d["beta"])
+    out["cost0"], out["g"] = o.calc_g(0, beta, d["a"])
+    out["cost1"], _ = o.calc_g(1, beta, d["a"])
+    return out
+
+
+COMPARED = ("en_st", "f", "E_force", "dE_force", "energy_calc", "c_d", "n_cd", "E_cd", "dE_cd", "cost0", "g",
+            "cost1")
+
+
+@pytest.mark.parametrize("path", PATHS, ids=[os.path.basename(p) for p in PATHS])
+@pytest.mark.parametrize("q,variant", R.RUNS)
+def test_oracle_matches_reference(path, q, variant):
+    d = R.load(path)
+    o = _oracle(d, q, variant)
+    n = o.natms
+    # geometry: ghosts, half list, 2nd list, crd
+    xa = o.x_all()
+    assert np.array_equal(xa.view(np.uint64), d[f"q{q}_x_all"].view(np.uint64))
+    off, nl = o.neighbor_list()
+    roff, rnl = R.csr(d, q, "list")
+    assert np.array_equal(off, roff) and np.array_equal(nl, rnl)
+    out = _run_sequence(o, d)
+    off2, nl2 = o.neighbor_list2()
+    roff2, rnl2 = R.csr(d, q, "list2")
+    assert np.array_equal(off2, roff2) and np.array_equal(nl2, rnl2)
+    assert np.array_equal(o.crd(), d[f"q{q}_crd"])
+    worst = {}
+    for k in COMPARED:
+        ref = np.asarray(d[f"{variant}_q{q}_{k}"], dtype=np.float64).reshape(-1)
+        got = np.asarray(out[k], dtype=np.float64).reshape(-1)
+        assert got.shape == ref.shape, k
+        worst[k] = (rel_inf(got, ref), int(np.count_nonzero(got != ref)))
+    print(os.path.basename(path), variant, q, {k: f"{e:.1e}/{m}" for k, (e, m) in worst.items()}, "natms", n,
+          flush=True)
+    for k, (e, m) in worst.items():
+        assert m == 0, (k, e, m)
+
+
+@pytest.mark.parametrize("path", PATHS[:1], ids=[os.path.basename(p) for p in PATHS[:1]])
+def test_plausible_mistakes_fail_the_bar(path):
+    """what a plausible restatement mistake produces is far from rounding:
+    swapping the density slots (a transposed index), one width per site
+    instead of one per (site, type) (what would hide the slot transposition)
+    or another variant's mobility moves forces or rates by more than 1e-6"""
+    d = R.load(path)
+    ref_f = d["DMD_I_q3_f"]
+    ref_cd = d["DMD_I_q3_c_d"]
+    # transposed density slots: rho[a][b] <-> rho[b][a]
+    t = R.tables(d)
+    t.rho = np.ascontiguousarray(np.swapaxes(t.rho, 0, 1))
+    from oracle import oracle as O
+    lat = R.lattice(d)
+    kw, skin = R.coef(d)
+    o = O.Oracle(nt=2, variant="DMD_I")
+    o.set_fs(t)
+    o.set_params(no_i=3, masses=t.masses, **kw)
+    o.set_system(lat.H, lat.x, lat.c)
+    o.init(skin)
+    o.force_calc(1)
+    assert rel_inf(o.get("f"), ref_f) > 1e-6
+    # the DMD_II mobility on DMD_I's data
+    o2 = _oracle(d, 3, "DMD_II")
+    o2.create_2nd_neigh_lst()
+    o2.force_calc(1)
+    o2.c_d_calc()
+    assert rel_inf(o2.get("c_d"), ref_cd) > 1e-3
+    # one width per site instead of one per (site, type)
+    lat2 = R.lattice(d)
+    lat2.x[:, 4] = lat2.x[:, 3]
+    o3 = O.Oracle(nt=2, variant="DMD_I")
+    o3.set_fs(R.tables(d))
+    o3.set_params(no_i=3, masses=t.masses, **kw)
+    o3.set_system(lat2.H, lat2.x, lat2.c)
+    o3.init(skin)
+    o3.force_calc(1)
+    assert rel_inf(o3.get("f"), ref_f) > 1e-6
+
+
+HARNESS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_harness")
+
+
+@pytest.mark.skipif(not os.path.exists(HARNESS), reason="reference build absent (make -f oracle/Makefile.ref)")
+@pytest.mark.parametrize("variant", ["DMD", "DMD_I", "DMD_II"])
+def test_oracle_matches_live_reference(variant):
+    """a system the fixtures do not hold: 4^3 cells, another seed, the 5-point
+    rule for DMD_I and DMD"""
+    import importlib.util
+    from mapp_amd import workload as W
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(R.GOLDEN, "make_ref_fixtures.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    t = W.johnson_tables(nr=1000, nrho=1000)
+    lat = W.fcc_alloy(4, seed=913)
+    q = 3 if variant == "DMD_II" else 5
+    with tempfile.TemporaryDirectory() as tmp:
+        ref = mk.run("live", lat, t, variant, q, tmp)
+    d = dict(F=t.F, rho=t.rho, rphi=t.rphi, grid=np.array([t.nrho, t.drho, t.nr, t.dr]), masses=np.array(t.masses),
+             beta=np.array(0.37), coef=np.array([3.0, 25.0, 1000.0, 600.0, 0.5]), H=ref["H"], x_read=ref["x_read"],
+             c_read=ref["c_read"], a=ref["a"])
+    o = _oracle(d, q, variant)
+    off, nl = o.neighbor_list()
+    assert np.array_equal(off, ref["list_off"].reshape(-1).astype(np.int32))
+    assert np.array_equal(nl, ref["list"].reshape(-1).astype(np.int32))
+    out = _run_sequence(o, d)
+    for k in COMPARED:
+        got = np.asarray(out[k], dtype=np.float64).reshape(-1)
+        assert np.array_equal(got.view(np.uint64), ref[k].reshape(-1).view(np.uint64)), (variant, k)
