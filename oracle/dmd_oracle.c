@@ -91,6 +91,7 @@ struct orc {
     double *rho, *drho_dr, *drho_dalpha, *phi, *dphi_dr, *dphi_dalpha;
     int fresh;
     int fix_dE_index; /* 1: dE[atom*nt+type] in the chemical-potential sweep */
+    int compensated;  /* 1: energy and virial sums of the DMD force_calc / energy_calc compensated */
     /* MD-mode potentials: per type pair (COMP order) list cutoffs and coefficients */
     double md_cut_sq[36], md_cut_sk_sq[36];
     int md_chk[36], md_set[36];
@@ -172,6 +173,27 @@ void orc_mark_params(orc_t* o, double rc) {
 
 void orc_set_fresh_caches(orc_t* o, int fresh) { o->fresh = fresh; }
 void orc_set_fix_dE_index(orc_t* o, int fix) { o->fix_dE_index = fix; }
+void orc_set_compensated(orc_t* o, int on) { o->compensated = on; }
+
+/* A running sum of the DMD energy and virial.  comp = 0: the reference's plain
+   double running sum (src/ff_dmd_I.cpp:250-257, :343-348, :403-430); comp = 1:
+   Neumaier's compensated sum of the same terms in the same order (the
+   rounding error of every addition is carried in c), whose error does not grow
+   with the number of terms -- the checker for sums over ~1e8 terms. */
+typedef struct {
+    double s, c;
+} acc_t;
+static inline void acc_add(acc_t* a, double x, int comp) {
+    if (!comp) {
+        a->s += x;
+        return;
+    }
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+static inline double acc_value(const acc_t* a) { return a->s + a->c; }
 void orc_set_energy_full(orc_t* o, int full) { o->energy_full = full; }
 
 /* ---------------------------------------------------------------- tables */
@@ -844,7 +866,8 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
     double* E = o->E;
     double* dE = o->dE;
     double* f = o->f;
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    acc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = o->compensated;
     ensure_caches(o);
     for (long i = 0; i < (long)natms * nt; i++) E[i] = 0.0;
     for (long i = 0; i < (long)natms * xd; i++) f[i] = 0.0;
@@ -884,9 +907,9 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
                                 E[(long)iatm * nt + it] += g[0] * cj;
                                 if (jatm < natms) {
                                     E[(long)jatm * nt + jt] += g[0] * ci;
-                                    if (it >= jt) ns[0] += 0.5 * XW(o, it, jt) * pairc * fq[0];
+                                    if (it >= jt) acc_add(&ns[0], 0.5 * XW(o, it, jt) * pairc * fq[0], kahan);
                                 } else {
-                                    if (it >= jt) ns[0] += 0.25 * XW(o, it, jt) * pairc * fq[0];
+                                    if (it >= jt) acc_add(&ns[0], 0.25 * XW(o, it, jt) * pairc * fq[0], kahan);
                                 }
                             }
                         } else if (o->alpha_max <= alpha) {
@@ -914,9 +937,9 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
                                 E[(long)iatm * nt + it] += o->rho[rho_start + rs] * cj;
                                 if (jatm < natms) {
                                     E[(long)jatm * nt + jt] += o->rho[rho_start + rs] * ci;
-                                    if (it >= jt) ns[0] += 0.5 * XW(o, it, jt) * pairc * o->phi[phi_start + ps];
+                                    if (it >= jt) acc_add(&ns[0], 0.5 * XW(o, it, jt) * pairc * o->phi[phi_start + ps], kahan);
                                 } else {
-                                    if (it >= jt) ns[0] += 0.25 * XW(o, it, jt) * pairc * o->phi[phi_start + ps];
+                                    if (it >= jt) acc_add(&ns[0], 0.25 * XW(o, it, jt) * pairc * o->phi[phi_start + ps], kahan);
                                 }
                             }
                         }
@@ -931,9 +954,9 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
             embed(o, it, E[k], &Fv, &dFv, NULL);
             E[k] = Fv;
             dE[k] = dFv;
-            ns[0] += c[k] * E[k];
-            ns[0] += o->kbT * calc_ent(c[k]);
-            ns[0] += 1.5 * o->kbT * c[k] * log(x[icomp + 3 + it]) + c[k] * o->c_0[it];
+            acc_add(&ns[0], c[k] * E[k], kahan);
+            acc_add(&ns[0], o->kbT * calc_ent(c[k]), kahan);
+            acc_add(&ns[0], 1.5 * o->kbT * c[k] * log(x[icomp + 3 + it]) + c[k] * o->c_0[it], kahan);
             f[icomp + 3 + it] -= 1.5 * o->kbT * c[k] / x[icomp + 3 + it];
         }
     }
@@ -972,20 +995,20 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
                             f[jcomp + 2] -= dx2 * fpair;
                             f[jcomp + 3 + jt] += apair * ai * ai;
                             if (st_clc) {
-                                ns[1] -= fpair * dx0 * dx0;
-                                ns[2] -= fpair * dx1 * dx1;
-                                ns[3] -= fpair * dx2 * dx2;
-                                ns[4] -= fpair * dx1 * dx2;
-                                ns[5] -= fpair * dx2 * dx0;
-                                ns[6] -= fpair * dx0 * dx1;
+                                acc_add(&ns[1], -(fpair * dx0 * dx0), kahan);
+                                acc_add(&ns[2], -(fpair * dx1 * dx1), kahan);
+                                acc_add(&ns[3], -(fpair * dx2 * dx2), kahan);
+                                acc_add(&ns[4], -(fpair * dx1 * dx2), kahan);
+                                acc_add(&ns[5], -(fpair * dx2 * dx0), kahan);
+                                acc_add(&ns[6], -(fpair * dx0 * dx1), kahan);
                             }
                         } else if (st_clc) {
-                            ns[1] -= 0.5 * fpair * dx0 * dx0;
-                            ns[2] -= 0.5 * fpair * dx1 * dx1;
-                            ns[3] -= 0.5 * fpair * dx2 * dx2;
-                            ns[4] -= 0.5 * fpair * dx1 * dx2;
-                            ns[5] -= 0.5 * fpair * dx2 * dx0;
-                            ns[6] -= 0.5 * fpair * dx0 * dx1;
+                            acc_add(&ns[1], -(0.5 * fpair * dx0 * dx0), kahan);
+                            acc_add(&ns[2], -(0.5 * fpair * dx1 * dx1), kahan);
+                            acc_add(&ns[3], -(0.5 * fpair * dx2 * dx2), kahan);
+                            acc_add(&ns[4], -(0.5 * fpair * dx1 * dx2), kahan);
+                            acc_add(&ns[5], -(0.5 * fpair * dx2 * dx0), kahan);
+                            acc_add(&ns[6], -(0.5 * fpair * dx0 * dx1), kahan);
                         }
                     }
                 }
@@ -993,8 +1016,8 @@ void orc_force_calc(orc_t* o, int st_clc, double* en_st) {
             rho_start += nt * nt;
         }
     }
-    if (st_clc) for (int i = 0; i < 7; i++) en_st[i] = ns[i];
-    else en_st[0] = ns[0];
+    if (st_clc) for (int i = 0; i < 7; i++) en_st[i] = acc_value(&ns[i]);
+    else en_st[0] = acc_value(&ns[0]);
 }
 
 /* ForceField_DMD_I::energy_calc, src/ff_dmd_I.cpp:454-668 (jtype from itype) */
@@ -1008,7 +1031,8 @@ double orc_energy_calc(orc_t* o) {
     double* x = o->x;
     double* c = o->c;
     double* E = o->E;
-    double en = 0.0;
+    acc_t en = {0, 0};
+    const int kahan = o->compensated;
     for (long i = 0; i < (long)natms * nt; i++) E[i] = 0.0;
     for (int iatm = 0; iatm < natms; iatm++) {
         long icomp = (long)xd * iatm;
@@ -1063,9 +1087,9 @@ double orc_energy_calc(orc_t* o) {
                             E[(long)iatm * nt + it] += g_0 * cj;
                             if (jatm < natms) {
                                 E[(long)jatm * nt + jt] += g_0 * ci;
-                                if (it >= jt) en += 0.5 * XW(o, it, jt) * pairc * f_0;
+                                if (it >= jt) acc_add(&en, 0.5 * XW(o, it, jt) * pairc * f_0, kahan);
                             } else {
-                                if (it >= jt) en += 0.25 * XW(o, it, jt) * pairc * f_0;
+                                if (it >= jt) acc_add(&en, 0.25 * XW(o, it, jt) * pairc * f_0, kahan);
                             }
                         }
                     } else if (o->alpha_max <= alpha) {
@@ -1088,9 +1112,9 @@ double orc_energy_calc(orc_t* o) {
                             E[(long)iatm * nt + it] += g_0 * cj;
                             if (jatm < natms) {
                                 E[(long)jatm * nt + jt] += g_0 * ci;
-                                if (it >= jt) en += 0.5 * XW(o, it, jt) * pairc * f_0;
+                                if (it >= jt) acc_add(&en, 0.5 * XW(o, it, jt) * pairc * f_0, kahan);
                             } else {
-                                if (it >= jt) en += 0.25 * XW(o, it, jt) * pairc * f_0;
+                                if (it >= jt) acc_add(&en, 0.25 * XW(o, it, jt) * pairc * f_0, kahan);
                             }
                         }
                     }
@@ -1101,12 +1125,12 @@ double orc_energy_calc(orc_t* o) {
             double Fv, dFv;
             embed(o, it, E[k], &Fv, &dFv, NULL);
             E[k] = Fv;
-            en += c[k] * E[k];
-            en += o->kbT * calc_ent(c[k]);
-            en += 1.5 * o->kbT * c[k] * log(x[icomp + 3 + it]) + c[k] * o->c_0[it];
+            acc_add(&en, c[k] * E[k], kahan);
+            acc_add(&en, o->kbT * calc_ent(c[k]), kahan);
+            acc_add(&en, 1.5 * o->kbT * c[k] * log(x[icomp + 3 + it]) + c[k] * o->c_0[it], kahan);
         }
     }
-    return en;
+    return acc_value(&en);
 }
 
 /* ForceField_DMD_I::create_2nd_neigh_lst, src/ff_dmd_I.cpp:2138-2217 */

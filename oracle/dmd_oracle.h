@@ -62,6 +62,10 @@ void orc_set_fresh_caches(orc_t* o, int fresh);
 /* 1: index dE by [atom*nt+type] in c_d_calc / calc_g instead of the reference's
    dE[jatm+jtype] (src/ff_dmd_I.cpp:2336,2339) -- a switch for tests only */
 void orc_set_fix_dE_index(orc_t* o, int fix);
+/* 1: the DMD force_calc / energy_calc sum energy and virial with Neumaier's
+   compensated summation (same terms, same order); 0 (default): the
+   reference's plain running sum */
+void orc_set_compensated(orc_t* o, int on);
 /* 1: consistent energy -- energy_calc sums every ordered type pair and both
    energies weight cross-type pair terms as the force loop does, so for
    symmetric density tables and one alpha per site f = -grad U;

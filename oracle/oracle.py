@@ -41,6 +41,7 @@ def lib():
         L.orc_set_fresh_caches.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_c.argtypes = [C.c_void_p, DP]
         L.orc_set_fix_dE_index.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_compensated.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_energy_full.argtypes = [C.c_void_p, C.c_int]
         L.orc_force_calc.argtypes = [C.c_void_p, C.c_int, DP]
         L.orc_energy_calc.argtypes = [C.c_void_p]
@@ -211,6 +212,10 @@ class Oracle:
 
     def set_fix_dE_index(self, fix):
         self.L.orc_set_fix_dE_index(self.o, 1 if fix else 0)
+
+    def set_compensated(self, on):
+        """energy and virial of force_calc / energy_calc by a compensated sum of the reference's terms"""
+        self.L.orc_set_compensated(self.o, 1 if on else 0)
 
     def set_energy_full(self, full):
         self.L.orc_set_energy_full(self.o, 1 if full else 0)
