@@ -230,7 +230,6 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx);
    (MAPP_DMD_VEC_A / MAPP_DMD_VEC_G); g is written only when chk == 0. */
 int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
                     double* g_dev, double* cost);
-
 /* energy mode (no counterpart in the reference; a switch):
    0 (default) the reference: energy_calc loops over type pairs jtype >= itype
    (src/ff_dmd_I.cpp:508), so its energy differs from force_calc's for two or
@@ -348,6 +347,10 @@ int mapp_dmd_sync(mapp_dmd_t* ctx);
 /* sizes: out[0] natms, out[1] nghost, out[2] half-list pairs, out[3] 2nd-list
    pairs, out[4] number of ghost swaps, out[5] pairs inside the quadrature cut */
 int mapp_dmd_sizes(mapp_dmd_t* ctx, long* out);
+/* cutoffs after coef (ForceField_DMD_I::coef, src/ff_dmd_I.cpp:786-792):
+   out[0] rc = (nr-1) dr, out[1] mod_rc = rc + xi[no_i-1]/sqrt(alpha_min) (the
+   quadrature cut; init's list cut is mod_rc + skin, :676-678), out[2] r_crd */
+int mapp_dmd_get_cutoffs(mapp_dmd_t* ctx, double* out);
 /* Neighbor::neighbor_list as CSR (offsets natms+1, entries npairs) */
 int mapp_dmd_get_list(mapp_dmd_t* ctx, int* offsets, int* entries);
 int mapp_dmd_get_list2(mapp_dmd_t* ctx, int* offsets, int* entries);

@@ -120,26 +120,50 @@ long exclusive_scan(mapp_dmd* ctx, const int* in, int* out, long n, int depth = 
     return (long)last[0] + last[1];
 }
 
-// CSR offsets (n+1) from counts (n)
+// CSR offsets (n+1) from counts (n); the offsets are int, so a total of 2^31
+// entries or more is refused before any caller sizes its entry array by it
 long counts_to_offsets(mapp_dmd* ctx, const int* cnt, DevBuf<int>& off, long n) {
     off.reserve(n + 1);
-    long tot = exclusive_scan(ctx, cnt, off.p, n);
+    unsigned long long total = 0;
+    if (n > 0) {
+        ctx->d_total.reserve(1);
+        HIP_OK(hipMemsetAsync(ctx->d_total.p, 0, sizeof(unsigned long long), ctx->stream));
+        hipLaunchKernelGGL(k_count_total, dim3((int)std::min<long>(nblk(n, kCountThreads), 1024)), dim3(kCountThreads),
+                           0, ctx->stream,
+                           cnt, n, ctx->d_total.p);
+        launch_check("k_count_total");
+        HIP_OK(hipMemcpyAsync(&total, ctx->d_total.p, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    long tot = exclusive_scan(ctx, cnt, off.p, n);  // synchronizes the stream: total is in
+    if (total >= (1ull << 31))
+        throw ApiError(MAPP_DMD_E_RANGE, "a list or table needs " + std::to_string(total) +
+                                             " entries; its int offsets hold fewer than 2^31");
     int t = (int)tot;
     HIP_OK(hipMemcpyAsync(off.p + n, &t, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return tot;
 }
 
-// out[0..nv) = sum over nparts partials (stride doubles each, starting at offset)
-void reduce_partials(mapp_dmd* ctx, const double* part, long nparts, int nv, int stride, int offset, double* out) {
-    const long chunk = 4096;
-    const long nb1 = (nparts + chunk - 1) / chunk;
-    ctx->d_stage.reserve((size_t)(nb1 + 1) * 8);
+// partials per first-stage block of reduce_partials
+constexpr long kReduceChunk = 4096;
+// doubles of stage memory reduce_partials needs for nparts partials
+inline long reduce_stage_doubles(long nparts) { return ((nparts + kReduceChunk - 1) / kReduceChunk + 1) * 8; }
+
+// out[0..nv) = sum over nparts partials (stride doubles each, starting at offset),
+// in two fixed-shape stages through stage[reduce_stage_doubles(nparts)]
+void reduce_partials_in(mapp_dmd* ctx, const double* part, long nparts, int nv, int stride, int offset, double* stage,
+                        double* out) {
+    const long nb1 = (nparts + kReduceChunk - 1) / kReduceChunk;
     hipLaunchKernelGGL(k_reduce_stage, dim3((int)nb1), dim3(kBlock), 0, ctx->stream, part, nparts, nv, stride, offset,
-                       chunk, ctx->d_stage.p, 8);
-    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->d_stage.p, nb1, nv, 8, 0,
-                       nb1, out, 1);
+                       kReduceChunk, stage, 8);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(kBlock), 0, ctx->stream, stage, nb1, nv, 8, 0, nb1, out, 1);
     launch_check("k_reduce_stage");
+}
+
+// the same with the context's stage buffer
+void reduce_partials(mapp_dmd* ctx, const double* part, long nparts, int nv, int stride, int offset, double* out) {
+    ctx->d_stage.reserve((size_t)reduce_stage_doubles(nparts));
+    reduce_partials_in(ctx, part, nparts, nv, stride, offset, ctx->d_stage.p, out);
 }
 
 void require(bool cond, int code, const char* msg) {
@@ -940,11 +964,25 @@ void run_c_d(mapp_dmd* ctx) {
     HIP_OK(hipStreamSynchronize(st));
 }
 
+// device memory calc_g needs beyond the per-atom vectors: the per-block partials
+// of the cost, the reduction's stage and the reduced cost
+struct GScratch {
+    double* part;
+    double* stage;
+    double* out;
+};
+
+// blocks of the 2nd-list sweeps of calc_g (one partial each)
+inline long calc_g_blocks(const mapp_dmd* ctx) { return nblk((long)ctx->natms * kList2G); }
+
+// fixed: caller-provided scratch; the site table must then be current, since
+// building it allocates
 template <int NT, int VAR>
-double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g) {
+double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g,
+                  const GScratch* fixed = nullptr) {
     hipStream_t st = ctx->stream;
-    ensure_pair_cache(ctx);
-    const int nbt = nblk((long)ctx->natms * kTableG), nb2 = nblk((long)ctx->natms * kList2G);
+    if (!fixed) ensure_pair_cache(ctx);
+    const int nbt = nblk((long)ctx->natms * kTableG), nb2 = (int)calc_g_blocks(ctx);
     const SiteTable T = site_table(ctx);
     hipLaunchKernelGGL((k_density_table<NT>), dim3(nbt), dim3(kBlock), 0, st, ctx->P, ctx->d_conc.p, T, ctx->d_F9.p,
                        ctx->d_E.p, ctx->d_n.p, ctx->d_dE.p, ctx->d_ddE.p);
@@ -955,14 +993,21 @@ double run_calc_g(mapp_dmd* ctx, int chk, double alpha_t, const double* a, doubl
                        T, ctx->d_dE.p, ctx->d_qidx.p, ctx->d_n.p);
     launch_check("k_chem_pot g");
     ghost_copy(ctx, ctx->d_n.p, NT);
-    ctx->d_part.reserve((size_t)nb2 * 9 + 16);
-    ctx->d_out.reserve(16);
+    GScratch S;
+    if (fixed) {
+        S = *fixed;
+    } else {
+        ctx->d_part.reserve((size_t)nb2 + 16);
+        ctx->d_out.reserve(16);
+        ctx->d_stage.reserve((size_t)reduce_stage_doubles(nb2));
+        S = GScratch{ctx->d_part.p, ctx->d_stage.p, ctx->d_out.p};
+    }
     hipLaunchKernelGGL((k_calc_g_s<NT, VAR>), dim3(nb2), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p, a,
-                       ctx->d_foff2.p, ctx->d_fidx2.p, ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, ctx->d_part.p);
+                       ctx->d_foff2.p, ctx->d_fidx2.p, ctx->d_n.p, ctx->d_s.p, ctx->d_cd.p, S.part);
     launch_check("k_calc_g_s");
-    reduce_partials(ctx, ctx->d_part.p, nb2, 1, 1, 0, ctx->d_out.p);
+    reduce_partials_in(ctx, S.part, nb2, 1, 1, 0, S.stage, S.out);
     double inner = 0.0;
-    HIP_OK(hipMemcpyAsync(&inner, ctx->d_out.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&inner, S.out, sizeof(double), hipMemcpyDeviceToHost, st));
     if (!chk) {
         ghost_copy(ctx, ctx->d_s.p, NT);
         hipLaunchKernelGGL((k_calc_g_t<NT, VAR>), dim3(nb2), dim3(kBlock), 0, st, ctx->P, alpha_t, ctx->d_conc.p,
@@ -989,10 +1034,11 @@ void c_d_variant(mapp_dmd* ctx) {
     else run_c_d<NT, 2>(ctx);
 }
 template <int NT>
-double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g) {
-    if (ctx->variant == 0) return run_calc_g<NT, 0>(ctx, chk, alpha_t, a, g);
-    if (ctx->variant == 1) return run_calc_g<NT, 1>(ctx, chk, alpha_t, a, g);
-    return run_calc_g<NT, 2>(ctx, chk, alpha_t, a, g);
+double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g,
+                      const GScratch* fixed = nullptr) {
+    if (ctx->variant == 0) return run_calc_g<NT, 0>(ctx, chk, alpha_t, a, g, fixed);
+    if (ctx->variant == 1) return run_calc_g<NT, 1>(ctx, chk, alpha_t, a, g, fixed);
+    return run_calc_g<NT, 2>(ctx, chk, alpha_t, a, g, fixed);
 }
 
 
@@ -1156,11 +1202,6 @@ int line_min(mapp_dmd* ctx, long n, double* nrgy, double* alph, long* n_energy) 
 }
 
 // ------------------------------------------------------------------ BDF clock
-template <int NT>
-void c_d_variant(mapp_dmd* ctx);
-template <int NT>
-double calc_g_variant(mapp_dmd* ctx, int chk, double alpha_t, const double* a, double* g);
-
 struct BdfState {
     mapp_dmd* ctx;
     long n;
@@ -2789,6 +2830,17 @@ int mapp_dmd_sizes(mapp_dmd_t* ctx, long* out) {
         out[3] = ctx->have_list2 ? ctx->npairs2 : 0;
         out[4] = ctx->swap_off.empty() ? 0 : (long)ctx->swap_off.size() - 1;
         out[5] = ctx->n_inrange;
+        return 0;
+    });
+}
+
+int mapp_dmd_get_cutoffs(mapp_dmd_t* ctx, double* out) {
+    return guarded(ctx, [&]() {
+        require(out != nullptr, MAPP_DMD_E_ARG, "null output");
+        require(ctx->have_params && !is_md(ctx), MAPP_DMD_E_STATE, "the DMD cutoffs follow coef");
+        out[0] = ctx->P.rc;
+        out[1] = ctx->mod_rc;
+        out[2] = std::sqrt(ctx->rsq_crd);
         return 0;
     });
 }

@@ -72,6 +72,7 @@ struct mapp_dmd {
     bool cache_valid = false;
     mappdmd::DevBuf<double> d_part, d_out, d_xaos, d_stage;
     mappdmd::DevBuf<int> d_iflag;
+    mappdmd::DevBuf<unsigned long long> d_total;  // 64-bit count total of counts_to_offsets
     long n_inrange = 0;
 
     // learned surrogate

@@ -24,6 +24,18 @@ __device__ __forceinline__ double comp3c(const double4& p, int d) {
     return d == 0 ? p.x : (d == 1 ? p.y : p.z);
 }
 
+// ---- total of n non-negative int counts in 64 bits (the int scan below wraps
+// past 2^31; integer sums do not depend on the order of the additions) ----
+constexpr int kCountThreads = 256;
+__global__ __launch_bounds__(kCountThreads) void k_count_total(const int* __restrict__ in, long n,
+                                                        unsigned long long* __restrict__ total) {
+    unsigned long long s = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += (unsigned)in[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(total, s);
+}
+
 // ---- exclusive scan over int (tile scan + recursive scan of tile sums) ----
 __global__ __launch_bounds__(kScanThreads) void k_scan_tile(const int* __restrict__ in, int* __restrict__ out,
                                                             int* __restrict__ sums, long n) {

@@ -51,6 +51,7 @@ _SIGS = {
     "mapp_dmd_set_vec": [CTX, C.c_int, DP, C.c_long],
     "mapp_dmd_sync": [CTX],
     "mapp_dmd_sizes": [CTX, LP],
+    "mapp_dmd_get_cutoffs": [CTX, DP],
     "mapp_dmd_get_list": [CTX, IP, IP],
     "mapp_dmd_get_list2": [CTX, IP, IP],
     "mapp_dmd_get_ghosts": [CTX, DP, IP],
@@ -321,6 +322,12 @@ class ForceFieldDMD:
         p = C.c_void_p()
         self._call("mapp_dmd_device_ptr", VEC[name], C.byref(p))
         return p.value
+
+    def cutoffs(self):
+        """(rc, mod_rc, r_crd) after coef"""
+        out = np.zeros(3)
+        self._call("mapp_dmd_get_cutoffs", _dp(out))
+        return tuple(float(v) for v in out)
 
     def neighbor_list(self):
         s = self.sizes()
