@@ -91,7 +91,7 @@ struct orc {
     double *rho, *drho_dr, *drho_dalpha, *phi, *dphi_dr, *dphi_dalpha;
     int fresh;
     int fix_dE_index; /* 1: dE[atom*nt+type] in the chemical-potential sweep */
-    int compensated;  /* 1: energy and virial sums of the DMD force_calc / energy_calc compensated */
+    int compensated;  /* 1: energy and virial sums of force_calc / energy_calc compensated */
     /* MD-mode potentials: per type pair (COMP order) list cutoffs and coefficients */
     double md_cut_sq[36], md_cut_sk_sq[36];
     int md_chk[36], md_set[36];
@@ -174,6 +174,7 @@ void orc_mark_params(orc_t* o, double rc) {
 void orc_set_fresh_caches(orc_t* o, int fresh) { o->fresh = fresh; }
 void orc_set_fix_dE_index(orc_t* o, int fix) { o->fix_dE_index = fix; }
 void orc_set_compensated(orc_t* o, int on) { o->compensated = on; }
+int orc_compensated(const orc_t* o) { return o->compensated; }
 
 /* A running sum of the DMD energy and virial.  comp = 0: the reference's plain
    double running sum (src/ff_dmd_I.cpp:250-257, :343-348, :403-430); comp = 1:
@@ -1458,7 +1459,8 @@ static void force_calc_eam(orc_t* o, int st_clc, double* en_st) {
     double* x = o->x;
     double* f = o->f;
     double* rho = o->E; /* per-atom rho, then F' */
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    acc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = o->compensated;
     long np = o->no_pairs;
     double* drhoi = (double*)calloc((size_t)np + 1, sizeof(double));
     double* drhoj = (double*)calloc((size_t)np + 1, sizeof(double));
@@ -1506,24 +1508,24 @@ static void force_calc_eam(orc_t* o, int st_clc, double* en_st) {
                     f[jcomp] -= fpair * dx0;
                     f[jcomp + 1] -= fpair * dx1;
                     f[jcomp + 2] -= fpair * dx2;
-                    ns[0] += phi;
+                    acc_add(&ns[0], phi, kahan);
                     if (st_clc) {
-                        ns[1] -= fpair * dx0 * dx0;
-                        ns[2] -= fpair * dx1 * dx1;
-                        ns[3] -= fpair * dx2 * dx2;
-                        ns[4] -= fpair * dx1 * dx2;
-                        ns[5] -= fpair * dx2 * dx0;
-                        ns[6] -= fpair * dx0 * dx1;
+                        acc_add(&ns[1], -(fpair * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(fpair * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(fpair * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(fpair * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(fpair * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(fpair * dx0 * dx1), kahan);
                     }
                 } else {
-                    ns[0] += 0.5 * phi;
+                    acc_add(&ns[0], 0.5 * phi, kahan);
                     if (st_clc) {
-                        ns[1] -= 0.5 * fpair * dx0 * dx0;
-                        ns[2] -= 0.5 * fpair * dx1 * dx1;
-                        ns[3] -= 0.5 * fpair * dx2 * dx2;
-                        ns[4] -= 0.5 * fpair * dx1 * dx2;
-                        ns[5] -= 0.5 * fpair * dx2 * dx0;
-                        ns[6] -= 0.5 * fpair * dx0 * dx1;
+                        acc_add(&ns[1], -(0.5 * fpair * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(0.5 * fpair * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(0.5 * fpair * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(0.5 * fpair * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(0.5 * fpair * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(0.5 * fpair * dx0 * dx1), kahan);
                     }
                 }
                 drhoi[istart] = -drho_i_dr * r_inv;
@@ -1541,7 +1543,7 @@ static void force_calc_eam(orc_t* o, int st_clc, double* en_st) {
         double tmp1 = (coef[6] * p + coef[5]) * p + coef[4];
         double tmp0 = ((coef[3] * p + coef[2]) * p + coef[1]) * p + coef[0];
         if (rho[iatm] > o->rho_max) tmp0 += tmp1 * (rho[iatm] - o->rho_max);
-        ns[0] += tmp0;
+        acc_add(&ns[0], tmp0, kahan);
         rho[iatm] = tmp1;
     }
     update_vec(o, rho, 1);
@@ -1564,20 +1566,20 @@ static void force_calc_eam(orc_t* o, int st_clc, double* en_st) {
                     f[jcomp + 1] -= dx1 * fpair;
                     f[jcomp + 2] -= dx2 * fpair;
                     if (st_clc) {
-                        ns[1] -= fpair * dx0 * dx0;
-                        ns[2] -= fpair * dx1 * dx1;
-                        ns[3] -= fpair * dx2 * dx2;
-                        ns[4] -= fpair * dx1 * dx2;
-                        ns[5] -= fpair * dx2 * dx0;
-                        ns[6] -= fpair * dx0 * dx1;
+                        acc_add(&ns[1], -(fpair * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(fpair * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(fpair * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(fpair * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(fpair * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(fpair * dx0 * dx1), kahan);
                     }
                 } else if (st_clc) {
-                    ns[1] -= 0.5 * fpair * dx0 * dx0;
-                    ns[2] -= 0.5 * fpair * dx1 * dx1;
-                    ns[3] -= 0.5 * fpair * dx2 * dx2;
-                    ns[4] -= 0.5 * fpair * dx1 * dx2;
-                    ns[5] -= 0.5 * fpair * dx2 * dx0;
-                    ns[6] -= 0.5 * fpair * dx0 * dx1;
+                    acc_add(&ns[1], -(0.5 * fpair * dx0 * dx0), kahan);
+                    acc_add(&ns[2], -(0.5 * fpair * dx1 * dx1), kahan);
+                    acc_add(&ns[3], -(0.5 * fpair * dx2 * dx2), kahan);
+                    acc_add(&ns[4], -(0.5 * fpair * dx1 * dx2), kahan);
+                    acc_add(&ns[5], -(0.5 * fpair * dx2 * dx0), kahan);
+                    acc_add(&ns[6], -(0.5 * fpair * dx0 * dx1), kahan);
                 }
             }
             istart++;
@@ -1585,8 +1587,8 @@ static void force_calc_eam(orc_t* o, int st_clc, double* en_st) {
     }
     free(drhoi);
     free(drhoj);
-    if (st_clc) for (int i = 0; i < 7; i++) en_st[i] = ns[i];
-    else en_st[0] = ns[0];
+    if (st_clc) for (int i = 0; i < 7; i++) en_st[i] = acc_value(&ns[i]);
+    else en_st[0] = acc_value(&ns[0]);
 }
 
 /* ForceField_eam::energy_calc, src/ff_eam.cpp:268-356 */
@@ -1657,7 +1659,8 @@ static void force_calc_lj(orc_t* o, int st_clc, double* en_st) {
     int natms = o->natms;
     double* x = o->x;
     double* f = o->f;
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    acc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = o->compensated;
     for (long i = 0; i < (long)natms * 3; i++) f[i] = 0.0;
     for (int iatm = 0; iatm < natms; iatm++) {
         int itype = (int)o->c[iatm];
@@ -1682,35 +1685,35 @@ static void force_calc_lj(orc_t* o, int st_clc, double* en_st) {
                 f[icomp + 1] += ft * dx1;
                 f[icomp + 2] += ft * dx2;
                 if (jatm < natms) {
-                    ns[0] += 4.0 * eps * (sig12 - sig6) + o->lj_offset[curs];
+                    acc_add(&ns[0], 4.0 * eps * (sig12 - sig6) + o->lj_offset[curs], kahan);
                     f[jcomp] -= ft * dx0;
                     f[jcomp + 1] -= ft * dx1;
                     f[jcomp + 2] -= ft * dx2;
                     if (st_clc) {
-                        ns[1] -= ft * dx0 * dx0;
-                        ns[2] -= ft * dx1 * dx1;
-                        ns[3] -= ft * dx2 * dx2;
-                        ns[4] -= ft * dx1 * dx2;
-                        ns[5] -= ft * dx2 * dx0;
-                        ns[6] -= ft * dx0 * dx1;
+                        acc_add(&ns[1], -(ft * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(ft * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(ft * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(ft * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(ft * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(ft * dx0 * dx1), kahan);
                     }
                 } else {
-                    ns[0] += 2.0 * eps * (sig12 - sig6) + o->lj_offset[curs] * 0.5;
+                    acc_add(&ns[0], 2.0 * eps * (sig12 - sig6) + o->lj_offset[curs] * 0.5, kahan);
                     if (st_clc) {
-                        ns[1] -= 0.5 * ft * dx0 * dx0;
-                        ns[2] -= 0.5 * ft * dx1 * dx1;
-                        ns[3] -= 0.5 * ft * dx2 * dx2;
-                        ns[4] -= 0.5 * ft * dx1 * dx2;
-                        ns[5] -= 0.5 * ft * dx2 * dx0;
-                        ns[6] -= 0.5 * ft * dx0 * dx1;
+                        acc_add(&ns[1], -(0.5 * ft * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(0.5 * ft * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(0.5 * ft * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(0.5 * ft * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(0.5 * ft * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(0.5 * ft * dx0 * dx1), kahan);
                     }
                 }
             }
         }
     }
-    en_st[0] = ns[0];
+    en_st[0] = acc_value(&ns[0]);
     if (st_clc)
-        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+        for (int k = 1; k < 7; k++) en_st[k] = acc_value(&ns[k]);
 }
 
 /* ForceField_lj::energy_calc (src/ff_lj.cpp:270-341) */
@@ -1752,7 +1755,8 @@ static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
     double* x = o->x;
     double* f = o->f;
     double* rho = o->E;
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    acc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = o->compensated;
     long np = o->no_pairs;
     double* drhoi = (double*)calloc((size_t)np + 1, sizeof(double));
     double* drhoj = (double*)calloc((size_t)np + 1, sizeof(double));
@@ -1796,24 +1800,24 @@ static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
                             f[jcomp] -= dx0 * phi_coef;
                             f[jcomp + 1] -= dx1 * phi_coef;
                             f[jcomp + 2] -= dx2 * phi_coef;
-                            ns[0] += dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                            acc_add(&ns[0], dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq), kahan);
                             if (st_clc) {
-                                ns[1] -= phi_coef * dx0 * dx0;
-                                ns[2] -= phi_coef * dx1 * dx1;
-                                ns[3] -= phi_coef * dx2 * dx2;
-                                ns[4] -= phi_coef * dx1 * dx2;
-                                ns[5] -= phi_coef * dx2 * dx0;
-                                ns[6] -= phi_coef * dx0 * dx1;
+                                acc_add(&ns[1], -(phi_coef * dx0 * dx0), kahan);
+                                acc_add(&ns[2], -(phi_coef * dx1 * dx1), kahan);
+                                acc_add(&ns[3], -(phi_coef * dx2 * dx2), kahan);
+                                acc_add(&ns[4], -(phi_coef * dx1 * dx2), kahan);
+                                acc_add(&ns[5], -(phi_coef * dx2 * dx0), kahan);
+                                acc_add(&ns[6], -(phi_coef * dx0 * dx1), kahan);
                             }
                         } else {
-                            ns[0] += 0.5 * dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq);
+                            acc_add(&ns[0], 0.5 * dr_phi * dr_phi * (k1 + k2 * r + k3 * rsq), kahan);
                             if (st_clc) {
-                                ns[1] -= 0.5 * phi_coef * dx0 * dx0;
-                                ns[2] -= 0.5 * phi_coef * dx1 * dx1;
-                                ns[3] -= 0.5 * phi_coef * dx2 * dx2;
-                                ns[4] -= 0.5 * phi_coef * dx1 * dx2;
-                                ns[5] -= 0.5 * phi_coef * dx2 * dx0;
-                                ns[6] -= 0.5 * phi_coef * dx0 * dx1;
+                                acc_add(&ns[1], -(0.5 * phi_coef * dx0 * dx0), kahan);
+                                acc_add(&ns[2], -(0.5 * phi_coef * dx1 * dx1), kahan);
+                                acc_add(&ns[3], -(0.5 * phi_coef * dx2 * dx2), kahan);
+                                acc_add(&ns[4], -(0.5 * phi_coef * dx1 * dx2), kahan);
+                                acc_add(&ns[5], -(0.5 * phi_coef * dx2 * dx0), kahan);
+                                acc_add(&ns[6], -(0.5 * phi_coef * dx0 * dx1), kahan);
                             }
                         }
                     }
@@ -1822,7 +1826,7 @@ static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
             istart++;
         }
         double rho_sqd = sqrt(rho[iatm]);
-        ns[0] -= o->fs_A[itype] * rho_sqd;
+        acc_add(&ns[0], -(o->fs_A[itype] * rho_sqd), kahan);
         rho[iatm] = -o->fs_A[itype] / rho_sqd;
     }
     update_vec(o, rho, 1);
@@ -1845,20 +1849,20 @@ static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
                     f[jcomp + 1] -= dx1 * rho_coef;
                     f[jcomp + 2] -= dx2 * rho_coef;
                     if (st_clc) {
-                        ns[1] -= rho_coef * dx0 * dx0;
-                        ns[2] -= rho_coef * dx1 * dx1;
-                        ns[3] -= rho_coef * dx2 * dx2;
-                        ns[4] -= rho_coef * dx1 * dx2;
-                        ns[5] -= rho_coef * dx2 * dx0;
-                        ns[6] -= rho_coef * dx0 * dx1;
+                        acc_add(&ns[1], -(rho_coef * dx0 * dx0), kahan);
+                        acc_add(&ns[2], -(rho_coef * dx1 * dx1), kahan);
+                        acc_add(&ns[3], -(rho_coef * dx2 * dx2), kahan);
+                        acc_add(&ns[4], -(rho_coef * dx1 * dx2), kahan);
+                        acc_add(&ns[5], -(rho_coef * dx2 * dx0), kahan);
+                        acc_add(&ns[6], -(rho_coef * dx0 * dx1), kahan);
                     }
                 } else if (st_clc) {
-                    ns[1] -= 0.5 * rho_coef * dx0 * dx0;
-                    ns[2] -= 0.5 * rho_coef * dx1 * dx1;
-                    ns[3] -= 0.5 * rho_coef * dx2 * dx2;
-                    ns[4] -= 0.5 * rho_coef * dx1 * dx2;
-                    ns[5] -= 0.5 * rho_coef * dx2 * dx0;
-                    ns[6] -= 0.5 * rho_coef * dx0 * dx1;
+                    acc_add(&ns[1], -(0.5 * rho_coef * dx0 * dx0), kahan);
+                    acc_add(&ns[2], -(0.5 * rho_coef * dx1 * dx1), kahan);
+                    acc_add(&ns[3], -(0.5 * rho_coef * dx2 * dx2), kahan);
+                    acc_add(&ns[4], -(0.5 * rho_coef * dx1 * dx2), kahan);
+                    acc_add(&ns[5], -(0.5 * rho_coef * dx2 * dx0), kahan);
+                    acc_add(&ns[6], -(0.5 * rho_coef * dx0 * dx1), kahan);
                 }
             }
             istart++;
@@ -1866,9 +1870,9 @@ static void force_calc_fs(orc_t* o, int st_clc, double* en_st) {
     }
     free(drhoi);
     free(drhoj);
-    en_st[0] = ns[0];
+    en_st[0] = acc_value(&ns[0]);
     if (st_clc)
-        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+        for (int k = 1; k < 7; k++) en_st[k] = acc_value(&ns[k]);
 }
 
 /* ForceField_fs::energy_calc (src/ff_fs.cpp:609-686) */
@@ -1914,13 +1918,13 @@ static double energy_calc_fs(orc_t* o) {
 
 /* virial share of a pair force coefficient c along d (w: 1, or 0.5 for a
    ghost partner) */
-static void pair_virial(double* ns, double w, double c, double d0, double d1, double d2) {
-    ns[1] -= w * c * d0 * d0;
-    ns[2] -= w * c * d1 * d1;
-    ns[3] -= w * c * d2 * d2;
-    ns[4] -= w * c * d1 * d2;
-    ns[5] -= w * c * d2 * d0;
-    ns[6] -= w * c * d0 * d1;
+static void pair_virial(acc_t* ns, double w, double c, double d0, double d1, double d2, int kahan) {
+    acc_add(&ns[1], -(w * c * d0 * d0), kahan);
+    acc_add(&ns[2], -(w * c * d1 * d1), kahan);
+    acc_add(&ns[3], -(w * c * d2 * d2), kahan);
+    acc_add(&ns[4], -(w * c * d1 * d2), kahan);
+    acc_add(&ns[5], -(w * c * d2 * d0), kahan);
+    acc_add(&ns[6], -(w * c * d0 * d1), kahan);
 }
 
 /* the density sweep of ForceField_fsm (src/ff_fsm.cpp:407-434 in force_calc,
@@ -1971,13 +1975,14 @@ static void force_calc_fsm(orc_t* o, int st_clc, double* en_st) {
     const double* x = o->x;
     double* f = o->f;
     double* rho = o->E;
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    acc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = o->compensated;
     for (long i = 0; i < (long)natms * 3; i++) f[i] = 0.0;
     fsm_density(o, rho, NULL);
     update_vec(o, rho, 1);
     for (int iatm = 0; iatm < natms; iatm++) {
         int itype = (int)o->c[iatm];
-        ns[0] += -o->fs_A[itype] * sqrt(rho[iatm]);
+        acc_add(&ns[0], -o->fs_A[itype] * sqrt(rho[iatm]), kahan);
         for (int jj = o->nl_off[iatm]; jj < o->nl_off[iatm + 1]; jj++) {
             int jatm = o->nl[jj];
             int jtype = (int)o->c[jatm];
@@ -2010,7 +2015,7 @@ static void force_calc_fsm(orc_t* o, int st_clc, double* en_st) {
                 coef[1] = c;
                 have[1] = 1;
                 double e = dp * dp * (k1 + k2 * r + k3 * rsq);
-                ns[0] += local ? e : 0.5 * e;
+                acc_add(&ns[0], local ? e : 0.5 * e, kahan);
             }
             for (int t = 0; t < 2; t++) {
                 if (!have[t]) continue;
@@ -2018,13 +2023,13 @@ static void force_calc_fsm(orc_t* o, int st_clc, double* en_st) {
                     f[3L * iatm + k] += d[k] * coef[t];
                     if (local) f[3L * jatm + k] -= d[k] * coef[t];
                 }
-                if (st_clc) pair_virial(ns, w, coef[t], d[0], d[1], d[2]);
+                if (st_clc) pair_virial(ns, w, coef[t], d[0], d[1], d[2], kahan);
             }
         }
     }
-    en_st[0] = ns[0];
+    en_st[0] = acc_value(&ns[0]);
     if (st_clc)
-        for (int k = 1; k < 7; k++) en_st[k] = ns[k];
+        for (int k = 1; k < 7; k++) en_st[k] = acc_value(&ns[k]);
 }
 
 /* ForceField_fsm::energy_calc (src/ff_fsm.cpp:582-664) */
@@ -2241,12 +2246,44 @@ static int bdf_interpolate(int q, double del_t, const double* t, double* const* 
    bounded predictor, then nonlinear conjugate gradients on calc_g's cost with a
    backtracking step that keeps c inside [0,1].  Returns cost / (m_tol dof) or
    -1 if c left [0,1]; *err is the local error estimate, *iters the CG count. */
-static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* const* y, const double* dy,
+/* The cells a BDF run integrates as one system: their dof concatenated in
+   cell order, calc_g's cost summed over the cells, every dot product, norm and
+   bound taken over the concatenation.  One cell is Clock_BDF itself; several
+   are the batched run's semantics (one step size and order for the batch). */
+typedef struct {
+    orc_t** cell;
+    int ncell;
+    long n;      /* dof of all cells */
+    long* off;   /* first dof of each cell */
+    double* c;   /* current concentrations of all cells */
+    double* c_d; /* current dc/dt of all cells */
+} bsys_t;
+
+static void bsys_set_c(bsys_t* S, const double* c) {
+    for (int k = 0; k < S->ncell; k++) orc_set_c(S->cell[k], c + S->off[k]);
+    memcpy(S->c, c, S->n * sizeof(double));
+}
+
+static void bsys_gather(bsys_t* S) {
+    for (int k = 0; k < S->ncell; k++) {
+        long m = S->off[k + 1] - S->off[k];
+        memcpy(S->c + S->off[k], S->cell[k]->c, m * sizeof(double));
+        memcpy(S->c_d + S->off[k], S->cell[k]->c_d, m * sizeof(double));
+    }
+}
+
+static double bsys_calc_g(bsys_t* S, int chk, double alpha, const double* a, double* g, int* n_calc_g) {
+    double cost = 0.0;
+    for (int k = 0; k < S->ncell; k++) cost += orc_calc_g(S->cell[k], chk, alpha, a + S->off[k], g + S->off[k]);
+    (*n_calc_g)++;
+    return cost;
+}
+
+static double bdf_solve(bsys_t* S, int q, double del_t, const double* t, double* const* y, const double* dy,
                         const double* a, const double* y0, double beta, double* err, int* iters, int* n_calc_g) {
     const double min_gamma = 1.0e-10, gamma_red = 0.8, slope = 0.4, m_tol = 1.0e-10, a_tol = 1.0e-6;
     const int max_iter = 50;
-    int nt = o->nt, natms = o->natms;
-    long n = (long)natms * nt;
+    long n = S->n;
     double* c = (double*)calloc(n, sizeof(double));
     double* g = (double*)calloc(n, sizeof(double));
     double* g0 = (double*)calloc(n, sizeof(double));
@@ -2259,9 +2296,8 @@ static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* 
         else if (tmp < 0.0) ratio = MINV((y[0][i] - 0.0) / (y[0][i] - tmp), ratio);
     }
     for (long i = 0; i < n; i++) c[i] = y[0][i] + dy[i] * del_t * ratio;
-    orc_set_c(o, c);
-    double curr_cost = orc_calc_g(o, 0, beta, a, g);
-    (*n_calc_g)++;
+    bsys_set_c(S, c);
+    double curr_cost = bsys_calc_g(S, 0, beta, a, g, n_calc_g);
     memcpy(h, g, n * sizeof(double));
     double g0_g0 = 0.0;
     for (long i = 0; i < n; i++) g0_g0 += g[i] * g[i];
@@ -2269,7 +2305,7 @@ static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* 
     int iter = 0;
     while (curr_cost > m_tol * (double)n && iter < max_iter && max_gamma > min_gamma) {
         memcpy(g0, g, n * sizeof(double));
-        memcpy(cc0, o->c, n * sizeof(double));
+        memcpy(cc0, S->c, n * sizeof(double));
         double gamma = 0.99;
         for (long i = 0; i < n; i++) {
             double tmp = cc0[i] + h[i];
@@ -2281,16 +2317,14 @@ static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* 
         double cost = curr_cost;
         while (chk && max_gamma > min_gamma) {
             for (long i = 0; i < n; i++) c[i] = cc0[i] + max_gamma * h[i];
-            orc_set_c(o, c);
-            curr_cost = orc_calc_g(o, 0, beta, a, g);
-            (*n_calc_g)++;
+            bsys_set_c(S, c);
+            curr_cost = bsys_calc_g(S, 0, beta, a, g, n_calc_g);
             double ideal = cost - slope * max_gamma * g_h;
             if (curr_cost < ideal) chk = 0;
             max_gamma *= gamma_red;
             if (max_gamma <= min_gamma) {
-                orc_set_c(o, cc0);
-                curr_cost = orc_calc_g(o, 1, beta, a, g);
-                (*n_calc_g)++;
+                bsys_set_c(S, cc0);
+                curr_cost = bsys_calc_g(S, 1, beta, a, g, n_calc_g);
             }
         }
         if (chk == 0) {
@@ -2322,14 +2356,14 @@ static double bdf_solve(orc_t* o, int q, double del_t, const double* t, double* 
     double err_pre_fac = -1.0 / (l1_q * (1.0 + a_1));
     double e = 0.0;
     for (long i = 0; i < n; i++) {
-        double d = y0[i] - o->c[i];
+        double d = y0[i] - S->c[i];
         e += d * d;
     }
     e = sqrt(e / ((double)n * a_tol));
     *err = e * fabs(err_pre_fac);
     double ret = curr_cost / (m_tol * (double)n);
     for (long i = 0; i < n; i++)
-        if (o->c[i] < 0.0 || o->c[i] > 1.0) ret = -1.0;
+        if (S->c[i] < 0.0 || S->c[i] > 1.0) ret = -1.0;
     free(c); free(g); free(g0); free(h); free(cc0);
     return ret;
 }
@@ -2410,16 +2444,18 @@ static void bdf_hi_lo(bdf_est_t* E, int q, double del_t, const double* t, double
    report[step*6 + {0..5}] = order, del_t, cost ratio, error, CG iterations,
    solve calls.  Returns the number of steps, -1 if c left [0,1], -2 on too
    many retries. */
-int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
-                double* report, int* n_calc_g) {
+static int bdf_run_sys(bsys_t* S, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
+                       double* report, int* n_calc_g) {
     const double a_tol = 1.0e-6, min_del_t = 1.0e-12, max_del_t = 1.0e4;
-    int nt = o->nt, natms = o->natms;
-    long n = (long)natms * nt;
+    long n = S->n;
     if (max_order < 1 || max_order > 7) return -3;
     double en[7];
-    orc_force_calc(o, 1, en);
-    orc_create_2nd_neigh_lst(o);
-    orc_c_d_calc(o);
+    for (int k = 0; k < S->ncell; k++) {
+        orc_force_calc(S->cell[k], 1, en);
+        orc_create_2nd_neigh_lst(S->cell[k]);
+        orc_c_d_calc(S->cell[k]);
+    }
+    bsys_gather(S);
     double t[8] = {0};
     double* y[8];
     for (int i = 0; i < max_order; i++) y[i] = (double*)calloc(n, sizeof(double));
@@ -2431,13 +2467,13 @@ int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int m
     E.e_n = (double*)calloc(n, sizeof(double));
     if (initial_del_t <= 0.0) {
         double sum = 0.0;
-        for (long i = 0; i < n; i++) sum += o->c_d[i] * o->c_d[i];
+        for (long i = 0; i < n; i++) sum += S->c_d[i] * S->c_d[i];
         sum = sqrt(sum / (double)n);
         initial_del_t = a_tol / sum > min_del_t ? a_tol / sum : min_del_t;
         initial_del_t = initial_del_t < max_del_t ? initial_del_t : max_del_t;
     }
-    memcpy(y[0], o->c, n * sizeof(double));
-    memcpy(dy, o->c_d, n * sizeof(double));
+    memcpy(y[0], S->c, n * sizeof(double));
+    memcpy(dy, S->c_d, n * sizeof(double));
     *n_calc_g = 0;
     double del_t = initial_del_t, beta = 0.0, err = 0.0, cost;
     int ord = 1, const_stp = 0, istep = 0, rc = 0;
@@ -2449,7 +2485,7 @@ int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int m
             chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
         }
         int iters, it_sum = 0, solves = 1, retries = 0;
-        cost = bdf_solve(o, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
+        cost = bdf_solve(S, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
         it_sum += iters;
         while (cost >= 0 && (err >= 1.0 || cost >= 1.0)) {
             if (++retries > max_retry) {
@@ -2466,7 +2502,7 @@ int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int m
                 del_t *= 0.8;
                 chk = bdf_interpolate(ord, del_t, t, y, dy, n, a, y0, &beta, fix);
             }
-            cost = bdf_solve(o, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
+            cost = bdf_solve(S, ord, del_t, t, y, dy, a, y0, beta, &err, &iters, n_calc_g);
             it_sum += iters;
             solves++;
         }
@@ -2485,11 +2521,12 @@ int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int m
         }
         y[0] = tmpy;
         t[0] += del_t;
-        memcpy(y[0], o->c, n * sizeof(double));
-        memcpy(dy, o->c_d, n * sizeof(double));
+        bsys_gather(S);
+        memcpy(y[0], S->c, n * sizeof(double));
+        memcpy(dy, S->c_d, n * sizeof(double));
         E.lo_avail = ord != 1 && const_stp > ord + 1;
         E.hi_avail = ord < max_order && istep != 0 && const_stp > ord + 1;
-        bdf_hi_lo(&E, ord, del_t, t, y, o->c, o->c_d, y0, n);
+        bdf_hi_lo(&E, ord, del_t, t, y, S->c, S->c_d, y0, n);
         double ratio, lo_ratio, hi_ratio;
         if (E.hi_avail && E.lo_avail) {
             lo_ratio = pow(0.5 / E.lo_err, 1.0 / (double)ord);
@@ -2536,6 +2573,30 @@ int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int m
     for (int i = 0; i < max_order; i++) free(y[i]);
     free(dy); free(a); free(y0); free(E.e_n);
     return rc ? rc : istep;
+}
+
+int orc_bdf_run_cells(orc_t** cells, int ncell, int nsteps, int max_order, double initial_del_t, int max_retry,
+                      int fix, double* report, int* n_calc_g) {
+    if (ncell < 1) return -3;
+    bsys_t S;
+    S.cell = cells;
+    S.ncell = ncell;
+    S.off = (long*)malloc((ncell + 1) * sizeof(long));
+    S.off[0] = 0;
+    for (int k = 0; k < ncell; k++) S.off[k + 1] = S.off[k] + (long)cells[k]->natms * cells[k]->nt;
+    S.n = S.off[ncell];
+    S.c = (double*)calloc(S.n, sizeof(double));
+    S.c_d = (double*)calloc(S.n, sizeof(double));
+    int rc = bdf_run_sys(&S, nsteps, max_order, initial_del_t, max_retry, fix, report, n_calc_g);
+    free(S.off);
+    free(S.c);
+    free(S.c_d);
+    return rc;
+}
+
+int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
+                double* report, int* n_calc_g) {
+    return orc_bdf_run_cells(&o, 1, nsteps, max_order, initial_del_t, max_retry, fix, report, n_calc_g);
 }
 
 int orc_natms(const orc_t* o) { return o->natms; }

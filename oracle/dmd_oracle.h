@@ -62,10 +62,11 @@ void orc_set_fresh_caches(orc_t* o, int fresh);
 /* 1: index dE by [atom*nt+type] in c_d_calc / calc_g instead of the reference's
    dE[jatm+jtype] (src/ff_dmd_I.cpp:2336,2339) -- a switch for tests only */
 void orc_set_fix_dE_index(orc_t* o, int fix);
-/* 1: the DMD force_calc / energy_calc sum energy and virial with Neumaier's
-   compensated summation (same terms, same order); 0 (default): the
-   reference's plain running sum */
+/* 1: force_calc / energy_calc of every potential (DMD, EAM, LJ, FS, FSM,
+   MEAM) sum energy and virial with Neumaier's compensated summation (same
+   terms, same order); 0 (default): the reference's plain running sum */
 void orc_set_compensated(orc_t* o, int on);
+int orc_compensated(const orc_t* o);
 /* 1: consistent energy -- energy_calc sums every ordered type pair and both
    energies weight cross-type pair terms as the force loop does, so for
    symmetric density tables and one alpha per site f = -grad U;
@@ -98,6 +99,11 @@ int orc_min_lbfgs(orc_t* o, int max_iter, int m_it, double e_tol, double* energi
    -1 if c left [0,1], -2 after more than max_retry retries of one step. */
 int orc_bdf_run(orc_t* o, int nsteps, int max_order, double initial_del_t, int max_retry, int fix,
                 double* report, int* n_calc_g);
+/* the same run over several independent cells integrated as one system (the
+   batched run): dof concatenated in cell order, calc_g's cost summed over the
+   cells, one step size and order for all of them */
+int orc_bdf_run_cells(orc_t** cells, int ncell, int nsteps, int max_order, double initial_del_t, int max_retry,
+                      int fix, double* report, int* n_calc_g);
 /* coefficients of Clock_BDF::interpolate: out[3 + 2q] = beta, alpha_dy_0,
    dalpha_dy_0, alpha_y[q], dalpha_y[q]; fix = 1 corrects the sign of the
    c1 c2 term of dalpha_y[0] (src/clock_bdf.cpp:327) */

@@ -839,6 +839,22 @@ static double density_sweep(meam_t* m, orc_t* o, int with_pair_energy) {
     return en;
 }
 
+/* energy and virial running sums: plain (the reference's) or Neumaier
+   compensated, as the DMD oracle's compensated mode selects */
+typedef struct {
+    double s, c;
+} macc_t;
+static inline void macc_add(macc_t* a, double x, int comp) {
+    if (!comp) {
+        a->s += x;
+        return;
+    }
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+
 double meam_energy_calc(orc_t* o) {
     meam_t* m = M(o);
     reset(m, o);
@@ -848,9 +864,10 @@ double meam_energy_calc(orc_t* o) {
 void meam_force_calc(orc_t* o, int st_clc, double* en_st) {
     meam_t* m = M(o);
     const orc_meam_params* p = &m->p;
-    double ns[7] = {0, 0, 0, 0, 0, 0, 0};
+    macc_t ns[7] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int kahan = orc_compensated(o);
     reset(m, o);
-    ns[0] = density_sweep(m, o, 0);
+    ns[0].s = density_sweep(m, o, 0);
     orc_update_vec(o, m->rec, RD);
 
     const int natms = orc_natms(o), ntot = natms + orc_nghost(o);
@@ -883,7 +900,7 @@ void meam_force_calc(orc_t* o, int st_clc, double* en_st) {
             const double sij = m->scr[ip] * m->fcp[ip];
             double phip;
             const double phi = table_phi(m, it, jt, rij, &phip);
-            ns[0] += j < natms ? phi * sij : 0.5 * phi * sij;
+            macc_add(&ns[0], j < natms ? phi * sij : 0.5 * phi * sij, kahan);
 
             /* atomic densities and their r-derivatives */
             const double recip = 1.0 / rij;
@@ -1107,17 +1124,17 @@ void meam_force_calc(orc_t* o, int st_clc, double* en_st) {
             if (st_clc) {
                 const double w = j < natms ? 1.0 : 0.5, h = j < natms ? 0.5 : 0.25;
                 if (j < natms) {
-                    ns[1] += d[0] * fi[0];
-                    ns[2] += d[1] * fi[1];
-                    ns[3] += d[2] * fi[2];
+                    macc_add(&ns[1], d[0] * fi[0], kahan);
+                    macc_add(&ns[2], d[1] * fi[1], kahan);
+                    macc_add(&ns[3], d[2] * fi[2], kahan);
                 } else {
-                    ns[1] += w * d[0] * fi[0];
-                    ns[2] += w * d[1] * fi[1];
-                    ns[3] += w * d[2] * fi[2];
+                    macc_add(&ns[1], w * d[0] * fi[0], kahan);
+                    macc_add(&ns[2], w * d[1] * fi[1], kahan);
+                    macc_add(&ns[3], w * d[2] * fi[2], kahan);
                 }
-                ns[4] += h * (d[1] * fi[2] + d[2] * fi[1]);
-                ns[5] += h * (d[0] * fi[2] + d[2] * fi[0]);
-                ns[6] += h * (d[0] * fi[1] + d[1] * fi[0]);
+                macc_add(&ns[4], h * (d[1] * fi[2] + d[2] * fi[1]), kahan);
+                macc_add(&ns[5], h * (d[0] * fi[2] + d[2] * fi[0]), kahan);
+                macc_add(&ns[6], h * (d[0] * fi[1] + d[1] * fi[0]), kahan);
             }
 
             /* three-body forces of the screening (:2033-2120) */
@@ -1175,11 +1192,11 @@ void meam_force_calc(orc_t* o, int st_clc, double* en_st) {
                             v[3] = -sixth * (dik[1] * fa[2] + djk[1] * fb[2] + dik[2] * fa[1] + djk[2] * fb[1]);
                             v[4] = -sixth * (dik[0] * fa[2] + djk[0] * fb[2] + dik[2] * fa[0] + djk[2] * fb[0]);
                             v[5] = -sixth * (dik[0] * fa[1] + djk[0] * fb[1] + dik[1] * fa[0] + djk[1] * fb[0]);
-                            for (int a = 0; a < 6; a++) ns[a + 1] -= v[a];
+                            for (int a = 0; a < 6; a++) macc_add(&ns[a + 1], -(v[a]), kahan);
                             if (j < natms)
-                                for (int a = 0; a < 6; a++) ns[a + 1] -= v[a];
+                                for (int a = 0; a < 6; a++) macc_add(&ns[a + 1], -(v[a]), kahan);
                             if (k < natms)
-                                for (int a = 0; a < 6; a++) ns[a + 1] -= v[a];
+                                for (int a = 0; a < 6; a++) macc_add(&ns[a + 1], -(v[a]), kahan);
                         }
                     }
                 }
@@ -1187,7 +1204,7 @@ void meam_force_calc(orc_t* o, int st_clc, double* en_st) {
         }
     }
     free(ty);
-    en_st[0] = ns[0];
+    en_st[0] = ns[0].s + ns[0].c;
     if (st_clc)
-        for (int a = 1; a < 7; a++) en_st[a] = ns[a];
+        for (int a = 1; a < 7; a++) en_st[a] = ns[a].s + ns[a].c;
 }

@@ -76,6 +76,8 @@ def lib():
         L.orc_min_lbfgs.argtypes = [C.c_void_p, C.c_int, C.c_int, D, DP, IP, IP, IP]
         L.orc_bdf_run.argtypes = [C.c_void_p, C.c_int, C.c_int, D, C.c_int, C.c_int, DP, IP]
         L.orc_bdf_coef.argtypes = [C.c_int, D, DP, C.c_int, DP]
+        L.orc_bdf_run_cells.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, D, C.c_int, C.c_int, DP,
+                                        IP]
         L.orc_lj_coef.argtypes = [C.c_void_p, C.c_int, C.c_int, D, D, D]
         L.orc_fs_coef.argtypes = [C.c_void_p] + [DP] * 8
         L.orc_meam_coef.argtypes = [C.c_void_p, C.POINTER(MeamParams)]
@@ -324,6 +326,17 @@ class Oracle:
         out = np.zeros(6)
         ok = self.L.orc_pair_quadrature(self.o, r, alpha, rho_slot, phi_slot, _dp(out))
         return out if ok else None
+
+
+def bdf_run_cells(oracles, nsteps, max_order=2, initial_del_t=-1.0, max_retry=100, fix=False):
+    """Clock_BDF::init + run over independent cells integrated as one system (the batched run's
+    semantics): (return code, report [steps][order, del_t, cost, err, iters, solves], calc_g calls)"""
+    rep = np.zeros((nsteps, 6))
+    ng = C.c_int(0)
+    arr = (C.c_void_p * len(oracles))(*[o.o for o in oracles])
+    rc = lib().orc_bdf_run_cells(arr, len(oracles), nsteps, max_order, initial_del_t, max_retry, 1 if fix else 0,
+                                 _dp(rep), C.byref(ng))
+    return rc, rep[: max(rc, 0)] if rc >= 0 else rep, ng.value
 
 
 def bdf_coef(q, del_t, t, fix=False):

@@ -4,15 +4,14 @@
 a few seconds per call at this size on one host core.
 
 Bars: neighbour lists bit-exact; forces (per-site sums of a few dozen terms)
-<= 1e-12 relative (infinity norm over the array).  The DMD force field's
-energy, virial and energy_calc <= 1e-12 relative against the oracle in its
-compensated mode: the same pair and site terms as the reference, in the same
-order, summed with Neumaier's compensated summation.  The reference's own
-single running sum over ~1e8 terms rounds at the scale of the 5.5e6 eV total
-and drifts by ~1e-4 eV (2e-11) at this size (pinned on the CPU by
-tests/test_oracle_compensated.py), so it is the wrong yardstick for a
-tree-summed total.  The other potentials keep the plain sum, which they
-match to 2e-13 or better; their bar for energy and virial is 1e-12 too."""
+<= 1e-12 relative (infinity norm over the array); energy, virial and
+energy_calc <= 1e-12 relative against the oracle in its compensated mode: the
+same pair and site terms as the reference, in the same order, summed with
+Neumaier's compensated summation.  The reference's own single running sum
+over ~1e8 terms rounds at the scale of the whole total: for the DMD force
+field it drifts by ~1e-4 eV (2e-11) at this size (pinned on the CPU by
+tests/test_oracle_compensated.py), and the plain EAM virial by 2.6e-12, so it
+is the wrong yardstick for a tree-summed total."""
 import numpy as np
 import pytest
 
@@ -66,6 +65,7 @@ def test_eam_full_size():
     orc.set_params(masses=tables.masses)
     orc.set_system(lat.H, s.copy(), t.reshape(-1, 1))
     orc.init(0.5)
+    orc.set_compensated(True)
     _check(setup_eam(lat.H, s, t, tables), orc, "eam")
 
 
@@ -86,6 +86,7 @@ def test_pair_potentials_full_size(variant):
         orc.fs_coef(W.fs_params())
     orc.set_system(H, s.copy(), t.reshape(-1, 1))
     orc.init(0.5)
+    orc.set_compensated(True)
     _check(ff, orc, variant)
 
 
@@ -99,4 +100,5 @@ def test_meam_full_size():
     orc.meam_coef(st)
     orc.set_system(H, s.copy(), t.reshape(-1, 1))
     orc.init(0.3)
+    orc.set_compensated(True)
     _check(setup_meam(H, s, t, st), orc, "meam")

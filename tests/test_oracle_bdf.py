@@ -61,3 +61,60 @@ def test_predictor_and_derivative_exact_for_polynomials(q):
         assert abs(der - dp(tn)) <= 1e-9 * max(1.0, abs(dp(tn)))
     assert abs(np.sum(cf["dalpha_y"])) < 1e-9
     assert abs(np.sum(ref["dalpha_y"])) > 1e-3
+
+
+def _cells(tables, sizes, variant):
+    from mapp_amd import workload as W
+    from oracle import oracle as O
+    return [O.make(W.fcc_alloy(n, seed=s), tables, no_i=3, variant=variant) for n, s in sizes]
+
+
+def test_batched_run_of_one_cell_is_the_run(small_tables):
+    """bdf_run_cells over one cell is Clock_BDF itself, bit for bit"""
+    from oracle import oracle as O
+    a = _cells(small_tables, [(3, 7)], "DMD_II")
+    b = _cells(small_tables, [(3, 7)], "DMD_II")
+    ra = a[0].bdf_run(10, 2)
+    rb = O.bdf_run_cells(b, 10, 2)
+    assert ra[0] == rb[0] == 10 and ra[2] == rb[2]
+    assert np.array_equal(ra[1], rb[1])
+    assert np.array_equal(a[0].get("c"), b[0].get("c"))
+
+
+def test_batched_copies_follow_the_cell(small_tables):
+    """k identical copies of a cell integrated as one system follow the cell's own
+    trajectory: the batch's cost is the sum of the cells' costs and every norm is
+    divided by the batch's dof count, so the copies change no ratio the step
+    control reads.  Summing costs per cell but normalising by one cell's dof, or
+    averaging norms per cell, moves the step sizes off the single-cell run."""
+    from oracle import oracle as O
+    one = _cells(small_tables, [(3, 7)], "DMD_II")
+    three = _cells(small_tables, [(3, 7)] * 3, "DMD_II")
+    r1 = one[0].bdf_run(14, 2)
+    r3 = O.bdf_run_cells(three, 14, 2)
+    assert r1[0] == r3[0] == 14
+    np.testing.assert_array_equal(r1[1][:, 0], r3[1][:, 0])   # orders
+    np.testing.assert_array_equal(r1[1][:, 5], r3[1][:, 5])   # solves per step
+    np.testing.assert_allclose(r3[1][:, 1], r1[1][:, 1], rtol=1e-9)
+    for o in three:
+        assert np.max(np.abs(o.get("c") - one[0].get("c"))) <= 1e-9
+
+
+def test_batched_uneven_cells_couple_through_the_step(small_tables):
+    """uneven cells share one step size and order: the batch's step sequence differs
+    from each cell's own, and each cell's concentrations stay in [0, 1] and conserve
+    the cell's solute to the solver's tolerance (the rates are antisymmetric within a
+    cell; the implicit solve stops at a residual of 1e-10 per dof, so the sums move
+    by ~1e-8 relative, against ~1e-2 if solute leaked between cells)"""
+    from oracle import oracle as O
+    sizes = [(3, 7), (4, 8), (3, 9)]
+    cells = _cells(small_tables, sizes, "DMD_II")
+    c0 = [o.get("c").copy() for o in cells]
+    rc, rep, ng = O.bdf_run_cells(cells, 12, 2)
+    assert rc == 12 and ng > 0
+    alone = _cells(small_tables, sizes[:1], "DMD_II")[0].bdf_run(12, 2)
+    assert not np.allclose(alone[1][:, 1], rep[:, 1], rtol=1e-6)
+    for o, c in zip(cells, c0):
+        c1 = o.get("c")
+        assert c1.min() >= 0.0 and c1.max() <= 1.0
+        np.testing.assert_allclose(c1.sum(axis=0), c.sum(axis=0), rtol=1e-6)
