@@ -8,15 +8,19 @@ Gauss-Hermite rule, FP64 quadrature.  value = sites / step time.
 Each path first runs the launch-schedule autotuner on its own system (or
 takes --schedule) and reports the schedule it kept.  Also reported on the
 same system: the mixed path (5-point rule, FP32
-quadrature sums), the learned surrogate (trained here on 4M quadrature
-samples, BF16 MFMA inference, quadrature fallback), a 50-step L-BFGS run
+quadrature sums; errors of energy, forces, alpha-forces, E, dE and dc/dt
+against the FP64 5-point path), the learned surrogate (trained here on 4M
+quadrature samples, BF16 MFMA inference, quadrature fallback wherever the
+networks miss the per-pair tolerances of a 5e-6 force bound; with the curve
+bound -> served share, errors, site-evals/s), a 50-step L-BFGS run
 with neighbour rebuilds (consistent energy mode), a batch of 96 uneven cells
 (surrogate force in one launch per kernel, autotuned, 20 BDF steps), MEAM,
 plain EAM, Lennard-Jones, Finnis-Sinclair and FSM force_calc on the same
 lattice size, the roofline of the
 dominant kernel (pass 1, timed with HIP events on the library's stream) and
 the FP64 CPU oracle timed on a bounded sample (32^3 cells per thread, one
-thread and then one box per thread of the process's CPU share, at most 16).
+thread and then one box per thread of the process's CPU share, at most 16,
+the share of one GPU on these boxes) and once on the full system, one thread.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-surrogate] [--no-lbfgs] [--no-batch] [--no-meam] [--fp64-only]
@@ -75,9 +79,10 @@ def time_force(ff, steps, warmup):
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, kernel_ms=(ms_kernels / steps).tolist(), energy=float(en[0]))
 
 
-def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobility=False):
+def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobility=False, keep=False):
     """force_calc timing after the launch-schedule autotuner (or with a given schedule);
-    with mobility, also one c_d_calc and one calc_g (the +dc/dt evaluation)"""
+    with mobility, also one c_d_calc and one calc_g (the +dc/dt evaluation); with keep,
+    E, dE and c_d of the system for error reports"""
     from mapp_amd.dmd import setup
     ff = setup(lat, tables, no_i=no_i, precision=precision)
     if schedule is None:
@@ -104,6 +109,12 @@ def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobilit
         r["calc_g_ms"] = (time.perf_counter() - t0) * 1e3 / reps
     r["sizes"] = ff.sizes()
     r["f"] = ff.get("f")
+    if keep:
+        ff.force_calc(1)
+        r["E"], r["dE"] = ff.get("E"), ff.get("dE")
+        ff.create_2nd_neigh_lst()
+        ff.c_d_calc()
+        r["c_d"] = ff.get("c_d")
     if mobility:
         # Atoms::update_0 on device-resident positions: one site moved past the
         # skin (ghosts, bins, half list, reverse list rebuilt), then small moves
@@ -128,20 +139,50 @@ def run_path(lat, tables, no_i, precision, steps, warmup, schedule=None, mobilit
     return r
 
 
-def run_surrogate(lat, tables, steps, warmup, n_train, f_ref):
-    """train on n_train quadrature samples (4 type pairs), then time force_calc"""
+def run_surrogate(lat, tables, steps, warmup, n_train, f_ref, e_ref, bound=5e-6,
+                  curve=(5e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1e30)):
+    """train on n_train quadrature samples (4 type pairs) with the fallback mask of a
+    relative force bound, time force_calc at that operating point, then the curve
+    bound -> (share of the quadrature-branch evaluations the networks serve, force and
+    energy error against the FP64 5-point path, site-evals/s); bound -1 is the
+    all-fallback path (the FP32 quadrature inside the surrogate kernel)"""
     from mapp_amd.dmd import setup
     ff = setup(lat, tables, no_i=5, precision="mixed")
+    ff.autotune(reps=2)  # the mixed pass stands in where no cell is served
     t0 = time.perf_counter()
-    rep = ff.surrogate_train(n_per_tp=n_train // 4, epochs=2, batch_per_tp=1024, lr=2e-3, seed=3, tol=0.05)
+    rep = ff.surrogate_train(n_per_tp=n_train // 4, epochs=4, batch_per_tp=1024, lr=2e-3, seed=3, tol=bound)
     train_s = time.perf_counter() - t0
     ff.set_precision("surrogate")
+
+    def point(b, k):
+        info = ff.surrogate_set_bound(b)
+        r = time_force(ff, k, 1)
+        f = ff.get("f")
+        return dict(bound=b, served_cells=info["served"], fallbacks=ff.surrogate_fallbacks(),
+                    ms_per_step=r["ms_per_step"], pass1_ms=r["kernel_ms"][0],
+                    site_evals_per_s=lat.natms / (r["ms_per_step"] / 1e3),
+                    force_rel_err_vs_fp64_q5=float(np.max(np.abs(f - f_ref)) / np.max(np.abs(f_ref))),
+                    energy_rel_err_vs_fp64_q5=abs(r["energy"] - e_ref) / abs(e_ref)), info
+
+    base, _ = point(-1.0, max(3, steps // 4))
+    total = base["fallbacks"]
+    rows = []
+    for b in curve:
+        row, info = point(b, max(3, steps // 4))
+        row["served_evals"] = 1.0 - row["fallbacks"] / total
+        rows.append(row)
+    ff.surrogate_set_bound(bound)
     r = time_force(ff, steps, warmup)
     f = ff.get("f")
-    r.update(train_seconds=train_s, train_report=rep, fallbacks=ff.surrogate_fallbacks(),
-             force_rel_err_vs_fp64=float(np.max(np.abs(f - f_ref)) / np.max(np.abs(f_ref))))
+    info = ff.surrogate_set_bound(bound)
+    out = dict(ms_per_step=r["ms_per_step"], kernel_ms=r["kernel_ms"], bound=bound, train_seconds=train_s,
+               train_report=rep, fallbacks=ff.surrogate_fallbacks(), all_fallback=base, curve=rows,
+               tolerances=info["eps"].tolist(), scales=info["scales"].tolist(),
+               force_rel_err_vs_fp64=float(np.max(np.abs(f - f_ref)) / np.max(np.abs(f_ref))),
+               energy_rel_err_vs_fp64=abs(r["energy"] - e_ref) / abs(e_ref))
+    out["served_evals"] = 1.0 - out["fallbacks"] / total
     ff.close()
-    return r
+    return out
 
 
 def run_lbfgs(lat, tables, iters):
@@ -171,7 +212,7 @@ def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
     t0 = time.perf_counter()
     ff = setup_cells(lats, tables, no_i=5, precision="mixed")
     build_s = time.perf_counter() - t0
-    rep = ff.surrogate_train(n_per_tp=1 << 20, epochs=2, batch_per_tp=1024, lr=2e-3, seed=3, tol=0.05)
+    rep = ff.surrogate_train(n_per_tp=1 << 20, epochs=2, batch_per_tp=1024, lr=2e-3, seed=3, tol=5e-6)
     ff.set_precision("surrogate")
     rows = ff.autotune(reps=2)
     r = time_force(ff, steps, warmup)
@@ -183,6 +224,7 @@ def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
                 sites_max=int(max(l.natms for l in lats)), build_seconds=build_s,
                 force_site_evals_per_s=natms / (r["ms_per_step"] / 1e3), force_ms=r["ms_per_step"],
                 kernel_ms=r["kernel_ms"][:4], schedule_candidates=int(len(rows)), surrogate_val_err=rep["val_mean_err"],
+                surrogate_bound=5e-6, surrogate_masked_cells=rep["masked_frac"],
                 bdf_steps=code, bdf_seconds=bdf_s, bdf_calc_g_calls=ncg,
                 bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [])
 
@@ -299,12 +341,26 @@ def _orc_fs(O, W, variant):
 
 def cpu_threads():
     """host threads for the multi-rank CPU baseline: the process's CPU share
-    (OMP_NUM_THREADS where set), at most 16"""
+    (OMP_NUM_THREADS where set), at most 16 -- the host share that comes with
+    one GPU on the MI355X boxes this runs on"""
     k = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         k = min(k, int(env))
     return max(1, min(k, 16))
+
+
+def cpu_full_size(lat, tables):
+    """the FP64 oracle's force_calc(1) on the full benchmark system, one thread, one call
+    (list build and caches set up before the timer)"""
+    from oracle import oracle as O
+    orc = O.make(lat, tables, no_i=3, variant="DMD_I")
+    t0 = time.perf_counter()
+    orc.force_calc(1)
+    dt = time.perf_counter() - t0
+    del orc
+    return dict(value=lat.natms / dt, cores=1, seconds=dt,
+                sample=f"one force_calc(1) on the full {lat.natms}-site system, 1 thread")
 
 
 def cpu_baseline(tables, n=32, reps=3, threads=None):
@@ -455,7 +511,8 @@ def main():
         "config": {"workload": workload, "sites": natms, "pairs_half_list": r64["sizes"]["npairs"],
                    "pairs_in_quadrature_cut": r64["sizes"]["n_inrange"],
                    "schedule": r64["schedule"]},
-        "roofline": {"bound": "mfma", "pipe": "fp64 vector (78.6 TFLOP/s, same as the fp64 matrix peak)",
+        "roofline": {"bound": "valu", "pipe": "fp64 vector ALU (78.6 TFLOP/s); no MFMA in this kernel",
+                     "limiter": "L2 latency of the table-row gathers (TA busy, pending L1-miss stalls)",
                      "kernel": "k_pass1", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None if traffic is None else traffic["bytes_per_launch"],
@@ -470,27 +527,51 @@ def main():
         "site_flops": site_flops(P, nt, 3),
         "site_bytes": bytes_per_site(P_list, nt),
     }
+    step_traffic = traffic_for(f"force_calc fp64 q3 {args.cells}^3 step")
+    if step_traffic is not None:
+        alg = bytes_per_site(P_list, nt) * natms
+        out["roofline"]["traffic_step"] = step_traffic["bytes_per_launch"]
+        out["roofline"]["traffic_step_kernels"] = step_traffic.get("kernels")
+        out["roofline"]["algorithmic_step_bytes"] = alg
+        out["roofline"]["traffic_step_over_algorithmic"] = step_traffic["bytes_per_launch"] / alg
     if not args.fp64_only:
-        rmix = run_path(lat, tables, 5, "mixed", args.steps, args.warmup, sched)
-        r64q5 = run_path(lat, tables, 5, "fp64", max(2, args.steps // 4), 1, sched)
+        rmix = run_path(lat, tables, 5, "mixed", args.steps, args.warmup, sched, keep=True)
+        r64q5 = run_path(lat, tables, 5, "fp64", max(2, args.steps // 4), 1, sched, keep=True)
+        xd = 3 + nt
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+        f5 = r64q5["f"].reshape(natms, xd)
+        fm = rmix["f"].reshape(natms, xd)
         paths = {
             "fp64_q3": {"site_evals_per_s": value, "ms_per_step": r64["ms_per_step"]},
             "fp64_q5": {"site_evals_per_s": natms / (r64q5["ms_per_step"] / 1e3), "ms_per_step": r64q5["ms_per_step"],
                         "kernel_ms": r64q5["kernel_ms"][:4]},
             "mixed_fp32_q5": {"site_evals_per_s": natms / (rmix["ms_per_step"] / 1e3),
                               "ms_per_step": rmix["ms_per_step"], "kernel_ms": rmix["kernel_ms"][:4],
-                              "force_rel_err_vs_fp64_q5": float(np.max(np.abs(rmix["f"] - r64q5["f"])) /
-                                                                np.max(np.abs(r64q5["f"])))},
+                              "force_rel_err_vs_fp64_q5": rel(rmix["f"], r64q5["f"]),
+                              "errors_vs_fp64_q5": {
+                                  "energy": abs(rmix["energy"] - r64q5["energy"]) / abs(r64q5["energy"]),
+                                  "force_xyz": rel(fm[:, :3], f5[:, :3]),
+                                  "force_alpha": rel(fm[:, 3:], f5[:, 3:]),
+                                  "E": rel(rmix["E"], r64q5["E"]), "dE": rel(rmix["dE"], r64q5["dE"]),
+                                  "dc_dt": rel(rmix["c_d"], r64q5["c_d"])}},
         }
         if not args.no_surrogate:
-            rs = run_surrogate(lat, tables, args.steps, args.warmup, 4 << 20, r64q5["f"])
+            rs = run_surrogate(lat, tables, args.steps, args.warmup, 4 << 20, r64q5["f"], r64q5["energy"])
             log("surrogate", rs)
             paths["surrogate_q5_fallback"] = {
                 "site_evals_per_s": natms / (rs["ms_per_step"] / 1e3), "ms_per_step": rs["ms_per_step"],
-                "kernel_ms": rs["kernel_ms"][:4], "force_rel_err_vs_fp64_q5": rs["force_rel_err_vs_fp64"],
-                "fallback_pair_evals": rs["fallbacks"], "train_seconds": rs["train_seconds"],
-                "train_report": rs["train_report"],
-                "speedup_vs_fp64_q5": r64q5["ms_per_step"] / rs["ms_per_step"]}
+                "kernel_ms": rs["kernel_ms"][:4], "bound": rs["bound"],
+                "force_rel_err_vs_fp64_q5": rs["force_rel_err_vs_fp64"],
+                "energy_rel_err_vs_fp64_q5": rs["energy_rel_err_vs_fp64"],
+                "served_evals": rs["served_evals"], "fallback_pair_evals": rs["fallbacks"],
+                "tolerances": rs["tolerances"], "bound_scales": rs["scales"],
+                "train_seconds": rs["train_seconds"], "train_report": rs["train_report"],
+                "speedup_vs_fp64_q5": r64q5["ms_per_step"] / rs["ms_per_step"],
+                "speedup_vs_mixed_q5": rmix["ms_per_step"] / rs["ms_per_step"],
+                "all_fallback": rs["all_fallback"], "curve": rs["curve"]}
         out["paths"] = paths
         if not args.no_lbfgs:
             out["lbfgs"] = run_lbfgs(lat, tables, 50)
@@ -505,6 +586,8 @@ def main():
             log("md", out["md"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tables)
+        out["cpu_baseline"]["single_thread_full_size"] = cpu_full_size(lat, tables)
+        log("cpu", out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 
 

@@ -315,13 +315,22 @@ int mapp_dmd_cfg_write(const char* path, const double* H, int natms, int nt, con
 /* Learned surrogate (no counterpart in the reference):
    one MLP 3->64->64->64->6 (SiLU) per ordered type pair maps (r, alpha_i,
    alpha_j) to {rho_bar, drho_bar/dr, drho_bar/dalpha, phi_bar, dphi_bar/dr,
-   dphi_bar/dalpha}.  Trains on the GPU on n_per_tp samples per type pair
-   labelled by the FP32 quadrature (current rule and tables), Adam with a cosine
-   schedule; then marks cells of (r, alpha_i, alpha_j) whose validation error
-   (max over outputs of |y - t| / std) exceeds tol for quadrature fallback.
+   dphi_bar/dalpha} as baseline + s(r) * net: the baseline is the second-order
+   Gaussian smoothing of the tabulated functions at t = r (one table row), s(r)
+   an envelope of the residual.  Trains on the GPU on n_per_tp samples per type
+   pair labelled by the FP32 quadrature (current rule and tables), Adam with a
+   cosine schedule.  The quadrature then judges the inference path (BF16 MFMA
+   networks) on validation samples, and a cell of (r, alpha_i, alpha_j) is
+   served by the networks only where every sample's error is within half of the
+   per-pair tolerances that a relative force bound tol (max |f_net - f| /
+   max |f| over the current system) implies (see mapp_dmd_surrogate_set_bound);
+   every other pair takes the FP32 quadrature.  tol < 0 serves no cell,
+   tol >= 1e30 every cell.  Needs an initialised two-type system (the bound's
+   scales are taken from it).
    box = {r_lo, r_hi, alpha_lo, alpha_hi} or NULL for {1.8, quadrature cut, 60, 120}.
-   report (8 doubles): final train loss, validation mean and max error, masked
-   cell fraction, steps, seconds, training samples, validation samples.
+   report (8 doubles): final train loss, validation mean and max error of the
+   networks in their standardised units, fraction of cells falling back, steps,
+   seconds, training samples, validation samples.
    Afterwards MAPP_DMD_PREC_SURROGATE runs pass 1 of force_calc on BF16 MFMA. */
 int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int batch_per_tp, double lr,
                              unsigned long long seed, double tol, const double* box, double* report);
@@ -333,8 +342,27 @@ int mapp_dmd_surrogate_gradient(mapp_dmd_t* ctx, const float* params, const floa
                                 const float* X, const float* T, float* grad, float* loss);
 /* trained parameters [4][8966] and normalisation [56] */
 int mapp_dmd_surrogate_params(mapp_dmd_t* ctx, float* params, float* norm);
-/* number of (pair, type pair) samples of the last surrogate force_calc that fell back to the quadrature */
+/* number of (pair, type pair) samples of the last surrogate force_calc that fell back to the quadrature
+   (when the mask serves no cell, force_calc runs the mixed path's pass 1 and every in-cut evaluation
+   counts) */
 int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count);
+/* re-derives the fallback mask of the trained networks for another relative
+   force bound without retraining.  Per-pair tolerances, from first-order error
+   propagation with the triangle inequality: with N the largest number of
+   neighbours of a site inside the quadrature cut, P_r, P_a the largest
+   per-site sums of |drho_bar/dr|, |drho_bar/dalpha| over them, F = max |f|,
+   F' = max |dE|, F'' = max |F''| over the densities present, U the energy:
+     phi_r, phi_alpha: bound F / (3N);  rho_r, rho_alpha: bound F / (6 N F');
+     rho: bound F / (6 N F'' max(P_r, P_a)) (and the energy's share);
+     phi: bound |U| / (natms N).
+   report (16 doubles): fraction of cells served, the six tolerances in output
+   order, the bound, then the scales F, F', F'', N, P_r, P_a, |U|, natms. */
+int mapp_dmd_surrogate_set_bound(mapp_dmd_t* ctx, double bound, double* report);
+/* the smallest relative force bound at which each cell of the trained networks
+   is served (every validation sample within half of each tolerance; infinity
+   for a cell without samples): out[tp][64 r cells][4 alpha_i cells][4 alpha_j
+   cells], cap >= 4096 doubles */
+int mapp_dmd_surrogate_cell_bounds(mapp_dmd_t* ctx, double* out, long cap);
 
 /* host <-> device copies of per-atom vectors (owned rows only) */
 int mapp_dmd_get_vec(mapp_dmd_t* ctx, int which, double* host, long count);

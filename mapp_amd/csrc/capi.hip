@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 
 #include "context.hpp"
@@ -638,7 +639,8 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     pack_atoms<NT>(ctx);
     if constexpr (NT == 2 && std::is_same<Q, double>::value) {
         if (sur) {
-            SurDev S{ctx->d_sp.p, reinterpret_cast<const __bf16*>(ctx->d_sfrag.p), ctx->d_smask.p, ctx->snorm};
+            SurDev S{ctx->d_sp.p, reinterpret_cast<const __bf16*>(ctx->d_sfrag.p), ctx->d_smask.p, ctx->d_senv.p,
+                     ctx->snorm};
             ctx->d_nfall.reserve(natms);
             HIP_OK(hipMemsetAsync(ctx->d_nfall.p, 0, sizeof(int) * natms, st));
             const int q = ctx->Pf.no_i;
@@ -730,11 +732,16 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
 template <int NT>
 void dispatch_force(mapp_dmd* ctx, int st_clc, double* en_st) {
     const int q = ctx->P.no_i;
-    if (ctx->prec == MAPP_DMD_PREC_SURROGATE) {
+    // the networks serve no cell at the current bound: every pair takes the FP32
+    // quadrature, which the mixed path's pass 1 evaluates faster than the
+    // surrogate kernel's per-lane fallback
+    const bool sur = ctx->prec == MAPP_DMD_PREC_SURROGATE && ctx->sur_served > 0.0;
+    ctx->sur_last_quadrature = ctx->prec == MAPP_DMD_PREC_SURROGATE && !sur;
+    if (sur) {
         run_force<NT, double, 0>(ctx, ctx->d_quad64.p, ctx->P, st_clc, en_st, true);
         return;
     }
-    if (ctx->prec == MAPP_DMD_PREC_MIXED) {
+    if (ctx->prec == MAPP_DMD_PREC_MIXED || ctx->prec == MAPP_DMD_PREC_SURROGATE) {
         if (q == 3) run_force<NT, float, 3>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
         else if (q == 5) run_force<NT, float, 5>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
         else run_force<NT, float, 0>(ctx, ctx->d_quad32.p, ctx->Pf, st_clc, en_st);
@@ -1059,21 +1066,22 @@ void norm_from_flat(SurNorm& N, const float* f) {
 }
 
 void sur_samples(mapp_dmd* ctx, const SurNorm& N, long n_per_tp, unsigned long long seed, DevBuf<float>& X,
-                 DevBuf<float>& T) {
+                 DevBuf<float>& T, DevBuf<float>& B) {
     const long n = n_per_tp * 4;
     X.reserve(n * 3);
     T.reserve(n * kOut);
+    B.reserve(n * kOut);
     const int q = ctx->Pf.no_i;
     const dim3 g(nblk(n)), b(kBlock);
     if (q == 3)
         hipLaunchKernelGGL((k_sur_samples<2, 3>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
-                           X.p, T.p);
+                           X.p, T.p, B.p);
     else if (q == 5)
         hipLaunchKernelGGL((k_sur_samples<2, 5>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
-                           X.p, T.p);
+                           X.p, T.p, B.p);
     else
         hipLaunchKernelGGL((k_sur_samples<2, 0>), g, b, 0, ctx->stream, ctx->Pf, N, ctx->d_quad32.p, n_per_tp, seed,
-                           X.p, T.p);
+                           X.p, T.p, B.p);
     launch_check("k_sur_samples");
 }
 
@@ -1527,6 +1535,189 @@ void init_batch(mapp_dmd* ctx, double skin) {
     ctx->cell_nghost.assign(ncell, 0);
     for (int k = 0; k < ncell; ++k) ctx->cell_nghost[k] = gh_base[k + 1] - gh_base[k];
 }
+
+// Scales of the current system that turn a relative force bound into per-pair
+// tolerances (sur_mask).  One FP32-quadrature force_calc gives F = max|f| over
+// all components, max|F'| and the energy; the per-site densities give the
+// range over which max|F''| is taken from the embedding table.  A sweep over
+// the half list (both ends of every pair inside the quadrature cut) gives N,
+// the largest number of neighbours of a site inside the cut, and P_r, P_a,
+// the largest per-site sums of |d rho_bar/dr| and |d rho_bar/dalpha| over
+// those neighbours, each pair taking the largest value any type pair and
+// width reaches in its r bin (prof_r, prof_a: over the training samples, 64
+// bins of the trained box, widened to the neighbouring bins).
+void sur_scales(mapp_dmd* ctx, const SurNorm& Nm, const std::vector<double>& prof_r,
+                const std::vector<double>& prof_a) {
+    const int nt = ctx->nt, natms = ctx->natms;
+    const int prec0 = ctx->prec;
+    ctx->prec = MAPP_DMD_PREC_MIXED;
+    ctx->cache_valid = false;
+    double en[7];
+    force_any(ctx, 1, en);
+    ctx->prec = prec0;
+    ctx->cache_valid = false;
+    const int xd = 3 + nt;
+    const long ntot = (long)natms + ctx->nghost;
+    std::vector<double> f((size_t)natms * xd), dE((size_t)natms * nt), Eown((size_t)natms * nt);
+    std::vector<int> off(natms + 1), roff(natms + 1), nl((size_t)std::max(ctx->npairs, 1L));
+    std::vector<double4> pos((size_t)ntot);
+    HIP_OK(hipMemcpy(f.data(), ctx->d_f.p, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(dE.data(), ctx->d_dE.p, dE.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(Eown.data(), ctx->d_Eown.p, Eown.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(off.data(), ctx->d_off.p, off.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(roff.data(), ctx->d_roff.p, roff.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(pos.data(), ctx->d_pos.p, pos.size() * sizeof(double4), hipMemcpyDeviceToHost));
+    if (ctx->npairs > 0)
+        HIP_OK(hipMemcpy(nl.data(), ctx->d_nl.p, (size_t)ctx->npairs * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> pairE((size_t)std::max(roff[natms], 1) * nt);
+    if (roff[natms] > 0)
+        HIP_OK(hipMemcpy(pairE.data(), ctx->d_pairE.p, (size_t)roff[natms] * nt * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    double fmax = 0.0, dfmax = 0.0, rho_lo = 1e300, rho_hi = -1e300;
+    for (double v : f) fmax = std::max(fmax, std::fabs(v));
+    for (double v : dE) dfmax = std::max(dfmax, std::fabs(v));
+    for (int i = 0; i < natms; ++i)
+        for (int t = 0; t < nt; ++t) {
+            double rho = Eown[(size_t)i * nt + t];
+            for (int q = roff[i]; q < roff[i + 1]; ++q) rho += pairE[(size_t)q * nt + t];
+            rho_lo = std::min(rho_lo, rho);
+            rho_hi = std::max(rho_hi, rho);
+        }
+    // neighbour counts and per-site sums of |rho_bar'| inside the quadrature cut
+    const int nb = (int)prof_r.size();
+    const double pr_all = *std::max_element(prof_r.begin(), prof_r.end());
+    const double pa_all = *std::max_element(prof_a.begin(), prof_a.end());
+    std::vector<int> cnt(natms, 0);
+    std::vector<double> sr(natms, 0.0), sa(natms, 0.0);
+    for (int i = 0; i < natms; ++i)
+        for (int q = off[i]; q < off[i + 1]; ++q) {
+            const int j = nl[q];
+            const double dx = pos[i].x - pos[j].x, dy = pos[i].y - pos[j].y, dz = pos[i].z - pos[j].z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 < ctx->P.cut_sq_mod * (1.0 + 1e-12))) continue;  // an ulp of margin: an upper bound
+            const double r = std::sqrt(r2);
+            double vr = pr_all, va = pa_all;  // below the trained box: the largest values of any bin
+            if (r >= Nm.r_lo) {
+                int b = (int)((r - Nm.r_lo) / (Nm.r_hi - Nm.r_lo) * nb);
+                b = std::min(b, nb - 1);
+                vr = prof_r[b];
+                va = prof_a[b];
+            }
+            ++cnt[i];
+            sr[i] += vr;
+            sa[i] += va;
+            if (j < natms) {
+                ++cnt[j];
+                sr[j] += vr;
+                sa[j] += va;
+            }
+        }
+    int nmax = 0;
+    double pr = 0.0, pa = 0.0;
+    for (int i = 0; i < natms; ++i) {
+        nmax = std::max(nmax, cnt[i]);
+        pr = std::max(pr, sr[i]);
+        pa = std::max(pa, sa[i]);
+    }
+    // |F''| = |c6 + 2 c7 p + 3 c8 p^2| / drho over the rows covering the densities
+    // present (5% margin), at both row ends and the interior extremum
+    const FFParams<double>& P = ctx->P;
+    const double margin = 0.05 * (rho_hi - rho_lo) + 1e-12;
+    const int m_lo = std::max(0, (int)((rho_lo - margin) * P.drho_inv));
+    const int m_hi = std::min(P.nrho - 1, (int)((rho_hi + margin) * P.drho_inv) + 1);
+    double ddf = 0.0;
+    for (int a = 0; a < nt; ++a)
+        for (int m = m_lo; m <= m_hi; ++m) {
+            const double* c = ctx->F9.data() + ((size_t)a * (P.nrho + 1) + m) * 9;
+            auto d2 = [&](double p) { return std::fabs(c[6] + 2.0 * c[7] * p + 3.0 * c[8] * p * p); };
+            double v = std::max(d2(0.0), d2(1.0));
+            if (c[8] != 0.0) {
+                const double ps = -c[7] / (3.0 * c[8]);
+                if (ps > 0.0 && ps < 1.0) v = std::max(v, d2(ps));
+            }
+            ddf = std::max(ddf, v * P.drho_inv);
+        }
+    double* S = ctx->sur_scale;
+    S[0] = fmax;
+    S[1] = dfmax;
+    S[2] = ddf;
+    S[3] = (double)nmax;
+    S[4] = pr;
+    S[5] = pa;
+    S[6] = std::fabs(en[0]);
+    S[7] = (double)natms;
+}
+
+// The per-pair tolerances of a bound tau, eps[o] = tau * unit[o] (sur_mask).
+void sur_eps_unit(const mapp_dmd* ctx, double* e) {
+    const double* S = ctx->sur_scale;
+    const double F = S[0], dF = std::max(S[1], 1e-300), ddF = std::max(S[2], 1e-300), N = std::max(S[3], 1.0);
+    const double Pd = std::max(std::max(S[4], S[5]), 1e-300), U = S[6], na = std::max(S[7], 1.0);
+    e[0] = std::min(F / (6.0 * N * ddF * Pd), U / (2.0 * na * N * dF));
+    e[1] = F / (6.0 * N * dF);
+    e[2] = F / (6.0 * N * dF);
+    e[3] = U / (na * N);
+    e[4] = F / (3.0 * N);
+    e[5] = F / (3.0 * N);
+}
+
+// Smallest bound at which each cell ([tp][cell]) is served: every validation
+// sample of the cell within half of each tolerance, i.e. tau >= 2 max err/unit;
+// infinity for a cell without samples.
+std::vector<double> sur_cell_bounds(const mapp_dmd* ctx) {
+    const int cells = kCellR * kCellA * kCellA;
+    double unit[6];
+    sur_eps_unit(ctx, unit);
+    std::vector<double> tc((size_t)4 * cells, 0.0);
+    std::vector<int> count((size_t)4 * cells, 0);
+    const long nv = ctx->sur_nval;
+    for (long k = 0; k < 4 * nv; ++k) {
+        const int tp = (int)(k / nv);
+        const size_t c = (size_t)tp * cells + ctx->h_scell[k];
+        for (int o = 0; o < kOut; ++o)
+            tc[c] = std::max(tc[c], 2.0 * (double)ctx->h_sjudge[(size_t)k * kOut + o] / unit[o]);
+        ++count[c];
+    }
+    for (size_t c = 0; c < tc.size(); ++c)
+        if (count[c] == 0) tc[c] = std::numeric_limits<double>::infinity();
+    return tc;
+}
+
+// Fallback mask for a relative force bound tau (max |f_net - f| / max |f|):
+// first-order propagation of per-pair output errors through the force
+// (pass 2: every ordered type pair weighted by c_i c_j, sum 1; the density
+// terms by F' at both ends; a density error of every neighbour moves F' by F''
+// at a site, and F' multiplies the site's density derivatives at both ends of
+// its pairs) with the triangle inequality, the budget split in thirds, gives
+//   eps(phi_r) = tau F / (3N),   eps(rho_r) = tau F / (6 N F'),
+//   eps(phi_a) = tau F / (3N),   eps(rho_a) = tau F / (6 N F')   (d alpha_pair / d alpha_i <= 1),
+//   eps(rho)   = tau F / (6 N F'' max(P_r, P_a)),
+//   eps(phi)   = tau |U| / (natms N)   (energy, N/2 pairs per site),
+// and eps(rho) is also held to tau |U| / (2 natms N F') for the energy
+// (scales: sur_scales).  A cell is served when it holds validation samples and
+// every sample's error is within half of each tolerance; tau < 0 masks every
+// cell, tau >= 1e30 serves every cell.
+void sur_mask(mapp_dmd* ctx, double tau) {
+    const int cells = kCellR * kCellA * kCellA;
+    double* e = ctx->sur_eps;
+    sur_eps_unit(ctx, e);
+    for (int o = 0; o < 6; ++o) e[o] *= tau;
+    std::vector<unsigned> mask((size_t)4 * cells / 32, 0u);
+    long served = 0;
+    if (tau >= 1e30) {
+        served = 4L * cells;
+    } else {
+        const std::vector<double> tc = sur_cell_bounds(ctx);
+        for (size_t c = 0; c < tc.size(); ++c)
+            if (tau >= 0.0 && tc[c] <= tau) ++served;
+            else mask[c / 32] |= 1u << (c % 32);
+    }
+    ctx->d_smask.reserve(mask.size());
+    HIP_OK(hipMemcpy(ctx->d_smask.p, mask.data(), mask.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+    ctx->sur_bound = tau;
+    ctx->sur_served = (double)served / (4.0 * cells);
+}
+
 }  // namespace
 
 // =================================================================== C ABI
@@ -2075,8 +2266,8 @@ int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
 int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int batch_per_tp, double lr,
                              unsigned long long seed, double tol, const double* box, double* report) {
     return guarded(ctx, [&]() {
-        require(ctx->have_params && ctx->nt == 2 && !is_md(ctx), MAPP_DMD_E_STATE,
-                "the surrogate covers the two-type DMD force field after coef");
+        require(ctx->have_params && ctx->nt == 2 && !is_md(ctx) && ctx->inited, MAPP_DMD_E_STATE,
+                "the surrogate covers the two-type DMD force field of an initialised system");
         require(n_per_tp >= kMB && batch_per_tp >= kMB && batch_per_tp % kMB == 0 && n_per_tp >= batch_per_tp &&
                     epochs >= 1 && lr > 0,
                 MAPP_DMD_E_ARG, "bad training sizes (batch a multiple of 64, samples >= batch)");
@@ -2092,12 +2283,58 @@ int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int bat
         N.a_c = 0.5f * (N.a_lo + N.a_hi);
         N.a_s = 0.5f * (N.a_hi - N.a_lo);
         const long n_val = std::max<long>(n_per_tp / 8, 4096);
-        sur_samples(ctx, N, n_per_tp, seed, ctx->d_sX, ctx->d_sT);
-        sur_samples(ctx, N, n_val, seed ^ 0x5bd1e995ull, ctx->d_sXv, ctx->d_sTv);
-        // output standardisation per type pair
-        std::vector<float> hT((size_t)n_per_tp * 4 * kOut);
+        sur_samples(ctx, N, n_per_tp, seed, ctx->d_sX, ctx->d_sT, ctx->d_sB);
+        sur_samples(ctx, N, n_val, seed ^ 0x5bd1e995ull, ctx->d_sXv, ctx->d_sTv, ctx->d_sBv);
+        // residual targets: (quadrature - baseline) / envelope, standardised per type pair
+        const size_t nt_s = (size_t)n_per_tp * 4;
+        std::vector<float> hT(nt_s * kOut), hB(nt_s * kOut), hX(nt_s * 3);
         HIP_OK(hipMemcpyAsync(hT.data(), ctx->d_sT.p, hT.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hB.data(), ctx->d_sB.p, hB.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hX.data(), ctx->d_sX.p, hX.size() * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
+        // largest |d rho_bar/dr|, |d rho_bar/dalpha| per r bin (for the bound's per-site sums)
+        constexpr int kProfBins = 64;
+        std::vector<double> prof_r(kProfBins, 0.0), prof_a(kProfBins, 0.0);
+        std::vector<double> emax((size_t)4 * kOut * kEnvBins, 0.0), gmax((size_t)4 * kOut, 0.0);
+        for (size_t k = 0; k < nt_s; ++k) {
+            const int tp = (int)(k / n_per_tp);
+            const float r = hX[k * 3];
+            int b = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kEnvBins);
+            b = std::min(std::max(b, 0), kEnvBins - 1);
+            int pb = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kProfBins);
+            pb = std::min(std::max(pb, 0), kProfBins - 1);
+            prof_r[pb] = std::max(prof_r[pb], (double)std::fabs(hT[k * kOut + 1]));
+            prof_a[pb] = std::max(prof_a[pb], (double)std::fabs(hT[k * kOut + 2]));
+            for (int o = 0; o < kOut; ++o) {
+                const double res = std::fabs((double)hT[k * kOut + o] - (double)hB[k * kOut + o]);
+                double& m = emax[((size_t)tp * kOut + o) * kEnvBins + b];
+                m = std::max(m, res);
+                gmax[(size_t)tp * kOut + o] = std::max(gmax[(size_t)tp * kOut + o], res);
+            }
+        }
+        // envelope: the largest residual of the bin and its two neighbours, floored
+        // at 1e-9 of the largest residual of the output
+        std::vector<float> lenv((size_t)4 * kOut * kEnvBins);
+        for (int tp = 0; tp < 4; ++tp)
+            for (int o = 0; o < kOut; ++o) {
+                const double* m = emax.data() + ((size_t)tp * kOut + o) * kEnvBins;
+                const double floor_v = std::max(1e-9 * gmax[(size_t)tp * kOut + o], 1e-30);
+                for (int b = 0; b < kEnvBins; ++b) {
+                    double v = m[b];
+                    if (b > 0) v = std::max(v, m[b - 1]);
+                    if (b + 1 < kEnvBins) v = std::max(v, m[b + 1]);
+                    lenv[((size_t)tp * kOut + o) * kEnvBins + b] = (float)std::log2(std::max(v, floor_v));
+                }
+            }
+        ctx->d_senv.reserve(lenv.size());
+        HIP_OK(hipMemcpy(ctx->d_senv.p, lenv.data(), lenv.size() * sizeof(float), hipMemcpyHostToDevice));
+        for (size_t k = 0; k < nt_s; ++k) {
+            const int tp = (int)(k / n_per_tp);
+            for (int o = 0; o < kOut; ++o) {
+                const float sc = std::exp2(sur_env_log2(lenv.data(), N, tp, o, hX[k * 3]));
+                hT[k * kOut + o] = (hT[k * kOut + o] - hB[k * kOut + o]) / sc;
+            }
+        }
         for (int tp = 0; tp < 4; ++tp)
             for (int o = 0; o < kOut; ++o) {
                 double m = 0, m2 = 0;
@@ -2111,6 +2348,7 @@ int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int bat
                 N.y_mean[tp][o] = (float)m;
                 N.y_std[tp][o] = var > 1e-30 ? (float)std::sqrt(var) : 1.0f;
             }
+        HIP_OK(hipMemcpy(ctx->d_sT.p, hT.data(), hT.size() * sizeof(float), hipMemcpyHostToDevice));
         // Glorot-uniform weights, zero biases
         std::mt19937_64 rng(seed);
         std::vector<float> hp((size_t)4 * kNParam, 0.0f);
@@ -2174,54 +2412,67 @@ int mapp_dmd_surrogate_train(mapp_dmd_t* ctx, long n_per_tp, int epochs, int bat
         }
         HIP_OK(hipStreamSynchronize(st));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        // validation error per cell -> fallback mask
-        ctx->d_serr.reserve((size_t)n_val * 4);
-        hipLaunchKernelGGL(k_sur_eval, dim3(nblk(n_val * 4)), dim3(kBlock), 0, st, ctx->d_sp.p, N, ctx->d_sXv.p,
-                           ctx->d_sTv.p, n_val, 4, ctx->d_serr.p);
-        launch_check("k_sur_eval");
-        std::vector<float> err((size_t)n_val * 4), hX((size_t)n_val * 4 * 3);
-        HIP_OK(hipMemcpyAsync(err.data(), ctx->d_serr.p, err.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(hX.data(), ctx->d_sXv.p, hX.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        // the inference path judged against the quadrature on the validation samples
+        ctx->d_sfrag.reserve((size_t)4 * kFragPerTp * 64 * 8);
+        hipLaunchKernelGGL(k_sur_pack_frags, dim3(nblk(4L * kFragPerTp * 64)), dim3(kBlock), 0, st, ctx->d_sp.p, 4,
+                           reinterpret_cast<__bf16*>(ctx->d_sfrag.p));
+        launch_check("k_sur_pack_frags");
+        ctx->d_serr.reserve((size_t)n_val * 4 * kOut);
+        {
+            SurDev S{ctx->d_sp.p, reinterpret_cast<const __bf16*>(ctx->d_sfrag.p), nullptr, ctx->d_senv.p, N};
+            HIP_OK(hipFuncSetAttribute((const void*)k_sur_judge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kSurLds));
+            hipLaunchKernelGGL(k_sur_judge, dim3(kSurBlocks), dim3(kSurBlock), kSurLds, st, ctx->Pf, S,
+                               ctx->d_quad32.p, ctx->d_sXv.p, ctx->d_sTv.p, n_val, ctx->d_serr.p);
+            launch_check("k_sur_judge");
+        }
+        ctx->h_sjudge.resize((size_t)n_val * 4 * kOut);
+        std::vector<float> hXv((size_t)n_val * 4 * 3);
+        HIP_OK(hipMemcpyAsync(ctx->h_sjudge.data(), ctx->d_serr.p, ctx->h_sjudge.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hXv.data(), ctx->d_sXv.p, hXv.size() * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        const int cells = kCellR * kCellA * kCellA;
-        std::vector<float> cmax((size_t)4 * cells, 0.0f);
-        double emean = 0.0, emax = 0.0;
-        for (long s2 = 0; s2 < n_val * 4; ++s2) {
-            const int tp = (int)(s2 / n_val);
-            const float r = hX[s2 * 3], ai = hX[s2 * 3 + 1], aj = hX[s2 * 3 + 2];
+        ctx->sur_nval = n_val;
+        ctx->h_scell.resize((size_t)n_val * 4);
+        // the networks' own error in standardised output units (|err| / (s(r) std))
+        double emean = 0.0, emax_v = 0.0;
+        for (long k = 0; k < n_val * 4; ++k) {
+            const int tp = (int)(k / n_val);
+            const float r = hXv[k * 3], ai = hXv[k * 3 + 1], aj = hXv[k * 3 + 2];
             int cr = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kCellR);
             int ci = (int)((ai - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
             int cj = (int)((aj - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
             cr = std::min(std::max(cr, 0), kCellR - 1);
             ci = std::min(std::max(ci, 0), kCellA - 1);
             cj = std::min(std::max(cj, 0), kCellA - 1);
-            const int cell = (cr * kCellA + ci) * kCellA + cj;
-            cmax[(size_t)tp * cells + cell] = std::max(cmax[(size_t)tp * cells + cell], err[s2]);
-            emean += err[s2];
-            emax = std::max(emax, (double)err[s2]);
+            ctx->h_scell[k] = (cr * kCellA + ci) * kCellA + cj;
+            double e = 0.0;
+            for (int o = 0; o < kOut; ++o) {
+                const double sc = std::exp2(sur_env_log2(lenv.data(), N, tp, o, r)) * N.y_std[tp][o];
+                e = std::max(e, (double)ctx->h_sjudge[(size_t)k * kOut + o] / sc);
+            }
+            emean += e;
+            emax_v = std::max(emax_v, e);
         }
         emean /= (double)(n_val * 4);
-        std::vector<unsigned> mask((size_t)4 * cells / 32, 0u);
-        long masked = 0;
-        for (int e = 0; e < 4 * cells; ++e)
-            if (cmax[e] > tol) {
-                mask[e / 32] |= 1u << (e % 32);
-                ++masked;
-            }
-        ctx->d_smask.reserve(mask.size());
-        HIP_OK(hipMemcpy(ctx->d_smask.p, mask.data(), mask.size() * sizeof(unsigned), hipMemcpyHostToDevice));
-        ctx->d_sfrag.reserve((size_t)4 * kFragPerTp * 64 * 8);
-        hipLaunchKernelGGL(k_sur_pack_frags, dim3(nblk(4L * kFragPerTp * 64)), dim3(kBlock), 0, st, ctx->d_sp.p, 4,
-                           reinterpret_cast<__bf16*>(ctx->d_sfrag.p));
-        launch_check("k_sur_pack_frags");
-        HIP_OK(hipStreamSynchronize(st));
+        {
+            std::vector<double> wr(prof_r), wa(prof_a);
+            for (int b = 0; b < kProfBins; ++b)
+                for (int d = -1; d <= 1; ++d)
+                    if (b + d >= 0 && b + d < kProfBins) {
+                        wr[b] = std::max(wr[b], prof_r[b + d]);
+                        wa[b] = std::max(wa[b], prof_a[b + d]);
+                    }
+            sur_scales(ctx, N, wr, wa);
+        }
+        sur_mask(ctx, tol);
         ctx->snorm = N;
         ctx->sur_trained = true;
         if (report) {
             report[0] = last_loss;
             report[1] = emean;
-            report[2] = emax;
-            report[3] = (double)masked / (4.0 * cells);
+            report[2] = emax_v;
+            report[3] = 1.0 - ctx->sur_served;
             report[4] = (double)total;
             report[5] = secs;
             report[6] = (double)(n_per_tp * 4);
@@ -2286,6 +2537,10 @@ int mapp_dmd_surrogate_params(mapp_dmd_t* ctx, float* params, float* norm) {
 int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count) {
     return guarded(ctx, [&]() {
         require(ctx->sur_trained && count, MAPP_DMD_E_STATE, "no trained surrogate");
+        if (ctx->sur_last_quadrature) {  // no cell served: the mixed pass 1 ran, every evaluation fell back
+            *count = ctx->n_inrange * ctx->nt * ctx->nt;
+            return 0;
+        }
         std::vector<int> h(ctx->natms);
         ctx->d_nfall.reserve(ctx->natms);
         HIP_OK(hipMemcpy(h.data(), ctx->d_nfall.p, sizeof(int) * ctx->natms, hipMemcpyDeviceToHost));
@@ -2296,6 +2551,31 @@ int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count) {
     });
 }
 
+
+int mapp_dmd_surrogate_cell_bounds(mapp_dmd_t* ctx, double* out, long cap) {
+    return guarded(ctx, [&]() {
+        require(ctx->sur_trained, MAPP_DMD_E_STATE, "no trained surrogate");
+        const std::vector<double> tc = sur_cell_bounds(ctx);
+        require(out && cap >= (long)tc.size(), MAPP_DMD_E_ARG, "out needs 4 * 64 * 4 * 4 doubles");
+        std::copy(tc.begin(), tc.end(), out);
+        return 0;
+    });
+}
+
+int mapp_dmd_surrogate_set_bound(mapp_dmd_t* ctx, double bound, double* report) {
+    return guarded(ctx, [&]() {
+        require(ctx->sur_trained, MAPP_DMD_E_STATE, "no trained surrogate");
+        HIP_OK(hipSetDevice(ctx->device));
+        sur_mask(ctx, bound);
+        if (report) {
+            report[0] = ctx->sur_served;
+            for (int o = 0; o < 6; ++o) report[1 + o] = ctx->sur_eps[o];
+            report[7] = bound;
+            for (int k = 0; k < 8; ++k) report[8 + k] = ctx->sur_scale[k];
+        }
+        return 0;
+    });
+}
 
 int mapp_dmd_set_energy_mode(mapp_dmd_t* ctx, int full) {
     return guarded(ctx, [&]() {

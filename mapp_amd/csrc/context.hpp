@@ -80,8 +80,20 @@ struct mapp_dmd {
     mappdmd::DevBuf<int> d_sidx, d_nfall;
     mappdmd::DevBuf<unsigned short> d_sfrag;  // bf16 MFMA weight fragments
     mappdmd::DevBuf<unsigned> d_smask;
+    mappdmd::DevBuf<float> d_sB, d_sBv, d_senv;  // sample baselines, log2 residual envelope
     mappdmd::SurNorm snorm{};
     bool sur_trained = false;
+    // the judge's validation errors (|prediction - quadrature| per output) and the
+    // cell of each validation sample, kept so the fallback mask can be re-derived
+    // for another force bound without retraining
+    std::vector<float> h_sjudge;
+    std::vector<int> h_scell;
+    long sur_nval = 0;
+    double sur_scale[8]{};  // force-bound scales of the system the networks were judged for
+    double sur_eps[6]{};    // per-pair tolerances of the current mask
+    double sur_bound = 0.0;
+    double sur_served = 0.0;  // fraction of cells the networks serve
+    bool sur_last_quadrature = false;  // the last surrogate force_calc served no cell and ran the mixed pass 1
     bool energy_full = false;  // energy_calc over every ordered type pair
     // launch shapes of the two force passes (waves per block; total waves, 0 = one per site)
     struct Shape {

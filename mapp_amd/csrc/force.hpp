@@ -223,6 +223,23 @@ constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
 constexpr int kDppRowRor8 = 0x128;   // row_ror:8 (rotate by 8 lanes within a row of 16)
 constexpr int kDppRowRor4 = 0x124;   // row_ror:4
 constexpr int kDppRowRor12 = 0x12C;  // row_ror:12: lane l reads lane (l + 4) mod 16 of its row
+constexpr int kDppQuad0022 = 0xA0;   // quad_perm [0,0,2,2]: lane q = 1 reads lane q = 0
+constexpr int kDppQuad0000 = 0x00;   // quad_perm [0,0,0,0]: every lane of a quad reads q = 0
+
+// Sums over the 16 quads of a wavefront, per position q in the quad, of two
+// registers at once: afterwards lane q of the wavefront holds, in A and B, the
+// totals of A and B over the lanes 4k + q (fixed tree: two rotations within each
+// row of 16, then the rows by xor 16 and 32; the result is read from lane q).
+__device__ __forceinline__ void quad_slot_sums(double& A, double& B) {
+    A += dpp_f64<kDppRowRor4>(A);
+    B += dpp_f64<kDppRowRor4>(B);
+    A += dpp_f64<kDppRowRor8>(A);
+    B += dpp_f64<kDppRowRor8>(B);
+    A += __shfl_xor(A, 16, 64);
+    B += __shfl_xor(B, 16, 64);
+    A += __shfl_xor(A, 32, 64);
+    B += __shfl_xor(B, 32, 64);
+}
 
 template <int NT>
 __device__ __forceinline__ double transpose_tp(double v) {
@@ -480,8 +497,10 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
 // only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
 // goes to pairE in reverse-CSR order (rpos), so the embedding reads it
 // contiguously.  Unless ENERGY_ONLY, the rho lane stores {drho/dr, drho/dalpha}
-// and the phi lane {dphi/dr, dphi/dalpha} at deriv[p][q] (the reference's
-// per-pair caches, src/ff_dmd_I.cpp:227-242, restricted to what the back-pass reads).
+// and the phi lane {dphi/dr, dphi/dalpha} at deriv[off[i] + k][q] for the k-th
+// pair of site i inside the quadrature cut (the reference's per-pair caches,
+// src/ff_dmd_I.cpp:227-242, restricted to what the back-pass reads): a site's
+// records are contiguous, and pass 2 ranks its inside pairs the same way.
 template <int NT, int PARTS = 4>
 struct Pass1Map {
     static constexpr int LPP = NT * NT * PARTS;
@@ -563,6 +582,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const Q* tab = quad + (size_t)q * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS));
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
+        int done = 0;  // inside entries of the earlier chunks: records go to off[i] + rank
         for (int chunk = p0; chunk < p1; chunk += 64) {
             // one list entry per lane: entries outside the cut get a zero partner
             // share; each entry inside it is staged in LDS at its rank among the
@@ -623,7 +643,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const bool inr = valid;  // inside the cut by construction
             double ej = 0.0;
             if (inr) {
-                if ((lane % LPP) == 0) npair += 1.0;
                 const double r_inv = rsqrt(rsq), r = rsq * r_inv;
                 const double ia = iai + iaj, asq = rsqrt(ia);
                 const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
@@ -746,7 +765,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v2[0];
                     }
                     if (!ENERGY_ONLY) {
-                        Q* d = deriv + ((size_t)p * NT * NT + q) * 4;
+                        Q* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + q) * 4;
                         __builtin_nontemporal_store((Q)v1[1], d);
                         __builtin_nontemporal_store((Q)v1[2], d + 1);
                         __builtin_nontemporal_store((Q)v2[1], d + 2);
@@ -762,7 +781,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v1[0];
                     }
                     if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
-                        Q* d = deriv + ((size_t)p * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
+                        Q* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
                         __builtin_nontemporal_store((Q)v1[1], d);
                         __builtin_nontemporal_store((Q)v1[2], d + 1);
                     }
@@ -775,14 +794,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             // an ordinary store: the 16-byte shares land scattered, and L2 merges them into lines
             if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rp * NT + jt] = ej;
         }
+        done += cnt;
         }
+        if (lane == 0) npair += (double)done;  // the site's pairs inside the cut
+        if constexpr (NT == 2 && PARTS == 2) {
+            // Ei[it] lives on the rho lanes 8k + 2q (q = 2 it + jt): join the jt lanes
+            // (then lanes 8k + 4 it hold slot k's share), add the two slots of each row
+            // and the four rows
+            double e = rho_lane ? Ei : 0.0;
+            e += dpp_f64<kDppXor2>(e);
+            e += dpp_f64<kDppRowRor8>(e);
+            e += __shfl_xor(e, 16, 64);
+            e += __shfl_xor(e, 32, 64);
+            const double e0 = read_lane(e, 0), e1 = read_lane(e, 4);
+            if (lane == 0) {
+                Eown[(size_t)i * NT] = e0;
+                Eown[(size_t)i * NT + 1] = e1;
+            }
+        } else {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const double s = wave_sum(rho_lane && it == t ? Ei : 0.0);
-            if (lane == 0) Eown[(size_t)i * NT + t] = s;
+            for (int t = 0; t < NT; ++t) {
+                const double s = wave_sum(rho_lane && it == t ? Ei : 0.0);
+                if (lane == 0) Eown[(size_t)i * NT + t] = s;
+            }
         }
     }
-    double acc[2] = {wave_sum(en), wave_sum(npair)};
+    double acc[2] = {wave_sum(en), npair};  // npair: lane 0 only
     block_store_waves<2>(acc, part);
 }
 
@@ -897,9 +934,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const double ai = ra.x, iai = ra.y, ci = ra.z, dEi = ra.w;
         double fx = 0.0, fy = 0.0, fz = 0.0, fa = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
+        int done = 0;  // inside entries of the earlier chunks (records at off[i] + rank)
         for (int chunk = p0; chunk < p1; chunk += 64) {
-            // entry, partner and separation of the inside pairs, packed
-            int pk_p, pk_j, pk_rp, cnt;
+            // partner and separation of the inside pairs, packed
+            int pk_j, pk_rp, cnt;
             double pk_d0, pk_d1, pk_d2, pk_r2;
             {
                 const int pc = chunk + lane;
@@ -924,7 +962,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                 }
                 const int dest = pack_dest(in, &cnt);
-                pk_p = packed(dest, pc);
                 pk_j = packed(dest, jc);
                 pk_rp = packed(dest, rp);
                 pk_d0 = packed(dest, e0);
@@ -936,7 +973,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const int slot = base + k;
             const bool valid = slot < cnt;
             const int src = valid ? slot : 0;
-            const int p = __shfl(pk_p, src, 64);
             const int j = __shfl(pk_j, src, 64);
             const int rp = __shfl(pk_rp, src, 64);
             const double dx0 = __shfl(pk_d0, src, 64), dx1 = __shfl(pk_d1, src, 64), dx2 = __shfl(pk_d2, src, 64);
@@ -965,7 +1001,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
                 const double r_inv = rsqrt(rsq);
                 // the lane's own record; the transposed type pair's comes from its quad lane
-                const Q* rec = deriv + ((size_t)p * LPP + q) * 4;
+                const Q* rec = deriv + ((size_t)(p0 + done + slot) * LPP + q) * 4;
                 const typename Vec2<Q>::T rg = *reinterpret_cast<const typename Vec2<Q>::T*>(rec);
                 const typename Vec2<Q>::T rf = *reinterpret_cast<const typename Vec2<Q>::T*>(rec + 2);
                 const double g_dr = (double)rg.x, g_da = (double)rg.y;
@@ -1010,11 +1046,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 if (it == 0) dst[3 + jt] = aJ;
             }
         }
+        done += cnt;
         }
-        const double sx = wave_sum(fx), sy = wave_sum(fy), sz = wave_sum(fz);
-        double sa[NT];
+        double sx, sy, sz, sa[NT];
+        if constexpr (NT == 2) {
+            // fx, fy, fz live on the lanes q = 0 and fa[it] on q = 2 it + jt: join the
+            // jt lanes (q = 0 and 2 then hold fa[0], fa[1]), pack the five sums into two
+            // registers -- q = 0: (fx, fy), q = 1: (fz, fa[0]), q = 2: (fa[1], 0) -- and
+            // sum both over the 16 pair slots at once
+            const double fa2 = fa + dpp_f64<kDppXor1>(fa);
+            const double fz1 = dpp_f64<kDppQuad0022>(fz), fa1 = dpp_f64<kDppQuad0022>(fa2);
+            double A = q == 0 ? fx : (q == 1 ? fz1 : (q == 2 ? fa2 : 0.0));
+            double B = q == 0 ? fy : (q == 1 ? fa1 : 0.0);
+            quad_slot_sums(A, B);
+            sx = read_lane(A, 0);
+            sy = read_lane(B, 0);
+            sz = read_lane(A, 1);
+            sa[0] = read_lane(B, 1);
+            sa[1] = read_lane(A, 2);
+        } else {
+            sx = wave_sum(fx);
+            sy = wave_sum(fy);
+            sz = wave_sum(fz);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) sa[t] = wave_sum(it == t ? fa : 0.0);
+            for (int t = 0; t < NT; ++t) sa[t] = wave_sum(it == t ? fa : 0.0);
+        }
         if (lane == 0) {
             fown[(size_t)i * XD + 0] = sx;
             fown[(size_t)i * XD + 1] = sy;
@@ -1023,8 +1079,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int t = 0; t < NT; ++t) fown[(size_t)i * XD + 3 + t] = sa[t];
         }
     }
+    if constexpr (NT == 2) {
+        // the virial lives on the lanes q = 0: q = 0 packs (v0, v1), q = 1 (v2, v3), q = 2 (v4, v5)
+        double w[4];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
+        for (int c = 0; c < 4; ++c) w[c] = dpp_f64<kDppQuad0000>(vir[2 + c]);
+        double A = q == 0 ? vir[0] : (q == 1 ? w[0] : (q == 2 ? w[2] : 0.0));
+        double B = q == 0 ? vir[1] : (q == 1 ? w[1] : (q == 2 ? w[3] : 0.0));
+        quad_slot_sums(A, B);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vir[2 * c] = read_lane(A, c);
+            vir[2 * c + 1] = read_lane(B, c);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
+    }
     block_store_waves<6>(vir, part);
 }
 

@@ -4,21 +4,33 @@
 // {rho_bar, d rho_bar/dr, d rho_bar/dalpha, phi_bar, d phi_bar/dr,
 // d phi_bar/dalpha} -- exactly what pass 1 of the force computes per
 // (pair, type pair) by quadrature (src/ff_dmd_I.cpp:180-242).  Shape
-// 3 -> 64 -> 64 -> 64 -> 6 with SiLU, inputs and outputs standardised.
+// 3 -> 64 -> 64 -> 64 -> 6 with SiLU, inputs standardised.  The networks do
+// not learn the averages themselves but their residual against a cheap
+// baseline read from one table row (second-order Gaussian smoothing of the
+// tabulated functions at t = r, sur_baseline), divided by an r-envelope of
+// that residual (so the networks' error is relative to the residual's local
+// size, which falls by orders of magnitude towards the cutoff):
+//     y = baseline(r, alpha) + s(r) * net(r, alpha_i, alpha_j).
 //
 //   k_sur_samples   training / validation samples labelled by the FP32
-//                   quadrature itself (the mixed path's tables and rule)
+//                   quadrature itself (the mixed path's tables and rule),
+//                   with the baseline of each sample
 //   k_sur_train     one micro-batch of 64 samples of one type pair per block:
 //                   forward, loss, backward in FP32 with everything in LDS;
 //                   writes the block's gradient (fixed-order sums, no atomics)
 //   k_sur_adam      sums block gradients in block order, Adam update
-//   k_sur_eval      FP32 forward for validation (error per cell -> fallback mask)
+//   k_sur_judge     the inference path itself (BF16 MFMA networks, baseline,
+//                   envelope) on validation samples: absolute error of every
+//                   output against the quadrature, from which the host marks
+//                   the (r, alpha_i, alpha_j) cells that meet the per-pair
+//                   tolerances derived from the force bound
 //   k_pass1_sur     pass 1 of the force with the networks on BF16 MFMA
 //                   (v_mfma_f32_16x16x32_bf16, FP32 accumulate): one wavefront
 //                   per site, lane = type pair * 16 + pair slot, so each MFMA
 //                   M tile holds the 16 samples of one type pair; samples
-//                   outside the trained box or in a cell whose validation error
-//                   exceeded the tolerance fall back to the quadrature.
+//                   outside the trained box or in a cell that failed the
+//                   tolerance fall back to the quadrature, and a network whose
+//                   16 samples all fall back is not run.
 #pragma once
 #include "force.hpp"
 #include "surrogate_types.hpp"
@@ -66,10 +78,69 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
 }
 __device__ __forceinline__ double u01(unsigned long long h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
 
-// samples n_per_tp per type pair: X[s] = {r, alpha_i, alpha_j}, T[s] = quadrature outputs
+// Baseline of the six outputs from the table row at t = r (the middle node of an
+// odd rule).  For a tabulated g (g(u) = u rho(u) for the density, the stored
+// r phi for the pair term) with derivatives g1, g2, g3, the Gaussian average
+// with variance 1/(2 alpha) is g + g2/(4 alpha) + O(alpha^-2), so
+//     b = (g + g2/(4 alpha)) / r,   db/dr = (g1 + g3/(4 alpha)) / r - b / r,
+//     db/dalpha = -g2 / (4 alpha^2 r),
+// with the derivatives of the row's cubic in p = r/dr - m.  Zero at and beyond
+// the cutoff.  ia = 1/alpha.
+__device__ __forceinline__ void sur_baseline(const FFParams<float>& P, const float* __restrict__ tab, int stride,
+                                             float r, float ia, float* B) {
+    if (!(r < P.rc)) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) B[c] = 0.0f;
+        return;
+    }
+    float p = r * P.dr_inv;
+    int m = (int)p;
+    m = m < P.nr - 2 ? m : P.nr - 2;
+    m = m > 0 ? m : 0;
+    p -= (float)m;
+    p = p < 1.0f ? p : 1.0f;
+    const float h = P.dr_inv;
+    const float4 cr = *reinterpret_cast<const float4*>(tab + (size_t)m * stride);
+    const float4 cp = *reinterpret_cast<const float4*>(tab + (size_t)m * stride + 4);
+    // value and first three r-derivatives of a cubic row
+    auto d4 = [&](float4 c, float* d) {
+        d[0] = ((c.w * p + c.z) * p + c.y) * p + c.x;
+        d[1] = ((3.0f * c.w * p + 2.0f * c.z) * p + c.y) * h;
+        d[2] = (6.0f * c.w * p + 2.0f * c.z) * h * h;
+        d[3] = 6.0f * c.w * h * h * h;
+    };
+    float v[4], w[4];
+    d4(cr, v);
+    d4(cp, w);
+    const float g0 = r * v[0], g1 = v[0] + r * v[1], g2 = 2.0f * v[1] + r * v[2], g3 = 3.0f * v[2] + r * v[3];
+    const float q = 0.25f * ia, ri = 1.0f / r;
+    B[0] = (g0 + q * g2) * ri;
+    B[1] = (g1 + q * g3) * ri - B[0] * ri;
+    B[2] = -4.0f * q * q * g2 * ri;
+    B[3] = (w[0] + q * w[2]) * ri;
+    B[4] = (w[1] + q * w[3]) * ri - B[3] * ri;
+    B[5] = -4.0f * q * q * w[2] * ri;
+}
+
+// envelope s(r) of output c of type pair tp: log2 s interpolated linearly
+// between the centres of kEnvBins equal r bins of the trained box
+__host__ __device__ inline float sur_env_log2(const float* __restrict__ lenv, const SurNorm& N, int tp, int c,
+                                              float r) {
+    const float* L = lenv + ((size_t)tp * kSurOut + c) * kEnvBins;
+    float u = (r - N.r_lo) / (N.r_hi - N.r_lo) * (float)kEnvBins - 0.5f;
+    int b = (int)floorf(u);
+    b = b < 0 ? 0 : (b > kEnvBins - 2 ? kEnvBins - 2 : b);
+    float f = u - (float)b;
+    f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+    return L[b] + f * (L[b + 1] - L[b]);
+}
+
+// samples n_per_tp per type pair: X[s] = {r, alpha_i, alpha_j}, T[s] = quadrature
+// outputs, B[s] = their baseline
 template <int NT, int NQ>
 __global__ void k_sur_samples(FFParams<float> PQ, SurNorm N, const float* __restrict__ quad, long n_per_tp,
-                              unsigned long long seed, float* __restrict__ X, float* __restrict__ T) {
+                              unsigned long long seed, float* __restrict__ X, float* __restrict__ T,
+                              float* __restrict__ B) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_per_tp * NT * NT) return;
     const int tp = (int)(s / n_per_tp);
@@ -89,6 +160,9 @@ __global__ void k_sur_samples(FFParams<float> PQ, SurNorm N, const float* __rest
         T[s * kOut + k] = g[k];
         T[s * kOut + 3 + k] = f[k];
     }
+    float b[6];
+    sur_baseline(PQ, tab, quad_stride(NT), (float)r, (float)ia, b);
+    for (int k = 0; k < 6; ++k) B[s * kOut + k] = b[k];
 }
 
 // one dense layer over the block's micro-batch: out[s][o] = b[o] + sum_k in[s][k] W[k][o]
@@ -252,46 +326,6 @@ __global__ void k_sur_sum_grad(const float* __restrict__ gpart, int ntp, int blo
     G[e] = g;
 }
 
-// FP32 forward of one sample (thread per sample), de-standardised outputs
-__device__ __forceinline__ void sur_forward_f32(const float* __restrict__ P, const SurNorm& N, int tp, float r,
-                                                float ai, float aj, float* y) {
-    float u[3] = {(r - N.r_c) / N.r_s, (ai - N.a_c) / N.a_s, (aj - N.a_c) / N.a_s};
-    float h1[kHid], h2[kHid];
-    for (int o = 0; o < kHid; ++o) {
-        float z = P[kB1 + o];
-        for (int k = 0; k < kIn; ++k) z += u[k] * P[kW1 + k * kHid + o];
-        h1[o] = silu_f(z);
-    }
-    for (int o = 0; o < kHid; ++o) {
-        float z = P[kB2 + o];
-        for (int k = 0; k < kHid; ++k) z += h1[k] * P[kW2 + k * kHid + o];
-        h2[o] = silu_f(z);
-    }
-    for (int o = 0; o < kHid; ++o) {
-        float z = P[kB3 + o];
-        for (int k = 0; k < kHid; ++k) z += h2[k] * P[kW3 + k * kHid + o];
-        h1[o] = silu_f(z);
-    }
-    for (int o = 0; o < kOut; ++o) {
-        float z = P[kB4 + o];
-        for (int k = 0; k < kHid; ++k) z += h1[k] * P[kW4 + k * kOut + o];
-        y[o] = z * N.y_std[tp][o] + N.y_mean[tp][o];
-    }
-}
-
-// validation: per-sample error (max over the 6 outputs of |y - t| / std), for the cell statistics
-__global__ void k_sur_eval(const float* __restrict__ params, SurNorm N, const float* __restrict__ X,
-                           const float* __restrict__ T, long n_per_tp, int ntp, float* __restrict__ err) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_per_tp * ntp) return;
-    const int tp = (int)(s / n_per_tp);
-    float y[kOut];
-    sur_forward_f32(params + (size_t)tp * kNParam, N, tp, X[s * 3], X[s * 3 + 1], X[s * 3 + 2], y);
-    float e = 0.0f;
-    for (int o = 0; o < kOut; ++o) e = fmaxf(e, fabsf(y[o] - T[s * kOut + o]) / N.y_std[tp][o]);
-    err[s] = e;
-}
-
 __device__ __forceinline__ int sur_cell(const SurNorm& N, float r, float ai, float aj) {
     int cr = (int)((r - N.r_lo) / (N.r_hi - N.r_lo) * kCellR);
     int ci = (int)((ai - N.a_lo) / (N.a_hi - N.a_lo) * kCellA);
@@ -341,20 +375,17 @@ struct SurDev {
     const float* params;   // [ntp][kNParam] FP32 (layer 1, biases)
     const __bf16* frag;    // MFMA weight fragments
     const unsigned* mask;  // fallback bit per cell, [ntp][cells/32]
+    const float* lenv;     // log2 of the residual envelope, [ntp][kSurOut][kEnvBins]
     SurNorm N;
 };
 
-// pass 1 with the surrogate (DMD force, NT == 2).  One block of kSurWaves
-// wavefronts per CU holds every network's MFMA weight fragments (72 KB), the
-// layer-1 weights and the biases in LDS; its wavefronts are persistent over
-// the sites of their XCD range and never synchronise after the set-up.  Per
-// chunk of 16 pairs, lane tp*16 + k prepares sample (pair k, type pair tp)
-// (partner record, distance, trained box, fallback cell) and writes its
-// standardised input to the wavefront's LDS slot; the wavefront then runs the
-// four networks one after the other, 16 samples per MFMA N tile (lane l:
-// sample l%16, feature group l/16), and lane tp*16 + k reads its six outputs
-// back.  Samples outside the trained box or in a cell whose validation error
-// exceeded the tolerance take the FP32 quadrature instead.
+// One block of kSurWaves wavefronts per CU holds every network's MFMA weight
+// fragments (72 KB), the layer-1 weights, the biases and the envelope in LDS.
+// Per chunk of 16 samples of each type pair, lane tp*16 + k stages the
+// standardised input of sample k of type pair tp in the wavefront's LDS slot;
+// the wavefront runs the networks one after the other, 16 samples per MFMA N
+// tile (lane l: sample l%16, feature group l/16), and lane tp*16 + k reads its
+// six outputs back.
 constexpr int kSurWaves = 16;
 constexpr int kSurBlock = 64 * kSurWaves;
 constexpr int kSurBlocks = 256;  // one block per CU
@@ -362,11 +393,146 @@ struct SurLds {
     __bf16 frag[4][kFragPerTp][64][8];  // MFMA A operands
     float4 w1[4][kHid];                 // layer 1: {w_r, w_ai, w_aj, bias} per output
     float bias[4][3][kHid];             // biases of layers 2, 3 and 4
+    float lenv[4][kSurOut][kEnvBins];   // log2 of the residual envelope
     float4 u[kSurWaves][64];            // standardised inputs, lane tp*16 + k
     float y[kSurWaves][64][8];          // standardised outputs, lane tp*16 + k
 };
 constexpr size_t kSurLds = sizeof(SurLds);
 
+__device__ __forceinline__ void sur_load_lds(SurLds& sh, const SurDev& S) {
+    const uint4* src = reinterpret_cast<const uint4*>(S.frag);
+    uint4* dst = reinterpret_cast<uint4*>(&sh.frag[0][0][0][0]);
+    for (int e = threadIdx.x; e < 4 * kFragPerTp * 64; e += kSurBlock) dst[e] = src[e];
+    for (int e = threadIdx.x; e < 4 * kHid; e += kSurBlock) {
+        const int m = e / kHid, o = e % kHid;
+        const float* Pm = S.params + (size_t)m * kNParam;
+        sh.w1[m][o] = make_float4(kKappa * Pm[kW1 + o], kKappa * Pm[kW1 + kHid + o], kKappa * Pm[kW1 + 2 * kHid + o],
+                                  kKappa * Pm[kB1 + o]);
+        sh.bias[m][0][o] = kKappa * Pm[kB2 + o];
+        sh.bias[m][1][o] = kKappa * Pm[kB3 + o];
+        sh.bias[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < 4 * kSurOut * kEnvBins; e += kSurBlock) (&sh.lenv[0][0][0])[e] = S.lenv[e];
+    __syncthreads();
+}
+
+// the networks of the type pairs whose bit is set in need (wave-uniform) on the
+// staged inputs sh.u[w][tp*16 + n] -> standardised outputs sh.y[w][tp*16 + n][0..5]
+__device__ __forceinline__ void sur_nets(SurLds& sh, int w, int lane, unsigned need) {
+    const int g = lane / 16, n = lane % 16;  // feature group, sample column
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        if (!((need >> t) & 1u)) continue;
+        const float4 uu = sh.u[w][t * 16 + n];
+        bf16x8_t b[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float4 wt = sh.w1[t][kfeat(ks, g, e)];
+                b[ks][e] = (__bf16)silu_scaled(wt.w + uu.x * wt.x + uu.y * wt.y + uu.z * wt.z);
+            }
+#pragma unroll
+        for (int layer = 0; layer < 2; ++layer) {
+            floatx4_t d[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                // the accumulator starts from the (scaled) biases of the lane's rows
+                floatx4_t c = *reinterpret_cast<const floatx4_t*>(&sh.bias[t][layer][mt * 16 + g * 4]);
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    const bf16x8_t a =
+                        *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(layer, mt, ks)][lane][0]);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
+                }
+                d[mt] = c;
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) b[mt / 2][(mt % 2) * 4 + e] = (__bf16)silu_scaled(d[mt][e]);
+        }
+        floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(2, 0, ks)][lane][0]);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
+        }
+        // rows g*4 + e are outputs 0..3 (g = 0) and 4, 5 (g = 1)
+        if (g < 2) {
+            const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][2][g * 4]);
+            float* y = &sh.y[w][t * 16 + n][g * 4];
+            y[0] = c[0] * kInvKappa + bs.x;
+            y[1] = c[1] * kInvKappa + bs.y;
+            if (g == 0) {
+                y[2] = c[2] * kInvKappa + bs.z;
+                y[3] = c[3] * kInvKappa + bs.w;
+            }
+        }
+    }
+}
+
+// which of the four 16-lane groups hold a lane with v set (wave-uniform)
+__device__ __forceinline__ unsigned group_any(bool v) {
+    const unsigned long long bal = __ballot(v);
+    unsigned need = 0u;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if ((bal >> (16 * t)) & 0xFFFFull) need |= 1u << t;
+    return need;
+}
+
+// lane's prediction from the networks' standardised outputs y and the baseline B:
+// out = B + s(r) * (y * std + mean)
+__device__ __forceinline__ void sur_predict(const SurLds& sh, const SurNorm& N, int tp, float r, const float* y,
+                                            const float* B, float* out) {
+#pragma unroll
+    for (int c = 0; c < kSurOut; ++c) {
+        const float sc = __builtin_amdgcn_exp2f(sur_env_log2(&sh.lenv[0][0][0], N, tp, c, r));
+        out[c] = B[c] + sc * (y[c] * N.y_std[tp][c] + N.y_mean[tp][c]);
+    }
+}
+
+// The inference path on validation samples: err[(tp * n_val + s) * 6 + c] =
+// |prediction - quadrature| of output c.  Persistent wavefronts over chunks of
+// 16 samples of every type pair.
+__global__ __launch_bounds__(kSurBlock) void k_sur_judge(FFParams<float> PQ, SurDev S, const float* __restrict__ quad,
+                                                         const float* __restrict__ X, const float* __restrict__ T,
+                                                         long n_val, float* __restrict__ err) {
+    constexpr int NT = 2;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sur_lds_raw[];
+    SurLds& sh = *reinterpret_cast<SurLds*>(sur_lds_raw);
+    sur_load_lds(sh, S);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tp = lane / 16, k = lane % 16;
+    const long nchunk = (n_val + 15) / 16;
+    for (long ch = (long)blockIdx.x * kSurWaves + w; ch < nchunk; ch += (long)gridDim.x * kSurWaves) {
+        const long s = ch * 16 + k;
+        const bool valid = s < n_val;
+        const long id = (long)tp * n_val + (valid ? s : 0);
+        const float r = X[id * 3], ai = X[id * 3 + 1], aj = X[id * 3 + 2];
+        const float ia = 1.0f / ai + 1.0f / aj;
+        float B[6];
+        sur_baseline(PQ, quad + (size_t)tp * kQuadRow, quad_stride(NT), r, ia, B);
+        sh.u[w][lane] = make_float4((r - S.N.r_c) / S.N.r_s, (ai - S.N.a_c) / S.N.a_s, (aj - S.N.a_c) / S.N.a_s, 0.0f);
+        __builtin_amdgcn_wave_barrier();
+        sur_nets(sh, w, lane, 0xFu);
+        __builtin_amdgcn_wave_barrier();
+        float out[6];
+        sur_predict(sh, S.N, tp, r, sh.y[w][lane], B, out);
+        __builtin_amdgcn_wave_barrier();
+        if (valid)
+            for (int c = 0; c < kSurOut; ++c) err[id * kSurOut + c] = fabsf(out[c] - T[id * kSurOut + c]);
+    }
+}
+
+// pass 1 with the surrogate (DMD force, NT == 2).  The block's wavefronts are
+// persistent over the sites of their XCD range and never synchronise after the
+// set-up.  Per chunk of 16 pairs, lane tp*16 + k prepares sample (pair k, type
+// pair tp) (partner record, distance, trained box, fallback cell, baseline row);
+// the networks of the type pairs that serve at least one sample of the chunk
+// run; samples outside the trained box or in a masked cell take the FP32
+// quadrature instead.
 template <int NQ>
 __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
                                                          const double4* __restrict__ pos,
@@ -379,25 +545,11 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
     constexpr int NT = 2;
     extern __shared__ __attribute__((aligned(16))) unsigned char sur_lds_raw[];
     SurLds& sh = *reinterpret_cast<SurLds*>(sur_lds_raw);
-    {
-        const uint4* src = reinterpret_cast<const uint4*>(S.frag);
-        uint4* dst = reinterpret_cast<uint4*>(&sh.frag[0][0][0][0]);
-        for (int e = threadIdx.x; e < 4 * kFragPerTp * 64; e += kSurBlock) dst[e] = src[e];
-        for (int e = threadIdx.x; e < 4 * kHid; e += kSurBlock) {
-            const int m = e / kHid, o = e % kHid;
-            const float* Pm = S.params + (size_t)m * kNParam;
-            sh.w1[m][o] = make_float4(kKappa * Pm[kW1 + o], kKappa * Pm[kW1 + kHid + o],
-                                      kKappa * Pm[kW1 + 2 * kHid + o], kKappa * Pm[kB1 + o]);
-            sh.bias[m][0][o] = kKappa * Pm[kB2 + o];
-            sh.bias[m][1][o] = kKappa * Pm[kB3 + o];
-            sh.bias[m][2][o] = o < kOut ? Pm[kB4 + o] : 0.0f;
-        }
-    }
-    __syncthreads();
+    sur_load_lds(sh, S);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tp = lane / 16, k = lane % 16, it = tp / NT, jt = tp % NT;
-    const int g = lane / 16, n = lane % 16;  // network lanes: feature group, sample column
     const bool want_phi = it >= jt;
+    const float* tab = quad + (size_t)tp * kQuadRow;
     double en = 0.0, npair = 0.0;
     const SiteLoop sl(P.natms);
     for (int i0 = sl.first; i0 < sl.end; i0 += sl.stride) {
@@ -411,6 +563,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
         const double ai = ra.x, iai = ra.y;
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
+        int done = 0;  // inside entries of the earlier chunks (records at off[i] + rank)
         for (int chunk = p0; chunk < p1; chunk += 64) {
             int packed;
             int cnt;
@@ -467,73 +620,31 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 const int per_tp = kCellR * kCellA * kCellA / 32;
                 if ((S.mask[tp * per_tp + cell / 32] >> (cell % 32)) & 1u) use_net = false;
             }
-            sh.u[w][lane] = make_float4((rf - S.N.r_c) / S.N.r_s, (aif - S.N.a_c) / S.N.a_s,
-                                        (ajf - S.N.a_c) / S.N.a_s, 0.0f);
-            __builtin_amdgcn_wave_barrier();
-            // the four networks, 16 samples each; lane (g, n) holds features kfeat(., g, .) of sample n
-#pragma unroll 1
-            for (int t = 0; t < 4; ++t) {
-                const float4 uu = sh.u[w][t * 16 + n];
-                bf16x8_t b[2];
-#pragma unroll
-                for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float4 wt = sh.w1[t][kfeat(ks, g, e)];
-                        b[ks][e] = (__bf16)silu_scaled(wt.w + uu.x * wt.x + uu.y * wt.y + uu.z * wt.z);
-                    }
-#pragma unroll
-                for (int layer = 0; layer < 2; ++layer) {
-                    floatx4_t d[4];
-#pragma unroll
-                    for (int mt = 0; mt < 4; ++mt) {
-                        // the accumulator starts from the (scaled) biases of the lane's rows
-                        floatx4_t c = *reinterpret_cast<const floatx4_t*>(&sh.bias[t][layer][mt * 16 + g * 4]);
-#pragma unroll
-                        for (int ks = 0; ks < 2; ++ks) {
-                            const bf16x8_t a =
-                                *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(layer, mt, ks)][lane][0]);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
-                        }
-                        d[mt] = c;
-                    }
-#pragma unroll
-                    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) b[mt / 2][(mt % 2) * 4 + e] = (__bf16)silu_scaled(d[mt][e]);
-                }
-                floatx4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int ks = 0; ks < 2; ++ks) {
-                    const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&sh.frag[t][frag_index(2, 0, ks)][lane][0]);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks], c, 0, 0, 0);
-                }
-                // rows g*4 + e are outputs 0..3 (g = 0) and 4, 5 (g = 1)
-                if (g < 2) {
-                    const float4 bs = *reinterpret_cast<const float4*>(&sh.bias[t][2][g * 4]);
-                    float* y = &sh.y[w][t * 16 + n][g * 4];
-                    y[0] = c[0] * kInvKappa + bs.x;
-                    y[1] = c[1] * kInvKappa + bs.y;
-                    if (g == 0) {
-                        y[2] = c[2] * kInvKappa + bs.z;
-                        y[3] = c[3] * kInvKappa + bs.w;
-                    }
-                }
+            const unsigned need = group_any(use_net);
+            float out[6];
+            if (need) {
+                float B[6];
+                sur_baseline(PQ, tab, quad_stride(NT), rf, (float)ia, B);
+                sh.u[w][lane] = make_float4((rf - S.N.r_c) / S.N.r_s, (aif - S.N.a_c) / S.N.a_s,
+                                            (ajf - S.N.a_c) / S.N.a_s, 0.0f);
+                __builtin_amdgcn_wave_barrier();
+                sur_nets(sh, w, lane, need);
+                __builtin_amdgcn_wave_barrier();
+                sur_predict(sh, S.N, tp, rf, sh.y[w][lane], B, out);
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
             double gv[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
             if (use_net) {
-                const float* y = sh.y[w][lane];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    gv[c] = y[c] * S.N.y_std[tp][c] + S.N.y_mean[tp][c];
-                    fv[c] = y[3 + c] * S.N.y_std[tp][3 + c] + S.N.y_mean[tp][3 + c];
+                    gv[c] = out[c];
+                    fv[c] = out[3 + c];
                 }
             } else if (valid && br == 1) {
                 ++fallbacks;
                 float gq[3] = {0, 0, 0}, fq[3] = {0, 0, 0};
-                gh_avg<float, NQ, true>(PQ, rf, (float)r_inv, (float)rsqrt(ia), (float)ia,
-                                        quad + (size_t)tp * kQuadRow, quad_stride(NT), gq, fq);
+                gh_avg<float, NQ, true>(PQ, rf, (float)r_inv, (float)rsqrt(ia), (float)ia, tab, quad_stride(NT), gq,
+                                        fq);
                 for (int c = 0; c < 3; ++c) {
                     gv[c] = gq[c];
                     fv[c] = fq[c];
@@ -551,7 +662,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                     const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
                     en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * fv[0];
                 }
-                double* d = deriv + ((size_t)p * NT * NT + tp) * 4;
+                double* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + tp) * 4;
                 __builtin_nontemporal_store(gv[1], d);
                 __builtin_nontemporal_store(gv[2], d + 1);
                 __builtin_nontemporal_store(fv[1], d + 2);
@@ -560,6 +671,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
             ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
             if (valid && it == 0 && j < P.natms) pairE[(size_t)rpos[p] * NT + jt] = ej;  // scattered: L2 merges lines
         }
+        done += cnt;
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {

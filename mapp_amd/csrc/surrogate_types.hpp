@@ -4,6 +4,9 @@
 namespace mappdmd {
 
 constexpr int kSurOut = 6;
+// r bins of the residual envelope: the networks learn (quadrature - baseline) / s(r),
+// with log2 s(r) interpolated linearly between bin centres
+constexpr int kEnvBins = 32;
 
 // trained box and standardisation of the surrogate networks
 struct SurNorm {

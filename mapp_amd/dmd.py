@@ -70,6 +70,8 @@ _SIGS = {
     "mapp_dmd_surrogate_gradient": [CTX, FP, FP, C.c_long, FP, FP, FP, FP],
     "mapp_dmd_surrogate_params": [CTX, FP, FP],
     "mapp_dmd_surrogate_fallbacks": [CTX, LP],
+    "mapp_dmd_surrogate_set_bound": [CTX, D, DP],
+    "mapp_dmd_surrogate_cell_bounds": [CTX, DP, C.c_long],
     "mapp_dmd_lj_coef": [CTX, C.c_int, C.c_int, D, D, D],
     "mapp_dmd_fs_coef": [CTX, DP, DP, DP, DP, DP, DP, DP, DP],
     "mapp_dmd_meam_coef": [CTX, C.c_void_p],
@@ -444,6 +446,19 @@ class ForceFieldDMD:
         norm = np.zeros(SUR_NNORM, dtype=np.float32)
         self._call("mapp_dmd_surrogate_params", _fp(params), _fp(norm))
         return params, norm
+
+    def surrogate_set_bound(self, bound):
+        """re-derive the fallback mask for a relative force bound: dict served (cell fraction), eps (six
+        per-pair tolerances), bound, scales (F, F', F'', N, max|drho/dr|, max|drho/dalpha|, |U|, natms)"""
+        rep = np.zeros(16)
+        self._call("mapp_dmd_surrogate_set_bound", float(bound), _dp(rep))
+        return dict(served=rep[0], eps=rep[1:7].copy(), bound=rep[7], scales=rep[8:16].copy())
+
+    def surrogate_cell_bounds(self):
+        """smallest relative force bound at which each cell is served: [type pair][r cell][alpha_i cell][alpha_j cell]"""
+        out = np.zeros(4 * 64 * 4 * 4)
+        self._call("mapp_dmd_surrogate_cell_bounds", _dp(out), out.size)
+        return out.reshape(4, 64, 4, 4)
 
     def surrogate_fallbacks(self):
         n = C.c_long(0)

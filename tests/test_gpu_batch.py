@@ -80,3 +80,67 @@ def test_autotune_keeps_results(batch):
             assert np.array_equal(f.view(np.uint64), f0.view(np.uint64)), sched
         else:  # same cubic form, another lane split: rounding may differ in the last bits
             assert rel_inf(f, f0) <= 1e-13, sched
+
+
+# eight uneven cells (108 to 864 sites), the shape of the batched workload at test size
+UNEVEN = [(3, 31), (4, 32), (5, 33), (3, 34), (4, 35), (6, 36), (3, 37), (5, 38)]
+
+
+def test_batch_bdf_matches_oracle(small_tables):
+    """the batched BDF run (one step size and order for the batch, costs and dot
+    products over all cells) against the oracle's restatement of the same
+    batch-total step control (orc_bdf_run_cells), with an initial step at which
+    the controller holds the step size long enough to raise the order to 2
+    (Clock_BDF::run raises it only after ord + 2 steps of constant size,
+    src/clock_bdf.cpp:582-664).  Bars as the single-cell BDF test: same orders and
+    solves per step, step sizes within 1e-8, every cell's final c within 1e-8."""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    from oracle import oracle as O
+    lats = [W.fcc_alloy(n, seed=s) for n, s in UNEVEN]
+    ff = setup_cells(lats, small_tables, no_i=3, variant="DMD_II")
+    orcs = [O.make(l, small_tables, no_i=3, variant="DMD_II") for l in lats]
+    n = 12
+    code_g, rep_g, ng_g = ff.bdf_run(n, 2, initial_del_t=8.0)
+    code_o, rep_o, ng_o = O.bdf_run_cells(orcs, n, 2, initial_del_t=8.0)
+    print("gpu", code_g, ng_g, "oracle", code_o, ng_o, flush=True)
+    print(np.column_stack([rep_g[:, [0, 1, 3, 5]], rep_o[:, [0, 1, 3, 5]]]), flush=True)
+    assert code_g == code_o == n
+    np.testing.assert_array_equal(rep_g[:, 0], rep_o[:, 0])
+    np.testing.assert_array_equal(rep_g[:, 5], rep_o[:, 5])
+    np.testing.assert_allclose(rep_g[:, 1], rep_o[:, 1], rtol=1e-8)
+    assert rep_g[:, 0].max() == 2
+    c_g = ff.get("c")
+    off = 0
+    for l, o in zip(lats, orcs):
+        assert np.max(np.abs(c_g[off:off + l.natms] - o.get("c"))) <= 1e-8
+        off += l.natms
+    ff.close()
+
+
+def test_batch_surrogate_all_fallback_matches_cells(tables):
+    """the batched surrogate + FP32 launch over eight uneven cells with every cell of
+    (r, alpha_i, alpha_j) marked for fallback: each cell's slice within the mixed
+    path's bars (forces 1e-5, energy 1e-6) of its own FP64 oracle run (5-point rule)"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    from oracle import oracle as O
+    lats = [W.fcc_alloy(n, seed=s) for n, s in UNEVEN]
+    ff = setup_cells(lats, tables, no_i=5, variant="DMD_I", precision="mixed")
+    rep = ff.surrogate_train(n_per_tp=4096, epochs=1, batch_per_tp=256, seed=5, tol=-1.0)
+    assert rep["masked_frac"] == 1.0
+    ff.set_precision("surrogate")
+    en = ff.force_calc(1)
+    assert ff.surrogate_fallbacks() > 0
+    f = ff.get("f")
+    off, e_tot = 0, 0.0
+    for l in lats:
+        o = O.make(l, tables, no_i=5, variant="DMD_I")
+        eo = o.force_calc(1)
+        e_tot += eo[0]
+        err = rel_inf(f[off:off + l.natms], o.get("f"))
+        print(l.natms, "sites: force rel", err, flush=True)
+        assert err <= 1e-5
+        off += l.natms
+    assert abs(en[0] - e_tot) <= 1e-6 * abs(e_tot)
+    ff.close()

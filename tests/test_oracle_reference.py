@@ -100,8 +100,9 @@ def test_oracle_matches_reference(path, q, variant):
 def test_plausible_mistakes_fail_the_bar(path):
     """what a plausible restatement mistake produces is far from rounding:
     swapping the density slots (a transposed index), one width per site
-    instead of one per (site, type) (what would hide the slot transposition)
-    or another variant's mobility moves forces or rates by more than 1e-6"""
+    instead of one per (site, type) (what would hide the slot transposition),
+    another variant's mobility or the corrected dE index of the chemical
+    potential moves forces or rates by more than 1e-6"""
     d = R.load(path)
     ref_f = d["DMD_I_q3_f"]
     ref_cd = d["DMD_I_q3_c_d"]
@@ -134,6 +135,13 @@ def test_plausible_mistakes_fail_the_bar(path):
     o3.init(skin)
     o3.force_calc(1)
     assert rel_inf(o3.get("f"), ref_f) > 1e-6
+    # the corrected dE[atom*nt+type] index in place of the reference's dE[jatm+jtype]
+    o4 = _oracle(d, 3, "DMD_I")
+    o4.set_fix_dE_index(True)
+    o4.create_2nd_neigh_lst()
+    o4.force_calc(1)
+    o4.c_d_calc()
+    assert rel_inf(o4.get("c_d"), ref_cd) > 1e-6
 
 
 HARNESS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_harness")
