@@ -346,6 +346,20 @@ int mapp_dmd_surrogate_params(mapp_dmd_t* ctx, float* params, float* norm);
    (when the mask serves no cell, force_calc runs the mixed path's pass 1 and every in-cut evaluation
    counts) */
 int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count);
+/* calc_g without any allocation: no counterpart in the reference.  For callers
+   that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc): the same computation and
+   result as mapp_dmd_calc_g, with the cost reduction's scratch in a
+   file-scope device array of the library (about 2.1 MB per GPU, for systems
+   of up to 16,777,216 owned sites; larger ones get MAPP_DMD_E_RANGE).  a_dev,
+   g_dev: device vectors [natms][nt] (both required).  The pair values must be
+   current: a mapp_dmd_c_d_calc or mapp_dmd_calc_g since the last change of
+   positions or widths (else MAPP_DMD_E_STATE); concentrations may change
+   (mapp_dmd_set_c).  Calls on one GPU share the array, so calls that overlap
+   on one GPU race. */
+int mapp_dmd_calc_g_noalloc(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev, double* g_dev,
+                            double* cost);
+
 /* re-derives the fallback mask of the trained networks for another relative
    force bound without retraining.  Per-pair tolerances, from first-order error
    propagation with the triangle inequality: with N the largest number of

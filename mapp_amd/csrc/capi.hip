@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 
 #include "context.hpp"
 #include "geometry.hpp"
@@ -1535,6 +1536,37 @@ void init_batch(mapp_dmd* ctx, double skin) {
     ctx->cell_nghost.assign(ncell, 0);
     for (int k = 0; k < ncell; ++k) ctx->cell_nghost[k] = gh_base[k + 1] - gh_base[k];
 }
+// ---------------------------------------------------------- no-alloc calc_g
+// Scratch of mapp_dmd_calc_g_noalloc: calc_g's per-block partials of the cost
+// sweep over the 2nd list, the two-stage reduction's stage and its output, for
+// systems of up to kNoAllocMaxSites owned sites (about 2.1 MB; each GPU the
+// library runs on holds its own copy until the process ends).  Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU race.  Each call
+// writes every element it reads (the partials of its own blocks, then the
+// stages), so nothing has to start at zero.
+constexpr long kNoAllocMaxSites = 1L << 24;
+constexpr long kNoAllocParts = (kNoAllocMaxSites * kList2G + kBlock - 1) / kBlock;
+constexpr long kNoAllocDoubles = kNoAllocParts + ((kNoAllocParts + kReduceChunk - 1) / kReduceChunk + 1) * 8 + 16;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// the device address of scratch on each GPU, looked up once per device at
+// context creation (the array's host-side name is not a device address)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Nothing else in the library uses the array, so a failed lookup does not fail
+// context creation: the no-alloc entry point then reports it.
+hipError_t g_scratch_err[kMaxDevices] = {};
+void scratch_setup(int device) {  // device current
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_ptr[device] = static_cast<double*>(p);
+    else (void)hipGetLastError();  // clear the sticky error for the context's other calls
+}
 
 // Scales of the current system that turn a relative force bound into per-pair
 // tolerances (sur_mask).  One FP32-quadrature force_calc gives F = max|f| over
@@ -1737,6 +1769,7 @@ int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
         HIP_OK(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, MAPP_DMD_E_ARG, "no such HIP device");
         HIP_OK(hipSetDevice(device));
+        scratch_setup(device);
         HIP_OK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto& e : ctx->ev) HIP_OK(hipEventCreate(&e));
         ctx->d_iflag.reserve(4);
@@ -2551,6 +2584,31 @@ int mapp_dmd_surrogate_fallbacks(mapp_dmd_t* ctx, long* count) {
     });
 }
 
+
+int mapp_dmd_calc_g_noalloc(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev, double* g_dev,
+                            double* cost) {
+    return guarded(ctx, [&]() {
+        require(!is_md(ctx), MAPP_DMD_E_STATE, "calc_g is a DMD entry point");
+        require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede calc_g");
+        require(ctx->cache_valid, MAPP_DMD_E_STATE,
+                "stale pair values: call mapp_dmd_c_d_calc or mapp_dmd_calc_g once after the last change of "
+                "positions or widths (building them allocates)");
+        require(a_dev && g_dev && cost, MAPP_DMD_E_ARG, "a and g (device pointers) and cost must be given");
+        require(ctx->natms <= kNoAllocMaxSites, MAPP_DMD_E_RANGE,
+                "more owned sites than the no-alloc scratch holds (16,777,216)");
+        HIP_OK(hipSetDevice(ctx->device));
+        require(ctx->device < kMaxDevices, MAPP_DMD_E_RANGE, "device index beyond the scratch table");
+        double* base = g_scratch_ptr[ctx->device];
+        if (!base)
+            throw ApiError(MAPP_DMD_E_HIP, std::string("no device address of the scratch array on this device: ") +
+                                               hipGetErrorString(g_scratch_err[ctx->device]));
+        const long nb2 = calc_g_blocks(ctx);
+        const GScratch S{base, base + nb2, base + nb2 + reduce_stage_doubles(nb2)};
+        *cost = ctx->nt == 1 ? calc_g_variant<1>(ctx, chk, alpha, a_dev, g_dev, &S)
+                             : calc_g_variant<2>(ctx, chk, alpha, a_dev, g_dev, &S);
+        return 0;
+    });
+}
 
 int mapp_dmd_surrogate_cell_bounds(mapp_dmd_t* ctx, double* out, long cap) {
     return guarded(ctx, [&]() {

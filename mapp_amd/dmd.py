@@ -72,6 +72,7 @@ _SIGS = {
     "mapp_dmd_surrogate_fallbacks": [CTX, LP],
     "mapp_dmd_surrogate_set_bound": [CTX, D, DP],
     "mapp_dmd_surrogate_cell_bounds": [CTX, DP, C.c_long],
+    "mapp_dmd_calc_g_noalloc": [CTX, C.c_int, D, C.c_void_p, C.c_void_p, DP],
     "mapp_dmd_lj_coef": [CTX, C.c_int, C.c_int, D, D, D],
     "mapp_dmd_fs_coef": [CTX, DP, DP, DP, DP, DP, DP, DP, DP],
     "mapp_dmd_meam_coef": [CTX, C.c_void_p],
@@ -298,6 +299,16 @@ class ForceFieldDMD:
         g_dev_ptr = self.device_ptr("g") if g_dev_ptr is None else g_dev_ptr
         cost = C.c_double(0.0)
         self._call("mapp_dmd_calc_g", int(chk), float(alpha), C.c_void_p(a_dev_ptr), C.c_void_p(g_dev_ptr),
+                   C.byref(cost))
+        return cost.value
+
+    def calc_g_noalloc(self, chk, alpha, a_dev_ptr=None, g_dev_ptr=None):
+        """calc_g with the library's static scratch (no device allocation); the pair values must be
+        current (a c_d_calc or calc_g since the last position change)"""
+        a_dev_ptr = self.device_ptr("a") if a_dev_ptr is None else a_dev_ptr
+        g_dev_ptr = self.device_ptr("g") if g_dev_ptr is None else g_dev_ptr
+        cost = C.c_double(0.0)
+        self._call("mapp_dmd_calc_g_noalloc", int(chk), float(alpha), C.c_void_p(a_dev_ptr), C.c_void_p(g_dev_ptr),
                    C.byref(cost))
         return cost.value
 
