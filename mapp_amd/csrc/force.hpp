@@ -226,19 +226,32 @@ constexpr int kDppRowRor12 = 0x12C;  // row_ror:12: lane l reads lane (l + 4) mo
 constexpr int kDppQuad0022 = 0xA0;   // quad_perm [0,0,2,2]: lane q = 1 reads lane q = 0
 constexpr int kDppQuad0000 = 0x00;   // quad_perm [0,0,0,0]: every lane of a quad reads q = 0
 
+// v summed with the same lane of the paired row (rows 0+1, 2+3: permlane16
+// swap) or of the other half of the wavefront (permlane32 swap); both lanes of
+// a pair add in the same order, so they hold the same bits
+template <bool HALVES>
+__device__ __forceinline__ double row_pair_sum(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const auto a = HALVES ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                          : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = HALVES ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                          : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+
 // Sums over the 16 quads of a wavefront, per position q in the quad, of two
 // registers at once: afterwards lane q of the wavefront holds, in A and B, the
 // totals of A and B over the lanes 4k + q (fixed tree: two rotations within each
-// row of 16, then the rows by xor 16 and 32; the result is read from lane q).
+// row of 16, then the rows in pairs and the halves; the result is read from lane q).
 __device__ __forceinline__ void quad_slot_sums(double& A, double& B) {
     A += dpp_f64<kDppRowRor4>(A);
     B += dpp_f64<kDppRowRor4>(B);
     A += dpp_f64<kDppRowRor8>(A);
     B += dpp_f64<kDppRowRor8>(B);
-    A += __shfl_xor(A, 16, 64);
-    B += __shfl_xor(B, 16, 64);
-    A += __shfl_xor(A, 32, 64);
-    B += __shfl_xor(B, 32, 64);
+    A = row_pair_sum<false>(A);
+    B = row_pair_sum<false>(B);
+    A = row_pair_sum<true>(A);
+    B = row_pair_sum<true>(B);
 }
 
 template <int NT>
@@ -804,8 +817,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             double e = rho_lane ? Ei : 0.0;
             e += dpp_f64<kDppXor2>(e);
             e += dpp_f64<kDppRowRor8>(e);
-            e += __shfl_xor(e, 16, 64);
-            e += __shfl_xor(e, 32, 64);
+            e = row_pair_sum<false>(e);
+            e = row_pair_sum<true>(e);
             const double e0 = read_lane(e, 0), e1 = read_lane(e, 4);
             if (lane == 0) {
                 Eown[(size_t)i * NT] = e0;
