@@ -220,13 +220,23 @@ def run_batch(tables, steps, warmup, ncell=96, bdf_steps=20, seed=5):
     code, brep, ncg = ff.bdf_run(bdf_steps, 2)
     bdf_s = time.perf_counter() - t0
     ff.close()
+    # the same cells with the DMD_II mobility, from an initial step at which the
+    # controller holds the step size and raises the order (the DMD_I runs above stay at
+    # order 1: their solver's cost test caps the step where the error is far below 1)
+    ff2 = setup_cells(lats, tables, no_i=5, variant="DMD_II", precision="fp64")
+    t0 = time.perf_counter()
+    code2, brep2, ncg2 = ff2.bdf_run(bdf_steps, 2, initial_del_t=8.0)
+    bdf2_s = time.perf_counter() - t0
+    ff2.close()
     return dict(cells=ncell, sites=natms, sites_min=int(min(l.natms for l in lats)),
                 sites_max=int(max(l.natms for l in lats)), build_seconds=build_s,
                 force_site_evals_per_s=natms / (r["ms_per_step"] / 1e3), force_ms=r["ms_per_step"],
                 kernel_ms=r["kernel_ms"][:4], schedule_candidates=int(len(rows)), surrogate_val_err=rep["val_mean_err"],
                 surrogate_bound=5e-6, surrogate_masked_cells=rep["masked_frac"],
                 bdf_steps=code, bdf_seconds=bdf_s, bdf_calc_g_calls=ncg,
-                bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [])
+                bdf_orders=[int(v) for v in brep[:, 0]] if code > 0 else [],
+                bdf_dmd_ii=dict(steps=code2, seconds=bdf2_s, calc_g_calls=ncg2, initial_del_t=8.0,
+                                orders=[int(v) for v in brep2[:, 0]] if code2 > 0 else []))
 
 
 def run_meam(n, steps, warmup, cpu_n=8):
@@ -454,6 +464,8 @@ def main():
     ap.add_argument("--no-meam", action="store_true")
     ap.add_argument("--meam-only", action="store_true", help="time MEAM force_calc alone (profiling)")
     ap.add_argument("--md-only", action="store_true", help="time the EAM, LJ, FS and FSM force_calc alone")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="train the surrogate and time force_calc with every cell served (profiling)")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -473,6 +485,17 @@ def main():
     if args.md_only:
         from mapp_amd import workload as W
         print(json.dumps({"md": run_md(args.cells, W.johnson_tables(), args.steps, args.warmup)}), flush=True)
+        return
+    if args.surrogate_only:
+        from mapp_amd import workload as W
+        from mapp_amd.dmd import setup
+        lat = W.fcc_alloy(args.cells, seed=1)
+        ff = setup(lat, W.johnson_tables(), no_i=5, precision="mixed")
+        ff.surrogate_train(n_per_tp=1 << 20, epochs=4, batch_per_tp=1024, lr=2e-3, seed=3, tol=1e30)
+        ff.set_precision("surrogate")
+        r = time_force(ff, args.steps, args.warmup)
+        ff.close()
+        print(json.dumps({"surrogate_all_served": {k: v for k, v in r.items()}}), flush=True)
         return
 
     from mapp_amd import workload as W
