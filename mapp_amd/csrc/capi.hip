@@ -630,11 +630,15 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     // the surrogate's pass 1: one block of kSurWaves wavefronts per CU, persistent
     const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
-    ctx->d_part.reserve((size_t)nb1 * 2 + (size_t)nbw * 6 + nb + 16);
+    // partials: one per wavefront in pass 1, the embedding and pass 2 (one per block
+    // in the surrogate's pass 1)
+    const long np1 = sur ? (long)nb1 : (long)nb1 * ctx->sh1.wpb;
+    const long np2 = (long)nb * (kBlock / 64), np3 = (long)nbw * ctx->sh2.wpb;
+    ctx->d_part.reserve((size_t)np1 * 2 + (size_t)np3 * 6 + (size_t)np2 + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
-    double* part2 = part1 + (size_t)nb1 * 2;
-    double* part3 = part2 + (size_t)nb;
+    double* part2 = part1 + (size_t)np1 * 2;
+    double* part3 = part2 + (size_t)np2;
     Q* deriv = reinterpret_cast<Q*>(ctx->d_deriv.p);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[0], st));
     pack_atoms<NT>(ctx);
@@ -681,9 +685,9 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
                        ctx->d_fa.p, ctx->d_f.p);
     launch_check("k_fgather");
     double* out = ctx->d_out.p;
-    reduce_partials(ctx, part1, nb1, 2, 2, 0, out);
-    reduce_partials(ctx, part2, nb, 1, 1, 0, out + 2);
-    reduce_partials(ctx, part3, nbw, 6, 6, 0, out + 3);
+    reduce_partials(ctx, part1, np1, 2, 2, 0, out);
+    reduce_partials(ctx, part2, np2, 1, 1, 0, out + 2);
+    reduce_partials(ctx, part3, np3, 6, 6, 0, out + 3);
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[4], st));
     double h[9];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -709,10 +713,11 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     const int nb = nblk((long)ctx->natms * kGatherG);
     const int nbw = shape_blocks(ctx->sh1, ctx->natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
-    ctx->d_part.reserve((size_t)nbw * 8 + nb + 16);
+    const long np1 = (long)nbw * ctx->sh1.wpb, np2 = (long)nb * (kBlock / 64);  // one partial per wavefront
+    ctx->d_part.reserve((size_t)np1 * 2 + (size_t)np2 + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
-    double* part2 = part1 + (size_t)nbw * 2;
+    double* part2 = part1 + (size_t)np1 * 2;
     pack_atoms<NT>(ctx);
     if (ctx->energy_full) launch_pass1<NT, Q, NQ, true, true>(ctx, dim3(nbw), quad, PQ, L, (Q*)nullptr, part1);
     else launch_pass1<NT, Q, NQ, true, false>(ctx, dim3(nbw), quad, PQ, L, (Q*)nullptr, part1);
@@ -722,8 +727,8 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
                        ctx->d_arec.p, part2);
     launch_check("k_embed energy");
     double* out = ctx->d_out.p;
-    reduce_partials(ctx, part1, nbw, 1, 2, 0, out);
-    reduce_partials(ctx, part2, nb, 1, 1, 0, out + 1);
+    reduce_partials(ctx, part1, np1, 1, 2, 0, out);
+    reduce_partials(ctx, part2, np2, 1, 1, 0, out + 1);
     double h[2];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));

@@ -432,6 +432,19 @@ __device__ __forceinline__ void block_store_waves(const double* v, double* __res
     }
 }
 
+// Per-wavefront partials without a block barrier: lane 0 of wavefront w of
+// block b stores the wave's NV totals at out[(b * waves per block + w) * NV].
+// A wave that finishes early leaves at once instead of holding its slot until
+// the block's slowest wave reaches a barrier.
+template <int NV>
+__device__ __forceinline__ void wave_store(const double* v, double* __restrict__ out) {
+    if ((threadIdx.x & 63) == 0) {
+        const size_t w = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[w * NV + k] = v[k];
+    }
+}
+
 // Sites of one wavefront.  Dispatch places block b on XCD b % 8; the sites are
 // split into 8 contiguous ranges, one per XCD, so spatial neighbours share an
 // L2, and the waves of an XCD stride through its range.  With one wave per
@@ -833,7 +846,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
     }
     double acc[2] = {wave_sum(en), npair};  // npair: lane 0 only
-    block_store_waves<2>(acc, part);
+    wave_store<2>(acc, part);
 }
 
 // ---------------------------------------------------------------- lane groups
@@ -918,7 +931,8 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
             acc[0] += 1.5 * P.kbT * c * log(a) + c * P.c0[t];
         }
     }
-    block_sum_store<1>(acc, part);
+    acc[0] = wave_sum(acc[0]);
+    wave_store<1>(acc, part);
 }
 
 // ---------------------------------------------------------------- pass 2
@@ -1109,7 +1123,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
         for (int c = 0; c < 6; ++c) vir[c] = wave_sum(vir[c]);
     }
-    block_store_waves<6>(vir, part);
+    wave_store<6>(vir, part);
 }
 
 
