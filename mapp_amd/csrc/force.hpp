@@ -572,6 +572,30 @@ struct Pass1Stage {
     }
 };
 
+// Quadrature-table rows through buffer loads: each point costs a 32-bit byte
+// offset (one multiply-add) instead of a 64-bit address, and the resource's
+// size bounds every access.
+template <typename Q2>
+struct RowLoad;
+template <>
+struct RowLoad<double2> {
+    static __device__ __forceinline__ double2 get(__amdgpu_buffer_rsrc_t r, int off) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        return make_double2(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+    }
+};
+template <>
+struct RowLoad<float2> {
+    static __device__ __forceinline__ float2 get(__amdgpu_buffer_rsrc_t r, int off) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return make_float2(__int_as_float((int)v[0]), __int_as_float((int)v[1]));
+    }
+};
+// raw buffer (stride 0, 32-bit data format) over n bytes
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, int nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, nbytes, 0x00020000);
+}
+
 // ENERGY_ONLY: the loop of ForceField_DMD_I::energy_calc (src/ff_dmd_I.cpp:454-668),
 // type pairs jt >= it only (:508) unless FULL, which sums every ordered pair so the
 // energy equals force_calc's.
@@ -605,7 +629,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
         const double4 ra = arec[(size_t)i * NT + it];
         const double ai = ra.x, iai = ra.y;
-        const Q* tab = quad + (size_t)q * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS));
+        const __amdgpu_buffer_rsrc_t qrs = raw_rsrc(quad, P.nr * quad_stride(NT) * (int)sizeof(Q));
+        const int tab_off = (q * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS))) * (int)sizeof(Q);
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         int done = 0;  // inside entries of the earlier chunks: records go to off[i] + rank
@@ -701,9 +726,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         m = m > 0 ? m : 0;
                         pp -= (Q)m;
                         ppv[kp] = pp < Q(1) ? pp : Q(1);
-                        const Q* row = tab + (size_t)m * quad_stride(NT);
+                        const int row = tab_off + m * (quad_stride(NT) * (int)sizeof(Q));
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) rowv[kp][v] = *reinterpret_cast<const Q2*>(row + 2 * v);
+                        for (int v = 0; v < NV; ++v) rowv[kp][v] = RowLoad<Q2>::get(qrs, row + v * (int)sizeof(Q2));
                     }
                     // keep the requests ahead of the arithmetic where the rows fit the
                     // register budget of four waves per SIMD (at most 24 registers)
