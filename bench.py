@@ -429,6 +429,48 @@ def cpu_baseline(tables, n=32, reps=3, threads=None):
                                    sample=f"the same call on one {n}^3 box, 1 thread, median of {reps}"))
 
 
+def run_small(tables, n=10, reps=5):
+    """The 4,000-site system (10^3 FCC cells), one energy+force+dE/dalpha+dc/dt evaluation:
+    force_calc(1) and c_d_calc on the GPU and, unless no_cpu, on one host thread with the
+    FP64 oracle; medians of reps calls."""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup
+    lat = W.fcc_alloy(n, seed=1)
+    ff = setup(lat, tables, no_i=3, precision="fp64")
+    ff.force_calc(1)
+    ff.create_2nd_neigh_lst()
+    ff.c_d_calc()
+    tf, tc = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ff.force_calc(1)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ff.c_d_calc()
+        tc.append(time.perf_counter() - t0)
+    ff.close()
+    return dict(sites=lat.natms, force_ms=float(np.median(tf)) * 1e3, c_d_ms=float(np.median(tc)) * 1e3,
+                site_evals_per_s=lat.natms / float(np.median(tf)), lat=lat)
+
+
+def cpu_small(lat, tables, reps=5):
+    """the same evaluation with the FP64 oracle on one host thread"""
+    from oracle import oracle as O
+    orc = O.make(lat, tables, no_i=3, variant="DMD_I")
+    orc.force_calc(1)
+    orc.create_2nd_neigh_lst()
+    tf, tc = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        orc.force_calc(1)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        orc.c_d_calc()
+        tc.append(time.perf_counter() - t0)
+    return dict(cores=1, force_ms=float(np.median(tf)) * 1e3, c_d_ms=float(np.median(tc)) * 1e3,
+                site_evals_per_s=lat.natms / float(np.median(tf)))
+
+
 def copy_calibration(mib=1024, reps=5):
     """a plain device copy of known size, profiled with the same HBM counters as the kernels"""
     import torch
@@ -607,7 +649,11 @@ def main():
             log("meam", out["meam"])
             out["md"] = run_md(args.cells, tables, max(3, args.steps // 4), 1)
             log("md", out["md"])
+    small = run_small(tables)
+    small_lat = small.pop("lat")
+    out["small_4000"] = small
     if not args.no_cpu:
+        out["small_4000"]["cpu_oracle"] = cpu_small(small_lat, tables)
         out["cpu_baseline"] = cpu_baseline(tables)
         out["cpu_baseline"]["single_thread_full_size"] = cpu_full_size(lat, tables)
         log("cpu", out["cpu_baseline"])
