@@ -144,3 +144,42 @@ def test_batch_surrogate_all_fallback_matches_cells(tables):
         off += l.natms
     assert abs(en[0] - e_tot) <= 1e-6 * abs(e_tot)
     ff.close()
+
+
+def test_batch_surrogate_trained_meets_bound(tables):
+    """the batched surrogate + FP32 launch over eight uneven cells with networks trained on
+    the batch, under the enforced relative force bound: the batch's forces within the bound
+    (plus the FP32 quadrature's own 3e-6) of the cells' FP64 oracle runs, at 5e-6 (then
+    within 1e-5) and at 1e-2, with the share of the quadrature evaluations the networks
+    served printed; a looser bound never serves less"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    from oracle import oracle as O
+    lats = [W.fcc_alloy(n, seed=s) for n, s in UNEVEN]
+    ff = setup_cells(lats, tables, no_i=5, variant="DMD_I", precision="mixed")
+    ff.surrogate_train(n_per_tp=1 << 18, epochs=8, batch_per_tp=512, seed=7, tol=5e-6)
+    f_o = []
+    for l in lats:
+        o = O.make(l, tables, no_i=5, variant="DMD_I")
+        o.force_calc(1)
+        f_o.append(o.get("f"))
+    f_o = np.concatenate(f_o)
+    ff.surrogate_set_bound(-1.0)
+    ff.set_precision("surrogate")
+    ff.force_calc(1)
+    total = ff.surrogate_fallbacks()
+    assert total > 0
+    shares = []
+    for bound in (5e-6, 1e-2):
+        info = ff.surrogate_set_bound(bound)
+        ff.force_calc(1)
+        share = 1.0 - ff.surrogate_fallbacks() / total
+        shares.append(share)
+        err = rel_inf(ff.get("f"), f_o)
+        print(f"bound {bound:.0e}: served {share:.4f} of the evaluations ({info['served']:.4f} of the cells), "
+              f"force rel {err:.3e}", flush=True)
+        assert err <= bound + 3e-6
+        if bound == 5e-6:
+            assert err <= 1e-5
+    assert shares[1] >= shares[0]
+    ff.close()

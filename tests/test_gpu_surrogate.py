@@ -111,7 +111,7 @@ def _served_share(ff, bound):
     return share, info
 
 
-@pytest.mark.parametrize("n", [6, 8])
+@pytest.mark.parametrize("n", [6, 8, 10])
 def test_trained_surrogate_meets_bound(tables, n):
     from mapp_amd import workload as W
     lat = W.fcc_alloy(n, seed=3 + n)
