@@ -205,6 +205,31 @@ struct PairBufs {
     const int* rev;
 };
 
+// Which half-list entries lay inside the quadrature cut at the last pass 1.
+// The partner shares (pairE of pass 1, pairF of pass 2) of an entry outside
+// the cut must read zero in the embedding and the force gather: they are
+// cleared when the list is built and again only when an entry leaves the cut,
+// instead of being rewritten with zeros at every evaluation.
+struct CutState {
+    unsigned char* st;
+    double* pairF;
+    // entry pc (partner jc, reverse position rp) is inside the cut now: record a change;
+    // on leaving, clear both shares of an owned partner
+    template <int NT>
+    __device__ __forceinline__ void update(int pc, bool in, int jc, int rp, int natms, double* pairE) const {
+        const bool was = st[pc] != 0;
+        if (was != in) {
+            st[pc] = in ? 1 : 0;
+            if (!in && jc < natms) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) pairE[(size_t)rp * NT + t] = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3 + NT; ++c) pairF[(size_t)rp * (3 + NT) + c] = 0.0;
+            }
+        }
+    }
+};
+
 // ---------------------------------------------------------------- lane mapping
 // NT*NT lanes per site, lane q = it*NT + jt owns the ordered type pair
 // (it, jt) of every pair of the site's list.  For NT = 2 the four lanes of a
@@ -606,7 +631,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                   const int* __restrict__ rpos, const Q* __restrict__ quad,
                                                   const double* __restrict__ rho7, const double* __restrict__ phi7,
                                                   double* __restrict__ Eown, double* __restrict__ pairE,
-                                                  Q* __restrict__ deriv, double* __restrict__ part) {
+                                                  Q* __restrict__ deriv, double* __restrict__ part,
+                                                  CutState C) {
     static_assert(PARTS == 1 || PARTS == 2 || PARTS == 4, "one, two or four lanes per (pair, type pair)");
     using Q2 = typename Vec2<Q>::T;
     constexpr int LPP = Pass1Map<NT, PARTS>::LPP, PPW = Pass1Map<NT, PARTS>::PPW;
@@ -655,10 +681,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                     }
                     in = r2 < P.cut_sq_mod;
-                    if (!in && jc < P.natms) {
-#pragma unroll
-                        for (int t = 0; t < NT; ++t) pairE[(size_t)rp * NT + t] = 0.0;
-                    }
+                    C.update<NT>(pc, in, jc, rp, P.natms, pairE);
                 }
                 const int dest = pack_dest(in, &cnt);
                 if (in) {
@@ -1007,11 +1030,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma clang fp contract(off)
                         r2 = e0 * e0 + e1 * e1 + e2 * e2;
                     }
-                    in = r2 < P.cut_sq_mod;
-                    if (!in && jc < P.natms) {
-#pragma unroll
-                        for (int c = 0; c < XD; ++c) pairF[(size_t)rp * XD + c] = 0.0;
-                    }
+                    in = r2 < P.cut_sq_mod;  // the shares of entries outside stay zero (CutState)
                 }
                 const int dest = pack_dest(in, &cnt);
                 pk_j = packed(dest, jc);

@@ -541,7 +541,8 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                                                          const double* __restrict__ rho7,
                                                          const double* __restrict__ phi7, double* __restrict__ Eown,
                                                          double* __restrict__ pairE, double* __restrict__ deriv,
-                                                         double* __restrict__ part, int* __restrict__ nfall) {
+                                                         double* __restrict__ part, int* __restrict__ nfall,
+                                                         CutState C) {
     constexpr int NT = 2;
     extern __shared__ __attribute__((aligned(16))) unsigned char sur_lds_raw[];
     SurLds& sh = *reinterpret_cast<SurLds*>(sur_lds_raw);
@@ -580,10 +581,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                         r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                     }
                     in = r2 < P.cut_sq_mod;
-                    if (!in && jc < P.natms) {
-#pragma unroll
-                        for (int t = 0; t < NT; ++t) pairE[(size_t)rpos[pc] * NT + t] = 0.0;
-                    }
+                    C.update<NT>(pc, in, jc, rpos[pc], P.natms, pairE);
                 }
                 cnt = pack_in_range(pc, in, &packed);
             }
