@@ -337,6 +337,12 @@ void finish_lists(mapp_dmd* ctx) {
     // reverse positions of the pairs with an owned partner (rpos) come with the ordering
     ctx->d_rpos.reserve(ctx->npairs + 1);
     sort_segments(ctx, ctx->d_roff.p, ctx->d_nl.p, ctx->d_rev.p, nrev, ctx->d_rpos.p);
+    ctx->d_nlrp.reserve(ctx->npairs + 1);
+    if (ctx->npairs > 0) {
+        hipLaunchKernelGGL(k_pack_nlrp, dim3(nblk(ctx->npairs)), dim3(kBlock), 0, st, ctx->d_nl.p, ctx->d_rpos.p,
+                           ctx->npairs, ctx->d_nlrp.p);
+        launch_check("k_pack_nlrp");
+    }
     ctx->d_pair_i.reserve(ctx->npairs + 1);
     hipLaunchKernelGGL(k_pair_owner, dim3(nblk(natms)), dim3(kBlock), 0, st, ctx->d_off.p, natms, ctx->d_pair_i.p);
     launch_check("k_pair_owner");
@@ -615,15 +621,15 @@ void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ
     const CutState C{ctx->d_cutst.p, ctx->d_pairF.p};
     if (ctx->sh1.parts == 1)
         hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_arec.p, L, ctx->d_nlrp.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part, C);
     else if (ctx->sh1.parts == 2)
         hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 2>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_arec.p, L, ctx->d_nlrp.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part, C);
     else
         hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 4>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
-                           ctx->d_arec.p, L, ctx->d_rpos.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
+                           ctx->d_arec.p, L, ctx->d_nlrp.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
                            ctx->d_pairE.p, deriv, part, C);
     launch_check("k_pass1");
 }
@@ -659,7 +665,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
             auto launch = [&](auto kern) {
                 HIP_OK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kSurLds));
                 hipLaunchKernelGGL(kern, dim3(nb1), dim3(kSurBlock), kSurLds, st, ctx->P, ctx->Pf, S, ctx->d_pos.p,
-                                   ctx->d_arec.p, L, ctx->d_rpos.p, ctx->d_quad32.p, ctx->d_rho7.p, ctx->d_phi7.p,
+                                   ctx->d_arec.p, L, ctx->d_nlrp.p, ctx->d_quad32.p, ctx->d_rho7.p, ctx->d_phi7.p,
                                    ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1, ctx->d_nfall.p,
                                    CutState{ctx->d_cutst.p, ctx->d_pairF.p});
             };
@@ -686,7 +692,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
     hipLaunchKernelGGL((k_pass2<NT, Q>), dim3(nbw), dim3(64 * ctx->sh2.wpb), 0, st, ctx->P, ctx->d_pos.p,
-                       ctx->d_arec.p, L, ctx->d_rpos.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
+                       ctx->d_arec.p, L, ctx->d_nlrp.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
     launch_check("k_pass2");
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,

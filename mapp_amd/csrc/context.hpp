@@ -67,6 +67,7 @@ struct mapp_dmd {
     // per half-list entry: inside the quadrature cut at the last pass 1; the shares of
     // entries outside it stay zero (cleared at list build, and when a pair leaves the cut)
     mappdmd::DevBuf<unsigned char> d_cutst;
+    mappdmd::DevBuf<int2> d_nlrp;  // per half-list entry {partner, reverse position}
     mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair values for c_d_calc / calc_g, per half-list pair
     mappdmd::DevBuf<int> d_toff, d_tidx;        // site table: both ends of every contributing pair
     mappdmd::DevBuf<double> d_trho, d_tphi;

@@ -422,6 +422,14 @@ __global__ void k_pack_atoms(const double* __restrict__ alpha, const double* __r
     }
 }
 
+// per half-list entry {partner, reverse position}: the force passes read both
+// with one load
+__global__ void k_pack_nlrp(const int* __restrict__ nl, const int* __restrict__ rpos, long n,
+                            int2* __restrict__ nlrp) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) nlrp[k] = make_int2(nl[k], rpos[k]);
+}
+
 // sum over the wavefront, the same value in every lane: a DPP rotation tree
 // inside each row of 16 lanes, then the four row sums in a fixed order
 __device__ __forceinline__ double read_lane(double v, int l) {
@@ -628,7 +636,7 @@ template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false, int P
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
-                                                  const int* __restrict__ rpos, const Q* __restrict__ quad,
+                                                  const int2* __restrict__ nlrp, const Q* __restrict__ quad,
                                                   const double* __restrict__ rho7, const double* __restrict__ phi7,
                                                   double* __restrict__ Eown, double* __restrict__ pairE,
                                                   Q* __restrict__ deriv, double* __restrict__ part,
@@ -672,8 +680,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 int jc = 0, rp = 0;
                 double r2 = 0.0;
                 if (pc < p1) {
-                    jc = L.nl[pc];
-                    rp = rpos[pc];  // meaningful for owned partners only
+                    const int2 e = nlrp[pc];
+                    jc = e.x;
+                    rp = e.y;  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
                     {
@@ -991,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
 template <int NT, typename Q>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
-                                                  const int* __restrict__ rpos, const Q* __restrict__ deriv,
+                                                  const int2* __restrict__ nlrp, const Q* __restrict__ deriv,
                                                   double* __restrict__ fown, double* __restrict__ pairF,
                                                   double* __restrict__ part) {
     constexpr int LPP = WaveMap<NT>::LPP, PPW = WaveMap<NT>::PPW;
@@ -1020,8 +1029,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 int jc = 0, rp = 0;
                 double e0 = 0.0, e1 = 0.0, e2 = 0.0, r2 = 0.0;
                 if (pc < p1) {
-                    jc = L.nl[pc];
-                    rp = rpos[pc];  // meaningful for owned partners only
+                    const int2 e = nlrp[pc];
+                    jc = e.x;
+                    rp = e.y;  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     e0 = xi.x - xj.x;
                     e1 = xi.y - xj.y;
