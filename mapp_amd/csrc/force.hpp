@@ -669,10 +669,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int p0 = L.off[i], p1 = L.off[i + 1];
         int done = 0;  // inside entries of the earlier chunks: records go to off[i] + rank
         for (int chunk = p0; chunk < p1; chunk += 64) {
-            // one list entry per lane: entries outside the cut get a zero partner
-            // share; each entry inside it is staged in LDS at its rank among the
-            // inside entries, with its partner's width, inverse width and
-            // concentration of every type, read from HBM once per entry
+            // one list entry per lane: the cut state of each entry is kept current
+            // (CutState); each entry inside the cut is staged in LDS at its rank
+            // among the inside entries, with its partner's width, inverse width
+            // and concentration of every type, read from HBM once per entry
             int cnt;
             {
                 const int pc = chunk + lane;
