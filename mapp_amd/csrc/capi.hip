@@ -611,6 +611,12 @@ inline int shape_blocks(const mapp_dmd::Shape& sh, int natms) {
     return (int)std::max<long>(1, (w + sh.wpb - 1) / sh.wpb);
 }
 
+// the DMD force shares inside d_pairF: nrev + 1 slots of 4 doubles, then nrev + 1 of nt - 1
+template <int NT>
+ForceShares force_shares(mapp_dmd* ctx) {
+    return ForceShares{ctx->d_pairF.p, ctx->d_pairF.p + (size_t)(ctx->nrev + 1) * 4};
+}
+
 // pass 1 with the schedule's lanes per (pair, type pair)
 template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL>
 void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ, const PairBufs& L, Q* deriv,
@@ -618,7 +624,7 @@ void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ
     const dim3 blk(64 * ctx->sh1.wpb);
     const size_t lds = (size_t)ctx->sh1.wpb * Pass1Stage<NT>::kBytes;  // one stage per wavefront
     hipStream_t st = ctx->stream;
-    const CutState C{ctx->d_cutst.p, ctx->d_pairF.p};
+    const CutState C{ctx->d_cutst.p, force_shares<NT>(ctx)};
     if (ctx->sh1.parts == 1)
         hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_nlrp.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
@@ -667,7 +673,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
                 hipLaunchKernelGGL(kern, dim3(nb1), dim3(kSurBlock), kSurLds, st, ctx->P, ctx->Pf, S, ctx->d_pos.p,
                                    ctx->d_arec.p, L, ctx->d_nlrp.p, ctx->d_quad32.p, ctx->d_rho7.p, ctx->d_phi7.p,
                                    ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1, ctx->d_nfall.p,
-                                   CutState{ctx->d_cutst.p, ctx->d_pairF.p});
+                                   CutState{ctx->d_cutst.p, force_shares<NT>(ctx)});
             };
             if (q == 5) launch(k_pass1_sur<5>);
             else if (q == 3) launch(k_pass1_sur<3>);
@@ -692,10 +698,10 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[2], st));
     hipLaunchKernelGGL((k_pass2<NT, Q>), dim3(nbw), dim3(64 * ctx->sh2.wpb), 0, st, ctx->P, ctx->d_pos.p,
-                       ctx->d_arec.p, L, ctx->d_nlrp.p, deriv, ctx->d_fown.p, ctx->d_pairF.p, part3);
+                       ctx->d_arec.p, L, ctx->d_nlrp.p, deriv, ctx->d_fown.p, force_shares<NT>(ctx), part3);
     launch_check("k_pass2");
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[3], st));
-    hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, ctx->d_pairF.p,
+    hipLaunchKernelGGL((k_fgather<NT>), dim3(nb), dim3(kBlock), 0, st, natms, L, ctx->d_fown.p, force_shares<NT>(ctx),
                        ctx->d_fa.p, ctx->d_f.p);
     launch_check("k_fgather");
     double* out = ctx->d_out.p;

@@ -205,6 +205,15 @@ struct PairBufs {
     const int* rev;
 };
 
+// Partner force shares of the DMD passes, one slot per reverse-list entry, in
+// two arrays: sh4[slot] = {fx, fy, fz, alpha force of type 0} (32 bytes, so a
+// pair's share is written with 16-byte stores) and shx[slot] = the alpha
+// forces of types 1..nt-1.
+struct ForceShares {
+    double* sh4;
+    double* shx;
+};
+
 // Which half-list entries lay inside the quadrature cut at the last pass 1.
 // The partner shares (pairE of pass 1, pairF of pass 2) of an entry outside
 // the cut must read zero in the embedding and the force gather: they are
@@ -212,7 +221,7 @@ struct PairBufs {
 // instead of being rewritten with zeros at every evaluation.
 struct CutState {
     unsigned char* st;
-    double* pairF;
+    ForceShares F;
     // entry pc (partner jc, reverse position rp) is inside the cut now: record a change;
     // on leaving, clear both shares of an owned partner
     template <int NT>
@@ -224,7 +233,9 @@ struct CutState {
 #pragma unroll
                 for (int t = 0; t < NT; ++t) pairE[(size_t)rp * NT + t] = 0.0;
 #pragma unroll
-                for (int c = 0; c < 3 + NT; ++c) pairF[(size_t)rp * (3 + NT) + c] = 0.0;
+                for (int c = 0; c < 4; ++c) F.sh4[(size_t)rp * 4 + c] = 0.0;
+#pragma unroll
+                for (int c = 0; c < NT - 1; ++c) F.shx[(size_t)rp * (NT - 1) + c] = 0.0;
             }
         }
     }
@@ -250,6 +261,7 @@ constexpr int kDppRowRor4 = 0x124;   // row_ror:4
 constexpr int kDppRowRor12 = 0x12C;  // row_ror:12: lane l reads lane (l + 4) mod 16 of its row
 constexpr int kDppQuad0022 = 0xA0;   // quad_perm [0,0,2,2]: lane q = 1 reads lane q = 0
 constexpr int kDppQuad0000 = 0x00;   // quad_perm [0,0,0,0]: every lane of a quad reads q = 0
+constexpr int kDppQuad0033 = 0xF0;   // quad_perm [0,0,3,3]: lane 1 reads lane 0, lane 2 reads lane 3
 
 // v summed with the same lane of the paired row (rows 0+1, 2+3: permlane16
 // swap) or of the other half of the wavefront (permlane32 swap); both lanes of
@@ -1001,7 +1013,7 @@ template <int NT, typename Q>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
                                                   const int2* __restrict__ nlrp, const Q* __restrict__ deriv,
-                                                  double* __restrict__ fown, double* __restrict__ pairF,
+                                                  double* __restrict__ fown, ForceShares F,
                                                   double* __restrict__ part) {
     constexpr int LPP = WaveMap<NT>::LPP, PPW = WaveMap<NT>::PPW;
     constexpr int XD = 3 + NT;
@@ -1117,14 +1129,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 vir[4] -= wf * dx2 * dx0;
                 vir[5] -= wf * dx0 * dx1;
             }
-            if (valid && owned) {
-                double* dst = pairF + (size_t)rp * XD;
-                if (q == 0) {
-                    dst[0] = -dx0 * S;
-                    dst[1] = -dx1 * S;
-                    dst[2] = -dx2 * S;
+            if constexpr (NT == 2) {
+                // lanes 0 and 1 of the pair store {fx, fy} and {fz, alpha force 0} (32
+                // contiguous bytes), lane 2 the alpha force of type 1: aJ of type jt
+                // lives on the lanes it = 0, 1 of that jt, moved to lanes 1 and 2
+                const double aJx = dpp_f64<kDppQuad0033>(aJ);
+                if (valid && owned) {
+                    if (q < 2) {
+                        const double2 v = q == 0 ? make_double2(-dx0 * S, -dx1 * S) : make_double2(-dx2 * S, aJx);
+                        reinterpret_cast<double2*>(F.sh4 + (size_t)rp * 4)[q] = v;
+                    } else if (q == 2) {
+                        F.shx[rp] = aJx;
+                    }
                 }
-                if (it == 0) dst[3 + jt] = aJ;
+            } else {
+                if (valid && owned) {
+                    double* dst = F.sh4 + (size_t)rp * 4;
+                    if (q == 0) {
+                        dst[0] = -dx0 * S;
+                        dst[1] = -dx1 * S;
+                        dst[2] = -dx2 * S;
+                    }
+                    if (it == 0) {
+                        if (jt == 0) dst[3] = aJ;
+                        else F.shx[(size_t)rp * (NT - 1) + jt - 1] = aJ;
+                    }
+                }
             }
         }
         done += cnt;
@@ -1184,7 +1214,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // ---------------------------------------------------------------- force gather
 template <int NT>
 __global__ __launch_bounds__(kBlock) void k_fgather(int natms, PairBufs L, const double* __restrict__ fown,
-                                                    const double* __restrict__ pairF,
+                                                    ForceShares F,
                                                     const double* __restrict__ fa, double* __restrict__ f) {
     constexpr int XD = 3 + NT;
     const SiteGroup<kGatherG> g;
@@ -1196,7 +1226,9 @@ __global__ __launch_bounds__(kBlock) void k_fgather(int natms, PairBufs L, const
     const int q1 = L.roff[i + 1];
     for (int q = L.roff[i] + g.l; q < q1; q += kGatherG) {
 #pragma unroll
-        for (int k = 0; k < XD; ++k) acc[k] += pairF[(size_t)q * XD + k];
+        for (int k = 0; k < 4; ++k) acc[k] += F.sh4[(size_t)q * 4 + k];
+#pragma unroll
+        for (int k = 4; k < XD; ++k) acc[k] += F.shx[(size_t)q * (NT - 1) + k - 4];
     }
 #pragma unroll
     for (int k = 0; k < XD; ++k) acc[k] = group_sum<kGatherG>(acc[k]);
