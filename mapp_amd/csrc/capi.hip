@@ -367,8 +367,6 @@ void finish_lists(mapp_dmd* ctx) {
     ctx->d_pairF.reserve((size_t)(nrev + 1) * (3 + NT));
     // every pair starts outside the quadrature cut with zero partner shares; pass 1
     // writes the shares of the pairs inside and clears those of a pair that leaves
-    ctx->d_cutst.reserve((size_t)ctx->npairs + 1);
-    HIP_OK(hipMemsetAsync(ctx->d_cutst.p, 0, (size_t)ctx->npairs + 1, st));
     HIP_OK(hipMemsetAsync(ctx->d_pairE.p, 0, sizeof(double) * (size_t)(nrev + 1) * NT, st));
     HIP_OK(hipMemsetAsync(ctx->d_pairF.p, 0, sizeof(double) * (size_t)(nrev + 1) * (3 + NT), st));
     ctx->d_crd.reserve(ntot);
@@ -624,7 +622,7 @@ void launch_pass1(mapp_dmd* ctx, dim3 grid, const Q* quad, const FFParams<Q>& PQ
     const dim3 blk(64 * ctx->sh1.wpb);
     const size_t lds = (size_t)ctx->sh1.wpb * Pass1Stage<NT>::kBytes;  // one stage per wavefront
     hipStream_t st = ctx->stream;
-    const CutState C{ctx->d_cutst.p, force_shares<NT>(ctx)};
+    const CutState C{ctx->d_nlrp.p, force_shares<NT>(ctx)};
     if (ctx->sh1.parts == 1)
         hipLaunchKernelGGL((k_pass1<NT, Q, NQ, ENERGY_ONLY, FULL, 1>), grid, blk, lds, st, ctx->P, PQ, ctx->d_pos.p,
                            ctx->d_arec.p, L, ctx->d_nlrp.p, quad, ctx->d_rho7.p, ctx->d_phi7.p, ctx->d_Eown.p,
@@ -673,7 +671,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
                 hipLaunchKernelGGL(kern, dim3(nb1), dim3(kSurBlock), kSurLds, st, ctx->P, ctx->Pf, S, ctx->d_pos.p,
                                    ctx->d_arec.p, L, ctx->d_nlrp.p, ctx->d_quad32.p, ctx->d_rho7.p, ctx->d_phi7.p,
                                    ctx->d_Eown.p, ctx->d_pairE.p, (double*)deriv, part1, ctx->d_nfall.p,
-                                   CutState{ctx->d_cutst.p, force_shares<NT>(ctx)});
+                                   CutState{ctx->d_nlrp.p, force_shares<NT>(ctx)});
             };
             if (q == 5) launch(k_pass1_sur<5>);
             else if (q == 3) launch(k_pass1_sur<3>);

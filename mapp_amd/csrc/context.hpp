@@ -64,10 +64,9 @@ struct mapp_dmd {
     mappdmd::DevBuf<double> d_E, d_dE, d_ddE, d_n, d_s, d_t, d_v, d_cd, d_fa, d_Eown, d_a, d_g;
     mappdmd::DevBuf<double> d_f, d_fown;        // [natms][3+nt]
     mappdmd::DevBuf<double> d_pairE, d_pairF;   // [npairs][nt], [npairs][3+nt]
-    // per half-list entry: inside the quadrature cut at the last pass 1; the shares of
-    // entries outside it stay zero (cleared at list build, and when a pair leaves the cut)
-    mappdmd::DevBuf<unsigned char> d_cutst;
-    mappdmd::DevBuf<int2> d_nlrp;  // per half-list entry {partner, reverse position}
+    // per half-list entry {partner, reverse position}; the partner's top bit: inside the
+    // quadrature cut at the last pass 1 (the shares of entries outside it stay zero)
+    mappdmd::DevBuf<int2> d_nlrp;
     mappdmd::DevBuf<double> d_rhoc, d_phic;     // pair values for c_d_calc / calc_g, per half-list pair
     mappdmd::DevBuf<int> d_toff, d_tidx;        // site table: both ends of every contributing pair
     mappdmd::DevBuf<double> d_trho, d_tphi;

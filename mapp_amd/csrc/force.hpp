@@ -214,22 +214,27 @@ struct ForceShares {
     double* shx;
 };
 
-// Which half-list entries lay inside the quadrature cut at the last pass 1.
-// The partner shares (pairE of pass 1, pairF of pass 2) of an entry outside
-// the cut must read zero in the embedding and the force gather: they are
-// cleared when the list is built and again only when an entry leaves the cut,
-// instead of being rewritten with zeros at every evaluation.
+// Which half-list entries lay inside the quadrature cut at the last pass 1:
+// the top bit of the partner index in the packed list {partner, reverse
+// position}, read with the entry at no extra cost.  The partner shares (pairE
+// of pass 1, the force shares of pass 2) of an entry outside the cut must read
+// zero in the embedding and the force gather: they are cleared when the list
+// is built and again only when an entry leaves the cut, instead of being
+// rewritten with zeros at every evaluation.
+constexpr int kInsideBit = (int)0x80000000u;
+__device__ __forceinline__ int list_partner(int2 e) { return e.x & 0x7fffffff; }
 struct CutState {
-    unsigned char* st;
+    int2* nlrp;
     ForceShares F;
-    // entry pc (partner jc, reverse position rp) is inside the cut now: record a change;
-    // on leaving, clear both shares of an owned partner
+    // entry pc (as read: e, partner jc) is inside the cut now: record a change; on
+    // leaving, clear both shares of an owned partner
     template <int NT>
-    __device__ __forceinline__ void update(int pc, bool in, int jc, int rp, int natms, double* pairE) const {
-        const bool was = st[pc] != 0;
+    __device__ __forceinline__ void update(int pc, int2 e, bool in, int jc, int natms, double* pairE) const {
+        const bool was = (e.x & kInsideBit) != 0;
         if (was != in) {
-            st[pc] = in ? 1 : 0;
+            nlrp[pc].x = in ? (jc | kInsideBit) : jc;
             if (!in && jc < natms) {
+                const int rp = e.y;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) pairE[(size_t)rp * NT + t] = 0.0;
 #pragma unroll
@@ -435,7 +440,7 @@ __global__ void k_pack_atoms(const double* __restrict__ alpha, const double* __r
 }
 
 // per half-list entry {partner, reverse position}: the force passes read both
-// with one load
+// with one load (every entry starts outside the quadrature cut)
 __global__ void k_pack_nlrp(const int* __restrict__ nl, const int* __restrict__ rpos, long n,
                             int2* __restrict__ nlrp) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -648,7 +653,7 @@ template <int NT, typename Q, int NQ, bool ENERGY_ONLY, bool FULL = false, int P
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass1(FFParams<double> P, FFParams<Q> PQ,
                                                   const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
-                                                  const int2* __restrict__ nlrp, const Q* __restrict__ quad,
+                                                  const int2* nlrp, const Q* __restrict__ quad,
                                                   const double* __restrict__ rho7, const double* __restrict__ phi7,
                                                   double* __restrict__ Eown, double* __restrict__ pairE,
                                                   Q* __restrict__ deriv, double* __restrict__ part,
@@ -693,7 +698,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 double r2 = 0.0;
                 if (pc < p1) {
                     const int2 e = nlrp[pc];
-                    jc = e.x;
+                    jc = list_partner(e);
                     rp = e.y;  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
@@ -702,7 +707,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                     }
                     in = r2 < P.cut_sq_mod;
-                    C.update<NT>(pc, in, jc, rp, P.natms, pairE);
+                    C.update<NT>(pc, e, in, jc, P.natms, pairE);
                 }
                 const int dest = pack_dest(in, &cnt);
                 if (in) {
@@ -1042,7 +1047,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 double e0 = 0.0, e1 = 0.0, e2 = 0.0, r2 = 0.0;
                 if (pc < p1) {
                     const int2 e = nlrp[pc];
-                    jc = e.x;
+                    jc = list_partner(e);
                     rp = e.y;  // meaningful for owned partners only
                     const double4 xj = pos[jc];
                     e0 = xi.x - xj.x;

@@ -537,7 +537,7 @@ template <int NQ>
 __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFParams<float> PQ, SurDev S,
                                                          const double4* __restrict__ pos,
                                                          const double4* __restrict__ arec, PairBufs L,
-                                                         const int2* __restrict__ nlrp, const float* __restrict__ quad,
+                                                         const int2* nlrp, const float* __restrict__ quad,
                                                          const double* __restrict__ rho7,
                                                          const double* __restrict__ phi7, double* __restrict__ Eown,
                                                          double* __restrict__ pairE, double* __restrict__ deriv,
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 bool in = false;
                 if (pc < p1) {
                     const int2 e = nlrp[pc];
-                    const int jc = e.x;
+                    const int jc = list_partner(e);
                     const double4 xj = pos[jc];
                     const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
                     double r2;
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                         r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
                     }
                     in = r2 < P.cut_sq_mod;
-                    C.update<NT>(pc, in, jc, e.y, P.natms, pairE);
+                    C.update<NT>(pc, e, in, jc, P.natms, pairE);
                 }
                 cnt = pack_in_range(pc, in, &packed);
             }
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
             double aj = 1.0, cj[NT] = {0.0, 0.0}, r = 1.0, r_inv = 1.0, ia = 1.0;
             int br = 0;
             if (valid) {  // inside the cut by construction
-                j = nlrp[p].x;
+                j = list_partner(nlrp[p]);
                 const double4 xj = pos[j];
                 const double dx0 = xi.x - xj.x, dx1 = xi.y - xj.y, dx2 = xi.z - xj.z;
                 {
