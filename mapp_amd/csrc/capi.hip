@@ -642,7 +642,8 @@ template <int NT, typename Q, int NQ>
 void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, double* en_st, bool sur = false) {
     hipStream_t st = ctx->stream;
     const int natms = ctx->natms;
-    const int nb = nblk((long)natms * kGatherG);             // embed and force gather: 16 lanes per site
+    const int nb = nblk((long)natms * kGatherG);             // force gather
+    const int nbe = nblk((long)natms * kEmbedG);             // embedding
     const int nbw = shape_blocks(ctx->sh2, natms);           // pass 2
     // the surrogate's pass 1: one block of kSurWaves wavefronts per CU, persistent
     const int nb1 = sur ? std::min(natms, kSurBlocks) : shape_blocks(ctx->sh1, natms);
@@ -650,7 +651,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
     // partials: one per wavefront in pass 1, the embedding and pass 2 (one per block
     // in the surrogate's pass 1)
     const long np1 = sur ? (long)nb1 : (long)nb1 * ctx->sh1.wpb;
-    const long np2 = (long)nb * (kBlock / 64), np3 = (long)nbw * ctx->sh2.wpb;
+    const long np2 = (long)nbe * (kBlock / 64), np3 = (long)nbw * ctx->sh2.wpb;
     ctx->d_part.reserve((size_t)np1 * 2 + (size_t)np3 * 6 + (size_t)np2 + 16);
     ctx->d_out.reserve(16);
     double* part1 = ctx->d_part.p;
@@ -684,7 +685,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
         launch_check("k_pass1");
     }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[1], st));
-    hipLaunchKernelGGL((k_embed<NT, false>), dim3(nb), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p,
+    hipLaunchKernelGGL((k_embed<NT, false>), dim3(nbe), dim3(kBlock), 0, st, ctx->P, ctx->d_alpha.p,
                        ctx->d_conc.p, L, ctx->d_Eown.p, ctx->d_pairE.p, ctx->d_F9.p, ctx->d_E.p, ctx->d_dE.p,
                        ctx->d_fa.p, ctx->d_arec.p, part2);
     launch_check("k_embed");
@@ -728,7 +729,7 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
 template <int NT, typename Q, int NQ>
 double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
     hipStream_t st = ctx->stream;
-    const int nb = nblk((long)ctx->natms * kGatherG);
+    const int nb = nblk((long)ctx->natms * kEmbedG);
     const int nbw = shape_blocks(ctx->sh1, ctx->natms);
     PairBufs L{ctx->d_off.p, ctx->d_nl.p, ctx->d_roff.p, ctx->d_rev.p};
     const long np1 = (long)nbw * ctx->sh1.wpb, np2 = (long)nb * (kBlock / 64);  // one partial per wavefront

@@ -960,6 +960,7 @@ __device__ __forceinline__ double pick(const double* v, int t) {
 }
 
 constexpr int kGatherG = 16;  // lanes per site of the reverse-CSR gathers
+constexpr int kEmbedG = 4;    // lanes per site of the embedding (its per-site arithmetic runs on NT lanes of each group)
 
 // ---------------------------------------------------------------- embed
 // rho_bar = own share + partner shares (16 lanes per site over the site's
@@ -972,7 +973,7 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
                                                   const double* __restrict__ F9, double* __restrict__ E,
                                                   double* __restrict__ dE, double* __restrict__ fa,
                                                   double4* __restrict__ arec, double* __restrict__ part) {
-    const SiteGroup<kGatherG> g;
+    const SiteGroup<kEmbedG> g;
     const int i = g.site;
     double acc[1] = {0.0};
     if (i < P.natms) {
@@ -980,12 +981,12 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
 #pragma unroll
         for (int t = 0; t < NT; ++t) e[t] = 0.0;
         const int q1 = L.roff[i + 1];
-        for (int q = L.roff[i] + g.l; q < q1; q += kGatherG) {
+        for (int q = L.roff[i] + g.l; q < q1; q += kEmbedG) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) e[t] += pairE[(size_t)q * NT + t];
         }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) e[t] = group_sum<kGatherG>(e[t]);
+        for (int t = 0; t < NT; ++t) e[t] = group_sum<kEmbedG>(e[t]);
         if (g.l < NT) {
             const int t = g.l;
             const size_t k = (size_t)i * NT + t;
