@@ -162,6 +162,29 @@ void reduce_partials_in(mapp_dmd* ctx, const double* part, long nparts, int nv, 
     launch_check("k_reduce_stage");
 }
 
+// several reduce_partials (parts[s], nparts[s], nv[s], stride[s] -> out[out_off[s]..]) in two launches
+void reduce_partials_segs(mapp_dmd* ctx, int nseg, const double* const* parts, const long* nparts, const int* nv,
+                          const int* stride, const int* out_off, double* out) {
+    ReduceSegs R{};
+    R.nseg = nseg;
+    int tot = 0;
+    for (int k = 0; k < nseg; ++k) {
+        R.part[k] = parts[k];
+        R.nparts[k] = nparts[k];
+        R.nv[k] = nv[k];
+        R.stride[k] = stride[k];
+        R.out_off[k] = out_off[k];
+        R.first[k] = tot;
+        R.nb[k] = (int)((nparts[k] + kReduceChunk - 1) / kReduceChunk);
+        tot += R.nb[k];
+    }
+    ctx->d_stage.reserve((size_t)(tot + 1) * 8);
+    if (tot > 0)
+        hipLaunchKernelGGL(k_reduce_segs1, dim3(tot), dim3(kBlock), 0, ctx->stream, R, kReduceChunk, ctx->d_stage.p);
+    hipLaunchKernelGGL(k_reduce_segs2, dim3(nseg), dim3(kBlock), 0, ctx->stream, R, ctx->d_stage.p, out);
+    launch_check("k_reduce_segs");
+}
+
 // the same with the context's stage buffer
 void reduce_partials(mapp_dmd* ctx, const double* part, long nparts, int nv, int stride, int offset, double* out) {
     ctx->d_stage.reserve((size_t)reduce_stage_doubles(nparts));
@@ -704,9 +727,12 @@ void run_force(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ, int st_clc, 
                        ctx->d_fa.p, ctx->d_f.p);
     launch_check("k_fgather");
     double* out = ctx->d_out.p;
-    reduce_partials(ctx, part1, np1, 2, 2, 0, out);
-    reduce_partials(ctx, part2, np2, 1, 1, 0, out + 2);
-    reduce_partials(ctx, part3, np3, 6, 6, 0, out + 3);
+    {
+        const double* parts[3] = {part1, part2, part3};
+        const long nparts[3] = {np1, np2, np3};
+        const int nv[3] = {2, 1, 6}, stride[3] = {2, 1, 6}, off[3] = {0, 2, 3};
+        reduce_partials_segs(ctx, 3, parts, nparts, nv, stride, off, out);
+    }
     if (ctx->timing) HIP_OK(hipEventRecord(ctx->ev[4], st));
     double h[9];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -746,8 +772,12 @@ double run_energy(mapp_dmd* ctx, const Q* quad, const FFParams<Q>& PQ) {
                        ctx->d_arec.p, part2);
     launch_check("k_embed energy");
     double* out = ctx->d_out.p;
-    reduce_partials(ctx, part1, np1, 1, 2, 0, out);
-    reduce_partials(ctx, part2, np2, 1, 1, 0, out + 1);
+    {
+        const double* parts[2] = {part1, part2};
+        const long nparts[2] = {np1, np2};
+        const int nv[2] = {1, 1}, stride[2] = {2, 1}, off[2] = {0, 1};
+        reduce_partials_segs(ctx, 2, parts, nparts, nv, stride, off, out);
+    }
     double h[2];
     HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));

@@ -1270,6 +1270,54 @@ __global__ __launch_bounds__(kBlock) void k_reduce_stage(const double* __restric
     }
 }
 
+// The same two-stage sums for up to kMaxSegs arrays of partials in one pair of
+// launches: segment s takes stage-1 blocks [first[s], first[s] + nb[s]) and
+// stage-2 block s; each sum keeps the fixed shape of k_reduce_stage.
+constexpr int kMaxSegs = 4;
+struct ReduceSegs {
+    const double* part[kMaxSegs];
+    long nparts[kMaxSegs];
+    int nv[kMaxSegs], stride[kMaxSegs], out_off[kMaxSegs], first[kMaxSegs], nb[kMaxSegs];
+    int nseg;
+};
+__device__ __forceinline__ double block_tree_sum(double s, double* sh) {
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(kBlock) void k_reduce_segs1(ReduceSegs R, long chunk, double* __restrict__ stage) {
+    __shared__ double sh[kBlock];
+    int sg = 0;
+    while (sg + 1 < R.nseg && (int)blockIdx.x >= R.first[sg + 1]) ++sg;
+    const long lo = (long)(blockIdx.x - R.first[sg]) * chunk;
+    long hi = lo + chunk;
+    hi = hi < R.nparts[sg] ? hi : R.nparts[sg];
+    const double* part = R.part[sg];
+    for (int k = 0; k < R.nv[sg]; ++k) {
+        double s = 0.0;
+        for (long b = lo + threadIdx.x; b < hi; b += kBlock) s += part[(size_t)b * R.stride[sg] + k];
+        s = block_tree_sum(s, sh);
+        if (threadIdx.x == 0) stage[(size_t)blockIdx.x * 8 + k] = s;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_reduce_segs2(ReduceSegs R, const double* __restrict__ stage,
+                                                         double* __restrict__ out) {
+    __shared__ double sh[kBlock];
+    const int sg = blockIdx.x;
+    for (int k = 0; k < R.nv[sg]; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < R.nb[sg]; b += kBlock) s += stage[(size_t)(R.first[sg] + b) * 8 + k];
+        s = block_tree_sum(s, sh);
+        if (threadIdx.x == 0) out[R.out_off[sg] + k] = s;
+    }
+}
+
 }  // namespace mappdmd
 
 #pragma clang fp contract(off)
