@@ -162,7 +162,7 @@ void reduce_partials_in(mapp_dmd* ctx, const double* part, long nparts, int nv, 
     launch_check("k_reduce_stage");
 }
 
-// several reduce_partials (parts[s], nparts[s], nv[s], stride[s] -> out[out_off[s]..]) in two launches
+// several reduce_partials (parts[s], nparts[s], nv[s] <= 8 values, stride[s] -> out[out_off[s]..]) in two launches
 void reduce_partials_segs(mapp_dmd* ctx, int nseg, const double* const* parts, const long* nparts, const int* nv,
                           const int* stride, const int* out_off, double* out) {
     ReduceSegs R{};

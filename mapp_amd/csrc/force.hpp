@@ -1270,9 +1270,10 @@ __global__ __launch_bounds__(kBlock) void k_reduce_stage(const double* __restric
     }
 }
 
-// The same two-stage sums for up to kMaxSegs arrays of partials in one pair of
-// launches: segment s takes stage-1 blocks [first[s], first[s] + nb[s]) and
-// stage-2 block s; each sum keeps the fixed shape of k_reduce_stage.
+// The same two-stage sums for up to kMaxSegs arrays of partials (at most 8
+// values each) in one pair of launches: segment s takes stage-1 blocks
+// [first[s], first[s] + nb[s]) and stage-2 block s; each sum keeps the fixed
+// shape of k_reduce_stage.
 constexpr int kMaxSegs = 4;
 struct ReduceSegs {
     const double* part[kMaxSegs];
@@ -1299,11 +1300,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce_segs1(ReduceSegs R, long chun
     long hi = lo + chunk;
     hi = hi < R.nparts[sg] ? hi : R.nparts[sg];
     const double* part = R.part[sg];
-    for (int k = 0; k < R.nv[sg]; ++k) {
-        double s = 0.0;
-        for (long b = lo + threadIdx.x; b < hi; b += kBlock) s += part[(size_t)b * R.stride[sg] + k];
-        s = block_tree_sum(s, sh);
-        if (threadIdx.x == 0) stage[(size_t)blockIdx.x * 8 + k] = s;
+    const int nv = R.nv[sg], stride = R.stride[sg];
+    // every value of a partial in one sweep (they share its cache line), then one tree per value
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long b = lo + threadIdx.x; b < hi; b += kBlock) {
+        const double* pb = part + (size_t)b * stride;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nv) s[k] += pb[k];
+    }
+    for (int k = 0; k < nv; ++k) {
+        const double t = block_tree_sum(s[k], sh);
+        if (threadIdx.x == 0) stage[(size_t)blockIdx.x * 8 + k] = t;
     }
 }
 __global__ __launch_bounds__(kBlock) void k_reduce_segs2(ReduceSegs R, const double* __restrict__ stage,
