@@ -963,7 +963,7 @@ constexpr int kGatherG = 16;  // lanes per site of the reverse-CSR gathers
 constexpr int kEmbedG = 4;    // lanes per site of the embedding (its per-site arithmetic runs on NT lanes of each group)
 
 // ---------------------------------------------------------------- embed
-// rho_bar = own share + partner shares (16 lanes per site over the site's
+// rho_bar = own share + partner shares (kEmbedG lanes per site over the site's
 // contiguous reverse range); F, F', energy terms, alpha entropic force
 template <int NT, bool ENERGY_ONLY>
 __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const double* __restrict__ alpha,
