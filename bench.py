@@ -509,6 +509,7 @@ def main():
     ap.add_argument("--surrogate-only", action="store_true",
                     help="train the surrogate and time force_calc with every cell served (profiling)")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true", help="time the mixed (FP32 quadrature, 5-point) force_calc alone")
     ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--schedule", type=str, default=None,
@@ -527,6 +528,13 @@ def main():
     if args.md_only:
         from mapp_amd import workload as W
         print(json.dumps({"md": run_md(args.cells, W.johnson_tables(), args.steps, args.warmup)}), flush=True)
+        return
+    if args.mixed_only:
+        from mapp_amd import workload as W
+        lat = W.fcc_alloy(args.cells, seed=1)
+        sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
+        r = run_path(lat, W.johnson_tables(), 5, "mixed", args.steps, args.warmup, sched)
+        print(json.dumps({"mixed_fp32_q5": {k: v for k, v in r.items() if k != "f"}}), flush=True)
         return
     if args.surrogate_only:
         from mapp_amd import workload as W
@@ -612,9 +620,10 @@ def main():
         paths = {
             "fp64_q3": {"site_evals_per_s": value, "ms_per_step": r64["ms_per_step"]},
             "fp64_q5": {"site_evals_per_s": natms / (r64q5["ms_per_step"] / 1e3), "ms_per_step": r64q5["ms_per_step"],
-                        "kernel_ms": r64q5["kernel_ms"][:4]},
+                        "kernel_ms": r64q5["kernel_ms"][:4], "schedule": r64q5["schedule"]},
             "mixed_fp32_q5": {"site_evals_per_s": natms / (rmix["ms_per_step"] / 1e3),
                               "ms_per_step": rmix["ms_per_step"], "kernel_ms": rmix["kernel_ms"][:4],
+                              "schedule": rmix["schedule"],
                               "force_rel_err_vs_fp64_q5": rel(rmix["f"], r64q5["f"]),
                               "errors_vs_fp64_q5": {
                                   "energy": abs(rmix["energy"] - r64q5["energy"]) / abs(r64q5["energy"]),
