@@ -1047,18 +1047,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 int jc = 0, rp = 0;
                 double e0 = 0.0, e1 = 0.0, e2 = 0.0, r2 = 0.0;
                 if (pc < p1) {
+                    // pass 1 of this evaluation left each entry's cut state in its
+                    // partner's top bit (same positions): only the entries inside
+                    // need a position; the shares of the others stay zero (CutState)
                     const int2 e = nlrp[pc];
-                    jc = list_partner(e);
-                    rp = e.y;  // meaningful for owned partners only
-                    const double4 xj = pos[jc];
-                    e0 = xi.x - xj.x;
-                    e1 = xi.y - xj.y;
-                    e2 = xi.z - xj.z;
-                    {
+                    in = (e.x & kInsideBit) != 0;
+                    if (in) {
+                        jc = list_partner(e);
+                        rp = e.y;  // meaningful for owned partners only
+                        const double4 xj = pos[jc];
+                        e0 = xi.x - xj.x;
+                        e1 = xi.y - xj.y;
+                        e2 = xi.z - xj.z;
+                        {
 #pragma clang fp contract(off)
-                        r2 = e0 * e0 + e1 * e1 + e2 * e2;
+                            r2 = e0 * e0 + e1 * e1 + e2 * e2;
+                        }
                     }
-                    in = r2 < P.cut_sq_mod;  // the shares of entries outside stay zero (CutState)
                 }
                 const int dest = pack_dest(in, &cnt);
                 pk_j = packed(dest, jc);
