@@ -429,13 +429,13 @@ struct WaveMap {
     static constexpr int PPW = 64 / LPP;
 };
 
-// per (site, type) record {alpha, 1/alpha, c, dE}, ghosts included
+// per (site, type) record {alpha, c, 1/alpha, dE}, ghosts included
 __global__ void k_pack_atoms(const double* __restrict__ alpha, const double* __restrict__ conc, long n,
                              double4* __restrict__ rec) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
         const double a = alpha[k];
-        rec[k] = make_double4(a, 1.0 / a, conc[k], 0.0);
+        rec[k] = make_double4(a, conc[k], 1.0 / a, 0.0);
     }
 }
 
@@ -677,9 +677,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const double4 xi = pos[i];
         double ci[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
+        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].y;
         const double4 ra = arec[(size_t)i * NT + it];
-        const double ai = ra.x, iai = ra.y;
+        const double ai = ra.x, iai = ra.z;
         const __amdgpu_buffer_rsrc_t qrs = raw_rsrc(quad, P.nr * quad_stride(NT) * (int)sizeof(Q));
         const int tab_off = (q * kQuadRow + (PARTS == 1 ? 0 : prt * (8 / PARTS))) * (int)sizeof(Q);
         double Ei = 0.0;
@@ -712,10 +712,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const int dest = pack_dest(in, &cnt);
                 if (in) {
                     st.idx[dest] = make_int4(pc, jc, rp, 0);
+                    // the partner's {alpha, c} of every type in one 16-byte load each; the
+                    // division gives the same bits as the stored 1/alpha (both correctly rounded)
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
-                        const double* rr = reinterpret_cast<const double*>(arec + (size_t)jc * NT + t);
-                        st.rec[dest * NT + t] = make_double4(rr[0], rr[1], rr[2], r2);
+                        const double2 ac = *reinterpret_cast<const double2*>(arec + (size_t)jc * NT + t);
+                        st.rec[dest * NT + t] = make_double4(ac.x, 1.0 / ac.x, ac.y, r2);
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -1033,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int i = __builtin_amdgcn_readfirstlane(i0);
         const double4 xi = pos[i];
         const double4 ra = arec[(size_t)i * NT + it];
-        const double ai = ra.x, iai = ra.y, ci = ra.z, dEi = ra.w;
+        const double ai = ra.x, ci = ra.y, iai = ra.z, dEi = ra.w;
         double fx = 0.0, fy = 0.0, fz = 0.0, fa = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         int done = 0;  // inside entries of the earlier chunks (records at off[i] + rank)
@@ -1088,19 +1090,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 double aj, iaj, cj, dEj;
                 if constexpr (NT == 2) {
                     // lanes (0, jt) and (1, jt) share the partner's record of type jt:
-                    // each loads one half ({alpha, 1/alpha} or {c, F'}) and the quad
+                    // each loads one half ({alpha, c} or {1/alpha, F'}) and the quad
                     // exchange (xor 2) brings the other
                     const double2 h = reinterpret_cast<const double2*>(arec + (size_t)j * NT + jt)[it];
                     const double o0 = dpp_f64<kDppXor2>(h.x), o1 = dpp_f64<kDppXor2>(h.y);
                     aj = it == 0 ? h.x : o0;
-                    iaj = it == 0 ? h.y : o1;
-                    cj = it == 0 ? o0 : h.x;
+                    cj = it == 0 ? h.y : o1;
+                    iaj = it == 0 ? o0 : h.x;
                     dEj = it == 0 ? o1 : h.y;
                 } else {
                     const double4 rj = arec[(size_t)j * NT + jt];
                     aj = rj.x;
-                    iaj = rj.y;
-                    cj = rj.z;
+                    cj = rj.y;
+                    iaj = rj.z;
                     dEj = rj.w;
                 }
                 const double r_inv = rsqrt(rsq);

@@ -559,9 +559,9 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
         const double4 xi = pos[i];
         double ci[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].z;
+        for (int t = 0; t < NT; ++t) ci[t] = arec[(size_t)i * NT + t].y;
         const double4 ra = arec[(size_t)i * NT + it];
-        const double ai = ra.x, iai = ra.y;
+        const double ai = ra.x, iai = ra.z;
         double Ei = 0.0;
         const int p0 = L.off[i], p1 = L.off[i + 1];
         int done = 0;  // inside entries of the earlier chunks (records at off[i] + rank)
@@ -604,11 +604,11 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                 }
                 const double4 rj = arec[(size_t)j * NT + jt];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.z : arec[(size_t)j * NT + t].z;
+                for (int t = 0; t < NT; ++t) cj[t] = t == jt ? rj.y : arec[(size_t)j * NT + t].y;
                 aj = rj.x;
                 r_inv = rsqrt(rsq);
                 r = rsq * r_inv;
-                ia = iai + rj.y;
+                ia = iai + rj.z;
                 br = alpha_branch(P, ia, ai, aj);
             }
             const float rf = (float)r, aif = (float)ai, ajf = (float)aj;
