@@ -291,7 +291,8 @@ int mapp_dmd_cells(mapp_dmd_t* ctx, int* ncell, int* natms, int* nghost, int cap
    persistent grid whose waves stride through XCD-contiguous site ranges),
    pass-2 waves per block, pass-2 total waves}.  mapp_dmd_autotune times every
    candidate on the context's own system with HIP events (reps runs each
-   after one warm-up force_calc) and keeps the fastest per pass; report
+   after one warm-up force_calc) and keeps the fastest per pass (until then:
+   {2, 4, 0, 2, 0}); report
    [cap][5] receives {lanes per pair, waves per block, total waves, pass-1 ms,
    pass-2 ms}. */
 int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* ncand);

@@ -98,12 +98,14 @@ struct mapp_dmd {
     double sur_served = 0.0;  // fraction of cells the networks serve
     bool sur_last_quadrature = false;  // the last surrogate force_calc served no cell and ran the mixed pass 1
     bool energy_full = false;  // energy_calc over every ordered type pair
-    // launch shapes of the two force passes (waves per block; total waves, 0 = one per site)
+    // launch shapes of the two force passes (waves per block; total waves, 0 = one per site);
+    // until autotuned, the shapes the autotuner keeps on MI355X for the 64^3 workloads
     struct Shape {
         int wpb = 4;
         long waves = 0;
-        int parts = 4;  // pass 1: lanes per (pair, type pair)
-    } sh1, sh2;
+        int parts = 2;  // pass 1: lanes per (pair, type pair)
+    };
+    Shape sh1{4, 0, 2}, sh2{2, 0, 2};
     mappdmd::DevBuf<double> d_lh, d_lx0, d_lf0, d_lS, d_lY;  // minimiser vectors
     mappdmd::DevBuf<double> d_bdf;  // BDF history and solver vectors
     mappdmd::DevBuf<int> d_qidx;    // element read by the dE[jatm + jtype] index quirk
