@@ -91,7 +91,8 @@ enum {
 };
 
 /* Constructor: ForceField_DMD_I::ForceField_DMD_I (src/ff_dmd_I.cpp:10-40).
-   no_types = nt; device = HIP device ordinal. */
+   no_types = nt (1..4; plain EAM 1..2, MAPP_DMD_E_RANGE beyond); device = HIP
+   device ordinal.  The surrogate path needs nt = 2. */
 int mapp_dmd_create(mapp_dmd_t** ctx, int no_types, int variant, int device);
 /* Destructor: ForceField_DMD_I::~ForceField_DMD_I (src/ff_dmd_I.cpp:44-85). */
 int mapp_dmd_destroy(mapp_dmd_t* ctx);

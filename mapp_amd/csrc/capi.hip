@@ -8,6 +8,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 
 #include "context.hpp"
 #include "geometry.hpp"
@@ -404,9 +405,19 @@ void finish_lists(mapp_dmd* ctx) {
 inline bool is_md(const mapp_dmd* ctx) { return ctx->variant >= MAPP_DMD_VARIANT_EAM; }
 inline int acols(const mapp_dmd* ctx) { return is_md(ctx) ? 1 : ctx->nt; }
 
+// f(std::integral_constant<int, NT>) for the number of DMD types nt = 1..kMaxTypes
+template <typename F>
+decltype(auto) with_nt(int nt, F&& f) {
+    switch (nt) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    default: return f(std::integral_constant<int, 4>{});
+    }
+}
+
 void rebuild_any(mapp_dmd* ctx) {
-    if (acols(ctx) == 1) rebuild<1>(ctx);
-    else rebuild<2>(ctx);
+    with_nt(acols(ctx), [&](auto c) { rebuild<decltype(c)::value>(ctx); });
 }
 
 template <int NT>
@@ -468,10 +479,11 @@ inline int xcols(const mapp_dmd* ctx) { return is_md(ctx) ? 3 : 3 + ctx->nt; }
 void gather_x_any(mapp_dmd* ctx, double* dst, int n, const double4* pos, const double* alpha) {
     if (is_md(ctx))
         hipLaunchKernelGGL((k_gather_x<1, 0>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
-    else if (ctx->nt == 1)
-        hipLaunchKernelGGL((k_gather_x<1>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
     else
-        hipLaunchKernelGGL((k_gather_x<2>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n, pos, alpha);
+        with_nt(ctx->nt, [&](auto c) {
+            hipLaunchKernelGGL((k_gather_x<decltype(c)::value>), dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dst, n,
+                               pos, alpha);
+        });
     launch_check("k_gather_x");
 }
 
@@ -946,13 +958,10 @@ SiteTable site_table(mapp_dmd* ctx) { return SiteTable{ctx->d_toff.p, ctx->d_tid
 
 void ensure_pair_cache(mapp_dmd* ctx) {
     if (ctx->cache_valid) return;
-    if (ctx->nt == 1) {
-        pair_values_dispatch<1>(ctx);
-        build_site_table<1>(ctx);
-    } else {
-        pair_values_dispatch<2>(ctx);
-        build_site_table<2>(ctx);
-    }
+    with_nt(ctx->nt, [&](auto c) {
+        pair_values_dispatch<decltype(c)::value>(ctx);
+        build_site_table<decltype(c)::value>(ctx);
+    });
     ctx->cache_valid = true;
 }
 
@@ -1157,8 +1166,7 @@ void sur_gradient(mapp_dmd* ctx, const SurNorm& N, const float* params, const fl
 int update_x_device(mapp_dmd* ctx) {
     const int nt = acols(ctx), natms = ctx->natms;
     if (is_md(ctx)) scatter_x<1, 0>(ctx, ctx->d_xaos.p);
-    else if (nt == 1) scatter_x<1>(ctx, ctx->d_xaos.p);
-    else scatter_x<2>(ctx, ctx->d_xaos.p);
+    else with_nt(nt, [&](auto c) { scatter_x<decltype(c)::value>(ctx, ctx->d_xaos.p); });
     HIP_OK(hipMemsetAsync(ctx->d_iflag.p, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_skin_check, dim3(nblk(natms)), dim3(kBlock), 0, ctx->stream, ctx->d_pos.p, ctx->d_x0.p, natms,
                        ctx->skin * ctx->skin, ctx->d_iflag.p);
@@ -1172,8 +1180,7 @@ int update_x_device(mapp_dmd* ctx) {
         rebuild_any(ctx);
         sync_sizes(ctx);
     } else {
-        if (nt == 1) ghost_update_x<1>(ctx);
-        else ghost_update_x<2>(ctx);  // EAM: alpha column unused
+        with_nt(nt, [&](auto c) { ghost_update_x<decltype(c)::value>(ctx); });  // EAM: one column
         HIP_OK(hipStreamSynchronize(ctx->stream));
     }
     ctx->cache_valid = false;
@@ -1183,12 +1190,11 @@ int update_x_device(mapp_dmd* ctx) {
 // ------------------------------------------------------------------ minimiser
 void force_any(mapp_dmd* ctx, int st_clc, double* en) {
     if (is_md(ctx)) run_eam(ctx, false, st_clc, en);
-    else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en);
-    else dispatch_force<2>(ctx, st_clc, en);
+    else with_nt(ctx->nt, [&](auto c) { dispatch_force<decltype(c)::value>(ctx, st_clc, en); });
 }
 double energy_any(mapp_dmd* ctx) {
     if (is_md(ctx)) return run_eam(ctx, true, 0, nullptr);
-    return ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
+    return with_nt(ctx->nt, [&](auto c) { return dispatch_energy<decltype(c)::value>(ctx); });
 }
 
 constexpr int kVecBlocks = 1024;  // grid of the grid-stride vector kernels (fixed: deterministic sums)
@@ -1278,7 +1284,7 @@ struct BdfState {
 
     double calc_g(int chk) {
         ++n_calc_g;
-        return nt == 1 ? calc_g_variant<1>(ctx, chk, beta, a, g) : calc_g_variant<2>(ctx, chk, beta, a, g);
+        return with_nt(nt, [&](auto c) { return calc_g_variant<decltype(c)::value>(ctx, chk, beta, a, g); });
     }
     void set_c_from(const double* src) {
         if (src != ctx->d_conc.p)
@@ -1813,7 +1819,8 @@ int mapp_dmd_create(mapp_dmd_t** out, int no_types, int variant, int device) {
     if (!out) return MAPP_DMD_E_ARG;
     *out = nullptr;
     if (variant < 0 || variant > MAPP_DMD_VARIANT_MEAM) return MAPP_DMD_E_ARG;
-    if (no_types < 1 || no_types > (variant >= MAPP_DMD_VARIANT_LJ ? kMaxTypes : 2)) return MAPP_DMD_E_RANGE;
+    // DMD variants and the MD potentials other than plain EAM take up to kMaxTypes types
+    if (no_types < 1 || no_types > (variant == MAPP_DMD_VARIANT_EAM ? 2 : kMaxTypes)) return MAPP_DMD_E_RANGE;
     std::unique_ptr<mapp_dmd> ctx(new mapp_dmd());
     ctx->nt = no_types;
     ctx->variant = variant;
@@ -2233,8 +2240,7 @@ int mapp_dmd_init(mapp_dmd_t* ctx, double skin) {
             require(ctx->bp.cut_ph_s[d] < 1.0, MAPP_DMD_E_RANGE,
                     "box thinner than cutoff+skin: needs more than one ghost swap per face");
         if (!ctx->cell_natms.empty()) {
-            if (nt == 1) init_batch<1>(ctx, skin);
-            else init_batch<2>(ctx, skin);
+            with_nt(nt, [&](auto c) { init_batch<decltype(c)::value>(ctx, skin); });
             sync_sizes(ctx);
             ctx->inited = true;
             return 0;
@@ -2296,8 +2302,7 @@ int mapp_dmd_force_calc(mapp_dmd_t* ctx, int st_clc, double* en_st) {
         require(en_st != nullptr, MAPP_DMD_E_ARG, "null en_st");
         HIP_OK(hipSetDevice(ctx->device));
         if (is_md(ctx)) run_eam(ctx, false, st_clc, en_st);
-        else if (ctx->nt == 1) dispatch_force<1>(ctx, st_clc, en_st);
-        else dispatch_force<2>(ctx, st_clc, en_st);
+        else with_nt(ctx->nt, [&](auto c) { dispatch_force<decltype(c)::value>(ctx, st_clc, en_st); });
         return 0;
     });
 }
@@ -2308,7 +2313,7 @@ int mapp_dmd_energy_calc(mapp_dmd_t* ctx, double* energy) {
         require(energy != nullptr, MAPP_DMD_E_ARG, "null energy");
         HIP_OK(hipSetDevice(ctx->device));
         if (is_md(ctx)) *energy = run_eam(ctx, true, 0, nullptr);
-        else *energy = ctx->nt == 1 ? dispatch_energy<1>(ctx) : dispatch_energy<2>(ctx);
+        else *energy = with_nt(ctx->nt, [&](auto c) { return dispatch_energy<decltype(c)::value>(ctx); });
         return 0;
     });
 }
@@ -2318,8 +2323,7 @@ int mapp_dmd_create_2nd_neigh_lst(mapp_dmd_t* ctx) {
     return guarded(ctx, [&]() {
         require(ctx->inited, MAPP_DMD_E_STATE, "init must precede create_2nd_neigh_lst");
         HIP_OK(hipSetDevice(ctx->device));
-        if (ctx->nt == 1) build_list2<1>(ctx);
-        else build_list2<2>(ctx);
+        with_nt(ctx->nt, [&](auto c) { build_list2<decltype(c)::value>(ctx); });
         return 0;
     });
 }
@@ -2329,8 +2333,7 @@ int mapp_dmd_c_d_calc(mapp_dmd_t* ctx) {
         require(!is_md(ctx), MAPP_DMD_E_STATE, "c_d_calc is a DMD entry point");
         require(ctx->inited && ctx->have_list2, MAPP_DMD_E_STATE, "init and create_2nd_neigh_lst must precede c_d_calc");
         HIP_OK(hipSetDevice(ctx->device));
-        if (ctx->nt == 1) c_d_variant<1>(ctx);
-        else c_d_variant<2>(ctx);
+        with_nt(ctx->nt, [&](auto c) { c_d_variant<decltype(c)::value>(ctx); });
         return 0;
     });
 }
@@ -2343,8 +2346,8 @@ int mapp_dmd_calc_g(mapp_dmd_t* ctx, int chk, double alpha, const double* a_dev,
         HIP_OK(hipSetDevice(ctx->device));
         if (!a_dev) a_dev = ctx->d_a.p;  // the library's own buffers (mapp_dmd_set_vec / get_vec A, G)
         if (!g_dev) g_dev = ctx->d_g.p;
-        *cost = ctx->nt == 1 ? calc_g_variant<1>(ctx, chk, alpha, a_dev, g_dev)
-                             : calc_g_variant<2>(ctx, chk, alpha, a_dev, g_dev);
+        *cost = with_nt(ctx->nt,
+                        [&](auto c) { return calc_g_variant<decltype(c)::value>(ctx, chk, alpha, a_dev, g_dev); });
         return 0;
     });
 }
@@ -2658,8 +2661,8 @@ int mapp_dmd_calc_g_noalloc(mapp_dmd_t* ctx, int chk, double alpha, const double
                                                hipGetErrorString(g_scratch_err[ctx->device]));
         const long nb2 = calc_g_blocks(ctx);
         const GScratch S{base, base + nb2, base + nb2 + reduce_stage_doubles(nb2)};
-        *cost = ctx->nt == 1 ? calc_g_variant<1>(ctx, chk, alpha, a_dev, g_dev, &S)
-                             : calc_g_variant<2>(ctx, chk, alpha, a_dev, g_dev, &S);
+        *cost = with_nt(ctx->nt,
+                        [&](auto c) { return calc_g_variant<decltype(c)::value>(ctx, chk, alpha, a_dev, g_dev, &S); });
         return 0;
     });
 }
@@ -2872,10 +2875,10 @@ int mapp_dmd_bdf_run(mapp_dmd_t* ctx, int nsteps, int max_order, double initial_
         // Clock_BDF::init (src/clock_bdf.cpp:201-275)
         double en[7];
         force_any(ctx, 1, en);
-        if (nt == 1) build_list2<1>(ctx);
-        else build_list2<2>(ctx);
-        if (nt == 1) c_d_variant<1>(ctx);
-        else c_d_variant<2>(ctx);
+        with_nt(nt, [&](auto c) {
+            build_list2<decltype(c)::value>(ctx);
+            c_d_variant<decltype(c)::value>(ctx);
+        });
         ctx->d_bdf.reserve((size_t)n * (max_order + 8));
         BdfState S{ctx, n, nt};
         S.fix = (flags & 1) != 0;
@@ -3028,8 +3031,7 @@ int mapp_dmd_autotune(mapp_dmd_t* ctx, int reps, double* report, int cap, int* n
             ctx->sh2 = c;
             double t1 = 1e300, t2 = 1e300;
             for (int r = 0; r < reps + 1; ++r) {  // the first run warms up
-                if (ctx->nt == 1) dispatch_force<1>(ctx, 1, en);
-                else dispatch_force<2>(ctx, 1, en);
+                with_nt(ctx->nt, [&](auto c) { dispatch_force<decltype(c)::value>(ctx, 1, en); });
                 if (r == 0) continue;
                 t1 = std::min(t1, ctx->last_ms[0]);
                 t2 = std::min(t2, ctx->last_ms[2]);

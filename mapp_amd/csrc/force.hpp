@@ -298,18 +298,42 @@ __device__ __forceinline__ void quad_slot_sums(double& A, double& B) {
 
 template <int NT>
 __device__ __forceinline__ double transpose_tp(double v) {
-    if constexpr (NT == 2) return dpp_f64<kDppTranspose>(v);
-    else return v;
+    if constexpr (NT == 2) {
+        return dpp_f64<kDppTranspose>(v);
+    } else if constexpr (NT == 1) {
+        return v;
+    } else {  // lane (it, jt) of the pair's NT*NT lanes reads lane (jt, it)
+        const int lane = threadIdx.x & 63, q = lane % (NT * NT);
+        return __shfl(v, lane - q + (q % NT) * NT + q / NT, 64);
+    }
 }
 template <int NT>
 __device__ __forceinline__ double sum_over_jt(double v) {
-    if constexpr (NT == 2) return v + dpp_f64<kDppXor1>(v);
-    else return v;
+    if constexpr (NT == 2) {
+        return v + dpp_f64<kDppXor1>(v);
+    } else if constexpr (NT == 1) {
+        return v;
+    } else {  // the NT lanes (it, 0..NT-1) in a fixed order
+        const int lane = threadIdx.x & 63, q = lane % (NT * NT), row = lane - q + (q / NT) * NT;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) s += __shfl(v, row + t, 64);
+        return s;
+    }
 }
 template <int NT>
 __device__ __forceinline__ double sum_over_it(double v) {
-    if constexpr (NT == 2) return v + dpp_f64<kDppXor2>(v);
-    else return v;
+    if constexpr (NT == 2) {
+        return v + dpp_f64<kDppXor2>(v);
+    } else if constexpr (NT == 1) {
+        return v;
+    } else {  // the NT lanes (0..NT-1, jt) in a fixed order
+        const int lane = threadIdx.x & 63, q = lane % (NT * NT), col = lane - q + q % NT;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) s += __shfl(v, col + t * NT, 64);
+        return s;
+    }
 }
 
 // which branch of the reference applies to the pair width alpha = 1/ia:
@@ -724,7 +748,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
-            const bool valid = slot < cnt;
+            const bool valid = slot < cnt && (64 % LPP == 0 || k < PPW);  // lanes past PPW pairs idle
             int p = 0, j = 0, rp = 0;
             double rsq = 1.0, aj = 1.0, iaj = 1.0;
             double cj[NT];
@@ -893,6 +917,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if constexpr (NT == 2 && PARTS == 4) ej += dpp_f64<kDppRowRor8>(ej);
             if constexpr (NT == 2 && PARTS == 2) ej += dpp_f64<kDppRowRor12>(ej);  // lane l + 4 for l % 8 < 4
             if constexpr (NT == 2 && PARTS == 1) ej += __shfl_xor(ej, 2, 64);
+            if constexpr (NT > 2) {  // the rho lanes (0..NT-1, jt) of the pair, in a fixed order
+                const int col = lane - lane % LPP + jt * PARTS;
+                double s = 0.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) s += __shfl(ej, col + t * NT * PARTS, 64);
+                ej = s;
+            }
             // an ordinary store: the 16-byte shares land scattered, and L2 merges them into lines
             if (valid && rho_lane && it == 0 && j < P.natms) pairE[(size_t)rp * NT + jt] = ej;
         }
@@ -1077,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         for (int base = 0; base < cnt; base += PPW) {
             const int slot = base + k;
-            const bool valid = slot < cnt;
+            const bool valid = slot < cnt && (64 % LPP == 0 || k < PPW);  // lanes past PPW pairs idle
             const int src = valid ? slot : 0;
             const int j = __shfl(pk_j, src, 64);
             const int rp = __shfl(pk_rp, src, 64);

@@ -15,6 +15,9 @@ import numpy as np
 ELEMENTS = {
     "Ni": dict(mass=58.6934, r_e=2.49, f_e=1.25, beta=6.10, phi_e=0.62, gamma=8.80, E_c=4.45, a=4.98),
     "Cu": dict(mass=63.546, r_e=2.556, f_e=1.00, beta=5.85, phi_e=0.59, gamma=8.00, E_c=3.54, a=5.09),
+    # third and fourth species for the three- and four-type alloys (same functional form)
+    "Pd": dict(mass=106.42, r_e=2.52, f_e=1.10, beta=6.00, phi_e=0.60, gamma=8.40, E_c=3.91, a=5.20),
+    "Ag": dict(mass=107.8682, r_e=2.58, f_e=0.90, beta=5.95, phi_e=0.55, gamma=8.60, E_c=2.85, a=5.40),
 }
 
 
@@ -152,7 +155,8 @@ def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0
               alpha_per_site=False):
     """n^3 FCC cells of lattice constant a; positions jittered N(0, jitter);
     alpha ~ U[alpha_lo, alpha_hi] per (site, type) (one value per site, shared
-    by the types, when alpha_per_site); c_A ~ U[0,1], c_B = 1 - c_A."""
+    by the types, when alpha_per_site); c_A ~ U[0,1], c_B = 1 - c_A for two
+    types, a flat Dirichlet draw over the types for three or four."""
     rng = np.random.default_rng(seed)
     basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
     g = np.arange(n)
@@ -168,11 +172,13 @@ def fcc_alloy(n, a=3.6, nt=2, seed=1, jitter=0.03, alpha_lo=60.0, alpha_hi=120.0
         alpha[:, 1:] = alpha[:, :1]
     if nt == 1:
         c = rng.uniform(0.3, 1.0, size=(natms, 1)) if c_per_site is None else np.full((natms, 1), c_per_site)
-    else:
+    elif nt == 2:
         cA = rng.uniform(0.0, 1.0, size=natms)
         c = np.zeros((natms, nt))
         c[:, 0] = cA
         c[:, 1] = 1.0 - cA
+    else:
+        c = rng.dirichlet(np.ones(nt), size=natms)
     x = np.concatenate([s, alpha], axis=1)
     return Lattice(H=H, x=np.ascontiguousarray(x), c=np.ascontiguousarray(c), cart=cart)
 

@@ -6,10 +6,11 @@ Runs only in the build container (where the reference sources and the image's
 MPI are); the fixtures it writes are data -- inputs and expected outputs -- and
 travel with the tests:
 
-    make -f oracle/Makefile.ref && python tests/golden/make_ref_fixtures.py
+    make -f oracle/Makefile.ref && python tests/golden/make_ref_fixtures.py [case ...]
 
-Each case stores the tables it ran on (Johnson-style Ni-Cu on 2000-point grids,
-the FS file's values), the box, the positions / widths / concentrations as
+Each case stores the tables it ran on (Johnson-style Ni-Cu on 2000-point grids
+for the two binary cases; Ni-Cu-Pd and Ni-Cu-Pd-Ag on 1000-point grids for the
+three- and four-type cases; the FS file's values), the box, the positions / widths / concentrations as
 the reference read them from the extended CFG, calc_g's a and beta, and per
 variant every array the harness dumps: ghosts' positions, the half list and
 the 2nd list, crd, force_calc's energy / virial / f / E / dE, energy_calc,
@@ -67,10 +68,11 @@ def read_dump(path):
 
 
 def lattice_cases():
+    """case name -> (lattice, element names, table grid points)"""
     from mapp_amd import workload as W
     cases = {}
     # 5^3 FCC Ni-Cu (500 sites): alpha per (site, type) ~ U[60, 120]
-    cases["fcc5"] = W.fcc_alloy(5, seed=101)
+    cases["fcc5"] = (W.fcc_alloy(5, seed=101), ("Ni", "Cu"), 2000)
     # widths from three bands so that pair widths fall in every regime of
     # src/ff_dmd_I.cpp:178-180, :261-310, and concentrations on the edges of
     # calc_ent's margin
@@ -84,7 +86,11 @@ def lattice_cases():
     cA[pick] = edge[np.arange(pick.sum()) % len(edge)]
     lat.c[:, 0] = cA
     lat.c[:, 1] = 1.0 - cA
-    cases["regimes5"] = lat
+    cases["regimes5"] = (lat, ("Ni", "Cu"), 2000)
+    # 4^3 FCC with three and four types (256 sites): concentrations drawn flat
+    # over the simplex, every density slot rho_ab distinct
+    cases["ternary4"] = (W.fcc_alloy(4, nt=3, seed=103), ("Ni", "Cu", "Pd"), 1000)
+    cases["quaternary4"] = (W.fcc_alloy(4, nt=4, seed=104), ("Ni", "Cu", "Pd", "Ag"), 1000)
     return cases
 
 
@@ -115,8 +121,11 @@ def main():
     from mapp_amd import workload as W
     if not os.path.exists(HARNESS):
         sys.exit("build the harness first: make -f oracle/Makefile.ref")
-    tables = W.johnson_tables(nr=2000, nrho=2000)
-    for case, lat in lattice_cases().items():
+    wanted = set(sys.argv[1:])
+    for case, (lat, elements, ngrid) in lattice_cases().items():
+        if wanted and case not in wanted:
+            continue
+        tables = W.johnson_tables(elements=elements, nr=ngrid, nrho=ngrid)
         arrays = dict(F=tables.F, rho=tables.rho, rphi=tables.rphi,
                       grid=np.array([tables.nrho, tables.drho, tables.nr, tables.dr]),
                       masses=np.array(tables.masses), beta=np.array(0.37),

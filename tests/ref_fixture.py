@@ -26,10 +26,15 @@ def load(path):
         return {k: z[k] for k in z.files}
 
 
+def elements(d):
+    """the species of a fixture, in type order (the count is the width of c)"""
+    return ["Ni", "Cu", "Pd", "Ag"][: d["c_read"].shape[1]]
+
+
 def tables(d):
     from mapp_amd.workload import FSTables
     nrho, drho, nr, dr = d["grid"]
-    return FSTables(elements=["Ni", "Cu"], nrho=int(nrho), drho=float(drho), nr=int(nr), dr=float(dr),
+    return FSTables(elements=elements(d), nrho=int(nrho), drho=float(drho), nr=int(nr), dr=float(dr),
                     F=d["F"], rho=d["rho"], rphi=d["rphi"], masses=list(d["masses"]))
 
 

@@ -35,4 +35,5 @@ def test_create_without_gpu_reports_error_not_crash():
         lib.mapp_dmd_destroy(ctx)
         return
     assert rc < 0 and not ctx.value
-    assert lib.mapp_dmd_create(ctypes.byref(ctx), 3, 1, 0) == -4  # E_RANGE: nt > 2
+    assert lib.mapp_dmd_create(ctypes.byref(ctx), 5, 1, 0) == -4  # E_RANGE: DMD takes nt <= 4
+    assert lib.mapp_dmd_create(ctypes.byref(ctx), 3, 3, 0) == -4  # E_RANGE: plain EAM takes nt <= 2

@@ -3,7 +3,8 @@ extended CFG file.
 
 The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
 _DMD_I and _DMD_II, compiled from the reference's sources, produced for two
-500-site binary systems (per-(site, type) widths, asymmetric density tables,
+500-site binary systems and two 256-site systems of three and four types
+(per-(site, type) widths, asymmetric density tables,
 so every slot and index quirk acts).  Here the system goes through the
 library's extended-CFG writer and reader (read_cfg.cpp:562-694 restated): the
 positions, widths and concentrations read back must equal what the reference
@@ -34,13 +35,13 @@ def _from_cfg(d, q, variant):
     kw, skin = R.coef(d)
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "system.cfg")
-        write_cfg(path, d["H"], d["x_read"], d["c_read"], list(d["masses"]), ["Ni", "Cu"])
+        write_cfg(path, d["H"], d["x_read"], d["c_read"], list(d["masses"]), t.elements)
         H, x, c, masses, names = read_cfg(path)
-    assert names == ["Ni", "Cu"]
+    assert names == t.elements
     assert np.array_equal(H.view(np.uint64), np.asarray(d["H"], dtype=np.float64).view(np.uint64))
     assert np.array_equal(x.view(np.uint64), d["x_read"].view(np.uint64))
     assert np.array_equal(c.view(np.uint64), d["c_read"].view(np.uint64))
-    ff = ForceFieldDMD(no_types=2, variant=variant)
+    ff = ForceFieldDMD(no_types=len(t.elements), variant=variant)
     ff.set_fs(t)
     ff.coef(kw["r_crd"], q, kw["alpha_min"], kw["alpha_max"], kw["T"], list(masses))
     ff.set_system(H, x, c)
@@ -55,7 +56,7 @@ def test_gpu_matches_reference_from_cfg(path, q, variant):
     ff = _from_cfg(d, q, variant)
     natms = d["x_read"].shape[0]
     gx, _ = ff.ghosts()
-    ref_x = d[f"q{q}_x_all"].reshape(-1, 5)
+    ref_x = d[f"q{q}_x_all"].reshape(-1, 3 + d["c_read"].shape[1])
     assert np.array_equal(gx.view(np.uint64), ref_x[natms:].view(np.uint64))
     off, nl = ff.neighbor_list()
     roff, rnl = R.csr(d, q, "list")

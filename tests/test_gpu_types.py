@@ -1,0 +1,107 @@
+"""Three and four types: the HIP path against the FP64 oracle beyond the
+256-site reference fixtures (which pin both at three and four types): 864-site
+FCC alloys with concentrations drawn over the simplex, every pass-1 schedule
+(one, two or four lanes per (pair, type pair); with nine type pairs some lanes
+of each wavefront idle), the 3- and 5-point rules, FP64 and mixed precision,
+the mobility calls, and positions moved within the skin and past it.
+
+Bars: FP64 <= 1e-12 relative (infinity norm over the array); mixed precision
+(FP32 quadrature sums) <= 1e-5."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_inf
+
+pytestmark = pytest.mark.gpu
+
+ELEMENTS = ("Ni", "Cu", "Pd", "Ag")
+
+
+@pytest.fixture(scope="module")
+def tables_n():
+    from mapp_amd import workload as W
+    return {nt: W.johnson_tables(elements=ELEMENTS[:nt], nr=2001, nrho=2001) for nt in (3, 4)}
+
+
+def _pair(nt, tables_n, no_i, variant="DMD_I", precision="fp64", seed=31):
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup
+    from oracle import oracle as O
+    lat = W.fcc_alloy(6, nt=nt, seed=seed)
+    ff = setup(lat, tables_n[nt], no_i=no_i, variant=variant, precision=precision)
+    orc = O.make(lat, tables_n[nt], no_i=no_i, variant=variant)
+    return lat, ff, orc
+
+
+def _check_force(ff, orc, tol, label):
+    en_g, en_o = ff.force_calc(1), orc.force_calc(1)
+    errs = {"f": rel_inf(ff.get("f"), orc.get("f")), "E": rel_inf(ff.get("E"), orc.get("E")),
+            "dE": rel_inf(ff.get("dE"), orc.get("dE")),
+            "energy": abs(en_g[0] - en_o[0]) / abs(en_o[0]),
+            "virial": rel_inf(np.asarray(en_g[1:7]), np.asarray(en_o[1:7]))}
+    print(label, {k: f"{e:.1e}" for k, e in errs.items()}, flush=True)
+    for k, e in errs.items():
+        assert e <= tol, (label, k, e)
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+@pytest.mark.parametrize("parts", [1, 2, 4])
+@pytest.mark.parametrize("precision,no_i,tol", [("fp64", 3, 1e-12), ("fp64", 5, 1e-12), ("mixed", 5, 1e-5)])
+def test_force_and_energy_every_schedule(tables_n, nt, parts, precision, no_i, tol):
+    lat, ff, orc = _pair(nt, tables_n, no_i, precision=precision)
+    ff.set_schedule([parts, 2, 0, 2, 0])
+    _check_force(ff, orc, tol, f"nt {nt} parts {parts} {precision} q{no_i}")
+    e_g, e_o = ff.energy_calc(), orc.energy_calc()
+    assert abs(e_g - e_o) <= tol * abs(e_o)
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+@pytest.mark.parametrize("variant", ["DMD", "DMD_I", "DMD_II"])
+def test_mobility_calls(tables_n, nt, variant):
+    """c_d_calc and calc_g (both chk) after a force evaluation"""
+    lat, ff, orc = _pair(nt, tables_n, 3, variant=variant, seed=32)
+    _check_force(ff, orc, 1e-12, f"nt {nt} {variant}")
+    ff.create_2nd_neigh_lst()
+    orc.create_2nd_neigh_lst()
+    ff.c_d_calc()
+    orc.c_d_calc()
+    assert rel_inf(ff.get("c_d"), orc.get("c_d")) <= 1e-12
+    rng = np.random.default_rng(nt)
+    a = rng.uniform(-0.2, 0.2, size=lat.c.shape)
+    ff.set_vec("a", a)
+    for chk in (0, 1):
+        cost_g = ff.calc_g(chk, 0.37)
+        cost_o, g_o = orc.calc_g(chk, 0.37, a)
+        print(f"nt {nt} {variant} calc_g chk {chk}: cost {cost_g!r} vs {cost_o!r}", flush=True)
+        assert abs(cost_g - cost_o) <= 1e-12 * abs(cost_o)
+        if chk == 0:
+            assert rel_inf(ff.get("g"), g_o) <= 1e-12
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+def test_moves_within_and_past_the_skin(tables_n, nt):
+    """set_x without a rebuild (pairs crossing the quadrature cut) and with one"""
+    lat, ff, orc = _pair(nt, tables_n, 3, seed=33)
+    _check_force(ff, orc, 1e-12, f"nt {nt} start")
+    x0 = orc.x_all()[: lat.natms].copy()
+    rng = np.random.default_rng(40 + nt)
+    for step, sigma in enumerate((0.07, 0.0, 0.25)):
+        x = x0.copy()
+        x[:, :3] += rng.normal(0.0, sigma, size=(lat.natms, 3))
+        rebuilt_g, rebuilt_o = ff.set_x(x), orc.set_x(x)
+        assert rebuilt_g == rebuilt_o
+        _check_force(ff, orc, 1e-12, f"nt {nt} move {step} (rebuild {rebuilt_g})")
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+def test_autotuned_schedule(tables_n, nt):
+    """every candidate schedule runs under the autotuner; the kept one meets the bar"""
+    lat, ff, orc = _pair(nt, tables_n, 3, seed=34)
+    rows = ff.autotune(reps=1)
+    assert len(rows) > 0
+    print(f"nt {nt} autotune kept {ff.schedule()} of {len(rows)} candidates", flush=True)
+    _check_force(ff, orc, 1e-12, f"nt {nt} autotuned")
+    ff.close()

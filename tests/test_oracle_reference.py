@@ -2,7 +2,8 @@
 
 The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
 _DMD_I and _DMD_II, compiled from the reference's sources, produced for two
-500-site binary systems with two types: per-(site, type) widths (the
+500-site binary systems and two 256-site systems with three and four types:
+per-(site, type) widths (the
 density-slot transposition of src/ff_dmd_I.cpp:206 vs :376 and the shared phi
 slot do not cancel), asymmetric density tables rho_ab != rho_ba, the
 dE[jatm+jtype] index of :2336 / :2515, energy_calc's type triangle (:508), and
@@ -36,7 +37,7 @@ def _oracle(d, q, variant):
     t = R.tables(d)
     lat = R.lattice(d)
     kw, skin = R.coef(d)
-    o = O.Oracle(nt=2, variant=variant)
+    o = O.Oracle(nt=len(t.elements), variant=variant)
     o.set_fs(t)
     o.set_params(no_i=q, masses=t.masses, **kw)
     o.set_system(lat.H, lat.x, lat.c)
