@@ -511,6 +511,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="time the mixed (FP32 quadrature, 5-point) force_calc alone")
     ap.add_argument("--mobility", action="store_true", help="time c_d_calc and calc_g also with --fp64-only")
+    ap.add_argument("--types", type=int, default=0,
+                    help="time FP64 force_calc alone on an alloy of this many types (1-4: Ni, Cu, Pd, Ag)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--schedule", type=str, default=None,
                     help="fixed launch schedule 'parts,wpb,waves,wpb2,waves2' instead of the autotuner (profiling)")
@@ -535,6 +537,18 @@ def main():
         sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
         r = run_path(lat, W.johnson_tables(), 5, "mixed", args.steps, args.warmup, sched)
         print(json.dumps({"mixed_fp32_q5": {k: v for k, v in r.items() if k != "f"}}), flush=True)
+        return
+    if args.types:
+        from mapp_amd import workload as W
+        nt = args.types
+        if not 1 <= nt <= 4:
+            sys.exit("--types takes 1 to 4")
+        lat = W.fcc_alloy(args.cells, nt=nt, seed=1)
+        tables = W.johnson_tables(elements=("Ni", "Cu", "Pd", "Ag")[:nt])
+        sched = None if args.schedule is None else [int(v) for v in args.schedule.split(",")]
+        r = run_path(lat, tables, 3, "fp64", args.steps, args.warmup, sched, mobility=True)
+        r["site_evals_per_s"] = lat.natms / (r["ms_per_step"] / 1e3)
+        print(json.dumps({f"fp64_q3_{nt}_types": {k: v for k, v in r.items() if k != "f"}}), flush=True)
         return
     if args.surrogate_only:
         from mapp_amd import workload as W
