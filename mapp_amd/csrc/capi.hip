@@ -387,7 +387,7 @@ void finish_lists(mapp_dmd* ctx) {
     ctx->d_f.reserve((size_t)natms * (3 + NT));
     ctx->d_fown.reserve((size_t)natms * (3 + NT));
     ctx->d_pairE.reserve((size_t)(nrev + 1) * NT);
-    ctx->d_deriv.reserve((size_t)(ctx->npairs + 1) * NT * NT * 4);
+    ctx->d_deriv.reserve((size_t)(ctx->npairs + 1) * DerivRec<NT>::kLen);
     ctx->d_pairF.reserve((size_t)(nrev + 1) * (3 + NT));
     // every pair starts outside the quadrature cut with zero partner shares; pass 1
     // writes the shares of the pairs inside and clears those of a pair that leaves

@@ -55,7 +55,7 @@ struct mapp_dmd {
     mappdmd::DevBuf<int> d_sort_tmp;  // out-of-place segment sort
     mappdmd::DevBuf<int> d_pair_i, d_pair_i2, d_rpos;
     mappdmd::DevBuf<double> d_inva, d_deriv;
-    mappdmd::DevBuf<double4> d_arec;  // [ntot][nt] {alpha, c, 1/alpha, dE}  // deriv: [npairs][nt*nt][4] of the quadrature type
+    mappdmd::DevBuf<double4> d_arec;  // [ntot][nt] {alpha, c, 1/alpha, dE}  // deriv: [npairs][DerivRec<nt>::kLen] of the quadrature type
     long nrev = 0;
     std::vector<mappdmd::DevBuf<int>*> scan_levels;
     long npairs = 0, npairs2 = 0;

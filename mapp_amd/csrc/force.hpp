@@ -597,14 +597,29 @@ __device__ __forceinline__ int pack_in_range(int p, bool in, int* out) {
 // only for itype == jtype; src/ff_dmd_I.cpp:508).  The partner's density share
 // goes to pairE in reverse-CSR order (rpos), so the embedding reads it
 // contiguously.  Unless ENERGY_ONLY, the rho lane stores {drho/dr, drho/dalpha}
-// and the phi lane {dphi/dr, dphi/dalpha} at deriv[off[i] + k][q] for the k-th
-// pair of site i inside the quadrature cut (the reference's per-pair caches,
-// src/ff_dmd_I.cpp:227-242, restricted to what the back-pass reads): a site's
-// records are contiguous, and pass 2 ranks its inside pairs the same way.
+// and the phi lane {dphi/dr, dphi/dalpha} in the record (DerivRec) of the k-th
+// pair of site i inside the quadrature cut, at off[i] + k (the reference's
+// per-pair caches, src/ff_dmd_I.cpp:227-242, restricted to what the back-pass
+// reads): a site's records are contiguous, and pass 2 ranks its inside pairs
+// the same way.  The phi lanes of the type pairs it < jt neither evaluate nor
+// store: the back-pass takes their phi from the transposed pair.
 template <int NT, int PARTS = 4>
 struct Pass1Map {
     static constexpr int LPP = NT * NT * PARTS;
     static constexpr int PPW = 64 / LPP;
+};
+
+// The derivative record of one pair inside the cut, kLen values of the
+// quadrature type: {drho/dr, drho/dalpha} of every ordered type pair
+// q = it*NT + jt, then {dphi/dr, dphi/dalpha} of the type pairs it >= jt only.
+// The back-pass uses the phi of (it, jt) for it >= jt on both (it, jt) and
+// (jt, it), as ForceField_DMD_I::force_calc reads one phi cache slot for both
+// orders (src/ff_dmd_I.cpp:244-248, :376), so the other half is never read.
+template <int NT>
+struct DerivRec {
+    static constexpr int kLen = 2 * NT * NT + NT * (NT + 1);
+    __host__ __device__ static constexpr int rho(int q) { return 2 * q; }
+    __host__ __device__ static constexpr int phi(int it, int jt) { return 2 * NT * NT + it * (it + 1) + 2 * jt; }
 };
 
 template <typename Q>
@@ -771,7 +786,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (inr) {
                 const double r_inv = rsqrt(rsq), r = rsq * r_inv;
                 const double ia = iai + iaj, asq = rsqrt(ia);
-                const int br = active ? alpha_branch(P, ia, ai, aj) : 0;
+                // the phi lanes of it < jt are idle (DerivRec)
+                const bool live = PARTS == 1 || prt < PARTS / 2 || want_phi;
+                const int br = active && live ? alpha_branch(P, ia, ai, aj) : 0;
                 // v1: the lane's function (rho on the rho lane, phi on the phi lane) or,
                 // with one lane per pair, rho; v2: phi with one lane per pair.
                 // Each holds value, d/dr, d/dalpha.
@@ -803,7 +820,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         ppv[kp] = pp < Q(1) ? pp : Q(1);
                         const int row = tab_off + m * (quad_stride(NT) * (int)sizeof(Q));
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) rowv[kp][v] = RowLoad<Q2>::get(qrs, row + v * (int)sizeof(Q2));
+                        for (int v = 0; v < NV; ++v)  // one lane per pair: the phi half only where it >= jt
+                            rowv[kp][v] = PARTS != 1 || v < 2 || want_phi
+                                              ? RowLoad<Q2>::get(qrs, row + v * (int)sizeof(Q2))
+                                              : Q2{Q(0), Q(0)};
                     }
                     // keep the requests ahead of the arithmetic where the rows fit the
                     // register budget of four waves per SIMD (at most 24 registers)
@@ -891,11 +911,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v2[0];
                     }
                     if (!ENERGY_ONLY) {
-                        Q* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + q) * 4;
-                        __builtin_nontemporal_store((Q)v1[1], d);
-                        __builtin_nontemporal_store((Q)v1[2], d + 1);
-                        __builtin_nontemporal_store((Q)v2[1], d + 2);
-                        __builtin_nontemporal_store((Q)v2[2], d + 3);
+                        Q* d = deriv + (size_t)(p0 + done + slot) * DerivRec<NT>::kLen;
+                        __builtin_nontemporal_store((Q)v1[1], d + DerivRec<NT>::rho(q));
+                        __builtin_nontemporal_store((Q)v1[2], d + DerivRec<NT>::rho(q) + 1);
+                        if (want_phi) {
+                            __builtin_nontemporal_store((Q)v2[1], d + DerivRec<NT>::phi(it, jt));
+                            __builtin_nontemporal_store((Q)v2[2], d + DerivRec<NT>::phi(it, jt) + 1);
+                        }
                     }
                 } else {
                     if (rho_lane) {
@@ -906,8 +928,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
                         en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * v1[0];
                     }
-                    if (!ENERGY_ONLY && (rho_lane || phi_lane)) {
-                        Q* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + q) * 4 + (rho_lane ? 0 : 2);
+                    if (!ENERGY_ONLY && (rho_lane || (phi_lane && want_phi))) {
+                        Q* d = deriv + (size_t)(p0 + done + slot) * DerivRec<NT>::kLen +
+                               (rho_lane ? DerivRec<NT>::rho(q) : DerivRec<NT>::phi(it, jt));
                         __builtin_nontemporal_store((Q)v1[1], d);
                         __builtin_nontemporal_store((Q)v1[2], d + 1);
                     }
@@ -1137,15 +1160,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     dEj = rj.w;
                 }
                 const double r_inv = rsqrt(rsq);
-                // the lane's own record; the transposed type pair's comes from its quad lane
-                const Q* rec = deriv + ((size_t)(p0 + done + slot) * LPP + q) * 4;
-                const typename Vec2<Q>::T rg = *reinterpret_cast<const typename Vec2<Q>::T*>(rec);
-                const typename Vec2<Q>::T rf = *reinterpret_cast<const typename Vec2<Q>::T*>(rec + 2);
+                // the lane's own density record; the transposed type pair's comes from
+                // its lane (transpose_tp)
+                const Q* rec = deriv + (size_t)(p0 + done + slot) * DerivRec<NT>::kLen;
+                const typename Vec2<Q>::T rg =
+                    *reinterpret_cast<const typename Vec2<Q>::T*>(rec + DerivRec<NT>::rho(q));
+                // one phi slot serves (it, jt) and (jt, it)
+                const typename Vec2<Q>::T rf = *reinterpret_cast<const typename Vec2<Q>::T*>(
+                    rec + DerivRec<NT>::phi(want_phi ? it : jt, want_phi ? jt : it));
                 const double g_dr = (double)rg.x, g_da = (double)rg.y;
                 const double f_dr = (double)rf.x, f_da = (double)rf.y;
                 const double gT_dr = transpose_tp<NT>(g_dr), gT_da = transpose_tp<NT>(g_da);
-                const double fT_dr = transpose_tp<NT>(f_dr), fT_da = transpose_tp<NT>(f_da);
-                const double fr_dr = want_phi ? f_dr : fT_dr, fr_da = want_phi ? f_da : fT_da;
+                const double fr_dr = f_dr, fr_da = f_da;
                 const double cc = ci * cj;
                 // 1/(alpha_i + alpha_j) = (1/alpha_i)(1/alpha_j) / (1/alpha_i + 1/alpha_j)
                 const double ia = iai + iaj;

@@ -661,11 +661,13 @@ __global__ __launch_bounds__(kSurBlock) void k_pass1_sur(FFParams<double> P, FFP
                     const double half = (j < P.natms ? 0.5 : 0.25) * (it != jt ? P.xw : 1.0);
                     en += half * (cj[jt] * ci[it] + cj[it] * ci[jt]) * fv[0];
                 }
-                double* d = deriv + ((size_t)(p0 + done + slot) * NT * NT + tp) * 4;
-                __builtin_nontemporal_store(gv[1], d);
-                __builtin_nontemporal_store(gv[2], d + 1);
-                __builtin_nontemporal_store(fv[1], d + 2);
-                __builtin_nontemporal_store(fv[2], d + 3);
+                double* d = deriv + (size_t)(p0 + done + slot) * DerivRec<NT>::kLen;
+                __builtin_nontemporal_store(gv[1], d + DerivRec<NT>::rho(tp));
+                __builtin_nontemporal_store(gv[2], d + DerivRec<NT>::rho(tp) + 1);
+                if (want_phi) {
+                    __builtin_nontemporal_store(fv[1], d + DerivRec<NT>::phi(it, jt));
+                    __builtin_nontemporal_store(fv[2], d + DerivRec<NT>::phi(it, jt) + 1);
+                }
             }
             ej += __shfl_xor(ej, 32, 64);  // sum over it: type pairs (0,jt) and (1,jt) are 32 lanes apart
             if (valid && it == 0 && j < P.natms) pairE[(size_t)nlrp[p].y * NT + jt] = ej;  // scattered: L2 merges lines
