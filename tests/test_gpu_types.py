@@ -3,7 +3,8 @@
 FCC alloys with concentrations drawn over the simplex, every pass-1 schedule
 (one, two or four lanes per (pair, type pair); with nine type pairs some lanes
 of each wavefront idle), the 3- and 5-point rules, FP64 and mixed precision,
-the mobility calls, and positions moved within the skin and past it.
+the mobility calls, positions moved within the skin and past it, and the
+device-resident BDF clock and L-BFGS minimiser.
 
 Bars: FP64 <= 1e-12 relative (infinity norm over the array); mixed precision
 (FP32 quadrature sums) <= 1e-5."""
@@ -104,4 +105,83 @@ def test_autotuned_schedule(tables_n, nt):
     assert len(rows) > 0
     print(f"nt {nt} autotune kept {ff.schedule()} of {len(rows)} candidates", flush=True)
     _check_force(ff, orc, 1e-12, f"nt {nt} autotuned")
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+def test_bdf_matches_oracle(tables_n, nt):
+    """the device BDF clock (DMD_II mobility, initial step 8: order-2 steps
+    occur) against the oracle's restatement; bars as for two types: same
+    orders and solve counts, step sizes within 1e-8, concentrations within 1e-8"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup
+    from oracle import oracle as O
+    lat = W.fcc_alloy(4, nt=nt, seed=4)
+    ff = setup(lat, tables_n[nt], no_i=3, variant="DMD_II")
+    orc = O.make(lat, tables_n[nt], no_i=3, variant="DMD_II")
+    n = 16
+    code_g, rep_g, ng_g = ff.bdf_run(n, 2, initial_del_t=8.0)
+    code_o, rep_o, ng_o = orc.bdf_run(n, 2, initial_del_t=8.0)
+    print(f"nt {nt} bdf gpu {code_g} {ng_g} oracle {code_o} {ng_o} orders {rep_g[:, 0].astype(int).tolist()}",
+          flush=True)
+    assert code_g == code_o == n
+    np.testing.assert_array_equal(rep_g[:, 0], rep_o[:, 0])
+    np.testing.assert_array_equal(rep_g[:, 5], rep_o[:, 5])
+    np.testing.assert_allclose(rep_g[:, 1], rep_o[:, 1], rtol=1e-8)
+    assert np.max(np.abs(ff.get("c") - orc.get("c"))) <= 1e-8
+    assert rep_g[:, 0].max() == 2
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+def test_lbfgs_consistent_energy_matches_oracle(nt):
+    """the device minimiser with the consistent energy (symmetric densities,
+    one width per site) against the oracle's restatement: same exit code,
+    step and call counts, energies within 1e-10, positions within 1e-9"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup
+    from oracle import oracle as O
+    tables = W.johnson_tables(elements=ELEMENTS[:nt], symmetric_density=True)
+    lat = W.fcc_alloy(4, nt=nt, seed=4, alpha_per_site=True)
+    ff = setup(lat, tables, no_i=3)
+    orc = O.make(lat, tables, no_i=3)
+    ff.set_energy_mode(True)
+    orc.set_energy_full(True)
+    code_g, en_g, nf_g, ne_g = ff.min_lbfgs(20, 2, 1e-8)
+    code_o, en_o, nf_o, ne_o = orc.min_lbfgs(20, 2, 1e-8)
+    print(f"nt {nt} lbfgs gpu {code_g} {len(en_g)} {nf_g} {ne_g} oracle {code_o} {len(en_o)} {nf_o} {ne_o}", flush=True)
+    assert (code_g, len(en_g), nf_g, ne_g) == (code_o, len(en_o), nf_o, ne_o)
+    assert np.max(np.abs(en_g - en_o) / np.abs(en_o)) <= 1e-10
+    x_o = orc.x_all()[: lat.natms]
+    assert np.max(np.abs(ff.get("x") - x_o)) <= 1e-9 * max(1.0, np.max(np.abs(x_o)))
+    ff.close()
+
+
+@pytest.mark.parametrize("nt", [3, 4])
+def test_batched_cells_match_each_cell(tables_n, nt):
+    """uneven cells in one batch (mapp_dmd_set_cells): each cell's slice of f,
+    E and c_d within 1e-12 of that cell's oracle run; batch energy = sum"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_cells
+    from oracle import oracle as O
+    lats = [W.fcc_alloy(n, nt=nt, seed=50 + n) for n in (3, 5, 4)]
+    ff = setup_cells(lats, tables_n[nt], no_i=3, variant="DMD_I")
+    orcs = [O.make(l, tables_n[nt], no_i=3, variant="DMD_I") for l in lats]
+    en = ff.force_calc(1)
+    f, E = ff.get("f"), ff.get("E")
+    ff.create_2nd_neigh_lst()
+    ff.c_d_calc()
+    cd = ff.get("c_d")
+    e_tot, off = 0.0, 0
+    for l, o in zip(lats, orcs):
+        e_tot += o.force_calc(1)[0]
+        sl = slice(off, off + l.natms)
+        e_f, e_E = rel_inf(f[sl], o.get("f")), rel_inf(E[sl], o.get("E"))  # E as force_calc leaves it
+        o.create_2nd_neigh_lst()
+        o.c_d_calc()
+        errs = (e_f, e_E, rel_inf(cd[sl], o.get("c_d")))
+        print(f"nt {nt} cell {l.natms} sites: f {errs[0]:.1e} E {errs[1]:.1e} c_d {errs[2]:.1e}", flush=True)
+        assert max(errs) <= 1e-12
+        off += l.natms
+    assert abs(en[0] - e_tot) <= 1e-12 * abs(e_tot)
     ff.close()
