@@ -1069,8 +1069,8 @@ __global__ __launch_bounds__(kBlock) void k_embed(FFParams<double> P, const doub
 // ---------------------------------------------------------------- pass 2
 // src/ff_dmd_I.cpp:357-436: fpair/apair of each ordered type pair from the
 // derivative records of pass 1 (slot type2rho[jt][it], i.e. the transposed
-// lane's record, pairs with dE_i; phi for it < jt lives in the transposed
-// lane's record as in the reference's shared COMP slot).  No table lookups.
+// lane's record, pairs with dE_i; phi for it < jt is the phi record of
+// (jt, it), as in the reference's shared COMP slot).  No table lookups.
 template <int NT, typename Q>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pass2(FFParams<double> P, const double4* __restrict__ pos,
                                                   const double4* __restrict__ arec, PairBufs L,
