@@ -91,6 +91,17 @@ def lattice_cases():
     # over the simplex, every density slot rho_ab distinct
     cases["ternary4"] = (W.fcc_alloy(4, nt=3, seed=103), ("Ni", "Cu", "Pd"), 1000)
     cases["quaternary4"] = (W.fcc_alloy(4, nt=4, seed=104), ("Ni", "Cu", "Pd", "Ag"), 1000)
+    # three types with widths in the three bands and concentration vectors on
+    # calc_ent's margins (types that are absent, nearly absent or nearly pure)
+    lat = W.fcc_alloy(4, nt=3, seed=105)
+    rng = np.random.default_rng(205)
+    band = rng.integers(0, 3, size=lat.c.shape)
+    lat.x[:, 3:] = rng.uniform(np.array([12.0, 60.0, 2000.0])[band], np.array([40.0, 120.0, 4000.0])[band])
+    edges = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 1.0], [1.0 - 5e-9, 5e-9, 0.0], [1e-8, 1.0 - 2e-8, 1e-8],
+                      [2e-8, 0.5, 0.5 - 2e-8], [0.0, 1.0 - 1e-8, 1e-8], [1.0 / 3, 1.0 / 3, 1.0 / 3]])
+    pick = np.flatnonzero(rng.integers(0, 3, size=lat.natms) == 0)
+    lat.c[pick] = edges[np.arange(pick.size) % len(edges)]
+    cases["regimes_ternary4"] = (lat, ("Ni", "Cu", "Pd"), 1000)
     return cases
 
 
