@@ -4,9 +4,9 @@ extended CFG file.
 The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
 _DMD_I and _DMD_II, compiled from the reference's sources, produced for two
 500-site binary systems and three 256-site systems of three and four types
-(one with widths in all three regimes and concentrations on the margins)
-(per-(site, type) widths, asymmetric density tables,
-so every slot and index quirk acts).  Here the system goes through the
+(per-(site, type) widths, asymmetric density tables, so every slot and index
+quirk acts; the second binary system and one three-type system with widths in
+all three regimes and concentrations on the margins).  Here the system goes through the
 library's extended-CFG writer and reader (read_cfg.cpp:562-694 restated): the
 positions, widths and concentrations read back must equal what the reference
 read, bit for bit; then the HIP path runs the harness's sequence -- 2nd list,
