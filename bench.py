@@ -429,6 +429,31 @@ def cpu_baseline(tables, n=32, reps=3, threads=None):
                                    sample=f"the same call on one {n}^3 box, 1 thread, median of {reps}"))
 
 
+def cpu_baseline_reference(tables, n=32, reps=3, procs=None):
+    """The reference's own ForceField_DMD_I::force_calc, built from its sources
+    (oracle/Makefile.ref) and driven by oracle/ref_harness: one process on one
+    n^3 box, then K processes at once, each on its own copy (K ranks with no
+    communication, as for the port); every process builds its system first and
+    the timed calls start together.  value = K * sites / the median over reps of
+    the slowest process's call."""
+    import tempfile
+    from mapp_amd import workload as W
+    from oracle import ref_run as RR
+    lat = W.fcc_alloy(n, seed=7)
+    with tempfile.TemporaryDirectory() as tmp:
+        one = RR.time_force(tmp, lat, tables, 1, reps)[0]
+    k = procs or cpu_threads()
+    with tempfile.TemporaryDirectory() as tmp:
+        tk = RR.time_force(tmp, lat, tables, k, reps)
+    wall = float(np.median(tk.max(axis=0)))
+    return dict(value=k * lat.natms / wall, unit="site-evals/s", cores=k, kind="reference",
+                sample=f"the reference's ForceField_DMD_I::force_calc(1) (built from its sources, -O2) in {k} "
+                       f"processes at once, each on its own {n}^3 FCC box ({lat.natms} sites), 3-point rule, "
+                       f"median of {reps} calls of the slowest process",
+                single_thread=dict(value=lat.natms / float(np.median(one)), cores=1,
+                                   sample=f"the same call in one process on one {n}^3 box, median of {reps}"))
+
+
 def run_small(tables, n=10, reps=5):
     """The 4,000-site system (10^3 FCC cells), one energy+force+dE/dalpha+dc/dt evaluation:
     force_calc(1) and c_d_calc on the GPU and, unless no_cpu, on one host thread with the
@@ -677,8 +702,20 @@ def main():
     out["small_4000"] = small
     if not args.no_cpu:
         out["small_4000"]["cpu_oracle"] = cpu_small(small_lat, tables)
-        out["cpu_baseline"] = cpu_baseline(tables)
-        out["cpu_baseline"]["single_thread_full_size"] = cpu_full_size(lat, tables)
+        port = cpu_baseline(tables)
+        port["single_thread_full_size"] = cpu_full_size(lat, tables)
+        from oracle import ref_run as RR
+        ref = None
+        if RR.available():
+            try:
+                ref = cpu_baseline_reference(tables)
+            except Exception as e:  # the build did not run here: the port stands in
+                log("reference baseline unavailable:", e)
+        if ref is not None:
+            ref["port"] = port
+            out["cpu_baseline"] = ref
+        else:
+            out["cpu_baseline"] = port
         log("cpu", out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 

@@ -1,8 +1,10 @@
 // TEST INFRASTRUCTURE ONLY -- drives the reference's own force field classes
 // (compiled from /root/reference/src by oracle/Makefile.ref into oracle/_ref/)
 // on one input and dumps every result of the hot path, to pin the oracle's
-// restatement against the reference bit for bit.  It runs only in the build
-// container; nothing on the GPU path links or loads it.
+// restatement against the reference bit for bit, and, with "time_force N",
+// to time the reference's own force_calc as the CPU baseline of bench.py
+// (where the build travels with the tree).  Nothing on the GPU path links or
+// loads it.
 //
 // Sequence (the one Clock_BDF::init runs, src/clock_bdf.cpp:201-228): an
 // input script reads the extended CFG (mode dmd, skin, read cfg), the force
@@ -22,8 +24,15 @@
 // usage: ref_harness <job file>; the job file holds "key value..." lines:
 //   cfg <path>  fs <path>  setfl <path>  variant DMD|DMD_I|DMD_II
 //   coef r_crd no_i alpha_min alpha_max T  skin s  a <path>  beta b  out <path>
+//   time_force N (optional: after create_2nd_neigh_lst, N calls of
+//   force_calc(1), each timed on the steady clock; output t_force [N] seconds
+//   and en_st only)  go <path> (optional, with time_force: create
+//   "<out>.ready" once the system is built, then start the timed calls when
+//   <path> exists, so that several processes time their calls together)
 // Output: for every array a line "name rows cols" followed by rows*cols
 // little-endian doubles.
+#include <chrono>
+#include <unistd.h>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -159,8 +168,9 @@ int main(int argc, char** argv) {
     const int natms = atoms->natms;
     const int xdim = atoms->vectors[0].dim;
 
+    const int n_time = job.count("time_force") ? atoi(job["time_force"].at(0).c_str()) : 0;
     Out out{fopen(job["out"].at(0).c_str(), "wb")};
-    {
+    if (n_time == 0) {
         TYPE0* x;
         atoms->vectors[0].ret(x);
         out.put("x_read", natms, xdim, x);
@@ -233,6 +243,28 @@ int main(int argc, char** argv) {
     mapp->neighbor->create_list(0, 1);
     atoms->store_0();
     ff->create_2nd_neigh_lst();
+
+    if (n_time > 0) {
+        if (job.count("go")) {
+            const std::string ready = job["out"].at(0) + ".ready";
+            FILE* fr = fopen(ready.c_str(), "w");
+            if (fr) fclose(fr);
+            const std::string go = job["go"].at(0);
+            while (access(go.c_str(), F_OK) != 0) usleep(1000);
+        }
+        double en[7] = {0, 0, 0, 0, 0, 0, 0};
+        std::vector<double> t(n_time);
+        for (int k = 0; k < n_time; k++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            ff->force_calc(1, en);
+            t[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        out.put("t_force", n_time, 1, t.data());
+        out.put("en_st", 1, 7, en);
+        fclose(out.fp);
+        MPI_Finalize();
+        return 0;
+    }
 
     const int nph = atoms->natms_ph;
     {

@@ -26,45 +26,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
-HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+from oracle.ref_run import COEF, HARNESS, read_dump, write_inputs, write_job  # noqa: E402
 
-COEF = dict(r_crd=3.0, alpha_min=25.0, alpha_max=1000.0, T=600.0, skin=0.5)
 # arrays that depend on the rule (through the list cut) but not on the variant
 GEOMETRY = ("x_all", "list_off", "list", "list2_off", "list2", "crd")
-
-
-def write_setfl_style(path, t):
-    """the layout ForceField_DMD_I / _DMD read for "FS" (src/ff_dmd_I.cpp:1069-1298):
-    3 comments, element line, grid line, one header line, then per element F
-    and its density, then r*phi for i >= j, five values per line.  Only its
-    grid matters here: the harness replaces the tables it produces."""
-    nt = len(t.elements)
-    with open(path, "w") as fh:
-        fh.write("# setfl-style copy of the same tables\n#\n#\n")
-        fh.write(f"{nt} " + " ".join(t.elements) + "\n")
-        fh.write(f"{t.nrho} {t.drho:.17g} {t.nr} {t.dr:.17g} {t.rc:.17g}\n")
-        fh.write("0 0.0 0.0 fcc\n")
-        blocks = []
-        for a in range(nt):
-            blocks += [t.F[a], t.rho[a, a]]
-        blocks += [t.rphi[k] for k in range(nt * (nt + 1) // 2)]
-        for b in blocks:
-            for s in range(0, b.size, 5):
-                fh.write(" ".join(f"{v:.17g}" for v in b[s:s + 5]) + "\n")
-
-
-def read_dump(path):
-    out = {}
-    with open(path, "rb") as fh:
-        while True:
-            head = fh.readline()
-            if not head:
-                break
-            name, rows, cols = head.decode().split()
-            n = int(rows) * int(cols)
-            out[name] = np.frombuffer(fh.read(8 * n), dtype="<f8").reshape(int(rows), int(cols)).copy()
-            fh.read(1)
-    return out
 
 
 def lattice_cases():
@@ -106,23 +71,15 @@ def lattice_cases():
 
 
 def run(case, lat, tables, variant, no_i, tmp):
-    from mapp_amd.dmd import write_cfg
-    from mapp_amd import workload as W
-    cfg = os.path.join(tmp, f"{case}.cfg")
-    write_cfg(cfg, lat.H, lat.x, lat.c, tables.masses, tables.elements)
-    fs = os.path.join(tmp, "tables.fs")
-    W.write_fs(fs, tables)
-    setfl = os.path.join(tmp, "tables.setfl")
-    write_setfl_style(setfl, tables)
+    cfg, fs, setfl = write_inputs(tmp, case, lat, tables)
     rng = np.random.default_rng(sum(case.encode()))  # the same a for every variant and rule of a case
     a = rng.uniform(-0.2, 0.2, size=lat.c.shape)
-    np.savetxt(os.path.join(tmp, "a.txt"), a.reshape(-1), fmt="%.17g")
+    a_path = os.path.join(tmp, "a.txt")
+    np.savetxt(a_path, a.reshape(-1), fmt="%.17g")
     out = os.path.join(tmp, f"{case}_{variant}_{no_i}.bin")
-    with open(os.path.join(tmp, "job"), "w") as fh:
-        fh.write(f"cfg {cfg}\nfs {fs}\nsetfl {setfl}\nvariant {variant}\n")
-        fh.write(f"coef {COEF['r_crd']!r} {no_i} {COEF['alpha_min']!r} {COEF['alpha_max']!r} {COEF['T']!r}\n")
-        fh.write(f"skin {COEF['skin']!r}\na {os.path.join(tmp, 'a.txt')}\nbeta 0.37\nout {out}\n")
-    subprocess.run([HARNESS, os.path.join(tmp, "job")], check=True, stdout=subprocess.DEVNULL)
+    job = os.path.join(tmp, "job")
+    write_job(job, cfg, fs, setfl, variant, no_i, out, a_path=a_path)
+    subprocess.run([HARNESS, job], check=True, stdout=subprocess.DEVNULL)
     d = read_dump(out)
     d["a"] = a
     return d
