@@ -2,20 +2,25 @@
 // src/ff_dmd_I.cpp:88-450, and energy_calc, :454-668).
 //
 // The reference scatters every half-list pair into both ends (E[j] +=,
-// f[j] -=).  Here one lane owns one site and never writes another site's row:
-//   pass 1   lane i walks its half list, evaluates the Gauss-Hermite averaged
-//            primitives, keeps its own density sum in registers and writes
-//            the partner's share of the pair (nt doubles) to a pair buffer;
-//   embed    lane i adds the shares of the lower-index sites that list i
-//            (reverse CSR, ascending pair index), then F(rho_bar), F', the
-//            per-site energy terms and the alpha entropic force;
-//   pass 2   lane i re-evaluates the primitives with their r- and alpha-
-//            derivatives, accumulates its own force row and writes the
-//            partner's share (3+nt doubles) to a pair buffer;
-//   gather   lane i adds the partner shares (reverse CSR).
+// f[j] -=).  Here one wavefront owns one site and never writes another
+// site's row; what a pair owes its partner goes to a per-pair share slot at
+// the pair's position in the partner's reverse list:
+//   pass 1   the site's list entries inside the quadrature cut are staged in
+//            LDS; lanes (pair, type pair, part) evaluate the Gauss-Hermite
+//            averaged density and pair terms, the site's own density sums
+//            stay in registers, the partner's density share (nt doubles)
+//            goes to its share slot, and the r- and alpha-derivatives go to
+//            the pair's derivative record (DerivRec);
+//   embed    a lane group per site adds the density shares of the sites that
+//            list it (reverse CSR, ascending pair index), then F(rho_bar), F',
+//            the per-site energy terms and the alpha entropic force;
+//   pass 2   lanes (pair, type pair) combine the derivative records with F'
+//            of both ends into the site's force row and the partner's force
+//            share (3 + nt doubles);
+//   gather   a lane group per site adds its force shares (reverse CSR).
 // No floating-point atomics: every sum has a fixed order, so results are
-// bitwise reproducible run to run.  Energy and virial partials are reduced
-// per block in a fixed tree and then over blocks in block order.
+// bitwise reproducible run to run.  Energy and virial partials are written per
+// wavefront and reduced in two fixed-shape stages.
 //
 // Q is the arithmetic of the quadrature (double, or float for the mixed
 // mode whose point sums run in FP32 with FP32 accumulation); site-level sums,
