@@ -1,5 +1,6 @@
 // MD-mode potentials of the reference on the DMD context's geometry (same
-// ghosts, half list and reverse CSR; one lane per site; partner shares through
+// ghosts, half list and reverse CSR; a group of kEamG = 16 lanes per site, each
+// lane a stride of the site's list, summed in a fixed tree; partner shares through
 // pair buffers in reverse-CSR order, no floating-point atomics):
 //   plain EAM   ForceField_eam::force_calc / energy_calc, src/ff_eam.cpp:44-356
 //               (F the cubic rows of ForceField_eam::interpolate, 7 coefficients;
