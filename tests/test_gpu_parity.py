@@ -241,10 +241,10 @@ def test_read_fs_matches_set_fs(tmp_path, tables):
     assert rel_inf(c.get("f")[:, :3], a2.get("f")[:, :3]) <= 1e-12
 
 
-@pytest.mark.parametrize("parts", [1, 2])
+@pytest.mark.parametrize("parts", [1, 2, 4])
 @pytest.mark.parametrize("precision,no_i,tol", [("fp64", 3, 1e-12), ("mixed", 5, 1e-5)])
 def test_two_lane_pass1_schedule(tables, precision, no_i, tol, parts):
-    """pass 1 with one or two lanes per (pair, type pair) and a persistent grid: same bars"""
+    """pass 1 with one, two or four lanes per (pair, type pair) and a persistent grid: same bars"""
     from mapp_amd import workload as W
     lat = W.fcc_alloy(6, seed=8)
     ff, orc = _pair(lat, tables, no_i=no_i, precision=precision)
