@@ -24,6 +24,17 @@ def build():
     return LIB
 
 
+def build_reference():
+    """the reference's own force field from its sources into oracle/_ref
+    (Makefile.ref; nothing to do where the sources are absent).  A failure is
+    reported, not raised: only the tests that pin the oracle against it and
+    bench.py's CPU baseline use it, and both fall back without it."""
+    r = subprocess.run(["make", "-s", "-j8", "-f", os.path.join(HERE, "Makefile.ref")], cwd=os.path.dirname(HERE))
+    if r.returncode != 0:
+        print(f"oracle/_ref: reference build failed (make exit {r.returncode})", flush=True)
+    return r.returncode == 0
+
+
 def lib():
     global _lib
     if _lib is None:
