@@ -1,5 +1,6 @@
 """GPU against the oracle directly at the full workload size: 64^3 FCC cells
-(1,048,576 sites) for the DMD force field, plain EAM, Lennard-Jones and MEAM,
+(1,048,576 sites) for the DMD force field (two and three types), plain EAM,
+Lennard-Jones and MEAM,
 64^3 BCC cells (524,288 sites) for Finnis-Sinclair and FSM.  The oracle takes
 a few seconds per call at this size on one host core.
 
@@ -50,6 +51,19 @@ def test_dmd_full_size():
     orc = O.make(lat, tables, no_i=3)
     orc.set_compensated(True)
     _check(setup(lat, tables, no_i=3), orc, "dmd")
+
+
+def test_dmd_three_types_full_size():
+    """Ni-Cu-Pd at 64^3 cells: nine type pairs, 30-value derivative records
+    per pair, three-column partner shares, all at the full size"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup
+    from oracle import oracle as O
+    tables = W.johnson_tables(elements=("Ni", "Cu", "Pd"))
+    lat = W.fcc_alloy(N, nt=3, seed=3)
+    orc = O.make(lat, tables, no_i=3)
+    orc.set_compensated(True)
+    _check(setup(lat, tables, no_i=3), orc, "dmd nt=3")
 
 
 def test_eam_full_size():
