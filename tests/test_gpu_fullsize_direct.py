@@ -1,8 +1,7 @@
 """GPU against the oracle directly at the full workload size: 64^3 FCC cells
 (1,048,576 sites) for the DMD force field (two and three types), plain EAM,
-Lennard-Jones and MEAM,
-64^3 BCC cells (524,288 sites) for Finnis-Sinclair and FSM.  The oracle takes
-a few seconds per call at this size on one host core.
+Lennard-Jones and MEAM, 64^3 BCC cells (524,288 sites) for Finnis-Sinclair and
+FSM.  The oracle takes a few seconds per call at this size on one host core.
 
 Bars: neighbour lists bit-exact; forces (per-site sums of a few dozen terms)
 <= 1e-12 relative (infinity norm over the array); energy, virial and
