@@ -24,12 +24,17 @@ def _inside(x, H, natms, cut):
     return r[iu] < cut
 
 
+@pytest.mark.parametrize("nt", [2, 3])
 @pytest.mark.parametrize("precision,no_i,tol", [("fp64", 3, 1e-12), ("mixed", 5, 1e-5)])
-def test_pairs_crossing_the_cut(tables, precision, no_i, tol):
+def test_pairs_crossing_the_cut(tables, precision, no_i, tol, nt):
+    """with three types a pair's share slot also holds two alpha-force columns
+    beyond the first, cleared with it when the pair leaves the cut"""
     from mapp_amd import workload as W
     from mapp_amd.dmd import setup
     from oracle import oracle as O
-    lat = W.fcc_alloy(6, seed=21)
+    if nt != 2:
+        tables = W.johnson_tables(elements=("Ni", "Cu", "Pd")[:nt], nr=2001, nrho=2001)
+    lat = W.fcc_alloy(6, nt=nt, seed=21)
     ff = setup(lat, tables, no_i=no_i, variant="DMD_I", precision=precision)
     orc = O.make(lat, tables, no_i=no_i, variant="DMD_I")
     _, mod_rc, _ = ff.cutoffs()
@@ -53,7 +58,7 @@ def test_pairs_crossing_the_cut(tables, precision, no_i, tol):
             assert abs(e_g - e_o) <= tol * abs(e_o)
         en_g, en_o = ff.force_calc(1), orc.force_calc(1)
         ef = rel_inf(ff.get("f"), orc.get("f"))
-        print(f"{precision} move {step}: pairs left the cut {left}, entered {entered}; "
+        print(f"nt {nt} {precision} move {step}: pairs left the cut {left}, entered {entered}; "
               f"force rel {ef:.3e}", flush=True)
         assert ef <= tol
         assert abs(en_g[0] - en_o[0]) <= tol * abs(en_o[0])
