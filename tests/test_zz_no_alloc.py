@@ -10,11 +10,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _prepared(n, seed, variant):
+def _prepared(n, seed, variant, nt=2):
     from mapp_amd import workload as W
     from mapp_amd.dmd import setup
-    lat = W.fcc_alloy(n, seed=seed)
-    ff = setup(lat, W.johnson_tables(nr=2001, nrho=2001), no_i=3, variant=variant)
+    lat = W.fcc_alloy(n, nt=nt, seed=seed)
+    tables = W.johnson_tables(elements=("Ni", "Cu", "Pd")[:nt], nr=2001, nrho=2001)
+    ff = setup(lat, tables, no_i=3, variant=variant)
     ff.force_calc(1)
     ff.create_2nd_neigh_lst()
     ff.c_d_calc()  # builds the pair values the no-alloc call needs
@@ -23,10 +24,10 @@ def _prepared(n, seed, variant):
     return ff
 
 
-@pytest.mark.parametrize("variant", ["DMD_I", "DMD_II"])
-def test_noalloc_calc_g_matches_calc_g(variant):
+@pytest.mark.parametrize("variant,nt", [("DMD_I", 2), ("DMD_II", 2), ("DMD_II", 3)])
+def test_noalloc_calc_g_matches_calc_g(variant, nt):
     for n, seed in ((6, 3), (4, 5)):  # the second call on a smaller system
-        ff = _prepared(n, seed, variant)
+        ff = _prepared(n, seed, variant, nt)
         for chk in (0, 1):
             ff.set_vec("g", np.zeros_like(ff.get("c")))
             c_ref = ff.calc_g(chk, 2e-3)
@@ -34,7 +35,7 @@ def test_noalloc_calc_g_matches_calc_g(variant):
             ff.set_vec("g", np.full_like(g_ref, np.nan))
             c_na = ff.calc_g_noalloc(chk, 2e-3)
             g_na = ff.get("g")
-            print(variant, n, chk, "cost", c_ref, c_na, flush=True)
+            print(variant, nt, n, chk, "cost", c_ref, c_na, flush=True)
             assert c_na == c_ref
             if chk == 0:
                 assert np.array_equal(g_na.view(np.uint64), g_ref.view(np.uint64))
