@@ -86,9 +86,10 @@ struct mapp_dmd {
     mappdmd::DevBuf<float> d_sB, d_sBv, d_senv;  // sample baselines, log2 residual envelope
     mappdmd::SurNorm snorm{};
     bool sur_trained = false;
-    // the judge's validation errors (|prediction - quadrature| per output) and the
-    // cell of each validation sample, kept so the fallback mask can be re-derived
-    // for another force bound without retraining
+    // the validation errors of the quadrature check (k_sur_judge: |prediction -
+    // quadrature| per output) and the cell of each validation sample, kept so
+    // the fallback mask can be re-derived for another force bound without
+    // retraining
     std::vector<float> h_sjudge;
     std::vector<int> h_scell;
     long sur_nval = 0;

@@ -1,5 +1,5 @@
-"""The reference-side binding in INTEGRATION.md compiles against the
-reference's own headers.
+"""The reference-side binding (the ForceField subclass of the integration
+notes at the repository root) compiles against the reference's own headers.
 
 The block between the binding markers is written out as ff_dmd_gpu.h and
 compiled with g++ -fsyntax-only next to src/ff.h, src/atoms.h, ... of the
@@ -28,7 +28,7 @@ MPI_INC = "/opt/conda/include"
 def _binding():
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     m = re.search(r"<!-- binding:begin -->\s*```cpp\n(.*?)```\s*<!-- binding:end -->", text, re.S)
-    assert m, "INTEGRATION.md has no binding block"
+    assert m, "no binding block between the binding markers"
     return m.group(1)
 
 
