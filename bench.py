@@ -685,6 +685,15 @@ def main():
                 "speedup_vs_fp64_q5": r64q5["ms_per_step"] / rs["ms_per_step"],
                 "speedup_vs_mixed_q5": rmix["ms_per_step"] / rs["ms_per_step"],
                 "all_fallback": rs["all_fallback"], "curve": rs["curve"]}
+        # the same FP64 step on a three-type alloy (Ni-Cu-Pd, concentrations
+        # drawn over the simplex): nine type pairs per list pair
+        lat3 = W.fcc_alloy(args.cells, nt=3, seed=1)
+        r3 = run_path(lat3, W.johnson_tables(elements=("Ni", "Cu", "Pd")), 3, "fp64", max(3, args.steps // 2), 1)
+        paths["fp64_q3_three_types"] = {"site_evals_per_s": lat3.natms / (r3["ms_per_step"] / 1e3),
+                                        "ms_per_step": r3["ms_per_step"], "kernel_ms": r3["kernel_ms"][:4],
+                                        "schedule": r3["schedule"], "elements": ["Ni", "Cu", "Pd"]}
+        del lat3, r3
+        log("fp64 three types", paths["fp64_q3_three_types"])
         out["paths"] = paths
         if not args.no_lbfgs:
             out["lbfgs"] = run_lbfgs(lat, tables, 50)
