@@ -175,3 +175,29 @@ def test_oracle_matches_live_reference(variant):
     for k in COMPARED:
         got = np.asarray(out[k], dtype=np.float64).reshape(-1)
         assert np.array_equal(got.view(np.uint64), ref[k].reshape(-1).view(np.uint64)), (variant, k)
+
+
+@pytest.mark.skipif(not os.path.exists(HARNESS), reason="reference build absent (make -f oracle/Makefile.ref)")
+def test_reference_timing_mode():
+    """oracle/ref_run.time_force (the CPU baseline of bench.py): two processes
+    with a common start, each timing its calls of the reference's force_calc
+    on its own copy; their energy is the one the harness dumps in its normal
+    sequence for the same system"""
+    from mapp_amd import workload as W
+    from oracle import ref_run as RR
+    t = W.johnson_tables(nr=1000, nrho=1000)
+    lat = W.fcc_alloy(3, seed=17)
+    with tempfile.TemporaryDirectory() as tmp:
+        ts = RR.time_force(tmp, lat, t, 2, 3)
+        timed = RR.read_dump(os.path.join(tmp, "rank1.bin"))["en_st"]
+    assert ts.shape == (2, 3) and np.all(ts > 0)
+    with tempfile.TemporaryDirectory() as tmp:
+        cfg, fs, setfl = RR.write_inputs(tmp, "sys", lat, t)
+        a_path = os.path.join(tmp, "a.txt")
+        np.savetxt(a_path, np.zeros(lat.c.size), fmt="%.17g")
+        out = os.path.join(tmp, "full.bin")
+        job = os.path.join(tmp, "job")
+        RR.write_job(job, cfg, fs, setfl, "DMD_I", 3, out, a_path=a_path)
+        subprocess.run([HARNESS, job], check=True, stdout=subprocess.DEVNULL)
+        full = RR.read_dump(out)["en_st"]
+    assert np.array_equal(timed.view(np.uint64), full.view(np.uint64))
