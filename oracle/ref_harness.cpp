@@ -24,6 +24,10 @@
 // usage: ref_harness <job file>; the job file holds "key value..." lines:
 //   cfg <path>  fs <path>  setfl <path>  variant DMD|DMD_I|DMD_II
 //   coef r_crd no_i alpha_min alpha_max T  skin s  a <path>  beta b  out <path>
+//   variant EAM: MD mode, ForceField_eam reading the FS file (no setfl,
+//   coef, a or beta): the sequence of Min_LBFGS::init up to force_calc
+//   (src/min_l-bfgs.cpp:101-178), then force_calc(1) and energy_calc; output
+//   x_read, type, H, x_all, list_off, list, en_st, f, energy_calc
 //   time_force N (optional: after create_2nd_neigh_lst, N calls of
 //   force_calc(1), each timed on the steady clock; output t_force [N] seconds
 //   and en_st only)  go <path> (optional, with time_force: create
@@ -48,6 +52,7 @@
 #include "ff_dmd.h"
 #include "ff_dmd_I.h"
 #include "ff_dmd_II.h"
+#include "ff_eam.h"
 #include "neighbor.h"
 
 using namespace MAPP_NS;
@@ -132,6 +137,75 @@ char* dup(const std::string& s) {
 
 }  // namespace
 
+// MD mode, plain EAM (ForceField_eam with its FS reader, src/ff_eam.cpp:400-430)
+int run_eam(std::map<std::string, std::vector<std::string>>& job) {
+    const std::string script = job["out"].at(0) + ".in";
+    {
+        FILE* fp = fopen(script.c_str(), "w");
+        fprintf(fp, "mode md\nskin %s\nread cfg %s\n", job["skin"].at(0).c_str(), job["cfg"].at(0).c_str());
+        fclose(fp);
+    }
+    char* margs[] = {dup("ref_harness"), dup("-i"), dup(script), dup("-o"), dup(job["out"].at(0) + ".log")};
+    MAPP* mapp = new MAPP(5, margs, MPI_COMM_WORLD);
+    Atoms* atoms = mapp->atoms;
+    const int natms = atoms->natms;
+    const int xdim = atoms->vectors[0].dim;
+    const int type_n = atoms->find("type");
+    Out out{fopen(job["out"].at(0).c_str(), "wb")};
+    {
+        TYPE0* x;
+        atoms->vectors[0].ret(x);
+        out.put("x_read", natms, xdim, x);
+        int* type;
+        atoms->vectors[type_n].ret(type);
+        out.put_vec("type", natms, 1, type);
+        double H[9];
+        for (int i = 0; i < 9; i++) H[i] = atoms->H[i / 3][i % 3];
+        out.put("H", 3, 3, H);
+    }
+    ForceField_eam* ff = new ForceField_eam(mapp);
+    mapp->forcefield = ff;
+    char* a[] = {dup("ff_coef"), dup("FS"), dup(job["fs"].at(0))};
+    ff->coef(3, a);
+    if (atoms->find_exist("f") < 0) atoms->add<TYPE0>(0, xdim, "f");
+    VecLst* vecs_comm = new VecLst(mapp, 2, 0, type_n);
+    vecs_comm->add_update(0);
+    atoms->reset_comm(vecs_comm);
+    ff->init();
+    atoms->ph_setup(1, vecs_comm);
+    mapp->neighbor->init();
+    mapp->neighbor->create_list(0, 1);
+    atoms->store_0();
+    {
+        TYPE0* x;
+        atoms->vectors[0].ret(x);
+        out.put("x_all", natms + atoms->natms_ph, xdim, x);
+        std::vector<double> off(natms + 1, 0.0), ent;
+        for (int i = 0; i < natms; i++) {
+            off[i + 1] = off[i] + mapp->neighbor->neighbor_list_size[i];
+            for (int k = 0; k < mapp->neighbor->neighbor_list_size[i]; k++)
+                ent.push_back(mapp->neighbor->neighbor_list[i][k]);
+        }
+        out.put("list_off", natms + 1, 1, off.data());
+        out.put("list", (int)ent.size(), 1, ent.data());
+    }
+    // ForceField_eam::force_calc adds to f; the drivers clear it before every
+    // call (src/min_l-bfgs.cpp:232-236, :690-696)
+    TYPE0* f;
+    atoms->vectors[atoms->find("f")].ret(f);
+    for (int i = 0; i < natms * xdim; i++) f[i] = 0.0;
+    double en[7] = {0, 0, 0, 0, 0, 0, 0};
+    ff->force_calc(1, en);
+    out.put("en_st", 1, 7, en);
+    atoms->vectors[atoms->find("f")].ret(f);
+    out.put("f", natms, xdim, f);
+    const double ue = ff->energy_calc();
+    out.put("energy_calc", 1, 1, &ue);
+    fclose(out.fp);
+    MPI_Finalize();
+    return 0;
+}
+
 int main(int argc, char** argv) {
     MPI_Init(&argc, &argv);
     if (argc != 2) {
@@ -152,6 +226,7 @@ int main(int argc, char** argv) {
         fclose(fp);
     }
     const std::string variant = job["variant"].at(0);
+    if (variant == "EAM") return run_eam(job);
     const std::vector<std::string>& coef = job["coef"];
 
     // the input script: mode, skin, read cfg

@@ -56,6 +56,30 @@ def read_dump(path):
     return out
 
 
+def write_cfg_md(path, H, s, types, masses, names):
+    """an MD-mode extended CFG as read_cfg.cpp:480-530 reads it: reduced
+    positions only (entry_count 3, no velocities); first the mass and element
+    lines of every type in type order (AtomTypes::add_type numbers types by
+    first appearance, src/atom_types.cpp:34-58), then a mass line and an
+    element line before every atom whose type differs from the previous one,
+    so the atoms keep their order; %.17g throughout"""
+    with open(path, "w") as fh:
+        fh.write(f"Number of particles = {len(s)}\nA = 1 Angstrom (basic length-scale)\n")
+        for i in range(3):
+            for j in range(3):
+                fh.write(f"H0({i + 1},{j + 1}) = {H[i][j]:.17g} A\n")
+        fh.write(".NO_VELOCITY.\nentry_count = 3\n")
+        for t in range(len(names)):
+            fh.write(f"{masses[t]:.17g}\n{names[t]}\n")
+        last = len(names) - 1
+        for p, t in zip(s, types):
+            t = int(t)
+            if t != last:
+                fh.write(f"{masses[t]:.17g}\n{names[t]}\n")
+                last = t
+            fh.write(f"{p[0]:.17g} {p[1]:.17g} {p[2]:.17g}\n")
+
+
 def write_inputs(tmp, name, lat, tables):
     """extended CFG of the system and both table files; returns their paths"""
     from mapp_amd.dmd import write_cfg

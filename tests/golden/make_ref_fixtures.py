@@ -8,7 +8,11 @@ travel with the tests:
 
     make -f oracle/Makefile.ref && python tests/golden/make_ref_fixtures.py [case ...]
 
-Each case stores the tables it ran on (Johnson-style Ni-Cu on 2000-point grids
+ref_eam_*.npz hold the same for the plain EAM force field in MD mode
+(ForceField_eam reading the FS file): tables, box, positions and types as
+read, ghosts, the half list, force_calc's energy / virial / f and energy_calc.
+
+Each DMD case stores the tables it ran on (Johnson-style Ni-Cu on 2000-point grids
 for the two binary cases; Ni-Cu-Pd and Ni-Cu-Pd-Ag on 1000-point grids for the
 three- and four-type cases; the FS file's values), the box, the positions / widths / concentrations as
 the reference read them from the extended CFG, calc_g's a and beta, and per
@@ -26,7 +30,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
-from oracle.ref_run import COEF, HARNESS, read_dump, write_inputs, write_job  # noqa: E402
+from oracle.ref_run import COEF, HARNESS, read_dump, write_cfg_md, write_inputs, write_job  # noqa: E402
 
 # arrays that depend on the rule (through the list cut) but not on the variant
 GEOMETRY = ("x_all", "list_off", "list", "list2_off", "list2", "crd")
@@ -85,6 +89,28 @@ def run(case, lat, tables, variant, no_i, tmp):
     return d
 
 
+def eam_cases():
+    """case name -> (box, reduced positions, integer types, table grid points)
+    for the plain EAM force field in MD mode"""
+    from mapp_amd import workload as W
+    lat = W.fcc_alloy(4, seed=21)
+    types = (np.arange(lat.natms) % 3 == 0).astype(np.int64)
+    return {"eam_fcc4": (lat.H, np.ascontiguousarray(lat.x[:, :3]), types, 2000)}
+
+
+def run_eam(case, H, s, types, tables, tmp):
+    import mapp_amd.workload as W
+    cfg = os.path.join(tmp, f"{case}.cfg")
+    write_cfg_md(cfg, H, s, types, tables.masses, tables.elements)
+    fs = os.path.join(tmp, "tables.fs")
+    W.write_fs(fs, tables)
+    out = os.path.join(tmp, f"{case}.bin")
+    job = os.path.join(tmp, "job")
+    write_job(job, cfg, fs, fs, "EAM", 1, out)
+    subprocess.run([HARNESS, job], check=True, stdout=subprocess.DEVNULL)
+    return read_dump(out)
+
+
 def main():
     from mapp_amd import workload as W
     if not os.path.exists(HARNESS):
@@ -121,6 +147,19 @@ def main():
                     arrays[f"{variant}_q{no_i}_{k}"] = v
             print(case, variant, no_i, "energy", d["en_st"][0, 0], "natms", d["x_read"].shape[0], flush=True)
         path = os.path.join(HERE, f"ref_dmd_{case}.npz")
+        np.savez_compressed(path, **arrays)
+        print("wrote", path, os.path.getsize(path), "bytes")
+    for case, (H, s, types, ngrid) in eam_cases().items():
+        if wanted and case not in wanted:
+            continue
+        tables = W.johnson_tables(nr=ngrid, nrho=ngrid)
+        with tempfile.TemporaryDirectory() as tmp:
+            d = run_eam(case, H, s, types, tables, tmp)
+        assert np.array_equal(d["x_read"], s) and np.array_equal(d["type"].reshape(-1), types), case
+        arrays = dict(F=tables.F, rho=tables.rho, rphi=tables.rphi,
+                      grid=np.array([tables.nrho, tables.drho, tables.nr, tables.dr]),
+                      masses=np.array(tables.masses), skin=np.array(COEF["skin"]), **d)
+        path = os.path.join(HERE, f"ref_{case}.npz")
         np.savez_compressed(path, **arrays)
         print("wrote", path, os.path.getsize(path), "bytes")
 

@@ -86,3 +86,25 @@ def test_gpu_matches_reference_from_cfg(path, q, variant):
     for k, e in errs.items():
         assert e <= 1e-12, (k, e)
     ff.close()
+
+
+@pytest.mark.parametrize("path", R.eam_fixture_paths(), ids=[os.path.basename(p) for p in R.eam_fixture_paths()])
+def test_gpu_eam_matches_reference(path):
+    """the plain EAM path (ForceField_eam, MD mode) against the reference's own
+    outputs: ghosts and half list bit-exact; energy, virial, forces and
+    energy_calc within 1e-12 relative"""
+    from mapp_amd.dmd import setup_eam
+    d = R.load(path)
+    ff = setup_eam(d["H"], np.ascontiguousarray(d["x_read"]), d["type"].reshape(-1), R.eam_tables(d),
+                   skin=float(d["skin"]))
+    off, nl = ff.neighbor_list()
+    assert np.array_equal(off, d["list_off"].reshape(-1).astype(np.int32))
+    assert np.array_equal(nl, d["list"].reshape(-1).astype(np.int32))
+    en = np.asarray(ff.force_calc(1), dtype=np.float64)
+    errs = {"en_st": rel_inf(en, d["en_st"].reshape(-1)),
+            "f": rel_inf(np.asarray(ff.get("f")).reshape(d["f"].shape), d["f"]),
+            "energy_calc": abs(ff.energy_calc() - d["energy_calc"][0, 0]) / abs(d["energy_calc"][0, 0])}
+    print(os.path.basename(path), {k: f"{e:.1e}" for k, e in errs.items()}, flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-12, (k, e)
+    ff.close()

@@ -201,3 +201,54 @@ def test_reference_timing_mode():
         subprocess.run([HARNESS, job], check=True, stdout=subprocess.DEVNULL)
         full = RR.read_dump(out)["en_st"]
     assert np.array_equal(timed.view(np.uint64), full.view(np.uint64))
+
+
+def _oracle_eam(t, H, s, types, skin):
+    from oracle import oracle as O
+    o = O.Oracle(nt=len(t.elements), variant="EAM")
+    o.set_fs(t)
+    o.set_params(masses=t.masses)
+    o.set_system(H, np.ascontiguousarray(s).copy(), np.asarray(types, dtype=np.float64).reshape(-1, 1))
+    o.init(skin)
+    return o
+
+
+def _eam_matches(o, d):
+    off, nl = o.neighbor_list()
+    assert np.array_equal(off, d["list_off"].reshape(-1).astype(np.int32))
+    assert np.array_equal(nl, d["list"].reshape(-1).astype(np.int32))
+    assert np.array_equal(o.x_all()[:, :3].view(np.uint64), d["x_all"].view(np.uint64))
+    en = np.asarray(o.force_calc(1), dtype=np.float64)
+    assert np.array_equal(en.view(np.uint64), d["en_st"].reshape(-1).view(np.uint64))
+    f = np.asarray(o.get("f"), dtype=np.float64).reshape(d["f"].shape)
+    assert np.array_equal(f.view(np.uint64), d["f"].view(np.uint64))
+    u = np.array([o.energy_calc()])
+    assert np.array_equal(u.view(np.uint64), d["energy_calc"].reshape(-1).view(np.uint64))
+
+
+@pytest.mark.parametrize("path", R.eam_fixture_paths(), ids=[os.path.basename(p) for p in R.eam_fixture_paths()])
+def test_oracle_eam_matches_reference(path):
+    """the oracle's plain EAM loop (ForceField_eam, src/ff_eam.cpp:44-356) against
+    the reference's build in MD mode, bit for bit: ghosts, half list, energy,
+    virial, forces, energy_calc"""
+    d = R.load(path)
+    o = _oracle_eam(R.eam_tables(d), d["H"], d["x_read"], d["type"].reshape(-1), float(d["skin"]))
+    _eam_matches(o, d)
+
+
+@pytest.mark.skipif(not os.path.exists(HARNESS), reason="reference build absent (make -f oracle/Makefile.ref)")
+def test_oracle_eam_matches_live_reference():
+    """a system the fixture does not hold: 3^3 cells, types in runs of two"""
+    import importlib.util
+    from mapp_amd import workload as W
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(R.GOLDEN, "make_ref_fixtures.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    t = W.johnson_tables(nr=1000, nrho=1000)
+    lat = W.fcc_alloy(3, seed=914)
+    s = np.ascontiguousarray(lat.x[:, :3])
+    types = (np.arange(lat.natms) // 2 % 2).astype(np.int64)
+    with tempfile.TemporaryDirectory() as tmp:
+        d = mk.run_eam("live", lat.H, s, types, t, tmp)
+    assert np.array_equal(d["type"].reshape(-1), types)
+    _eam_matches(_oracle_eam(t, lat.H, s, types, 0.5), d)
