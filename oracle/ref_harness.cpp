@@ -24,10 +24,13 @@
 // usage: ref_harness <job file>; the job file holds "key value..." lines:
 //   cfg <path>  fs <path>  setfl <path>  variant DMD|DMD_I|DMD_II
 //   coef r_crd no_i alpha_min alpha_max T  skin s  a <path>  beta b  out <path>
-//   variant EAM: MD mode, ForceField_eam reading the FS file (no setfl,
-//   coef, a or beta): the sequence of Min_LBFGS::init up to force_calc
-//   (src/min_l-bfgs.cpp:101-178), then force_calc(1) and energy_calc; output
-//   x_read, type, H, x_all, list_off, list, en_st, f, energy_calc
+//   variant EAM | LJ | FS | FSM | MEAM: MD mode (no setfl, coef, a or beta):
+//   the sequence of Min_LBFGS::init up to force_calc (src/min_l-bfgs.cpp:101-178),
+//   then force_calc(1) and energy_calc; output x_read, type, H, x_all,
+//   list_off, list, en_st, f, energy_calc.  Coefficients: EAM the FS file
+//   ("fs"), LJ the ff_coef lines ("lj": five tokens a line, one line a type
+//   pair), FS and FSM the parameter file ("params"), MEAM the library and
+//   local files ("lib", "local")
 //   time_force N (optional: after create_2nd_neigh_lst, N calls of
 //   force_calc(1), each timed on the steady clock; output t_force [N] seconds
 //   and en_st only)  go <path> (optional, with time_force: create
@@ -53,6 +56,10 @@
 #include "ff_dmd_I.h"
 #include "ff_dmd_II.h"
 #include "ff_eam.h"
+#include "ff_fs.h"
+#include "ff_fsm.h"
+#include "ff_lj.h"
+#include "ff_meam.h"
 #include "neighbor.h"
 
 using namespace MAPP_NS;
@@ -137,8 +144,10 @@ char* dup(const std::string& s) {
 
 }  // namespace
 
-// MD mode, plain EAM (ForceField_eam with its FS reader, src/ff_eam.cpp:400-430)
-int run_eam(std::map<std::string, std::vector<std::string>>& job) {
+// MD mode: plain EAM (ForceField_eam with its FS reader, src/ff_eam.cpp:400-430),
+// LJ (src/ff_lj.cpp:70-100), FS / FSM (src/ff_fs.cpp:113, src/ff_fsm.cpp:111),
+// MEAM (src/ff_meam.cpp:2593)
+int run_md(std::map<std::string, std::vector<std::string>>& job, const std::string& variant) {
     const std::string script = job["out"].at(0) + ".in";
     {
         FILE* fp = fopen(script.c_str(), "w");
@@ -163,10 +172,40 @@ int run_eam(std::map<std::string, std::vector<std::string>>& job) {
         for (int i = 0; i < 9; i++) H[i] = atoms->H[i / 3][i % 3];
         out.put("H", 3, 3, H);
     }
-    ForceField_eam* ff = new ForceField_eam(mapp);
-    mapp->forcefield = ff;
-    char* a[] = {dup("ff_coef"), dup("FS"), dup(job["fs"].at(0))};
-    ff->coef(3, a);
+    ForceField* ff = nullptr;
+    if (variant == "EAM") {
+        ForceField_eam* f = new ForceField_eam(mapp);
+        mapp->forcefield = ff = f;
+        char* a[] = {dup("ff_coef"), dup("FS"), dup(job["fs"].at(0))};
+        f->coef(3, a);
+    } else if (variant == "LJ") {
+        ForceField_lj* f = new ForceField_lj(mapp);
+        mapp->forcefield = ff = f;
+        const std::vector<std::string>& t = job["lj"];
+        for (size_t k = 0; k + 5 <= t.size(); k += 5) {
+            char* a[] = {dup("ff_coef"), dup(t[k]), dup(t[k + 1]), dup(t[k + 2]), dup(t[k + 3]), dup(t[k + 4])};
+            f->coef(6, a);
+        }
+    } else if (variant == "FS" || variant == "FSM") {
+        char* a[] = {dup("ff_coef"), dup(job["params"].at(0))};
+        if (variant == "FS") {
+            ForceField_fs* f = new ForceField_fs(mapp);
+            mapp->forcefield = ff = f;
+            f->coef(2, a);
+        } else {
+            ForceField_fsm* f = new ForceField_fsm(mapp);
+            mapp->forcefield = ff = f;
+            f->coef(2, a);
+        }
+    } else if (variant == "MEAM") {
+        ForceField_meam* f = new ForceField_meam(mapp);
+        mapp->forcefield = ff = f;
+        char* a[] = {dup("ff_coef"), dup(job["lib"].at(0)), dup(job["local"].at(0))};
+        f->coef(3, a);
+    } else {
+        fprintf(stderr, "ref_harness: unknown variant %s\n", variant.c_str());
+        return 2;
+    }
     if (atoms->find_exist("f") < 0) atoms->add<TYPE0>(0, xdim, "f");
     VecLst* vecs_comm = new VecLst(mapp, 2, 0, type_n);
     vecs_comm->add_update(0);
@@ -189,8 +228,8 @@ int run_eam(std::map<std::string, std::vector<std::string>>& job) {
         out.put("list_off", natms + 1, 1, off.data());
         out.put("list", (int)ent.size(), 1, ent.data());
     }
-    // ForceField_eam::force_calc adds to f; the drivers clear it before every
-    // call (src/min_l-bfgs.cpp:232-236, :690-696)
+    // the MD force fields add to f; the drivers clear it before every call
+    // (src/min_l-bfgs.cpp:232-236, :690-696)
     TYPE0* f;
     atoms->vectors[atoms->find("f")].ret(f);
     for (int i = 0; i < natms * xdim; i++) f[i] = 0.0;
@@ -226,7 +265,8 @@ int main(int argc, char** argv) {
         fclose(fp);
     }
     const std::string variant = job["variant"].at(0);
-    if (variant == "EAM") return run_eam(job);
+    if (variant == "EAM" || variant == "LJ" || variant == "FS" || variant == "FSM" || variant == "MEAM")
+        return run_md(job, variant);
     const std::vector<std::string>& coef = job["coef"];
 
     // the input script: mode, skin, read cfg

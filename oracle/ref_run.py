@@ -106,6 +106,15 @@ def write_job(path, cfg, fs, setfl, variant, no_i, out, a_path=None, beta=0.37, 
             fh.write(f"go {go}\n")
 
 
+def write_md_job(path, cfg, variant, out, skin, **coef):
+    """an MD-mode job (variant EAM, LJ, FS, FSM or MEAM): the coefficient
+    files, or the LJ ff_coef tokens, as key -> value or list of values"""
+    with open(path, "w") as fh:
+        fh.write(f"cfg {cfg}\nvariant {variant}\nskin {skin!r}\nout {out}\n")
+        for k, v in coef.items():
+            fh.write(f"{k} " + (" ".join(str(e) for e in v) if isinstance(v, (list, tuple)) else str(v)) + "\n")
+
+
 def time_force(tmp, lat, tables, procs, reps, variant="DMD_I", no_i=3, timeout=600):
     """force_calc(1) of the reference on `procs` processes at once, each on its
     own copy of the system: every process builds its system, signals ready,

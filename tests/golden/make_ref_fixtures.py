@@ -8,9 +8,10 @@ travel with the tests:
 
     make -f oracle/Makefile.ref && python tests/golden/make_ref_fixtures.py [case ...]
 
-ref_eam_*.npz hold the same for the plain EAM force field in MD mode
-(ForceField_eam reading the FS file): tables, box, positions and types as
-read, ghosts, the half list, force_calc's energy / virial / f and energy_calc.
+ref_md_*.npz hold the same for the MD-mode force fields (plain EAM reading the
+FS file, LJ, FS, FSM, MEAM): variant, box, positions and types as read, ghosts,
+the half (MEAM: full) list, force_calc's energy / virial / f and energy_calc;
+the EAM case also its tables (the others are rebuilt from workload.py).
 
 Each DMD case stores the tables it ran on (Johnson-style Ni-Cu on 2000-point grids
 for the two binary cases; Ni-Cu-Pd and Ni-Cu-Pd-Ag on 1000-point grids for the
@@ -30,7 +31,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
-from oracle.ref_run import COEF, HARNESS, read_dump, write_cfg_md, write_inputs, write_job  # noqa: E402
+from oracle.ref_run import COEF, HARNESS, read_dump, write_cfg_md, write_inputs, write_job, write_md_job  # noqa: E402
 
 # arrays that depend on the rule (through the list cut) but not on the variant
 GEOMETRY = ("x_all", "list_off", "list", "list2_off", "list2", "crd")
@@ -89,24 +90,64 @@ def run(case, lat, tables, variant, no_i, tmp):
     return d
 
 
-def eam_cases():
-    """case name -> (box, reduced positions, integer types, table grid points)
-    for the plain EAM force field in MD mode"""
+MD_SKIN = 0.5
+
+
+def md_cases():
+    """case name -> (variant, box, reduced positions, integer types) for the
+    MD-mode force fields"""
     from mapp_amd import workload as W
+    cases = {}
     lat = W.fcc_alloy(4, seed=21)
-    types = (np.arange(lat.natms) % 3 == 0).astype(np.int64)
-    return {"eam_fcc4": (lat.H, np.ascontiguousarray(lat.x[:, :3]), types, 2000)}
+    cases["eam_fcc4"] = ("EAM", lat.H, np.ascontiguousarray(lat.x[:, :3]), np.arange(lat.natms) % 3 == 0)
+    cases["lj_fcc4"] = ("LJ",) + W.md_lattice(4, 3.6, "fcc", seed=3)
+    H, s, t = W.md_lattice(5, 2.87, "bcc", seed=5)
+    cases["fs_bcc5"] = ("FS", H, s, t)
+    cases["fsm_bcc5"] = ("FSM", H, s, t)
+    cases["meam_fcc4"] = ("MEAM",) + W.md_lattice(4, 3.58, "fcc", nt=2, seed=41, jitter=0.05)
+    return {k: (v, H, s, np.asarray(t).astype(np.int64)) for k, (v, H, s, t) in cases.items()}
 
 
-def run_eam(case, H, s, types, tables, tmp):
-    import mapp_amd.workload as W
+def md_coefficients(variant):
+    """element names, masses and the coefficient inputs of each MD variant, as
+    the tests rebuild them: EAM the Johnson tables on 2000-point grids, LJ
+    workload.LJ_COEFS, FS / FSM workload.fs_params(), MEAM workload.meam_state
+    of Aa and Bb with workload.meam_local(2)"""
+    from mapp_amd import workload as W
+    if variant == "EAM":
+        t = W.johnson_tables(nr=2000, nrho=2000)
+        return list(t.elements), list(t.masses), t
+    if variant == "LJ":
+        return ["A", "B"], [1.0, 1.0], W.LJ_COEFS
+    if variant in ("FS", "FSM"):
+        p = W.fs_params()
+        return list(p.elements), [1.0, 1.0], p
+    names = ["Aa", "Bb"]
+    return names, [W.MEAM_LIB[e]["atwt"] for e in names], W.meam_local(2)
+
+
+def run_md(case, variant, H, s, types, tmp):
+    from mapp_amd import workload as W
+    names, masses, co = md_coefficients(variant)
     cfg = os.path.join(tmp, f"{case}.cfg")
-    write_cfg_md(cfg, H, s, types, tables.masses, tables.elements)
-    fs = os.path.join(tmp, "tables.fs")
-    W.write_fs(fs, tables)
+    write_cfg_md(cfg, H, s, types, masses, names)
+    kw = {}
+    if variant == "EAM":
+        kw["fs"] = os.path.join(tmp, "tables.fs")
+        W.write_fs(kw["fs"], co)
+    elif variant == "LJ":
+        kw["lj"] = [v for a, b, eps, sig, cut in co for v in (names[a], names[b], repr(eps), repr(sig), repr(cut))]
+    elif variant in ("FS", "FSM"):
+        kw["params"] = os.path.join(tmp, "fs.params")
+        W.write_fs_params(kw["params"], co)
+    else:
+        kw["lib"] = os.path.join(tmp, "meam.library")
+        W.write_meam_lib(kw["lib"], names)
+        kw["local"] = os.path.join(tmp, "meam.local")
+        W.write_meam_local(kw["local"], co)
     out = os.path.join(tmp, f"{case}.bin")
     job = os.path.join(tmp, "job")
-    write_job(job, cfg, fs, fs, "EAM", 1, out)
+    write_md_job(job, cfg, variant, out, MD_SKIN, **kw)
     subprocess.run([HARNESS, job], check=True, stdout=subprocess.DEVNULL)
     return read_dump(out)
 
@@ -149,18 +190,20 @@ def main():
         path = os.path.join(HERE, f"ref_dmd_{case}.npz")
         np.savez_compressed(path, **arrays)
         print("wrote", path, os.path.getsize(path), "bytes")
-    for case, (H, s, types, ngrid) in eam_cases().items():
+    for case, (variant, H, s, types) in md_cases().items():
         if wanted and case not in wanted:
             continue
-        tables = W.johnson_tables(nr=ngrid, nrho=ngrid)
         with tempfile.TemporaryDirectory() as tmp:
-            d = run_eam(case, H, s, types, tables, tmp)
+            d = run_md(case, variant, H, s, types, tmp)
         assert np.array_equal(d["x_read"], s) and np.array_equal(d["type"].reshape(-1), types), case
-        arrays = dict(F=tables.F, rho=tables.rho, rphi=tables.rphi,
-                      grid=np.array([tables.nrho, tables.drho, tables.nr, tables.dr]),
-                      masses=np.array(tables.masses), skin=np.array(COEF["skin"]), **d)
-        path = os.path.join(HERE, f"ref_{case}.npz")
+        arrays = dict(variant=np.array(variant), skin=np.array(MD_SKIN), **d)
+        if variant == "EAM":
+            t = md_coefficients(variant)[2]
+            arrays.update(F=t.F, rho=t.rho, rphi=t.rphi, grid=np.array([t.nrho, t.drho, t.nr, t.dr]),
+                          masses=np.array(t.masses))
+        path = os.path.join(HERE, f"ref_md_{case}.npz")
         np.savez_compressed(path, **arrays)
+        print(case, variant, "energy", d["en_st"][0, 0], "natms", d["x_read"].shape[0], flush=True)
         print("wrote", path, os.path.getsize(path), "bytes")
 
 

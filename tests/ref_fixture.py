@@ -21,10 +21,11 @@ def fixture_paths():
     return sorted(glob.glob(os.path.join(GOLDEN, "ref_dmd_*.npz")))
 
 
-def eam_fixture_paths():
-    """the plain EAM force field in MD mode (ref_eam_*.npz): tables, box,
-    positions and types as read, x_all, list_off, list, en_st, f, energy_calc"""
-    return sorted(glob.glob(os.path.join(GOLDEN, "ref_eam_*.npz")))
+def md_fixture_paths():
+    """the MD-mode force fields (ref_md_*.npz: plain EAM, LJ, FS, FSM, MEAM):
+    variant, skin, box, positions and types as read, x_all, list_off, list,
+    en_st, f, energy_calc; the EAM case also its tables"""
+    return sorted(glob.glob(os.path.join(GOLDEN, "ref_md_*.npz")))
 
 
 def eam_tables(d):
@@ -32,6 +33,44 @@ def eam_tables(d):
     nrho, drho, nr, dr = d["grid"]
     return FSTables(elements=["Ni", "Cu", "Pd", "Ag"][: d["F"].shape[0]], nrho=int(nrho), drho=float(drho),
                     nr=int(nr), dr=float(dr), F=d["F"], rho=d["rho"], rphi=d["rphi"], masses=list(d["masses"]))
+
+
+def md_oracle(d):
+    """the oracle set up as the fixture's force field (coefficients rebuilt from
+    mapp_amd.workload as tests/golden/make_ref_fixtures.py wrote them)"""
+    from mapp_amd import workload as W
+    from oracle import oracle as O
+    variant = str(d["variant"])
+    o = O.Oracle(nt=2, variant=variant)
+    if variant == "EAM":
+        t = eam_tables(d)
+        o.set_fs(t)
+        o.set_params(masses=t.masses)
+    elif variant == "LJ":
+        for c in W.LJ_COEFS:
+            o.lj_coef(*c)
+    elif variant in ("FS", "FSM"):
+        o.fs_coef(W.fs_params())
+    else:
+        o.meam_coef(W.meam_state(["Aa", "Bb"], local=W.meam_local(2)))
+    o.set_system(d["H"], np.ascontiguousarray(d["x_read"]).copy(), d["type"].reshape(-1, 1).astype(np.float64))
+    o.init(float(d["skin"]))
+    return o
+
+
+def md_gpu(d):
+    """the HIP path set up as the fixture's force field"""
+    from mapp_amd import workload as W
+    from mapp_amd.dmd import setup_eam, setup_fs, setup_lj, setup_meam
+    variant, skin = str(d["variant"]), float(d["skin"])
+    H, s, t = d["H"], np.ascontiguousarray(d["x_read"]), d["type"].reshape(-1)
+    if variant == "EAM":
+        return setup_eam(H, s, t, eam_tables(d), skin=skin)
+    if variant == "LJ":
+        return setup_lj(H, s, t, W.LJ_COEFS, nt=2, skin=skin)
+    if variant in ("FS", "FSM"):
+        return setup_fs(H, s, t, W.fs_params(), skin=skin, variant=variant)
+    return setup_meam(H, s, t, W.meam_state(["Aa", "Bb"], local=W.meam_local(2)), skin=skin)
 
 
 def load(path):

@@ -88,15 +88,16 @@ def test_gpu_matches_reference_from_cfg(path, q, variant):
     ff.close()
 
 
-@pytest.mark.parametrize("path", R.eam_fixture_paths(), ids=[os.path.basename(p) for p in R.eam_fixture_paths()])
-def test_gpu_eam_matches_reference(path):
-    """the plain EAM path (ForceField_eam, MD mode) against the reference's own
-    outputs: ghosts and half list bit-exact; energy, virial, forces and
-    energy_calc within 1e-12 relative"""
-    from mapp_amd.dmd import setup_eam
+MD_PATHS = R.md_fixture_paths()
+
+
+@pytest.mark.parametrize("path", MD_PATHS, ids=[os.path.basename(p) for p in MD_PATHS])
+def test_gpu_md_matches_reference(path):
+    """the MD-mode paths (plain EAM, LJ, FS, FSM, MEAM) against the reference's
+    own outputs: the half (MEAM: full) list bit-exact; energy, virial, forces
+    and energy_calc within 1e-12 relative"""
     d = R.load(path)
-    ff = setup_eam(d["H"], np.ascontiguousarray(d["x_read"]), d["type"].reshape(-1), R.eam_tables(d),
-                   skin=float(d["skin"]))
+    ff = R.md_gpu(d)
     off, nl = ff.neighbor_list()
     assert np.array_equal(off, d["list_off"].reshape(-1).astype(np.int32))
     assert np.array_equal(nl, d["list"].reshape(-1).astype(np.int32))

@@ -203,17 +203,7 @@ def test_reference_timing_mode():
     assert np.array_equal(timed.view(np.uint64), full.view(np.uint64))
 
 
-def _oracle_eam(t, H, s, types, skin):
-    from oracle import oracle as O
-    o = O.Oracle(nt=len(t.elements), variant="EAM")
-    o.set_fs(t)
-    o.set_params(masses=t.masses)
-    o.set_system(H, np.ascontiguousarray(s).copy(), np.asarray(types, dtype=np.float64).reshape(-1, 1))
-    o.init(skin)
-    return o
-
-
-def _eam_matches(o, d):
+def _md_matches(o, d):
     off, nl = o.neighbor_list()
     assert np.array_equal(off, d["list_off"].reshape(-1).astype(np.int32))
     assert np.array_equal(nl, d["list"].reshape(-1).astype(np.int32))
@@ -226,29 +216,37 @@ def _eam_matches(o, d):
     assert np.array_equal(u.view(np.uint64), d["energy_calc"].reshape(-1).view(np.uint64))
 
 
-@pytest.mark.parametrize("path", R.eam_fixture_paths(), ids=[os.path.basename(p) for p in R.eam_fixture_paths()])
-def test_oracle_eam_matches_reference(path):
-    """the oracle's plain EAM loop (ForceField_eam, src/ff_eam.cpp:44-356) against
-    the reference's build in MD mode, bit for bit: ghosts, half list, energy,
-    virial, forces, energy_calc"""
+MD_PATHS = R.md_fixture_paths()
+
+
+@pytest.mark.parametrize("path", MD_PATHS, ids=[os.path.basename(p) for p in MD_PATHS])
+def test_oracle_md_matches_reference(path):
+    """the oracle's MD-mode loops (ForceField_eam src/ff_eam.cpp:44-356, _lj
+    src/ff_lj.cpp:160-341, _fs src/ff_fs.cpp:379-686, _fsm src/ff_fsm.cpp,
+    _meam src/ff_meam.cpp) against the reference's build, bit for bit: ghosts,
+    half (MEAM: full) list, energy, virial, forces, energy_calc"""
     d = R.load(path)
-    o = _oracle_eam(R.eam_tables(d), d["H"], d["x_read"], d["type"].reshape(-1), float(d["skin"]))
-    _eam_matches(o, d)
+    _md_matches(R.md_oracle(d), d)
 
 
 @pytest.mark.skipif(not os.path.exists(HARNESS), reason="reference build absent (make -f oracle/Makefile.ref)")
-def test_oracle_eam_matches_live_reference():
-    """a system the fixture does not hold: 3^3 cells, types in runs of two"""
+@pytest.mark.parametrize("variant", ["EAM", "LJ", "FS", "MEAM"])
+def test_oracle_md_matches_live_reference(variant):
+    """systems the fixtures do not hold: 3^3 cells, other seeds"""
     import importlib.util
     from mapp_amd import workload as W
     spec = importlib.util.spec_from_file_location("mk", os.path.join(R.GOLDEN, "make_ref_fixtures.py"))
     mk = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mk)
-    t = W.johnson_tables(nr=1000, nrho=1000)
-    lat = W.fcc_alloy(3, seed=914)
-    s = np.ascontiguousarray(lat.x[:, :3])
-    types = (np.arange(lat.natms) // 2 % 2).astype(np.int64)
+    kind, a = {"EAM": ("fcc", 3.6), "LJ": ("fcc", 3.6), "FS": ("bcc", 2.87), "MEAM": ("fcc", 3.58)}[variant]
+    H, s, t = W.md_lattice(3, a, kind, seed=915, frac=0.4)
+    t = np.asarray(t).astype(np.int64)
     with tempfile.TemporaryDirectory() as tmp:
-        d = mk.run_eam("live", lat.H, s, types, t, tmp)
-    assert np.array_equal(d["type"].reshape(-1), types)
-    _eam_matches(_oracle_eam(t, lat.H, s, types, 0.5), d)
+        d = mk.run_md("live", variant, H, s, t, tmp)
+    d.update(variant=np.array(variant), skin=np.array(mk.MD_SKIN), H=H)
+    if variant == "EAM":
+        tb = mk.md_coefficients("EAM")[2]
+        d.update(F=tb.F, rho=tb.rho, rphi=tb.rphi, grid=np.array([tb.nrho, tb.drho, tb.nr, tb.dr]),
+                 masses=np.array(tb.masses))
+    assert np.array_equal(d["type"].reshape(-1), t)
+    _md_matches(R.md_oracle(d), d)
