@@ -61,6 +61,8 @@ def lattice_cases():
     # over the simplex, every density slot rho_ab distinct
     cases["ternary4"] = (W.fcc_alloy(4, nt=3, seed=103), ("Ni", "Cu", "Pd"), 1000)
     cases["quaternary4"] = (W.fcc_alloy(4, nt=4, seed=104), ("Ni", "Cu", "Pd", "Ag"), 1000)
+    # one type (pure Ni, c ~ U[0.3, 1]: the vacancy concentration is 1 - c)
+    cases["unary4"] = (W.fcc_alloy(4, nt=1, seed=106), ("Ni",), 1000)
     # three types with widths in the three bands and concentration vectors on
     # calc_ent's margins (types that are absent, nearly absent or nearly pure)
     lat = W.fcc_alloy(4, nt=3, seed=105)

@@ -3,7 +3,7 @@ extended CFG file.
 
 The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
 _DMD_I and _DMD_II, compiled from the reference's sources, produced for two
-500-site binary systems and three 256-site systems of three and four types
+500-site binary systems and four 256-site systems of one, three and four types
 (per-(site, type) widths, asymmetric density tables, so every slot and index
 quirk acts; the second binary system and one three-type system with widths in
 all three regimes and concentrations on the margins).  Here the system goes through the

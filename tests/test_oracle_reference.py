@@ -2,7 +2,7 @@
 
 The golden fixtures (tests/golden/ref_dmd_*.npz) hold what ForceField_DMD,
 _DMD_I and _DMD_II, compiled from the reference's sources, produced for two
-500-site binary systems and three 256-site systems with three and four types
+500-site binary systems and four 256-site systems with one, three and four types
 (one of them with widths in all three regimes and concentrations on the margins):
 per-(site, type) widths (the
 density-slot transposition of src/ff_dmd_I.cpp:206 vs :376 and the shared phi
